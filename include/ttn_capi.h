@@ -1,0 +1,155 @@
+/*
+ * ttn_capi.h - C ABI of the MI355X TDVP library for spin-boson tree tensor networks.
+ *
+ * A drop-in for the one-site projector-splitting TDVP time step of the reference
+ * (llindoy/ttns_lib).  The library owns the tree layout, the per-node complex128
+ * tensor storage in HBM, the compiled sum-of-products operator tables and the sweep
+ * schedule; every time step runs on the GPU.  Plain C types only: no C++ types and no
+ * exceptions cross this boundary.  Complex numbers are interleaved (re, im) doubles;
+ * matrices are row-major.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   ttn_tree_create      httensor<T>(topology, capacity)            include/ttns_lib/httensor.hpp:128-288
+ *   ttn_sop_create       sop_operator<T>(nterms, mode_dims) + bind() include/ttns_lib/operators/sop_operator.hpp:252-392
+ *   ttn_integrator_create  {fixed_step_,subspace_expansion_}projector_splitting_integrator(A, H, krylov_dim,
+ *                        krylov_tol, nthreads) and initialise()     include/ttns_lib/integrators/projector_splitting_integrator.hpp:86-116,236-261
+ *                        (operator_container::resize compiles the SOP:  tdvp_core/operator_container.hpp:43-96)
+ *   ttn_state_set/get    httensor node matrix access (A[i]().as_matrix())  include/ttns_lib/ttn_nodes/httensor_node.hpp:20-285
+ *   ttn_step             integrator operator()(A, H, coeff)           projector_splitting_integrator.hpp:136-169,281-326
+ *   ttn_time             integrator t()                               projector_splitting_integrator.hpp:188,359
+ *   ttn_counters         nh_applications(), engine().Nonesite/Ntwosite  projector_splitting_integrator.hpp:190,361-365
+ *   ttn_expect           mel()(op, mode, A)                           include/ttns_lib/observables/matrix_element.hpp:264-314
+ *   ttn_traversal        tdvp_tree_traversal order                   include/ttns_lib/tdvp_core/tdvp_tree_traversal.hpp:37-87
+ *   ttn_sb_problem, ttn_sb_initial_state
+ *                        the spin-boson driver's topology, SOP binding and initial state
+ *                                                                     src/spin_boson.cpp:447-649, src/mode_data.hpp:136-361
+ * Error convention: every call returns TTN_OK or an error code; ttn_last_error() gives
+ * the message of the last failure on a context (reference: CALL_AND_HANDLE / RAISE_*
+ * exception chains).
+ */
+#ifndef TTN_CAPI_H
+#define TTN_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TTN_OK 0
+#define TTN_ERR_NUMERIC 1
+#define TTN_ERR_CAPACITY 2
+#define TTN_ERR_INVALID 3
+#define TTN_ERR_DEVICE 4
+
+#define TTN_OP_DENSE 0
+#define TTN_OP_DIAG 1
+#define TTN_OP_CSR 2
+#define TTN_OP_IDENTITY 3
+
+#define TTN_MODE_FIXED_STEP 0
+#define TTN_MODE_SUBSPACE_EXPANSION 1
+
+typedef struct ttn_ctx ttn_ctx;
+typedef struct ttn_tree ttn_tree;
+typedef struct ttn_sop ttn_sop;
+typedef struct ttn_integ ttn_integ;
+typedef struct ttn_batch ttn_batch;
+
+/* ---- context: one per GPU; owns a HIP stream ---- */
+int ttn_ctx_create(int device, ttn_ctx** out);
+int ttn_ctx_destroy(ttn_ctx* ctx);
+const char* ttn_last_error(const ttn_ctx* ctx);
+int ttn_version(int* major, int* minor);
+
+/* ---- tree (nodes in pre-order) ----
+ * parent[i]: parent id (-1 for node 0 only); hrank[i], cap_hrank[i]: bond dimension to
+ * the parent and its capacity; leaf_dim[i] > 0 marks a leaf node carrying a primitive
+ * mode of that dimension (modes are numbered in pre-order of the leaves). */
+int ttn_tree_create(ttn_ctx* ctx, int64_t n_nodes, const int64_t* parent, const int64_t* hrank,
+                    const int64_t* cap_hrank, const int64_t* leaf_dim, ttn_tree** out);
+int ttn_tree_destroy(ttn_tree* tree);
+/* node shape: number of children, rows (= product of mode dims), hrank, leaf mode index (-1 if branch) */
+int ttn_tree_node(const ttn_tree* tree, int64_t node, int64_t* nchild, int64_t* rows, int64_t* hrank,
+                  int64_t* leaf_index);
+int ttn_tree_size(const ttn_tree* tree, int64_t* n_nodes, int64_t* n_modes);
+/* sweep order; order may be NULL to query the length */
+int ttn_traversal(const ttn_tree* tree, int64_t* len, int64_t* order);
+
+/* ---- sum-of-products operator ----
+ * nops primitive operators; operator k acts on mode op_mode[k], has kind op_kind[k]
+ * (TTN_OP_*), complex data cdata[2*op_coff[k] .. 2*op_coff[k+1]) (dense d*d row-major,
+ * diagonal d, CSR values nnz), integer data idata[op_ioff[k] .. op_ioff[k+1]) (CSR only:
+ * rowptr d+1 then colind nnz), and belongs to the terms r[op_roff[k] .. op_roff[k+1]).
+ * Operators are bound in the order given (this fixes the per-mode operator index). */
+int ttn_sop_create(ttn_ctx* ctx, int64_t nmodes, const int64_t* mode_dims, int64_t nterms,
+                   const double* coeff, int64_t nops, const int32_t* op_mode, const int32_t* op_kind,
+                   const int64_t* op_coff, const double* cdata, const int64_t* op_ioff, const int64_t* idata,
+                   const int64_t* op_roff, const int64_t* r, ttn_sop** out);
+int ttn_sop_destroy(ttn_sop* sop);
+/* Compiled per-node term tables (operator_container's integer bookkeeping) serialised
+ * as int64, buf may be NULL to query the length.  Layout, node by node in pre-order:
+ *   ncommon_spf, ncommon_mf, nstandard, then per term:
+ *     id_spf, id_mf, nspf, {len, (nu, cri) x len} x nspf, nmf, {parent, nsib, (nu, cri) x nsib} x nmf */
+int ttn_sop_compiled_tables(const ttn_tree* tree, const ttn_sop* sop, int64_t* len, int64_t* buf);
+
+/* ---- integrator ---- */
+typedef struct {
+    int64_t krylov_dim;          /* maximum Krylov subspace dimension (reference default 4) */
+    double krylov_tol;           /* Krylov error tolerance (reference default 1e-12) */
+    double dt;                   /* time step */
+    int32_t mode;                /* TTN_MODE_* */
+    double spawning_threshold;   /* subspace expansion knobs (projector_splitting_integrator.hpp:341-357) */
+    double unoccupied_threshold;
+    int64_t minimum_unoccupied;
+    int64_t neigenvalues;
+    int64_t team_size;           /* workgroups cooperating on this instance (0 = automatic) */
+} ttn_integ_params;
+
+int ttn_integrator_create(ttn_ctx* ctx, const ttn_tree* tree, const ttn_sop* sop, const ttn_integ_params* p,
+                          ttn_integ** out);
+int ttn_integrator_destroy(ttn_integ* integ);
+/* node matrix, rows x cols must equal the node's current shape */
+int ttn_state_set(ttn_integ* integ, int64_t node, const double* data, int64_t rows, int64_t cols);
+int ttn_state_get(ttn_integ* integ, int64_t node, double* data, int64_t rows, int64_t cols);
+/* nsteps integrator calls A <- TDVP(A, H, coeff) (coeff = -i for real time) */
+int ttn_step(ttn_integ* integ, double coeff_re, double coeff_im, int64_t nsteps);
+int ttn_time(const ttn_integ* integ, double* t);
+/* cumulative node-propagation H applications and completed steps */
+int ttn_counters(const ttn_integ* integ, int64_t* nh_applications, int64_t* nsteps);
+/* <psi|O|psi> for a dense operator O (d x d complex) on primitive mode `mode` */
+int ttn_expect(ttn_integ* integ, int64_t mode, const double* op, double* out);
+/* <psi|psi> */
+int ttn_norm2(ttn_integ* integ, double* out);
+
+/* ---- spin-boson driver helpers (src/spin_boson.cpp:447-649) ---- */
+typedef struct {
+    int64_t nmodes;             /* bath modes, one per leaf */
+    const double* wk;           /* frequencies [nmodes] */
+    const double* gk;           /* couplings [nmodes] */
+    const int64_t* boson_dim;   /* Fock truncation [nmodes] */
+    int64_t nspf, nspf_lower, nspf_cap, nspf_lower_cap;
+    int64_t nhilb;              /* system dimension */
+    const double* hsys;         /* nhilb x nhilb complex */
+    const double* scoup;        /* nhilb x nhilb complex */
+    const double* psi0;         /* nhilb complex */
+    uint64_t seed;              /* std::mt19937 seed of the initial state */
+} ttn_sb_params;
+
+int ttn_sb_problem(ttn_ctx* ctx, const ttn_sb_params* p, ttn_tree** tree, ttn_sop** sop);
+int ttn_sb_initial_state(ttn_integ* integ, const ttn_sb_params* p);
+
+/* ---- batched persistent sweep: many instances advance in one launch ---- */
+int ttn_batch_create(ttn_ctx* ctx, int64_t n, ttn_integ* const* integs, ttn_batch** out);
+int ttn_batch_destroy(ttn_batch* batch);
+int ttn_batch_step(ttn_batch* batch, double coeff_re, double coeff_im, int64_t nsteps);
+
+/* ---- timing helpers (device-side event timing of the last ttn_step / ttn_batch_step) ---- */
+int ttn_last_step_ms(const ttn_integ* integ, double* ms);
+int ttn_batch_last_step_ms(const ttn_batch* batch, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TTN_CAPI_H */

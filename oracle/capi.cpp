@@ -1,0 +1,304 @@
+// ORACLE — test infrastructure only (see oracle.hpp).  C-ABI of the CPU restatement.
+#include "orc_capi.h"
+
+#include "../ttns_lib_amd/csrc/host/marshal.hpp"
+#include "../ttns_lib_amd/csrc/host/tables.hpp"
+#include "oracle.hpp"
+
+#include <memory>
+#include <string>
+
+using namespace ttnh;
+
+struct orc_ctx {
+    std::string err;
+};
+struct orc_tree {
+    orc_ctx* ctx;
+    Topology topo;
+};
+struct orc_sop {
+    orc_ctx* ctx;
+    SOP sop;
+};
+struct orc_integ {
+    orc_ctx* ctx;
+    std::unique_ptr<orc::Engine> eng;
+};
+
+namespace {
+template <class F>
+int guard(orc_ctx* ctx, F&& f) {
+    try {
+        f();
+        return TTN_OK;
+    } catch (const Error& e) {
+        if (ctx) ctx->err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->err = e.what();
+        return TTN_ERR_INVALID;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int orc_ctx_create(int, orc_ctx** out) {
+    *out = new orc_ctx();
+    return TTN_OK;
+}
+int orc_ctx_destroy(orc_ctx* ctx) {
+    delete ctx;
+    return TTN_OK;
+}
+const char* orc_last_error(const orc_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int orc_tree_create(orc_ctx* ctx, int64_t n, const int64_t* parent, const int64_t* hrank, const int64_t* cap_hrank,
+                    const int64_t* leaf_dim, orc_tree** out) {
+    return guard(ctx, [&] {
+        auto t = std::make_unique<orc_tree>();
+        t->ctx = ctx;
+        t->topo = topology_from_arrays(static_cast<int>(n), parent, hrank, cap_hrank, leaf_dim);
+        *out = t.release();
+    });
+}
+int orc_tree_destroy(orc_tree* t) {
+    delete t;
+    return TTN_OK;
+}
+int orc_tree_node(const orc_tree* t, int64_t node, int64_t* nchild, int64_t* rows, int64_t* hrank, int64_t* leaf) {
+    return guard(t->ctx, [&] {
+        check(node >= 0 && node < t->topo.size(), ERR_INVALID, "node out of range");
+        const Node& nd = t->topo.nodes[node];
+        if (nchild) *nchild = static_cast<int64_t>(nd.children.size());
+        if (rows) *rows = nd.dimen();
+        if (hrank) *hrank = nd.hrank;
+        if (leaf) *leaf = nd.leaf_index;
+    });
+}
+int orc_tree_size(const orc_tree* t, int64_t* n, int64_t* m) {
+    if (n) *n = t->topo.size();
+    if (m) *m = t->topo.nmodes();
+    return TTN_OK;
+}
+int orc_traversal(const orc_tree* t, int64_t* len, int64_t* order) {
+    return guard(t->ctx, [&] {
+        auto tour = euler_tour(t->topo);
+        *len = static_cast<int64_t>(tour.size());
+        if (order)
+            for (size_t i = 0; i < tour.size(); ++i) order[i] = tour[i];
+    });
+}
+
+int orc_sop_create(orc_ctx* ctx, int64_t nmodes, const int64_t* mode_dims, int64_t nterms, const double* coeff,
+                   int64_t nops, const int32_t* op_mode, const int32_t* op_kind, const int64_t* op_coff,
+                   const double* cdata, const int64_t* op_ioff, const int64_t* idata, const int64_t* op_roff,
+                   const int64_t* r, orc_sop** out) {
+    return guard(ctx, [&] {
+        auto s = std::make_unique<orc_sop>();
+        s->ctx = ctx;
+        s->sop = sop_from_arrays(nmodes, mode_dims, nterms, coeff, nops, op_mode, op_kind, op_coff, cdata, op_ioff,
+                                 idata, op_roff, r);
+        *out = s.release();
+    });
+}
+int orc_sop_destroy(orc_sop* s) {
+    delete s;
+    return TTN_OK;
+}
+int orc_sop_compiled_tables(const orc_tree* t, const orc_sop* s, int64_t* len, int64_t* buf) {
+    return guard(t->ctx, [&] {
+        auto v = serialise_tables(compile_sop(t->topo, s->sop));
+        if (buf) {
+            check(*len >= static_cast<int64_t>(v.size()), ERR_CAPACITY, "table buffer too small");
+            std::copy(v.begin(), v.end(), buf);
+        }
+        *len = static_cast<int64_t>(v.size());
+    });
+}
+
+int orc_integrator_create(orc_ctx* ctx, const orc_tree* t, const orc_sop* s, const ttn_integ_params* p,
+                          orc_integ** out) {
+    return guard(ctx, [&] {
+        check(p->mode == TTN_MODE_FIXED_STEP, ERR_INVALID, "oracle: only the fixed-step integrator is restated");
+        check(p->krylov_dim >= 1 && p->krylov_dim <= 64, ERR_INVALID, "krylov_dim must be in [1, 64]");
+        orc::Params pp;
+        pp.krylov_dim = static_cast<int>(p->krylov_dim);
+        pp.krylov_tol = p->krylov_tol;
+        pp.dt = p->dt;
+        auto in = std::make_unique<orc_integ>();
+        in->ctx = ctx;
+        in->eng = std::make_unique<orc::Engine>(t->topo, s->sop, pp);
+        *out = in.release();
+    });
+}
+int orc_integrator_destroy(orc_integ* in) {
+    delete in;
+    return TTN_OK;
+}
+
+int orc_state_set(orc_integ* in, int64_t node, const double* data, int64_t rows, int64_t cols) {
+    return guard(in->ctx, [&] {
+        const auto& topo = in->eng->topo();
+        check(node >= 0 && node < topo.size(), ERR_INVALID, "node out of range");
+        const Node& nd = topo.nodes[node];
+        check(rows == nd.dimen() && cols == nd.hrank, ERR_INVALID, "state_set: shape mismatch");
+        auto& A = in->eng->A(static_cast<int>(node));
+        for (int64_t i = 0; i < rows * cols; ++i) A[i] = cget(data, i);
+    });
+}
+int orc_state_get(orc_integ* in, int64_t node, double* data, int64_t rows, int64_t cols) {
+    return guard(in->ctx, [&] {
+        const auto& topo = in->eng->topo();
+        check(node >= 0 && node < topo.size(), ERR_INVALID, "node out of range");
+        const Node& nd = topo.nodes[node];
+        check(rows == nd.dimen() && cols == nd.hrank, ERR_INVALID, "state_get: shape mismatch");
+        const auto& A = in->eng->A(static_cast<int>(node));
+        for (int64_t i = 0; i < rows * cols; ++i) {
+            data[2 * i] = A[i].real();
+            data[2 * i + 1] = A[i].imag();
+        }
+    });
+}
+int orc_step(orc_integ* in, double cr, double ci, int64_t nsteps) {
+    return guard(in->ctx, [&] {
+        for (int64_t s = 0; s < nsteps; ++s) in->eng->step(cplx(cr, ci));
+    });
+}
+int orc_time(const orc_integ* in, double* t) {
+    *t = in->eng->time();
+    return TTN_OK;
+}
+int orc_counters(const orc_integ* in, int64_t* nh, int64_t* ns) {
+    if (nh) *nh = in->eng->nh_applications();
+    if (ns) *ns = in->eng->nsteps();
+    return TTN_OK;
+}
+int orc_expect(orc_integ* in, int64_t mode, const double* op, double* out) {
+    return guard(in->ctx, [&] {
+        const auto& topo = in->eng->topo();
+        check(mode >= 0 && mode < topo.nmodes(), ERR_INVALID, "mode out of range");
+        const i64 d = topo.mode_dims[mode];
+        std::vector<cplx> o(d * d);
+        for (i64 i = 0; i < d * d; ++i) o[i] = cget(op, i);
+        const cplx v = in->eng->expect(static_cast<int>(mode), o.data());
+        out[0] = v.real();
+        out[1] = v.imag();
+    });
+}
+int orc_norm2(orc_integ* in, double* out) {
+    return guard(in->ctx, [&] { *out = in->eng->expect(-1, nullptr).real(); });
+}
+
+int orc_sb_problem(orc_ctx* ctx, const ttn_sb_params* p, orc_tree** tree, orc_sop** sop) {
+    return guard(ctx, [&] {
+        SBProblem pr = sb_build(sb_from_c(p));
+        auto t = std::make_unique<orc_tree>();
+        t->ctx = ctx;
+        t->topo = pr.topo;
+        auto s = std::make_unique<orc_sop>();
+        s->ctx = ctx;
+        s->sop = pr.sop;
+        *tree = t.release();
+        *sop = s.release();
+    });
+}
+int orc_sb_initial_state(orc_integ* in, const ttn_sb_params* p) {
+    return guard(in->ctx, [&] {
+        SBParams sp = sb_from_c(p);
+        const auto& topo = in->eng->topo();
+        auto st = sb_initial_state(topo, sp.psi0, sp.seed);
+        for (int n = 0; n < topo.size(); ++n) in->eng->A(n) = st[n];
+    });
+}
+
+// ---- building blocks ----
+static std::vector<cplx> cvec(const double* p, i64 n) {
+    std::vector<cplx> v(n);
+    for (i64 i = 0; i < n; ++i) v[i] = cget(p, i);
+    return v;
+}
+static void cput(const std::vector<cplx>& v, double* p) {
+    for (size_t i = 0; i < v.size(); ++i) {
+        p[2 * i] = v[i].real();
+        p[2 * i + 1] = v[i].imag();
+    }
+}
+
+int orc_heff_node(orc_integ* in, int64_t node, const double* v, double* out) {
+    return guard(in->ctx, [&] {
+        const i64 sz = in->eng->topo().nodes.at(node).size();
+        auto x = cvec(v, sz);
+        std::vector<cplx> y(sz);
+        in->eng->heff_node(static_cast<int>(node), x.data(), y.data());
+        cput(y, out);
+    });
+}
+int orc_heff_bond(orc_integ* in, int64_t node, const double* v, double* out) {
+    return guard(in->ctx, [&] {
+        const i64 r = in->eng->topo().nodes.at(node).hrank;
+        auto x = cvec(v, r * r);
+        std::vector<cplx> y(r * r);
+        in->eng->heff_bond(static_cast<int>(node), x.data(), y.data());
+        cput(y, out);
+    });
+}
+int orc_update_spf(orc_integ* in, int64_t node) {
+    return guard(in->ctx, [&] { in->eng->update_spf(static_cast<int>(node)); });
+}
+int orc_update_mf(orc_integ* in, int64_t node) {
+    return guard(in->ctx, [&] { in->eng->update_mf(static_cast<int>(node)); });
+}
+int orc_env_get(orc_integ* in, int64_t node, int64_t term, int32_t which_mf, double* out) {
+    return guard(in->ctx, [&] {
+        const auto& m = which_mf ? in->eng->mf(static_cast<int>(node), static_cast<int>(term))
+                                 : in->eng->spf(static_cast<int>(node), static_cast<int>(term));
+        check(!m.empty(), ERR_INVALID, "identity environment has no matrix");
+        cput(m, out);
+    });
+}
+int orc_env_set(orc_integ* in, int64_t node, int64_t term, int32_t which_mf, const double* src) {
+    return guard(in->ctx, [&] {
+        auto& m = which_mf ? in->eng->mf_mut(static_cast<int>(node), static_cast<int>(term))
+                           : in->eng->spf_mut(static_cast<int>(node), static_cast<int>(term));
+        check(!m.empty(), ERR_INVALID, "identity environment has no matrix");
+        for (size_t i = 0; i < m.size(); ++i) m[i] = cget(src, static_cast<i64>(i));
+    });
+}
+int orc_expv(int64_t n, const double* Hd, double* x, double cr, double ci, double tau, int64_t kdim, double tol,
+             int64_t* napply) {
+    return guard(nullptr, [&] {
+        auto H = cvec(Hd, n * n);
+        auto v = cvec(x, n);
+        auto f = [&](const cplx* a, cplx* b) {
+            for (i64 i = 0; i < n; ++i) {
+                cplx s(0.0, 0.0);
+                for (i64 j = 0; j < n; ++j) s += H[i * n + j] * a[j];
+                b[i] = s;
+            }
+        };
+        const i64 na = orc::expv(f, v.data(), n, cplx(cr, ci), tau, static_cast<int>(kdim), tol);
+        if (napply) *napply = na;
+        cput(v, x);
+    });
+}
+int orc_expm(int64_t n, const double* M, double* E) {
+    return guard(nullptr, [&] {
+        auto m = cvec(M, n * n);
+        std::vector<cplx> e(n * n);
+        orc::expm(m.data(), static_cast<int>(n), e.data());
+        cput(e, E);
+    });
+}
+int orc_qr(int64_t m, int64_t n, const double* A, double* Q, double* R) {
+    return guard(nullptr, [&] {
+        auto a = cvec(A, m * n);
+        std::vector<cplx> q(m * n), r(n * n);
+        orc::householder_qr(a.data(), m, n, q.data(), r.data());
+        cput(q, Q);
+        cput(r, R);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,66 @@
+/*
+ * ORACLE — test infrastructure only.  CPU restatement of the TDVP step behind the same
+ * call signatures as include/ttn_capi.h, with the prefix orc_ instead of ttn_.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ */
+#ifndef ORC_CAPI_H
+#define ORC_CAPI_H
+
+#include "../include/ttn_capi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_ctx orc_ctx;
+typedef struct orc_tree orc_tree;
+typedef struct orc_sop orc_sop;
+typedef struct orc_integ orc_integ;
+
+int orc_ctx_create(int device, orc_ctx** out);
+int orc_ctx_destroy(orc_ctx* ctx);
+const char* orc_last_error(const orc_ctx* ctx);
+int orc_tree_create(orc_ctx* ctx, int64_t n_nodes, const int64_t* parent, const int64_t* hrank,
+                    const int64_t* cap_hrank, const int64_t* leaf_dim, orc_tree** out);
+int orc_tree_destroy(orc_tree* tree);
+int orc_tree_node(const orc_tree* tree, int64_t node, int64_t* nchild, int64_t* rows, int64_t* hrank,
+                  int64_t* leaf_index);
+int orc_tree_size(const orc_tree* tree, int64_t* n_nodes, int64_t* n_modes);
+int orc_traversal(const orc_tree* tree, int64_t* len, int64_t* order);
+int orc_sop_create(orc_ctx* ctx, int64_t nmodes, const int64_t* mode_dims, int64_t nterms, const double* coeff,
+                   int64_t nops, const int32_t* op_mode, const int32_t* op_kind, const int64_t* op_coff,
+                   const double* cdata, const int64_t* op_ioff, const int64_t* idata, const int64_t* op_roff,
+                   const int64_t* r, orc_sop** out);
+int orc_sop_destroy(orc_sop* sop);
+int orc_sop_compiled_tables(const orc_tree* tree, const orc_sop* sop, int64_t* len, int64_t* buf);
+int orc_integrator_create(orc_ctx* ctx, const orc_tree* tree, const orc_sop* sop, const ttn_integ_params* p,
+                          orc_integ** out);
+int orc_integrator_destroy(orc_integ* integ);
+int orc_state_set(orc_integ* integ, int64_t node, const double* data, int64_t rows, int64_t cols);
+int orc_state_get(orc_integ* integ, int64_t node, double* data, int64_t rows, int64_t cols);
+int orc_step(orc_integ* integ, double coeff_re, double coeff_im, int64_t nsteps);
+int orc_time(const orc_integ* integ, double* t);
+int orc_counters(const orc_integ* integ, int64_t* nh_applications, int64_t* nsteps);
+int orc_expect(orc_integ* integ, int64_t mode, const double* op, double* out);
+int orc_norm2(orc_integ* integ, double* out);
+int orc_sb_problem(orc_ctx* ctx, const ttn_sb_params* p, orc_tree** tree, orc_sop** sop);
+int orc_sb_initial_state(orc_integ* integ, const ttn_sb_params* p);
+
+/* building blocks for unit tests: apply the node / bond effective Hamiltonian with the
+ * current environments, refresh environments, and the dense kernels */
+int orc_heff_node(orc_integ* integ, int64_t node, const double* v, double* out);
+int orc_heff_bond(orc_integ* integ, int64_t node, const double* v, double* out);
+int orc_update_spf(orc_integ* integ, int64_t node);
+int orc_update_mf(orc_integ* integ, int64_t node);
+int orc_env_get(orc_integ* integ, int64_t node, int64_t term, int32_t which_mf, double* out);
+int orc_env_set(orc_integ* integ, int64_t node, int64_t term, int32_t which_mf, const double* in);
+int orc_expv(int64_t n, const double* Hdense, double* x, double c_re, double c_im, double tau, int64_t kdim,
+             double tol, int64_t* napply);
+int orc_expm(int64_t n, const double* M, double* E);
+int orc_qr(int64_t m, int64_t n, const double* A, double* Q, double* R);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ORC_CAPI_H */

@@ -1,0 +1,374 @@
+"""ctypes bindings of the C ABI declared in include/ttn_capi.h.
+
+`Library(path, prefix)` binds every entry point of a shared library that exports the
+ttn_* call signatures under `prefix` ("ttn_" for the HIP library).  `Problem`,
+`Integrator` and `Batch` are thin Python handles over those calls, used by the tests,
+the benchmark and the autograd harness.  Failures raise `TTNError` with the library's
+own message, mirroring the reference's exception chains
+(integrators/projector_splitting_integrator.hpp:321-325).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+i64 = C.c_int64
+i64p = C.POINTER(C.c_int64)
+i32p = C.POINTER(C.c_int32)
+dblp = C.POINTER(C.c_double)
+vp = C.c_void_p
+
+STATUS = {0: "OK", 1: "ERR_NUMERIC", 2: "ERR_CAPACITY", 3: "ERR_INVALID", 4: "ERR_DEVICE"}
+OP_DENSE, OP_DIAG, OP_CSR, OP_IDENTITY = 0, 1, 2, 3
+MODE_FIXED_STEP, MODE_SUBSPACE_EXPANSION = 0, 1
+
+
+class TTNError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class IntegParams(C.Structure):
+    _fields_ = [
+        ("krylov_dim", C.c_int64),
+        ("krylov_tol", C.c_double),
+        ("dt", C.c_double),
+        ("mode", C.c_int32),
+        ("spawning_threshold", C.c_double),
+        ("unoccupied_threshold", C.c_double),
+        ("minimum_unoccupied", C.c_int64),
+        ("neigenvalues", C.c_int64),
+        ("team_size", C.c_int64),
+    ]
+
+
+class SBParamsC(C.Structure):
+    _fields_ = [
+        ("nmodes", C.c_int64),
+        ("wk", dblp),
+        ("gk", dblp),
+        ("boson_dim", i64p),
+        ("nspf", C.c_int64),
+        ("nspf_lower", C.c_int64),
+        ("nspf_cap", C.c_int64),
+        ("nspf_lower_cap", C.c_int64),
+        ("nhilb", C.c_int64),
+        ("hsys", dblp),
+        ("scoup", dblp),
+        ("psi0", dblp),
+        ("seed", C.c_uint64),
+    ]
+
+
+# name -> argtypes (every function returns int status unless listed in _RESTYPE)
+_SIGS = {
+    "ctx_create": [C.c_int, C.POINTER(vp)],
+    "ctx_destroy": [vp],
+    "last_error": [vp],
+    "tree_create": [vp, i64, i64p, i64p, i64p, i64p, C.POINTER(vp)],
+    "tree_destroy": [vp],
+    "tree_node": [vp, i64, i64p, i64p, i64p, i64p],
+    "tree_size": [vp, i64p, i64p],
+    "traversal": [vp, i64p, i64p],
+    "sop_create": [vp, i64, i64p, i64, dblp, i64, i32p, i32p, i64p, dblp, i64p, i64p, i64p, i64p, C.POINTER(vp)],
+    "sop_destroy": [vp],
+    "sop_compiled_tables": [vp, vp, i64p, i64p],
+    "integrator_create": [vp, vp, vp, C.POINTER(IntegParams), C.POINTER(vp)],
+    "integrator_destroy": [vp],
+    "state_set": [vp, i64, dblp, i64, i64],
+    "state_get": [vp, i64, dblp, i64, i64],
+    "step": [vp, C.c_double, C.c_double, i64],
+    "time": [vp, dblp],
+    "counters": [vp, i64p, i64p],
+    "expect": [vp, i64, dblp, dblp],
+    "norm2": [vp, dblp],
+    "sb_problem": [vp, C.POINTER(SBParamsC), C.POINTER(vp), C.POINTER(vp)],
+    "sb_initial_state": [vp, C.POINTER(SBParamsC)],
+    "batch_create": [vp, i64, C.POINTER(vp), C.POINTER(vp)],
+    "batch_destroy": [vp],
+    "batch_step": [vp, C.c_double, C.c_double, i64],
+    "last_step_ms": [vp, dblp],
+    "batch_last_step_ms": [vp, dblp],
+}
+_RESTYPE = {"last_error": C.c_char_p}
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(dblp)
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(i64p)
+
+
+def _cplx_in(a) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.complex128))
+    return a.view(np.float64)
+
+
+class Library:
+    """Binds the ttn_* signatures exported under `prefix` by the shared library `path`."""
+
+    def __init__(self, path: str, prefix: str = "ttn_", extra: Optional[dict] = None):
+        self.path = path
+        self.prefix = prefix
+        self.so = C.CDLL(path)
+        self.fn = {}
+        sigs = dict(_SIGS)
+        if extra:
+            sigs.update(extra)
+        for name, args in sigs.items():
+            sym = prefix + name
+            if not hasattr(self.so, sym):
+                continue
+            f = getattr(self.so, sym)
+            f.argtypes = args
+            f.restype = _RESTYPE.get(name, C.c_int)
+            self.fn[name] = f
+        self.ctx = vp()
+        self._call("ctx_create", 0, C.byref(self.ctx))
+
+    def has(self, name: str) -> bool:
+        return name in self.fn
+
+    def error(self) -> str:
+        msg = self.fn["last_error"](self.ctx)
+        return msg.decode() if msg else ""
+
+    def _call(self, name, *args):
+        rc = self.fn[name](*args)
+        if rc != 0:
+            raise TTNError(rc, f"{self.prefix}{name}: " + (self.error() if name != "ctx_create" else ""))
+        return rc
+
+    def close(self):
+        if self.ctx:
+            self.fn["ctx_destroy"](self.ctx)
+            self.ctx = vp()
+
+
+@dataclass
+class SpinBoson:
+    """Spin-boson model with one bath mode per leaf (src/spin_boson.cpp:447-649)."""
+
+    wk: Sequence[float]
+    gk: Sequence[float]
+    boson_dim: Sequence[int]
+    nspf: int = 8
+    nspf_lower: int = 8
+    nspf_cap: int = 0
+    nspf_lower_cap: int = 0
+    hsys: np.ndarray = field(default_factory=lambda: np.array([[0, 1], [1, 0]], dtype=complex))
+    scoup: np.ndarray = field(default_factory=lambda: np.array([[1, 0], [0, -1]], dtype=complex))
+    psi0: np.ndarray = field(default_factory=lambda: np.array([1, 0], dtype=complex))
+    seed: int = 0
+
+    def to_c(self) -> SBParamsC:
+        self._keep = [
+            np.ascontiguousarray(self.wk, dtype=np.float64),
+            np.ascontiguousarray(self.gk, dtype=np.float64),
+            np.ascontiguousarray(self.boson_dim, dtype=np.int64),
+            _cplx_in(self.hsys),
+            _cplx_in(self.scoup),
+            _cplx_in(self.psi0),
+        ]
+        wk, gk, bd, hs, sc, ps = self._keep
+        nh = int(np.asarray(self.hsys).shape[0])
+        return SBParamsC(
+            len(wk), _dp(wk), _dp(gk), _ip(bd), self.nspf, self.nspf_lower,
+            max(self.nspf_cap, self.nspf), max(self.nspf_lower_cap, self.nspf_lower), nh,
+            _dp(hs), _dp(sc), _dp(ps), self.seed,
+        )
+
+
+class Tree:
+    def __init__(self, lib: Library, handle: vp):
+        self.lib, self.h = lib, handle
+
+    @classmethod
+    def create(cls, lib: Library, parent, hrank, cap_hrank, leaf_dim):
+        arrs = [np.ascontiguousarray(x, dtype=np.int64) for x in (parent, hrank, cap_hrank, leaf_dim)]
+        h = vp()
+        lib._call("tree_create", lib.ctx, len(arrs[0]), *[_ip(a) for a in arrs], C.byref(h))
+        return cls(lib, h)
+
+    def size(self):
+        n, m = i64(), i64()
+        self.lib._call("tree_size", self.h, C.byref(n), C.byref(m))
+        return n.value, m.value
+
+    def node(self, i):
+        a, b, c, d = i64(), i64(), i64(), i64()
+        self.lib._call("tree_node", self.h, i, C.byref(a), C.byref(b), C.byref(c), C.byref(d))
+        return dict(nchild=a.value, rows=b.value, hrank=c.value, leaf_index=d.value)
+
+    def shapes(self):
+        n, _ = self.size()
+        return [(self.node(i)["rows"], self.node(i)["hrank"]) for i in range(n)]
+
+    def traversal(self):
+        n = i64()
+        self.lib._call("traversal", self.h, C.byref(n), None)
+        out = np.zeros(n.value, dtype=np.int64)
+        self.lib._call("traversal", self.h, C.byref(n), _ip(out))
+        return out
+
+    def destroy(self):
+        if self.h:
+            self.lib.fn["tree_destroy"](self.h)
+            self.h = vp()
+
+
+class SOPHandle:
+    def __init__(self, lib: Library, handle: vp):
+        self.lib, self.h = lib, handle
+
+    @classmethod
+    def create(cls, lib: Library, mode_dims, nterms, ops, coeff=None):
+        """ops: list of dicts {mode, kind, data (complex array), idata (int array, CSR), r (terms)}."""
+        mode_dims = np.ascontiguousarray(mode_dims, dtype=np.int64)
+        coeff = np.ones(nterms, dtype=complex) if coeff is None else np.asarray(coeff, dtype=complex)
+        op_mode = np.array([o["mode"] for o in ops], dtype=np.int32)
+        op_kind = np.array([o["kind"] for o in ops], dtype=np.int32)
+        cd, idd, rr = [], [], []
+        coff, ioff, roff = [0], [0], [0]
+        for o in ops:
+            d = np.asarray(o.get("data", np.zeros(0)), dtype=complex).ravel()
+            cd.append(d)
+            coff.append(coff[-1] + d.size)
+            ii = np.asarray(o.get("idata", np.zeros(0)), dtype=np.int64).ravel()
+            idd.append(ii)
+            ioff.append(ioff[-1] + ii.size)
+            r = np.asarray(o["r"], dtype=np.int64).ravel()
+            rr.append(r)
+            roff.append(roff[-1] + r.size)
+        cdata = _cplx_in(np.concatenate(cd) if cd else np.zeros(0, complex))
+        idata = np.ascontiguousarray(np.concatenate(idd) if idd else np.zeros(0), dtype=np.int64)
+        rdat = np.ascontiguousarray(np.concatenate(rr) if rr else np.zeros(0), dtype=np.int64)
+        coff, ioff, roff = (np.array(x, dtype=np.int64) for x in (coff, ioff, roff))
+        cf = _cplx_in(coeff)
+        h = vp()
+        lib._call(
+            "sop_create", lib.ctx, len(mode_dims), _ip(mode_dims), nterms, _dp(cf), len(ops),
+            op_mode.ctypes.data_as(i32p), op_kind.ctypes.data_as(i32p), _ip(coff), _dp(cdata),
+            _ip(ioff), _ip(idata), _ip(roff), _ip(rdat), C.byref(h),
+        )
+        return cls(lib, h)
+
+    def destroy(self):
+        if self.h:
+            self.lib.fn["sop_destroy"](self.h)
+            self.h = vp()
+
+
+def compiled_tables(lib: Library, tree: Tree, sop: SOPHandle) -> np.ndarray:
+    n = i64(0)
+    lib._call("sop_compiled_tables", tree.h, sop.h, C.byref(n), None)
+    buf = np.zeros(n.value, dtype=np.int64)
+    lib._call("sop_compiled_tables", tree.h, sop.h, C.byref(n), _ip(buf))
+    return buf
+
+
+def sb_problem(lib: Library, model: SpinBoson):
+    t, s = vp(), vp()
+    p = model.to_c()
+    lib._call("sb_problem", lib.ctx, C.byref(p), C.byref(t), C.byref(s))
+    return Tree(lib, t), SOPHandle(lib, s)
+
+
+class Integrator:
+    def __init__(self, lib: Library, tree: Tree, sop: SOPHandle, *, dt: float, krylov_dim: int = 12,
+                 krylov_tol: float = 1e-12, mode: int = MODE_FIXED_STEP, team_size: int = 0,
+                 spawning_threshold: float = 1e-12, unoccupied_threshold: float = 1e-12,
+                 minimum_unoccupied: int = 0, neigenvalues: int = 2):
+        self.lib, self.tree = lib, tree
+        self.params = IntegParams(krylov_dim, krylov_tol, dt, mode, spawning_threshold, unoccupied_threshold,
+                                  minimum_unoccupied, neigenvalues, team_size)
+        self.h = vp()
+        lib._call("integrator_create", lib.ctx, tree.h, sop.h, C.byref(self.params), C.byref(self.h))
+        self.shapes = tree.shapes()
+
+    def set_sb_initial_state(self, model: SpinBoson):
+        p = model.to_c()
+        self.lib._call("sb_initial_state", self.h, C.byref(p))
+
+    def set_node(self, i, a):
+        a = np.asarray(a, dtype=complex)
+        rows, cols = self.shapes[i]
+        assert a.shape == (rows, cols), (a.shape, rows, cols)
+        buf = _cplx_in(a)
+        self.lib._call("state_set", self.h, i, _dp(buf), rows, cols)
+
+    def get_node(self, i):
+        rows, cols = self.shapes[i]
+        buf = np.zeros(2 * rows * cols, dtype=np.float64)
+        self.lib._call("state_get", self.h, i, _dp(buf), rows, cols)
+        return buf.view(np.complex128).reshape(rows, cols)
+
+    def step(self, nsteps: int = 1, coeff: complex = -1j):
+        self.lib._call("step", self.h, coeff.real, coeff.imag, nsteps)
+
+    def time(self):
+        t = C.c_double()
+        self.lib._call("time", self.h, C.byref(t))
+        return t.value
+
+    def counters(self):
+        a, b = i64(), i64()
+        self.lib._call("counters", self.h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def expect(self, mode: int, op) -> complex:
+        o = _cplx_in(op)
+        out = np.zeros(2)
+        self.lib._call("expect", self.h, mode, _dp(o), _dp(out))
+        return complex(out[0], out[1])
+
+    def norm2(self) -> float:
+        out = C.c_double()
+        self.lib._call("norm2", self.h, C.byref(out))
+        return out.value
+
+    def last_step_ms(self) -> float:
+        out = C.c_double()
+        self.lib._call("last_step_ms", self.h, C.byref(out))
+        return out.value
+
+    def destroy(self):
+        if self.h:
+            self.lib.fn["integrator_destroy"](self.h)
+            self.h = vp()
+
+
+class Batch:
+    def __init__(self, lib: Library, integs: Sequence[Integrator]):
+        self.lib = lib
+        self.integs = list(integs)
+        arr = (vp * len(integs))(*[i.h for i in integs])
+        self.h = vp()
+        lib._call("batch_create", lib.ctx, len(integs), arr, C.byref(self.h))
+
+    def step(self, nsteps: int = 1, coeff: complex = -1j):
+        self.lib._call("batch_step", self.h, coeff.real, coeff.imag, nsteps)
+
+    def last_step_ms(self) -> float:
+        out = C.c_double()
+        self.lib._call("batch_last_step_ms", self.h, C.byref(out))
+        return out.value
+
+    def destroy(self):
+        if self.h:
+            self.lib.fn["batch_destroy"](self.h)
+            self.h = vp()
+
+
+SIGMA = {
+    "x": np.array([[0, 1], [1, 0]], dtype=complex),
+    "y": np.array([[0, -1j], [1j, 0]], dtype=complex),
+    "z": np.array([[1, 0], [0, -1]], dtype=complex),
+    "1": np.eye(2, dtype=complex),
+}

@@ -1,0 +1,153 @@
+#include "spin_boson.hpp"
+
+#include <cmath>
+#include <random>
+
+namespace ttnh {
+
+Topology sb_topology(const SBParams& p) {
+    const size_t nblocks = p.boson_dim.size();
+    check(nblocks >= 1, ERR_INVALID, "spin-boson model needs at least one bath mode");
+    const size_t nlevels = static_cast<size_t>(std::log2(static_cast<double>(nblocks)));
+    auto level_fn = [nlevels](i64 top, i64 bottom) {
+        return [nlevels, top, bottom](size_t l) -> i64 {
+            if (l >= nlevels) return bottom;
+            if (l == 0) return top;
+            const double rmax = std::log2(static_cast<double>(top));
+            const double rmin = std::log2(static_cast<double>(bottom));
+            return static_cast<i64>(
+                std::pow(2.0, ((nlevels - l) * static_cast<double>(rmax - rmin)) / nlevels + rmin));
+        };
+    };
+    auto build = [&](i64 top, i64 bottom) {
+        NTree t(1);
+        t.insert(p.nhilb).insert(p.nhilb);
+        NTree& bath = t.insert(p.nhilb);
+        htucker_subtree(bath, p.boson_dim, 2, level_fn(top, bottom));
+        sanitise_tree(t);
+        return t;
+    };
+    NTree topo = build(p.nspf, p.nspf_lower);
+    NTree cap = build(std::max(p.nspf, p.nspf_cap), std::max(p.nspf_lower, p.nspf_lower_cap));
+    return topology_from_ntree(topo, cap);
+}
+
+PrimOp harmonic_h0(double w, i64 d) {
+    PrimOp op;
+    op.kind = OP_DIAG;
+    op.dim = d;
+    op.diag.resize(d);
+    for (i64 n = 0; n < d; ++n) op.diag[n] = cplx(w * (static_cast<double>(n) + 0.5 - 0.5), 0.0);
+    return op;
+}
+
+PrimOp harmonic_hc(double g, i64 d) {
+    // row n: lowering entry (n-1) then raising entry (n+1)
+    PrimOp op;
+    op.kind = OP_CSR;
+    op.dim = d;
+    op.rowptr.push_back(0);
+    for (i64 n = 0; n < d; ++n) {
+        if (n > 0) {
+            op.colind.push_back(n - 1);
+            op.vals.push_back(cplx(std::sqrt(static_cast<double>(n)) * g, 0.0));
+        }
+        if (n + 1 < d) {
+            op.colind.push_back(n + 1);
+            op.vals.push_back(cplx(std::sqrt(static_cast<double>(n) + 1.0) * g, 0.0));
+        }
+        op.rowptr.push_back(static_cast<i64>(op.vals.size()));
+    }
+    return op;
+}
+
+std::vector<std::vector<cplx>> sb_initial_state(const Topology& t, const std::vector<cplx>& psi0,
+                                                uint64_t seed) {
+    std::mt19937 rng(static_cast<std::mt19937::result_type>(seed));
+    std::vector<std::vector<cplx>> A(t.size());
+    for (int n = 0; n < t.size(); ++n) {
+        const Node& nd = t.nodes[n];
+        const i64 D = nd.dimen(), r = nd.hrank;
+        A[n].assign(D * r, cplx(0.0, 0.0));
+        if (nd.is_leaf()) {
+            // ct: r x D, row i = i-th single-particle function
+            std::vector<cplx> ct(r * D, cplx(0.0, 0.0));
+            if (nd.leaf_index == 0) {
+                check(static_cast<i64>(psi0.size()) == D, ERR_INVALID, "psi0 has the wrong size");
+                for (i64 j = 0; j < D; ++j) ct[j] = psi0[j];
+            } else {
+                ct[0] = cplx(1.0, 0.0);
+            }
+            std::uniform_real_distribution<double> dist(0, 2.0 * std::acos(-1.0));
+            std::normal_distribution<double> length_dist(0, 1);
+            for (i64 i = 1; i < r; ++i) {
+                bool generated = false;
+                while (!generated) {
+                    for (i64 j = 0; j < D; ++j) {
+                        const double theta = dist(rng);
+                        const double len = length_dist(rng);
+                        ct[i * D + j] = len * cplx(std::cos(theta), std::sin(theta));
+                    }
+                    double nrm = 0.0;
+                    for (i64 j = 0; j < D; ++j) nrm += std::norm(ct[i * D + j]);
+                    nrm = std::sqrt(nrm);
+                    for (i64 j = 0; j < D; ++j) ct[i * D + j] /= nrm;
+                    for (i64 k = 0; k < i; ++k) {
+                        cplx ov(0.0, 0.0);
+                        for (i64 j = 0; j < D; ++j) ov += std::conj(ct[k * D + j]) * ct[i * D + j];
+                        for (i64 j = 0; j < D; ++j) ct[i * D + j] -= ov * ct[k * D + j];
+                    }
+                    double nn = 0.0;
+                    for (i64 j = 0; j < D; ++j) nn += std::norm(ct[i * D + j]);
+                    nn = std::sqrt(nn);
+                    if (nn > 1e-12) {
+                        for (i64 j = 0; j < D; ++j) ct[i * D + j] /= nn;
+                        generated = true;
+                    }
+                }
+            }
+            for (i64 j = 0; j < D; ++j)
+                for (i64 i = 0; i < r; ++i) A[n][j * r + i] = ct[i * D + j];
+        } else if (!nd.is_root()) {
+            for (i64 i = 0; i < std::min(D, r); ++i) A[n][i * r + i] = cplx(1.0, 0.0);
+        } else {
+            A[n][0] = cplx(1.0, 0.0);
+        }
+    }
+    return A;
+}
+
+SBProblem sb_build(const SBParams& p) {
+    const size_t nb = p.boson_dim.size();
+    check(p.wk.size() == nb && p.gk.size() == nb, ERR_INVALID, "wk, gk and boson_dim must have equal length");
+    check(static_cast<i64>(p.hsys.size()) == p.nhilb * p.nhilb &&
+              static_cast<i64>(p.scoup.size()) == p.nhilb * p.nhilb,
+          ERR_INVALID, "Hsys / Scoup must be nhilb x nhilb");
+    SBProblem prob;
+    prob.topo = sb_topology(p);
+
+    std::vector<i64> mode_dims(nb + 1);
+    mode_dims[0] = p.nhilb;
+    for (size_t i = 0; i < nb; ++i) mode_dims[i + 1] = p.boson_dim[i];
+    SOP H(static_cast<int>(1 + 2 * nb), mode_dims);
+
+    PrimOp hs;
+    hs.kind = OP_DENSE;
+    hs.dim = p.nhilb;
+    hs.dense = p.hsys;
+    H.bind(hs, {0}, 0);
+    PrimOp sc = hs;
+    sc.dense = p.scoup;
+    std::vector<int> zr(nb);
+    for (size_t i = 0; i < nb; ++i) zr[i] = static_cast<int>(2 * i + 2);
+    H.bind(sc, zr, 0);
+    for (size_t i = 0; i < nb; ++i) {
+        H.bind(harmonic_h0(p.wk[i], p.boson_dim[i]), {static_cast<int>(1 + 2 * i)}, static_cast<int>(i + 1));
+        H.bind(harmonic_hc(p.gk[i], p.boson_dim[i]), {static_cast<int>(2 + 2 * i)}, static_cast<int>(i + 1));
+    }
+    prob.sop = std::move(H);
+    prob.state = sb_initial_state(prob.topo, p.psi0, p.seed);
+    return prob;
+}
+
+}  // namespace ttnh

@@ -1,0 +1,53 @@
+// Spin-boson problem construction: tree topology, SOP Hamiltonian and initial state,
+// following the reference's driver (src/spin_boson.cpp:447-649) and harmonic mode
+// operators (src/mode_data.hpp:136-361) for the case of one bath mode per leaf.
+//
+//   H = Hsys (x) 1  +  sum_k  w_k n_k  +  sum_k Scoup (x) g_k (a_k + a_k^dagger)
+//
+// Terms (sop index r): 0 = Hsys on mode 0; 1+2k = diag(w_k n) on mode k+1;
+// 2+2k = Scoup on mode 0 times the CSR coupling g_k sqrt(n) on mode k+1.
+#pragma once
+
+#include "common.hpp"
+#include "sop.hpp"
+#include "topology.hpp"
+
+#include <cstdint>
+
+namespace ttnh {
+
+struct SBParams {
+    std::vector<double> wk, gk;  // bath frequencies and couplings (one mode per leaf)
+    std::vector<i64> boson_dim;  // Fock truncation per mode
+    i64 nspf = 8, nspf_lower = 8;           // bond dims at the top / bottom bath levels
+    i64 nspf_cap = 8, nspf_lower_cap = 8;   // bond-dim capacities
+    i64 nhilb = 2;
+    std::vector<cplx> hsys;   // nhilb x nhilb row-major
+    std::vector<cplx> scoup;  // nhilb x nhilb row-major
+    std::vector<cplx> psi0;   // nhilb
+    uint64_t seed = 0;
+};
+
+struct SBProblem {
+    Topology topo;
+    SOP sop;
+    std::vector<std::vector<cplx>> state;  // per node, row-major dimen x hrank
+};
+
+// Spin-boson topology (spin_boson.cpp:447-489): root(1) -> [system node(nhilb) ->
+// primitive(nhilb)], [bath node(nhilb) -> balanced binary subtree over the modes]
+// with level-dependent bond dimensions, sanitised.
+Topology sb_topology(const SBParams& p);
+
+// Harmonic mode operators for one mode (mode_data.hpp:239-253, 309-361).
+PrimOp harmonic_h0(double w, i64 d);
+PrimOp harmonic_hc(double g, i64 d);
+
+SBProblem sb_build(const SBParams& p);
+
+// Random orthonormal leaf functions and identity interior nodes, drawn with
+// std::mt19937(seed) in pre-order (spin_boson.cpp:578-649).
+std::vector<std::vector<cplx>> sb_initial_state(const Topology& t, const std::vector<cplx>& psi0,
+                                                uint64_t seed);
+
+}  // namespace ttnh
