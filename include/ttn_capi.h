@@ -121,6 +121,11 @@ int ttn_counters(const ttn_integ* integ, int64_t* nh_applications, int64_t* nste
 int ttn_expect(ttn_integ* integ, int64_t mode, const double* op, double* out);
 /* <psi|psi> */
 int ttn_norm2(ttn_integ* integ, double* out);
+/* Record Re<psi|O|psi> and <psi|psi> on the device after every completed step, as the
+ * reference driver does per step (src/spin_boson.cpp:81-105).  O: d x d complex on mode. */
+int ttn_set_observable(ttn_integ* integ, int64_t mode, const double* op);
+/* recorded (Re<O>, <1>) pairs, one per completed step; out may be NULL to query *len */
+int ttn_trajectory(const ttn_integ* integ, int64_t* len, double* out);
 
 /* ---- spin-boson driver helpers (src/spin_boson.cpp:447-649) ---- */
 typedef struct {

@@ -86,6 +86,8 @@ _SIGS = {
     "counters": [vp, i64p, i64p],
     "expect": [vp, i64, dblp, dblp],
     "norm2": [vp, dblp],
+    "set_observable": [vp, i64, dblp],
+    "trajectory": [vp, i64p, dblp],
     "sb_problem": [vp, C.POINTER(SBParamsC), C.POINTER(vp), C.POINTER(vp)],
     "sb_initial_state": [vp, C.POINTER(SBParamsC)],
     "batch_create": [vp, i64, C.POINTER(vp), C.POINTER(vp)],

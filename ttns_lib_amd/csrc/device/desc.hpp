@@ -1,0 +1,93 @@
+// Device-side descriptors of one TTNS instance: node shapes and arena offsets, the
+// compiled SOP term tables flattened into arrays, and the sweep order.  Built on the host
+// (capi.cpp) and read by the persistent sweep kernel (sweep.hip).  Shared by host and
+// device code, so plain structs only.
+//
+// All complex128 data of an instance lives in one HBM arena of double2 values; every
+// "offset" below is an index into that arena.  Node tensors are stored row-major
+// (prod dims) x hrank, child 0 slowest (reference ttn_nodes/httensor_node.hpp:20-285).
+#pragma once
+
+#include <cstdint>
+
+namespace ttnd {
+
+constexpr int MAXC = 4;        // max children of a branch node on the device path
+constexpr int MAXK = 16;       // max Krylov dimension on the device path
+constexpr int NWORK = 6;       // generic work buffers, each max node size
+
+struct DNode {
+    int parent, child_id, nchild, leaf;  // leaf = primitive mode index, -1 for branch nodes
+    int nmodes, hrank, dimen, size;      // size = dimen * hrank
+    int dims[MAXC];
+    int children[MAXC];
+    int64_t A, R;                         // arena offsets: node tensor, gauge matrix (rmax^2)
+    int term0, nterms;                    // slice of DTerm
+};
+
+// operator_term (reference ttn_nodes/operator_node.hpp:78-229)
+struct DTerm {
+    int id_spf, id_mf;
+    double cre, cim;       // coeff()
+    double ire, iim;       // identity-spf coefficient: nspf > 1 ? sum(accum) : coeff
+    int sub0, nsub;        // spf sub-terms (DSub)
+    int mf0, nmf;          // mean-field contributions (DMf)
+    int64_t spf, mf;       // arena offsets of the hrank x hrank matrices, -1 for identity
+};
+
+// one spf sub-term: accumulation coefficient and the chain of mode operators
+struct DSub {
+    double are, aim;       // accum[i]
+    int k0, nk;            // slice of DK
+};
+
+// one operator of a chain.  Branch nodes: nu = child slot, mat = child's spf matrix
+// (dim x dim, kind = DENSE).  Leaf nodes: nu = primitive mode, kind/mat/idx/nnz describe
+// the primitive operator (dense d x d, diagonal d, or CSR values + (rowptr, colind) in the
+// integer table).
+struct DK {
+    int nu, kind, dim, nnz;
+    int64_t mat, idx;
+};
+
+// one mean-field contribution (operator_node.hpp:23-74 mf_index)
+struct DMf {
+    double are, aim;       // accum[it]
+    int64_t pmf;           // parent's mf matrix offset, -1 for identity
+    int k0, nk;            // sibling spf chain (DK, branch form, applied to the parent tensor)
+};
+
+struct DInst {
+    double2* base;                  // arena
+    const DNode* nodes;
+    const DTerm* terms;
+    const DSub* subs;
+    const DK* ks;
+    const DMf* mfs;
+    const int* tour;
+    const int64_t* ints;           // CSR rowptr/colind
+    int nnodes, tour_len;
+    int kdim, pad0;
+    double tol;
+    int64_t maxsize;               // largest node size (work-buffer stride)
+    int64_t work[NWORK];           // work buffer offsets
+    int64_t krylov;                // Krylov basis (kdim + 1) * maxsize
+    int64_t obs_op;                // observable operator (d x d) offset, -1 if none
+    int obs_leaf;                  // node id of the observable's leaf, -1 if none
+    int pad1;
+    double* traj;                  // per-step <O>, <1> (2 doubles per step) or null
+    int64_t* counters;             // [0] H applications (node updates), [1] steps done
+    int* status;                   // 0 ok, 1 numeric
+};
+
+struct RunArgs {
+    double2 coeff;                 // -i for real time
+    int prepare;                   // run prepare_evolution first
+    int nladder;                   // ladder sub-steps (first step)
+    double ladder_dt[8];
+    int nregular;                  // regular steps after the ladder
+    double dt;
+    int64_t step0;                 // index of the first step of this call (trajectory slot)
+};
+
+}  // namespace ttnd

@@ -1,0 +1,535 @@
+// Persistent TDVP sweep kernel for gfx950.
+//
+// One workgroup owns one TTNS instance and runs whole integrator calls on the device:
+// prepare_evolution, the first-step ladder and any number of regular steps, each step a
+// forward and a backward projector-splitting sweep (reference
+// tdvp_core/tdvp_algorithm_engine.hpp:207-501) with the Krylov propagators, environment
+// updates and QR gauge moves in between.  No host round trip happens inside a call.  A
+// batch is a grid of such workgroups, one per instance.
+#include <hip/hip_runtime.h>
+
+#include "desc.hpp"
+#include "wg_ops.hpp"
+
+namespace ttnd {
+
+struct Ctx {
+    const DInst& I;
+    Shared& sh;
+    double2* base;
+    __device__ Ctx(const DInst& inst, Shared& s) : I(inst), sh(s), base(inst.base) {}
+    __device__ double2* W(int k) const { return base + I.work[k]; }
+    __device__ const DNode& node(int n) const { return I.nodes[n]; }
+    // failures are detected on workgroup-uniform values, so every thread takes the same
+    // branch; the LDS flag is read only after a barrier
+    __device__ bool failed() const { return sh.fail != 0; }
+    __device__ void fail() const { sh.fail = 1; }
+};
+
+// (L, R) of the rank-3 view of mode `mode` of a node
+__device__ __forceinline__ void mode_view(const DNode& nd, int mode, int64_t& L, int64_t& R) {
+    L = 1;
+    R = nd.hrank;
+    for (int c = 0; c < mode; ++c) L *= nd.dims[c];
+    for (int c = mode + 1; c < nd.nmodes; ++c) R *= nd.dims[c];
+}
+
+// kronecker_product_operator::apply (kronecker_product_operator_helper.hpp:261-315):
+// chain of mode contractions with spf matrices; the last contraction lands in out.
+__device__ void kpo(Ctx& c, const DNode& nd, int k0, int nk, const double2* v, double2* out, double2* tmp) {
+    if (nk == 0) {
+        wg_copy(out, v, nd.size);
+        return;
+    }
+    const double2* src = v;
+    for (int q = 0; q < nk; ++q) {
+        double2* dst = ((nk - 1 - q) % 2 == 0) ? out : tmp;
+        const DK& k = c.I.ks[k0 + q];
+        int64_t L, R;
+        mode_view(nd, k.nu, L, R);
+        wg_mode_contract(c.base + k.mat, L, nd.dims[k.nu], R, src, dst, cmk(1.0, 0.0), false);
+        src = dst;
+    }
+}
+
+// coefficient_evolution_{leaf,branch} (projector_splitting_evolution_functions.hpp:65-258)
+// uses W1..W4
+__device__ void heff_node(Ctx& c, int n, const double2* v, double2* res) {
+    const DNode& nd = c.node(n);
+    const int64_t sz = nd.size;
+    const int r = nd.hrank;
+    const int64_t D = nd.dimen;
+    double2 *t1 = c.W(1), *t2 = c.W(2), *t3 = c.W(3);
+    wg_zero(res, sz);
+    for (int ti = 0; ti < nd.nterms; ++ti) {
+        const DTerm& tm = c.I.terms[nd.term0 + ti];
+        if (tm.id_spf) {
+            const double2 coeff = cmk(tm.ire, tm.iim);
+            if (tm.id_mf)
+                wg_axpy(res, coeff, v, sz);
+            else
+                wg_mul_transposed(v, D, r, c.base + tm.mf, res, coeff, true);
+            continue;
+        }
+        for (int i = 0; i < tm.nsub; ++i) {
+            const DSub& s = c.I.subs[tm.sub0 + i];
+            const double2 ci = (i == 0) ? (tm.nsub > 1 ? cmk(s.are, s.aim) : cmk(tm.cre, tm.cim)) : cmk(s.are, s.aim);
+            if (nd.leaf >= 0 && s.nk > 0)
+                wg_prim_apply(c.I.ks[s.k0], c.base, c.I.ints, v, t2, r);
+            else
+                kpo(c, nd, s.k0, s.nk, v, t2, t1);
+            if (i == 0)
+                wg_scale_to(t3, ci, t2, sz);
+            else
+                wg_axpy(t3, ci, t2, sz);
+        }
+        if (tm.id_mf)
+            wg_axpy(res, cmk(1.0, 0.0), t3, sz);
+        else
+            wg_mul_transposed(t3, D, r, c.base + tm.mf, res, cmk(1.0, 0.0), true);
+    }
+}
+
+// full_hamiltonian_action (projector_splitting_evolution_functions.hpp:7-62); uses W1, W2
+__device__ void heff_bond(Ctx& c, int n, const double2* v, double2* res) {
+    const DNode& nd = c.node(n);
+    const int r = nd.hrank;
+    const int64_t sz = static_cast<int64_t>(r) * r;
+    double2 *t1 = c.W(1), *t2 = c.W(2);
+    wg_zero(res, sz);
+    for (int ti = 0; ti < nd.nterms; ++ti) {
+        const DTerm& tm = c.I.terms[nd.term0 + ti];
+        const double2* x = v;
+        if (!tm.id_spf) {
+            wg_mode_contract(c.base + tm.spf, 1, r, r, v, t1, cmk(1.0, 0.0), false);
+            x = t1;
+        }
+        const double2 coeff = cmk(tm.cre, tm.cim);
+        if (!tm.id_mf)
+            wg_mul_transposed(x, r, r, c.base + tm.mf, res, coeff, true);
+        else
+            wg_axpy(res, coeff, x, sz);
+    }
+}
+
+// single_particle_operator_engine::evaluate (single_particle_operator.hpp:26-103); uses W1, W2
+__device__ void update_spf(Ctx& c, int n) {
+    const DNode& nd = c.node(n);
+    const int r = nd.hrank;
+    const int64_t D = nd.dimen;
+    const double2* a = c.base + nd.A;
+    double2 *HA = c.W(1), *tmp = c.W(2);
+    for (int ti = 0; ti < nd.nterms; ++ti) {
+        const DTerm& tm = c.I.terms[nd.term0 + ti];
+        if (tm.id_spf) continue;
+        double2* out = c.base + tm.spf;
+        for (int i = 0; i < tm.nsub; ++i) {
+            const DSub& s = c.I.subs[tm.sub0 + i];
+            if (nd.leaf >= 0 && s.nk > 0)
+                wg_prim_apply(c.I.ks[s.k0], c.base, c.I.ints, a, HA, r);
+            else
+                kpo(c, nd, s.k0, s.nk, a, HA, tmp);
+            if (tm.nsub == 1)
+                wg_adjoint_mul(a, HA, D, r, out, cmk(1.0, 0.0), false);
+            else
+                wg_adjoint_mul(a, HA, D, r, out, cmk(s.are, s.aim), i > 0);
+        }
+    }
+}
+
+// mean_field_operator_engine::evaluate (mean_field_operator.hpp:37-135) for child n;
+// uses W1, W2, W3
+__device__ void update_mf(Ctx& c, int n) {
+    const DNode& nd = c.node(n);
+    if (nd.parent < 0) return;
+    const DNode& pn = c.node(nd.parent);
+    const int mode = nd.child_id;
+    int64_t L, R;
+    mode_view(pn, mode, L, R);
+    const int d = pn.dims[mode];
+    const int64_t pD = pn.dimen;
+    const int pr = pn.hrank;
+    const double2* Ap = c.base + pn.A;
+    double2 *t1 = c.W(1), *t2 = c.W(2), *t3 = c.W(3);
+    for (int ti = 0; ti < nd.nterms; ++ti) {
+        const DTerm& tm = c.I.terms[nd.term0 + ti];
+        if (tm.id_mf) continue;
+        double2* out = c.base + tm.mf;
+        for (int it = 0; it < tm.nmf; ++it) {
+            const DMf& m = c.I.mfs[tm.mf0 + it];
+            const double2* HA = Ap;
+            if (m.nk > 0) {
+                kpo(c, pn, m.k0, m.nk, Ap, t1, t2);
+                HA = t1;
+            }
+            if (m.pmf >= 0) {
+                wg_mul_transposed(HA, pD, pr, c.base + m.pmf, t3, cmk(1.0, 0.0), false);
+                HA = t3;
+            }
+            if (tm.nmf == 1)
+                wg_outer_contract(Ap, HA, L, d, R, out, cmk(1.0, 0.0), false);
+            else
+                wg_outer_contract(Ap, HA, L, d, R, out, cmk(m.are, m.aim), it > 0);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- gauge moves
+// root_to_leaf_decomposition_engine::evaluate + apply_to_node (root_to_leaf_decomposition.hpp:112-204)
+__device__ void r2l(Ctx& c, int n, int mode) {
+    const DNode& nd = c.node(n);
+    const int64_t sz = nd.size;
+    int64_t id1 = 1;
+    for (int q = 0; q <= mode; ++q) id1 *= nd.dims[q];
+    const int64_t id2 = sz / id1;
+    const int d2 = nd.dims[mode];
+    const int d1 = static_cast<int>(sz / d2);
+    double2* A = c.base + nd.A;
+    double2 *U = c.W(5), *Q = c.W(1), *Rm = c.W(2);
+    for (int64_t e = threadIdx.x; e < sz; e += NT) {
+        const int64_t i = e / id2, j = e % id2;
+        U[j * id1 + i] = A[e];
+    }
+    __syncthreads();
+    wg_householder_qr(c.sh, U, d1, d2, Q, Rm);
+    for (int64_t e = threadIdx.x; e < sz; e += NT) {
+        const int64_t i = e / id2, j = e % id2;
+        A[e] = Q[j * id1 + i];
+    }
+    double2* Rn = c.base + nd.R;
+    for (int64_t e = threadIdx.x; e < static_cast<int64_t>(d2) * d2; e += NT) {
+        const int i = static_cast<int>(e / d2), j = static_cast<int>(e % d2);
+        Rn[e] = Rm[static_cast<int64_t>(j) * d2 + i];
+    }
+    __syncthreads();
+}
+
+// leaf_to_root_decomposition_engine::evaluate + apply_to_node (leaf_to_root_decomposition.hpp:56-101)
+__device__ void l2r(Ctx& c, int n) {
+    const DNode& nd = c.node(n);
+    double2* A = c.base + nd.A;
+    double2* U = c.W(5);
+    wg_copy(U, A, nd.size);
+    wg_householder_qr(c.sh, U, nd.dimen, nd.hrank, A, c.base + nd.R);
+}
+
+// root_to_leaf_decomposition_engine::apply_from_parent (:208-233): A <- A * R_parent
+__device__ void apply_from_parent(Ctx& c, int n) {
+    const DNode& nd = c.node(n);
+    double2* A = c.base + nd.A;
+    double2* T = c.W(5);
+    wg_mul(A, nd.dimen, nd.hrank, c.base + c.node(nd.parent).R, T);
+    wg_copy(A, T, nd.size);
+}
+
+// leaf_to_root_decomposition_engine::apply_to_parent (leaf_to_root_decomposition.hpp:105-138)
+__device__ void apply_to_parent(Ctx& c, int n) {
+    const DNode& nd = c.node(n);
+    const DNode& pn = c.node(nd.parent);
+    int64_t L, R;
+    mode_view(pn, nd.child_id, L, R);
+    double2* T = c.W(5);
+    double2* Ap = c.base + pn.A;
+    wg_mode_contract(c.base + nd.R, L, pn.dims[nd.child_id], R, Ap, T, cmk(1.0, 0.0), false);
+    wg_copy(Ap, T, pn.size);
+}
+
+// ---------------------------------------------------------------- Krylov propagator
+// x <- exp(coeff * tau * H) x: Arnoldi with CGS2, Saad error estimate, step halving,
+// Pade(13) small exponential (same algorithm as the oracle's expv).  Returns the number
+// of H applications.  H(v, w) may use W1..W4; the Krylov vector w is W0.
+template <class HF>
+__device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, double tau) {
+    if (tau == 0.0 || n == 0) return 0;
+    const int kdim = c.I.kdim;
+    const double tol = c.I.tol;
+    Shared& sh = c.sh;
+    double2* V = c.base + c.I.krylov;
+    const int64_t ld = c.I.maxsize;
+    double2* w = c.W(0);
+    int64_t napply = 0;
+    double s_done = 0.0, s_step = 1.0;
+    int guard = 0;
+    while (s_done < 1.0 && guard++ < 4096) {
+        s_step = fmin(s_step, 1.0 - s_done);
+        const double beta = sqrt(block_norm2(sh, x, n));
+        if (!isfinite(beta)) {
+            c.fail();
+            return napply;
+        }
+        if (beta == 0.0) return napply;
+        wg_scale_to(V, cmk(1.0 / beta, 0.0), x, n);
+        for (int e = threadIdx.x; e < (MAXK + 1) * MAXK; e += NT) sh.Hm[e] = cz();
+        __syncthreads();
+        int m = 0;
+        double hn = 0.0;
+        bool found = false;
+        auto small_exp = [&](int mm, double s) -> double {
+            const double2 f = cscale(coeff, tau * s);
+            for (int e = threadIdx.x; e < mm * mm; e += NT) {
+                const int i = e / mm, j = e % mm;
+                sh.S[e] = cmul(f, sh.Hm[i * MAXK + j]);
+            }
+            __syncthreads();
+            sm_expm(sh, sh.S, sh.F, mm);
+            return hn * cabsv(sh.F[(mm - 1) * mm]);
+        };
+        for (int j = 0; j < kdim; ++j) {
+            H(V + j * ld, w);
+            ++napply;
+            for (int pass = 0; pass < 2; ++pass) {
+                double p[2 * MAXK], o[2 * MAXK];
+#pragma unroll
+                for (int i = 0; i < 2 * MAXK; ++i) p[i] = 0.0;
+                for (int64_t q = threadIdx.x; q < n; q += NT) {
+                    const double2 wq = w[q];
+#pragma unroll
+                    for (int i = 0; i < MAXK; ++i) {
+                        if (i <= j) {
+                            const double2 pr = cmulc(V[i * ld + q], wq);
+                            p[2 * i] += pr.x;
+                            p[2 * i + 1] += pr.y;
+                        }
+                    }
+                }
+                block_sum<2 * MAXK>(sh, p, o);
+                for (int64_t q = threadIdx.x; q < n; q += NT) {
+                    double2 wq = w[q];
+                    for (int i = 0; i <= j; ++i) wq = csub(wq, cmul(cmk(o[2 * i], o[2 * i + 1]), V[i * ld + q]));
+                    w[q] = wq;
+                }
+                if (threadIdx.x <= j) {
+                    const int i = threadIdx.x;
+                    sh.Hm[i * MAXK + j] = cadd(sh.Hm[i * MAXK + j], cmk(o[2 * i], o[2 * i + 1]));
+                }
+                __syncthreads();
+            }
+            hn = sqrt(block_norm2(sh, w, n));
+            if (!isfinite(hn)) {
+                c.fail();
+                return napply;
+            }
+            if (j + 1 < kdim && threadIdx.x == 0) sh.Hm[(j + 1) * MAXK + j] = cmk(hn, 0.0);
+            __syncthreads();
+            m = j + 1;
+            const double err = small_exp(m, s_step);
+            if (err <= tol) {
+                found = true;
+                break;
+            }
+            if (j + 1 < kdim) wg_scale_to(V + (j + 1) * ld, cmk(1.0 / hn, 0.0), w, n);
+        }
+        for (int k = 0; !found && k < 60; ++k) {
+            s_step *= 0.5;
+            if (small_exp(m, s_step) <= tol) found = true;
+        }
+        if (!found) {
+            c.fail();
+            return napply;
+        }
+        for (int64_t q = threadIdx.x; q < n; q += NT) {
+            double2 s = cz();
+            for (int i = 0; i < m; ++i) s = cadd(s, cmul(sh.F[i * m], V[i * ld + q]));
+            x[q] = cscale(s, beta);
+        }
+        __syncthreads();
+        s_done += s_step;
+    }
+    return napply;
+}
+
+// ---------------------------------------------------------------- sweeps
+struct SweepState {
+    double2 coeff;
+    double sub_dt;
+    int64_t nh;
+};
+
+__device__ void node_update(Ctx& c, SweepState& st, int n) {
+    const DNode& nd = c.node(n);
+    st.nh += expv(
+        c, [&](const double2* v, double2* res) { heff_node(c, n, v, res); }, c.base + nd.A, nd.size, st.coeff,
+        st.sub_dt / 2.0);
+}
+
+__device__ void bond_update(Ctx& c, SweepState& st, int n, int ops) {
+    const int r = c.node(ops).hrank;
+    expv(
+        c, [&](const double2* v, double2* res) { heff_bond(c, ops, v, res); }, c.base + c.node(n).R,
+        static_cast<int64_t>(r) * r, st.coeff, -st.sub_dt / 2.0);
+}
+
+__device__ void forward(Ctx& c, SweepState& st, bool evolve, int* visits) {
+    for (int e = threadIdx.x; e < c.I.nnodes; e += NT) visits[e] = 0;
+    __syncthreads();
+    for (int s = 0; s < c.I.tour_len; ++s) {
+        if (c.failed()) return;
+        const int id = c.I.tour[s];
+        const DNode& nd = c.node(id);
+        const int tv = visits[id];
+        __syncthreads();
+        if (threadIdx.x == 0) visits[id] = tv + 1;
+        __syncthreads();
+        if (tv != nd.nmodes) {
+            if (tv == 0 && nd.parent >= 0) apply_from_parent(c, id);
+            if (nd.leaf < 0) {
+                r2l(c, id, tv);
+                update_mf(c, nd.children[tv]);
+            }
+        } else {
+            if (evolve) node_update(c, st, id);
+            if (nd.parent >= 0) {
+                l2r(c, id);
+                update_spf(c, id);
+                if (evolve) bond_update(c, st, id, id);
+                apply_to_parent(c, id);
+            }
+        }
+    }
+}
+
+__device__ void backward(Ctx& c, SweepState& st, bool evolve, int* visits) {
+    for (int e = threadIdx.x; e < c.I.nnodes; e += NT) visits[e] = 0;
+    __syncthreads();
+    for (int s = c.I.tour_len - 1; s >= 0; --s) {
+        if (c.failed()) return;
+        const int id = c.I.tour[s];
+        const DNode& nd = c.node(id);
+        const int tv = visits[id];
+        __syncthreads();
+        if (threadIdx.x == 0) visits[id] = tv + 1;
+        __syncthreads();
+        if (tv != nd.nmodes) {
+            const int mode = nd.nmodes - (tv + 1);
+            if (tv == 0) {
+                if (nd.parent >= 0) {
+                    if (evolve) bond_update(c, st, nd.parent, id);
+                    apply_from_parent(c, id);
+                }
+                if (evolve) node_update(c, st, id);
+            }
+            if (nd.leaf < 0) {
+                r2l(c, id, mode);
+                update_mf(c, nd.children[mode]);
+            }
+        } else if (nd.parent >= 0) {
+            l2r(c, id);
+            apply_to_parent(c, id);
+            update_spf(c, id);
+        }
+    }
+}
+
+// <psi|O|psi> for O on the leaf node `leaf` (path contraction to the root, the other
+// subtrees orthonormal as after every sweep) and <psi|psi> = |A_root|^2; uses W1..W4
+__device__ void path_expect(Ctx& c, int leaf, int64_t op_off, double2& val, double& nrm) {
+    const DNode& ln = c.node(leaf);
+    double2 *HA = c.W(1), *M = c.W(2), *T = c.W(3), *M2 = c.W(4);
+    DK k;
+    k.nu = ln.leaf;
+    k.kind = 0;
+    k.dim = ln.dimen;
+    k.nnz = 0;
+    k.mat = op_off;
+    k.idx = 0;
+    wg_prim_apply(k, c.base, c.I.ints, c.base + ln.A, HA, ln.hrank);
+    wg_adjoint_mul(c.base + ln.A, HA, ln.dimen, ln.hrank, M, cmk(1.0, 0.0), false);
+    int n = leaf;
+    while (c.node(n).parent >= 0) {
+        const DNode& nd = c.node(n);
+        const DNode& pn = c.node(nd.parent);
+        int64_t L, R;
+        mode_view(pn, nd.child_id, L, R);
+        wg_mode_contract(M, L, pn.dims[nd.child_id], R, c.base + pn.A, T, cmk(1.0, 0.0), false);
+        wg_adjoint_mul(c.base + pn.A, T, pn.dimen, pn.hrank, M2, cmk(1.0, 0.0), false);
+        double2* tmp = M;
+        M = M2;
+        M2 = tmp;
+        n = nd.parent;
+    }
+    val = M[0];
+    const DNode& root = c.node(0);
+    nrm = block_norm2(c.sh, c.base + root.A, root.size);
+}
+
+__device__ void step_done(Ctx& c, int64_t step_index) {
+    if (c.I.obs_leaf >= 0 && c.I.traj != nullptr) {
+        double2 v;
+        double nrm;
+        path_expect(c, c.I.obs_leaf, c.I.obs_op, v, nrm);
+        if (threadIdx.x == 0) {
+            c.I.traj[2 * step_index] = v.x;
+            c.I.traj[2 * step_index + 1] = nrm;
+        }
+    }
+    if (threadIdx.x == 0) c.I.counters[1] += 1;
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(NT) void tdvp_run_kernel(const DInst* insts, RunArgs a) {
+    __shared__ Shared sh;
+    __shared__ int visits[1024];
+    const DInst& I = insts[blockIdx.x];
+    if (threadIdx.x == 0) sh.fail = 0;
+    __syncthreads();
+    Ctx c(I, sh);
+    SweepState st;
+    st.coeff = a.coeff;
+    st.nh = 0;
+    int64_t step_index = a.step0;
+    if (a.prepare) {
+        // tdvp_algorithm_base::prepare_evolution (:257-321) without subspace expansion
+        for (int n = I.nnodes - 1; n >= 0; --n)
+            if (I.nodes[n].parent >= 0) update_spf(c, n);
+        st.sub_dt = a.nladder > 0 ? a.ladder_dt[0] : a.dt;
+        forward(c, st, false, visits);
+        backward(c, st, false, visits);
+    }
+    for (int l = 0; l < a.nladder && !c.failed(); ++l) {
+        st.sub_dt = a.ladder_dt[l];
+        forward(c, st, true, visits);
+        backward(c, st, true, visits);
+    }
+    if (a.nladder > 0 && !c.failed()) step_done(c, step_index++);
+    for (int s = 0; s < a.nregular && !c.failed(); ++s) {
+        st.sub_dt = a.dt;
+        forward(c, st, true, visits);
+        backward(c, st, true, visits);
+        step_done(c, step_index++);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        I.counters[0] += st.nh;
+        if (sh.fail) *I.status = 1;
+    }
+}
+
+// <psi|O|psi> on one instance (O stored at obs_op); out: re, im, norm2
+__global__ __launch_bounds__(NT) void expect_kernel(const DInst* inst, int leaf, int64_t op_off, double* out) {
+    __shared__ Shared sh;
+    if (threadIdx.x == 0) sh.fail = 0;
+    __syncthreads();
+    Ctx c(*inst, sh);
+    double2 v;
+    double nrm;
+    path_expect(c, leaf, op_off, v, nrm);
+    if (threadIdx.x == 0) {
+        out[0] = v.x;
+        out[1] = v.y;
+        out[2] = nrm;
+    }
+}
+
+}  // namespace ttnd
+
+// ---------------------------------------------------------------- host launchers
+extern "C" hipError_t ttnd_launch_run(const ttnd::DInst* d_insts, int n, const ttnd::RunArgs* a, hipStream_t s) {
+    hipLaunchKernelGGL(ttnd::tdvp_run_kernel, dim3(n), dim3(ttnd::NT), 0, s, d_insts, *a);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t ttnd_launch_expect(const ttnd::DInst* d_inst, int leaf, int64_t op_off, double* d_out,
+                                         hipStream_t s) {
+    hipLaunchKernelGGL(ttnd::expect_kernel, dim3(1), dim3(ttnd::NT), 0, s, d_inst, leaf, op_off, d_out);
+    return hipGetLastError();
+}

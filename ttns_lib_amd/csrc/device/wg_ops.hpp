@@ -1,0 +1,445 @@
+// Workgroup-level complex128 primitives for the persistent TDVP sweep.
+//
+// Every function is executed by all NT threads of one workgroup on data in HBM/L2
+// (node tensors, work buffers) or LDS (small Krylov matrices).  Functions that write
+// results other threads read end with __syncthreads(); callers never need their own
+// barrier between two primitives.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "desc.hpp"
+
+namespace ttnd {
+
+constexpr int NT = 256;          // threads per workgroup (4 wave64)
+constexpr int NWAVE = NT / 64;
+
+// ---------------------------------------------------------------- complex helpers
+__device__ __forceinline__ double2 cz() { return make_double2(0.0, 0.0); }
+__device__ __forceinline__ double2 cmk(double a, double b) { return make_double2(a, b); }
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// conj(a) * b
+__device__ __forceinline__ double2 cmulc(double2 a, double2 b) {
+    return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ __forceinline__ double cabs2(double2 a) { return a.x * a.x + a.y * a.y; }
+__device__ __forceinline__ double cabsv(double2 a) { return hypot(a.x, a.y); }
+// a / b with Smith's range reduction
+__device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
+    if (fabs(b.x) >= fabs(b.y)) {
+        const double r = b.y / b.x, den = b.x + b.y * r;
+        return make_double2((a.x + a.y * r) / den, (a.y - a.x * r) / den);
+    }
+    const double r = b.x / b.y, den = b.x * r + b.y;
+    return make_double2((a.x * r + a.y) / den, (a.y * r - a.x) / den);
+}
+
+// ---------------------------------------------------------------- reductions
+struct Shared {
+    double red[NWAVE * 2 * (MAXK + 1)];
+    double2 bc[4];      // broadcast slots
+    double2 qtau[64];   // Householder tau (up to 64 columns)
+    double2 qw[64];     // Householder trailing dot products
+    // Krylov small matrices
+    double2 Hm[(MAXK + 1) * MAXK];
+    double2 S[MAXK * MAXK], F[MAXK * MAXK];
+    double2 E[8][MAXK * MAXK];
+    double2 lu_f[MAXK];
+    int ipiv;
+    int fail;           // set (uniformly) when a numerical failure is detected
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// sum of K doubles over the workgroup; every thread receives the totals in out[]
+template <int K>
+__device__ void block_sum(Shared& sh, double (&v)[K], double (&out)[K]) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double s = wave_sum(v[k]);
+        if (lane == 0) sh.red[wv * K + k] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < NWAVE; ++w) s += sh.red[w * K + k];
+        out[k] = s;
+    }
+    __syncthreads();
+}
+
+__device__ double block_norm2(Shared& sh, const double2* x, int64_t n) {
+    double p[1] = {0.0}, o[1];
+    for (int64_t i = threadIdx.x; i < n; i += NT) p[0] += cabs2(x[i]);
+    block_sum<1>(sh, p, o);
+    return o[0];
+}
+
+// ---------------------------------------------------------------- elementwise
+__device__ void wg_zero(double2* y, int64_t n) {
+    for (int64_t i = threadIdx.x; i < n; i += NT) y[i] = cz();
+    __syncthreads();
+}
+__device__ void wg_copy(double2* y, const double2* x, int64_t n) {
+    for (int64_t i = threadIdx.x; i < n; i += NT) y[i] = x[i];
+    __syncthreads();
+}
+// y += a * x
+__device__ void wg_axpy(double2* y, double2 a, const double2* x, int64_t n) {
+    for (int64_t i = threadIdx.x; i < n; i += NT) y[i] = cadd(y[i], cmul(a, x[i]));
+    __syncthreads();
+}
+// y = a * x
+__device__ void wg_scale_to(double2* y, double2 a, const double2* x, int64_t n) {
+    for (int64_t i = threadIdx.x; i < n; i += NT) y[i] = cmul(a, x[i]);
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------- contractions
+// dst(l,i,r) (=|+=) alpha * sum_j op(i,j) src(l,j,r) on the (L, d, R) view.
+// Reference: contract(op, 1, A3, 1) in kronecker_product_operator_helper.hpp:292-303.
+__device__ void wg_mode_contract(const double2* op, int64_t L, int d, int64_t R, const double2* src, double2* dst,
+                                 double2 alpha, bool accumulate) {
+    const int64_t total = L * d * R;
+    for (int64_t idx = threadIdx.x; idx < total; idx += NT) {
+        const int64_t r = idx % R;
+        const int64_t li = idx / R;
+        const int i = static_cast<int>(li % d);
+        const int64_t l = li / d;
+        const double2* s = src + l * d * R + r;
+        const double2* o = op + static_cast<int64_t>(i) * d;
+        double2 acc = cz();
+        for (int j = 0; j < d; ++j) acc = cadd(acc, cmul(o[j], s[static_cast<int64_t>(j) * R]));
+        acc = cmul(alpha, acc);
+        dst[idx] = accumulate ? cadd(dst[idx], acc) : acc;
+    }
+    __syncthreads();
+}
+
+// dst (=|+=) alpha * X M^T, X: rows x r, M: r x r  (v * trans(mf))
+__device__ void wg_mul_transposed(const double2* X, int64_t rows, int r, const double2* M, double2* dst,
+                                  double2 alpha, bool accumulate) {
+    const int64_t total = rows * r;
+    for (int64_t idx = threadIdx.x; idx < total; idx += NT) {
+        const int64_t x = idx / r;
+        const int i = static_cast<int>(idx % r);
+        const double2* xr = X + x * r;
+        const double2* mr = M + static_cast<int64_t>(i) * r;
+        double2 acc = cz();
+        for (int j = 0; j < r; ++j) acc = cadd(acc, cmul(xr[j], mr[j]));
+        acc = cmul(alpha, acc);
+        dst[idx] = accumulate ? cadd(dst[idx], acc) : acc;
+    }
+    __syncthreads();
+}
+
+// dst = X M, X: rows x r, M: r x r
+__device__ void wg_mul(const double2* X, int64_t rows, int r, const double2* M, double2* dst) {
+    const int64_t total = rows * r;
+    for (int64_t idx = threadIdx.x; idx < total; idx += NT) {
+        const int64_t x = idx / r;
+        const int j = static_cast<int>(idx % r);
+        const double2* xr = X + x * r;
+        double2 acc = cz();
+        for (int k = 0; k < r; ++k) acc = cadd(acc, cmul(xr[k], M[static_cast<int64_t>(k) * r + j]));
+        dst[idx] = acc;
+    }
+    __syncthreads();
+}
+
+// out(i,j) (=|+=) alpha * sum_{l,r} conj(A(l,i,r)) B(l,j,r) on the (L, d, R) view.
+// For L = 1, R = 1 this is A^H B of (rows x d) matrices with rows = ... see callers.
+__device__ void wg_outer_contract(const double2* A, const double2* B, int64_t L, int d, int64_t R, double2* out,
+                                  double2 alpha, bool accumulate) {
+    const int nout = d * d;
+    // threads per output element
+    int tpo = NT / nout;
+    if (tpo < 1) tpo = 1;
+    if (tpo > 64) tpo = 64;
+    // round down to a power of two so that groups stay inside a wave
+    int p2 = 1;
+    while (p2 * 2 <= tpo) p2 *= 2;
+    tpo = p2;
+    const int groups = NT / tpo;
+    const int g = threadIdx.x / tpo, sub = threadIdx.x % tpo;
+    const int64_t K = L * R;
+    for (int base = 0; base < nout; base += groups) {
+        const int o = base + g;
+        double2 acc = cz();
+        if (o < nout) {
+            const int i = o / d, j = o % d;
+            for (int64_t k = sub; k < K; k += tpo) {
+                const int64_t l = k / R, r = k % R;
+                const int64_t off = l * d * R + r;
+                acc = cadd(acc, cmulc(A[off + static_cast<int64_t>(i) * R], B[off + static_cast<int64_t>(j) * R]));
+            }
+        }
+        // reduce within the tpo-sized group (inside one wave)
+        for (int s = tpo / 2; s > 0; s >>= 1) {
+            acc.x += __shfl_xor(acc.x, s, 64);
+            acc.y += __shfl_xor(acc.y, s, 64);
+        }
+        if (o < nout && sub == 0) {
+            const double2 v = cmul(alpha, acc);
+            out[o] = accumulate ? cadd(out[o], v) : v;
+        }
+    }
+    __syncthreads();
+}
+
+// out = A^H B for A, B: rows x r
+__device__ void wg_adjoint_mul(const double2* A, const double2* B, int64_t rows, int r, double2* out, double2 alpha,
+                               bool accumulate) {
+    wg_outer_contract(A, B, rows, r, 1, out, alpha, accumulate);
+}
+
+// ---------------------------------------------------------------- primitive leaf operators
+// y = op x, x/y: dim x ncols row-major; kind: 0 dense, 1 diag, 2 csr
+__device__ void wg_prim_apply(const DK& k, const double2* base, const int64_t* ints, const double2* x, double2* y,
+                              int ncols) {
+    const int d = k.dim;
+    const double2* mat = base + k.mat;
+    const int64_t total = static_cast<int64_t>(d) * ncols;
+    for (int64_t idx = threadIdx.x; idx < total; idx += NT) {
+        const int i = static_cast<int>(idx / ncols), c = static_cast<int>(idx % ncols);
+        double2 acc = cz();
+        if (k.kind == 0) {
+            for (int j = 0; j < d; ++j) acc = cadd(acc, cmul(mat[i * d + j], x[static_cast<int64_t>(j) * ncols + c]));
+        } else if (k.kind == 1) {
+            acc = cmul(mat[i], x[idx]);
+        } else {
+            const int64_t* rowptr = ints + k.idx;
+            const int64_t* colind = rowptr + d + 1;
+            for (int64_t q = rowptr[i]; q < rowptr[i + 1]; ++q)
+                acc = cadd(acc, cmul(mat[q], x[colind[q] * ncols + c]));
+        }
+        y[idx] = acc;
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------- Householder QR
+// A (m x n, row-major, destroyed: holds the reflectors) = Q (m x n) R (n x n).
+// LAPACK zgeqr2/zlarfg/zung2r conventions; for m < n, Q gets zero columns and R zero
+// rows beyond m (decomposition_engine.hpp:124-158).
+__device__ void wg_householder_qr(Shared& sh, double2* A, int m, int n, double2* Q, double2* Rout) {
+    const int k = m < n ? m : n;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int i = 0; i < k; ++i) {
+        // |x|^2 of the column below the diagonal
+        double p[1] = {0.0}, o[1];
+        for (int rr = i + 1 + threadIdx.x; rr < m; rr += NT) p[0] += cabs2(A[static_cast<int64_t>(rr) * n + i]);
+        block_sum<1>(sh, p, o);
+        const double xn2 = o[0];
+        const double2 alpha = A[static_cast<int64_t>(i) * n + i];
+        double2 tau = cz(), scal = cz();
+        double beta = alpha.x;
+        if (!(xn2 == 0.0 && alpha.y == 0.0)) {
+            beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2), alpha.x);
+            tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+            scal = cdiv(make_double2(1.0, 0.0), make_double2(alpha.x - beta, alpha.y));
+        }
+        if (threadIdx.x == 0) sh.qtau[i] = tau;
+        if (tau.x != 0.0 || tau.y != 0.0) {
+            for (int rr = i + 1 + threadIdx.x; rr < m; rr += NT) {
+                double2& v = A[static_cast<int64_t>(rr) * n + i];
+                v = cmul(v, scal);
+            }
+            __syncthreads();
+            // w_c = v^H A(i:m, c) with v0 = 1, for c > i; one wave per column
+            for (int c = i + 1 + wv; c < n; c += NWAVE) {
+                double2 acc = cz();
+                for (int rr = i + 1 + lane; rr < m; rr += 64)
+                    acc = cadd(acc, cmulc(A[static_cast<int64_t>(rr) * n + i], A[static_cast<int64_t>(rr) * n + c]));
+                acc.x = wave_sum(acc.x);
+                acc.y = wave_sum(acc.y);
+                if (lane == 0) sh.qw[c] = cadd(acc, A[static_cast<int64_t>(i) * n + c]);
+            }
+            __syncthreads();
+            const double2 tc = make_double2(tau.x, -tau.y);
+            const int ncol = n - i - 1;
+            const int64_t tot = static_cast<int64_t>(m - i) * ncol;
+            for (int64_t e = threadIdx.x; e < tot; e += NT) {
+                const int rr = i + static_cast<int>(e / ncol);
+                const int c = i + 1 + static_cast<int>(e % ncol);
+                const double2 v = rr == i ? make_double2(1.0, 0.0) : A[static_cast<int64_t>(rr) * n + i];
+                double2& a = A[static_cast<int64_t>(rr) * n + c];
+                a = csub(a, cmul(tc, cmul(v, sh.qw[c])));
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) A[static_cast<int64_t>(i) * n + i] = make_double2(beta, 0.0 * alpha.y);
+        if (threadIdx.x == 0 && tau.x == 0.0 && tau.y == 0.0) A[static_cast<int64_t>(i) * n + i] = alpha;
+        __syncthreads();
+    }
+    // R
+    for (int64_t e = threadIdx.x; e < static_cast<int64_t>(n) * n; e += NT) {
+        const int i = static_cast<int>(e / n), j = static_cast<int>(e % n);
+        Rout[e] = (i < k && j >= i) ? A[static_cast<int64_t>(i) * n + j] : cz();
+    }
+    // Q = H_0 ... H_{k-1} [I_k; 0]
+    for (int64_t e = threadIdx.x; e < static_cast<int64_t>(m) * n; e += NT) {
+        const int i = static_cast<int>(e / n), j = static_cast<int>(e % n);
+        Q[e] = (i == j && j < k) ? make_double2(1.0, 0.0) : cz();
+    }
+    __syncthreads();
+    for (int i = k - 1; i >= 0; --i) {
+        const double2 tau = sh.qtau[i];
+        if (tau.x == 0.0 && tau.y == 0.0) continue;
+        for (int c = wv; c < k; c += NWAVE) {
+            double2 acc = cz();
+            for (int rr = i + 1 + lane; rr < m; rr += 64)
+                acc = cadd(acc, cmulc(A[static_cast<int64_t>(rr) * n + i], Q[static_cast<int64_t>(rr) * n + c]));
+            acc.x = wave_sum(acc.x);
+            acc.y = wave_sum(acc.y);
+            if (lane == 0) sh.qw[c] = cmul(tau, cadd(acc, Q[static_cast<int64_t>(i) * n + c]));
+        }
+        __syncthreads();
+        const int64_t tot = static_cast<int64_t>(m - i) * k;
+        for (int64_t e = threadIdx.x; e < tot; e += NT) {
+            const int rr = i + static_cast<int>(e / k);
+            const int c = static_cast<int>(e % k);
+            const double2 v = rr == i ? make_double2(1.0, 0.0) : A[static_cast<int64_t>(rr) * n + i];
+            double2& q = Q[static_cast<int64_t>(rr) * n + c];
+            q = csub(q, cmul(v, sh.qw[c]));
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- small dense expm (LDS)
+// C = A B for n x n matrices in LDS, n*n <= NT
+__device__ void sm_matmul(const double2* A, const double2* B, double2* C, int n) {
+    const int t = threadIdx.x;
+    double2 acc = cz();
+    if (t < n * n) {
+        const int i = t / n, j = t % n;
+        for (int q = 0; q < n; ++q) acc = cadd(acc, cmul(A[i * n + q], B[q * n + j]));
+    }
+    __syncthreads();
+    if (t < n * n) C[t] = acc;
+    __syncthreads();
+}
+
+// E = exp(S), Pade(13) scaling and squaring (Higham 2005); S, E in LDS; uses sh.E[*]
+__device__ void sm_expm(Shared& sh, const double2* S, double2* Eout, int n) {
+    const double b[14] = {64764752532480000.0, 32382376266240000.0, 7771770303897600.0, 1187353796428800.0,
+                          129060195264000.0,   10559470521600.0,   670442572800.0,     33522128640.0,
+                          1323241920.0,        40840800.0,         960960.0,           16380.0,
+                          182.0,               1.0};
+    const double theta13 = 5.371920351148152;
+    const int t = threadIdx.x, nn = n * n;
+    double2 *A = sh.E[0], *A2 = sh.E[1], *A4 = sh.E[2], *A6 = sh.E[3], *T1 = sh.E[4], *T2 = sh.E[5], *U = sh.E[6],
+            *V = sh.E[7];
+    // 1-norm and scaling (every thread computes the same value from LDS)
+    double norm1 = 0.0;
+    for (int j = 0; j < n; ++j) {
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s += cabsv(S[i * n + j]);
+        norm1 = fmax(norm1, s);
+    }
+    int sc = 0;
+    if (norm1 > theta13) sc = static_cast<int>(ceil(log2(norm1 / theta13)));
+    const double scale = ldexp(1.0, -sc);
+    if (t < nn) A[t] = cscale(S[t], scale);
+    __syncthreads();
+    sm_matmul(A, A, A2, n);
+    sm_matmul(A2, A2, A4, n);
+    sm_matmul(A4, A2, A6, n);
+    if (t < nn) T1[t] = cadd(cadd(cscale(A6[t], b[13]), cscale(A4[t], b[11])), cscale(A2[t], b[9]));
+    __syncthreads();
+    sm_matmul(A6, T1, T2, n);
+    if (t < nn) {
+        double2 v = cadd(T2[t], cadd(cadd(cscale(A6[t], b[7]), cscale(A4[t], b[5])), cscale(A2[t], b[3])));
+        if (t / n == t % n) v.x += b[1];
+        T2[t] = v;
+    }
+    __syncthreads();
+    sm_matmul(A, T2, U, n);
+    if (t < nn) T1[t] = cadd(cadd(cscale(A6[t], b[12]), cscale(A4[t], b[10])), cscale(A2[t], b[8]));
+    __syncthreads();
+    sm_matmul(A6, T1, V, n);
+    if (t < nn) {
+        double2 v = cadd(V[t], cadd(cadd(cscale(A6[t], b[6]), cscale(A4[t], b[4])), cscale(A2[t], b[2])));
+        if (t / n == t % n) v.x += b[0];
+        V[t] = v;
+    }
+    __syncthreads();
+    // P = V - U (into A), X = V + U (into T1)
+    double2 *P = A, *X = T1;
+    if (t < nn) {
+        const double2 u = U[t], v = V[t];
+        P[t] = csub(v, u);
+        X[t] = cadd(v, u);
+    }
+    __syncthreads();
+    for (int kk = 0; kk < n; ++kk) {
+        if (t == 0) {
+            int piv = kk;
+            double best = cabsv(P[kk * n + kk]);
+            for (int i = kk + 1; i < n; ++i) {
+                const double a = cabsv(P[i * n + kk]);
+                if (a > best) {
+                    best = a;
+                    piv = i;
+                }
+            }
+            sh.ipiv = piv;
+        }
+        __syncthreads();
+        const int piv = sh.ipiv;
+        if (piv != kk && t < n) {
+            double2 a = P[kk * n + t];
+            P[kk * n + t] = P[piv * n + t];
+            P[piv * n + t] = a;
+            a = X[kk * n + t];
+            X[kk * n + t] = X[piv * n + t];
+            X[piv * n + t] = a;
+        }
+        __syncthreads();
+        if (t > kk && t < n) sh.lu_f[t] = cdiv(P[t * n + kk], P[kk * n + kk]);
+        __syncthreads();
+        if (t < nn) {
+            const int i = t / n, j = t % n;
+            if (i > kk) {
+                const double2 f = sh.lu_f[i];
+                if (j >= kk) P[i * n + j] = csub(P[i * n + j], cmul(f, P[kk * n + j]));
+                X[i * n + j] = csub(X[i * n + j], cmul(f, X[kk * n + j]));
+            }
+        }
+        __syncthreads();
+    }
+    for (int kk = n - 1; kk >= 0; --kk) {
+        if (t < n) {
+            double2 s = X[kk * n + t];
+            for (int i = kk + 1; i < n; ++i) s = csub(s, cmul(P[kk * n + i], X[i * n + t]));
+            X[kk * n + t] = cdiv(s, P[kk * n + kk]);
+        }
+        __syncthreads();
+    }
+    double2* cur = X;
+    double2* nxt = T2;
+    for (int s = 0; s < sc; ++s) {
+        sm_matmul(cur, cur, nxt, n);
+        double2* tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+    }
+    if (t < nn) Eout[t] = cur[t];
+    __syncthreads();
+}
+
+}  // namespace ttnd
