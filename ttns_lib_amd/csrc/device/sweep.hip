@@ -54,40 +54,73 @@ __device__ void kpo(Ctx& c, const DNode& nd, int k0, int nk, const double2* v, d
 
 // coefficient_evolution_{leaf,branch} (projector_splitting_evolution_functions.hpp:65-258)
 // uses W1..W4
+// out (=|+=) alpha * K v for one spf sub-term: a leaf's primitive operator or a chain of
+// mode contractions (intermediates in t1/t2, the last one lands in out with the epilogue)
+__device__ void subterm_into(Ctx& c, const DNode& nd, const DSub& s, const double2* v, double2* out, double2 alpha,
+                             bool accumulate, double2* t1, double2* t2) {
+    if (s.nk == 0) {
+        if (accumulate)
+            wg_axpy(out, alpha, v, nd.size);
+        else
+            wg_scale_to(out, alpha, v, nd.size);
+        return;
+    }
+    if (nd.leaf >= 0) {
+        wg_prim_apply(c.I.ks[s.k0], c.base, c.I.ints, v, out, nd.hrank, alpha, accumulate);
+        return;
+    }
+    const double2* src = v;
+    for (int q = 0; q < s.nk; ++q) {
+        const bool last = q == s.nk - 1;
+        double2* dst = last ? out : (q % 2 == 0 ? t1 : t2);
+        const DK& k = c.I.ks[s.k0 + q];
+        int64_t L, R;
+        mode_view(nd, k.nu, L, R);
+        wg_mode_contract(c.base + k.mat, L, nd.dims[k.nu], R, src, dst, last ? alpha : cmk(1.0, 0.0),
+                         last && accumulate);
+        src = dst;
+    }
+}
+
+// res = sum_t [sum_i c_i K_{t,i} v] mf_t^T with the identity shortcuts of the reference;
+// accumulation and coefficients are folded into the GEMM epilogues.  Uses W1..W3.
 __device__ void heff_node(Ctx& c, int n, const double2* v, double2* res) {
     const DNode& nd = c.node(n);
-    const int64_t sz = nd.size;
     const int r = nd.hrank;
     const int64_t D = nd.dimen;
     double2 *t1 = c.W(1), *t2 = c.W(2), *t3 = c.W(3);
-    wg_zero(res, sz);
+    bool first = true;  // res not written yet
     for (int ti = 0; ti < nd.nterms; ++ti) {
         const DTerm& tm = c.I.terms[nd.term0 + ti];
         if (tm.id_spf) {
             const double2 coeff = cmk(tm.ire, tm.iim);
-            if (tm.id_mf)
-                wg_axpy(res, coeff, v, sz);
-            else
-                wg_mul_transposed(v, D, r, c.base + tm.mf, res, coeff, true);
+            if (tm.id_mf) {
+                if (first)
+                    wg_scale_to(res, coeff, v, nd.size);
+                else
+                    wg_axpy(res, coeff, v, nd.size);
+            } else {
+                wg_mul_transposed(v, D, r, c.base + tm.mf, res, coeff, !first);
+            }
+            first = false;
             continue;
         }
         for (int i = 0; i < tm.nsub; ++i) {
             const DSub& s = c.I.subs[tm.sub0 + i];
             const double2 ci = (i == 0) ? (tm.nsub > 1 ? cmk(s.are, s.aim) : cmk(tm.cre, tm.cim)) : cmk(s.are, s.aim);
-            if (nd.leaf >= 0 && s.nk > 0)
-                wg_prim_apply(c.I.ks[s.k0], c.base, c.I.ints, v, t2, r);
-            else
-                kpo(c, nd, s.k0, s.nk, v, t2, t1);
-            if (i == 0)
-                wg_scale_to(t3, ci, t2, sz);
-            else
-                wg_axpy(t3, ci, t2, sz);
+            if (tm.id_mf) {
+                subterm_into(c, nd, s, v, res, ci, !first, t1, t2);
+                first = false;
+            } else {
+                subterm_into(c, nd, s, v, t3, ci, i > 0, t1, t2);
+            }
         }
-        if (tm.id_mf)
-            wg_axpy(res, cmk(1.0, 0.0), t3, sz);
-        else
-            wg_mul_transposed(t3, D, r, c.base + tm.mf, res, cmk(1.0, 0.0), true);
+        if (!tm.id_mf) {
+            wg_mul_transposed(t3, D, r, c.base + tm.mf, res, cmk(1.0, 0.0), !first);
+            first = false;
+        }
     }
+    if (first) wg_zero(res, nd.size);
 }
 
 // full_hamiltonian_action (projector_splitting_evolution_functions.hpp:7-62); uses W1, W2
@@ -130,9 +163,9 @@ __device__ void update_spf(Ctx& c, int n) {
             else
                 kpo(c, nd, s.k0, s.nk, a, HA, tmp);
             if (tm.nsub == 1)
-                wg_adjoint_mul(a, HA, D, r, out, cmk(1.0, 0.0), false);
+                wg_adjoint_mul(c.sh, a, HA, D, r, out, cmk(1.0, 0.0), false);
             else
-                wg_adjoint_mul(a, HA, D, r, out, cmk(s.are, s.aim), i > 0);
+                wg_adjoint_mul(c.sh, a, HA, D, r, out, cmk(s.are, s.aim), i > 0);
         }
     }
 }
@@ -167,9 +200,9 @@ __device__ void update_mf(Ctx& c, int n) {
                 HA = t3;
             }
             if (tm.nmf == 1)
-                wg_outer_contract(Ap, HA, L, d, R, out, cmk(1.0, 0.0), false);
+                wg_outer_contract(c.sh, Ap, HA, L, d, R, out, cmk(1.0, 0.0), false);
             else
-                wg_outer_contract(Ap, HA, L, d, R, out, cmk(m.are, m.aim), it > 0);
+                wg_outer_contract(c.sh, Ap, HA, L, d, R, out, cmk(m.are, m.aim), it > 0);
         }
     }
 }
@@ -277,34 +310,66 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
         for (int j = 0; j < kdim; ++j) {
             H(V + j * ld, w);
             ++napply;
-            for (int pass = 0; pass < 2; ++pass) {
-                double p[2 * MAXK], o[2 * MAXK];
+            // classical Gram-Schmidt applied twice (CGS2): the second pass's inner
+            // products accumulate while the first correction is written, the norm while
+            // the second one is written (three sweeps over the basis instead of five)
+            double p[2 * MAXK];
+            const double* h1 = sh.hb[0];
+            const double* h2 = sh.hb[1];
 #pragma unroll
-                for (int i = 0; i < 2 * MAXK; ++i) p[i] = 0.0;
-                for (int64_t q = threadIdx.x; q < n; q += NT) {
-                    const double2 wq = w[q];
+            for (int i = 0; i < 2 * MAXK; ++i) p[i] = 0.0;
+            for (int64_t q = threadIdx.x; q < n; q += NT) {
+                const double2 wq = w[q];
 #pragma unroll
-                    for (int i = 0; i < MAXK; ++i) {
-                        if (i <= j) {
-                            const double2 pr = cmulc(V[i * ld + q], wq);
-                            p[2 * i] += pr.x;
-                            p[2 * i + 1] += pr.y;
-                        }
+                for (int i = 0; i < MAXK; ++i) {
+                    if (i <= j) {
+                        const double2 pr = cmulc(V[i * ld + q], wq);
+                        p[2 * i] += pr.x;
+                        p[2 * i + 1] += pr.y;
                     }
                 }
-                block_sum<2 * MAXK>(sh, p, o);
-                for (int64_t q = threadIdx.x; q < n; q += NT) {
-                    double2 wq = w[q];
-                    for (int i = 0; i <= j; ++i) wq = csub(wq, cmul(cmk(o[2 * i], o[2 * i + 1]), V[i * ld + q]));
-                    w[q] = wq;
-                }
-                if (threadIdx.x <= j) {
-                    const int i = threadIdx.x;
-                    sh.Hm[i * MAXK + j] = cadd(sh.Hm[i * MAXK + j], cmk(o[2 * i], o[2 * i + 1]));
-                }
-                __syncthreads();
             }
-            hn = sqrt(block_norm2(sh, w, n));
+            block_sum_lds<2 * MAXK>(sh, p, sh.hb[0]);
+#pragma unroll
+            for (int i = 0; i < 2 * MAXK; ++i) p[i] = 0.0;
+            for (int64_t q = threadIdx.x; q < n; q += NT) {
+                double2 vi[MAXK];
+                double2 wq = w[q];
+#pragma unroll
+                for (int i = 0; i < MAXK; ++i) {
+                    if (i <= j) {
+                        vi[i] = V[i * ld + q];
+                        wq = csub(wq, cmul(cmk(h1[2 * i], h1[2 * i + 1]), vi[i]));
+                    }
+                }
+                w[q] = wq;
+#pragma unroll
+                for (int i = 0; i < MAXK; ++i) {
+                    if (i <= j) {
+                        const double2 pr = cmulc(vi[i], wq);
+                        p[2 * i] += pr.x;
+                        p[2 * i + 1] += pr.y;
+                    }
+                }
+            }
+            block_sum_lds<2 * MAXK>(sh, p, sh.hb[1]);
+            double nn[1] = {0.0}, nno[1];
+            for (int64_t q = threadIdx.x; q < n; q += NT) {
+                double2 wq = w[q];
+#pragma unroll
+                for (int i = 0; i < MAXK; ++i)
+                    if (i <= j) wq = csub(wq, cmul(cmk(h2[2 * i], h2[2 * i + 1]), V[i * ld + q]));
+                w[q] = wq;
+                nn[0] += cabs2(wq);
+            }
+            block_sum<1>(sh, nn, nno);
+            if (threadIdx.x <= j) {
+                const int i = threadIdx.x;
+                sh.Hm[i * MAXK + j] =
+                    cadd(cadd(sh.Hm[i * MAXK + j], cmk(h1[2 * i], h1[2 * i + 1])), cmk(h2[2 * i], h2[2 * i + 1]));
+            }
+            __syncthreads();
+            hn = sqrt(nno[0]);
             if (!isfinite(hn)) {
                 c.fail();
                 return napply;
@@ -433,7 +498,7 @@ __device__ void path_expect(Ctx& c, int leaf, int64_t op_off, double2& val, doub
     k.mat = op_off;
     k.idx = 0;
     wg_prim_apply(k, c.base, c.I.ints, c.base + ln.A, HA, ln.hrank);
-    wg_adjoint_mul(c.base + ln.A, HA, ln.dimen, ln.hrank, M, cmk(1.0, 0.0), false);
+    wg_adjoint_mul(c.sh, c.base + ln.A, HA, ln.dimen, ln.hrank, M, cmk(1.0, 0.0), false);
     int n = leaf;
     while (c.node(n).parent >= 0) {
         const DNode& nd = c.node(n);
@@ -441,7 +506,7 @@ __device__ void path_expect(Ctx& c, int leaf, int64_t op_off, double2& val, doub
         int64_t L, R;
         mode_view(pn, nd.child_id, L, R);
         wg_mode_contract(M, L, pn.dims[nd.child_id], R, c.base + pn.A, T, cmk(1.0, 0.0), false);
-        wg_adjoint_mul(c.base + pn.A, T, pn.dimen, pn.hrank, M2, cmk(1.0, 0.0), false);
+        wg_adjoint_mul(c.sh, c.base + pn.A, T, pn.dimen, pn.hrank, M2, cmk(1.0, 0.0), false);
         double2* tmp = M;
         M = M2;
         M2 = tmp;

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include "desc.hpp"
+#include "zgemm.hpp"
 
 namespace ttnd {
 
@@ -53,6 +54,7 @@ struct Shared {
     double2 lu_f[MAXK];
     int ipiv;
     int fail;           // set (uniformly) when a numerical failure is detected
+    double hb[2][2 * MAXK];  // Gram-Schmidt coefficients of the two CGS passes
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -77,6 +79,24 @@ __device__ void block_sum(Shared& sh, double (&v)[K], double (&out)[K]) {
 #pragma unroll
         for (int w = 0; w < NWAVE; ++w) s += sh.red[w * K + k];
         out[k] = s;
+    }
+    __syncthreads();
+}
+
+// sum of K doubles over the workgroup, totals left in the LDS array dst[K]
+template <int K>
+__device__ void block_sum_lds(Shared& sh, double (&v)[K], double* dst) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double s = wave_sum(v[k]);
+        if (lane == 0) sh.red[wv * K + k] = s;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < K; k += NT) {
+        double s = 0.0;
+        for (int w = 0; w < NWAVE; ++w) s += sh.red[w * K + k];
+        dst[k] = s;
     }
     __syncthreads();
 }
@@ -108,23 +128,28 @@ __device__ void wg_scale_to(double2* y, double2 a, const double2* x, int64_t n) 
     __syncthreads();
 }
 
-// ---------------------------------------------------------------- contractions
+// ---------------------------------------------------------------- contractions (FP64 MFMA)
 // dst(l,i,r) (=|+=) alpha * sum_j op(i,j) src(l,j,r) on the (L, d, R) view.
 // Reference: contract(op, 1, A3, 1) in kronecker_product_operator_helper.hpp:292-303.
+// One 16x16 tile of (i, r) per wave work item; op rows come from L1.
 __device__ void wg_mode_contract(const double2* op, int64_t L, int d, int64_t R, const double2* src, double2* dst,
                                  double2 alpha, bool accumulate) {
-    const int64_t total = L * d * R;
-    for (int64_t idx = threadIdx.x; idx < total; idx += NT) {
-        const int64_t r = idx % R;
-        const int64_t li = idx / R;
-        const int i = static_cast<int>(li % d);
-        const int64_t l = li / d;
-        const double2* s = src + l * d * R + r;
-        const double2* o = op + static_cast<int64_t>(i) * d;
-        double2 acc = cz();
-        for (int j = 0; j < d; ++j) acc = cadd(acc, cmul(o[j], s[static_cast<int64_t>(j) * R]));
-        acc = cmul(alpha, acc);
-        dst[idx] = accumulate ? cadd(dst[idx], acc) : acc;
+    const int wv = threadIdx.x >> 6;
+    const int mt = (d + 15) >> 4;
+    const int64_t nt = (R + 15) >> 4;
+    const int64_t items = L * mt * nt;
+    for (int64_t it = wv; it < items; it += NWAVE) {
+        const int64_t n = it % nt;
+        const int m = static_cast<int>((it / nt) % mt);
+        const int64_t l = it / (nt * mt);
+        ZAcc acc;
+        zacc_zero(acc);
+        const int mv = min(16, d - 16 * m);
+        const int nv = static_cast<int>(R - 16 * n < 16 ? R - 16 * n : 16);
+        const double2* s = src + l * d * R + 16 * n;
+        ztile<false>(acc, op + static_cast<int64_t>(16 * m) * d, d, 1, mv, s, R, 1, nv, d);
+        ztile_store(acc, dst + l * d * R + static_cast<int64_t>(16 * m) * R + 16 * n, R, 1, mv, nv, alpha,
+                    accumulate);
     }
     __syncthreads();
 }
@@ -132,84 +157,126 @@ __device__ void wg_mode_contract(const double2* op, int64_t L, int d, int64_t R,
 // dst (=|+=) alpha * X M^T, X: rows x r, M: r x r  (v * trans(mf))
 __device__ void wg_mul_transposed(const double2* X, int64_t rows, int r, const double2* M, double2* dst,
                                   double2 alpha, bool accumulate) {
-    const int64_t total = rows * r;
-    for (int64_t idx = threadIdx.x; idx < total; idx += NT) {
-        const int64_t x = idx / r;
-        const int i = static_cast<int>(idx % r);
-        const double2* xr = X + x * r;
-        const double2* mr = M + static_cast<int64_t>(i) * r;
-        double2 acc = cz();
-        for (int j = 0; j < r; ++j) acc = cadd(acc, cmul(xr[j], mr[j]));
-        acc = cmul(alpha, acc);
-        dst[idx] = accumulate ? cadd(dst[idx], acc) : acc;
+    const int wv = threadIdx.x >> 6;
+    const int64_t mt = (rows + 15) >> 4;
+    const int nt = (r + 15) >> 4;
+    const int64_t items = mt * nt;
+    for (int64_t it = wv; it < items; it += NWAVE) {
+        const int n = static_cast<int>(it % nt);
+        const int64_t m = it / nt;
+        ZAcc acc;
+        zacc_zero(acc);
+        const int mv = static_cast<int>(rows - 16 * m < 16 ? rows - 16 * m : 16);
+        const int nv = min(16, r - 16 * n);
+        ztile<false>(acc, X + 16 * m * r, r, 1, mv, M + static_cast<int64_t>(16 * n) * r, 1, r, nv, r);
+        ztile_store(acc, dst + 16 * m * r + 16 * n, r, 1, mv, nv, alpha, accumulate);
     }
     __syncthreads();
 }
 
 // dst = X M, X: rows x r, M: r x r
 __device__ void wg_mul(const double2* X, int64_t rows, int r, const double2* M, double2* dst) {
-    const int64_t total = rows * r;
-    for (int64_t idx = threadIdx.x; idx < total; idx += NT) {
-        const int64_t x = idx / r;
-        const int j = static_cast<int>(idx % r);
-        const double2* xr = X + x * r;
-        double2 acc = cz();
-        for (int k = 0; k < r; ++k) acc = cadd(acc, cmul(xr[k], M[static_cast<int64_t>(k) * r + j]));
-        dst[idx] = acc;
+    const int wv = threadIdx.x >> 6;
+    const int64_t mt = (rows + 15) >> 4;
+    const int nt = (r + 15) >> 4;
+    const int64_t items = mt * nt;
+    for (int64_t it = wv; it < items; it += NWAVE) {
+        const int n = static_cast<int>(it % nt);
+        const int64_t m = it / nt;
+        ZAcc acc;
+        zacc_zero(acc);
+        const int mv = static_cast<int>(rows - 16 * m < 16 ? rows - 16 * m : 16);
+        const int nv = min(16, r - 16 * n);
+        ztile<false>(acc, X + 16 * m * r, r, 1, mv, M + 16 * n, r, 1, nv, r);
+        ztile_store(acc, dst + 16 * m * r + 16 * n, r, 1, mv, nv, make_double2(1.0, 0.0), false);
     }
     __syncthreads();
 }
 
-// out(i,j) (=|+=) alpha * sum_{l,r} conj(A(l,i,r)) B(l,j,r) on the (L, d, R) view.
-// For L = 1, R = 1 this is A^H B of (rows x d) matrices with rows = ... see callers.
-__device__ void wg_outer_contract(const double2* A, const double2* B, int64_t L, int d, int64_t R, double2* out,
-                                  double2 alpha, bool accumulate) {
-    const int nout = d * d;
-    // threads per output element
-    int tpo = NT / nout;
-    if (tpo < 1) tpo = 1;
-    if (tpo > 64) tpo = 64;
-    // round down to a power of two so that groups stay inside a wave
-    int p2 = 1;
-    while (p2 * 2 <= tpo) p2 *= 2;
-    tpo = p2;
-    const int groups = NT / tpo;
-    const int g = threadIdx.x / tpo, sub = threadIdx.x % tpo;
-    const int64_t K = L * R;
-    for (int base = 0; base < nout; base += groups) {
-        const int o = base + g;
-        double2 acc = cz();
-        if (o < nout) {
-            const int i = o / d, j = o % d;
-            for (int64_t k = sub; k < K; k += tpo) {
-                const int64_t l = k / R, r = k % R;
-                const int64_t off = l * d * R + r;
-                acc = cadd(acc, cmulc(A[off + static_cast<int64_t>(i) * R], B[off + static_cast<int64_t>(j) * R]));
+// out(i,j) (=|+=) alpha * sum_{l,r} conj(A(l,i,r)) B(l,j,r) on the (L, d, R) view: a
+// long-K product.  With several output tiles the waves take whole tiles; with a single
+// tile (d <= 16) they split K and the partial tiles are summed through LDS.
+__device__ void wg_outer_contract(Shared& sh, const double2* A, const double2* B, int64_t L, int d, int64_t R,
+                                  double2* out, double2 alpha, bool accumulate) {
+    const int wv = threadIdx.x >> 6;
+    const int mt = (d + 15) >> 4;
+    const int tiles = mt * mt;
+    const bool split = tiles < NWAVE;
+    for (int t0 = 0; t0 < tiles; t0 += split ? 1 : NWAVE) {
+        const int t = split ? t0 : t0 + wv;
+        ZAcc acc;
+        zacc_zero(acc);
+        const int m = t / mt, n = t % mt;
+        const int mv = t < tiles ? min(16, d - 16 * m) : 0;
+        const int nv = t < tiles ? min(16, d - 16 * n) : 0;
+        if (t < tiles) {
+            if (R == 1) {
+                // element (l, i) at l*d + i: K runs over l
+                const int64_t k0 = split ? (L * wv) / NWAVE : 0, k1 = split ? (L * (wv + 1)) / NWAVE : L;
+                ztile<true>(acc, A + k0 * d + 16 * m, 1, d, mv, B + k0 * d + 16 * n, d, 1, nv, k1 - k0);
+            } else {
+                for (int64_t l = split ? wv : 0; l < L; l += split ? NWAVE : 1) {
+                    const double2* a = A + l * d * R + static_cast<int64_t>(16 * m) * R;
+                    const double2* b = B + l * d * R + static_cast<int64_t>(16 * n) * R;
+                    if (split && L < NWAVE) {
+                        // too few slices: split each slice's K range instead (handled below)
+                        break;
+                    }
+                    ztile<true>(acc, a, R, 1, mv, b, 1, R, nv, R);
+                }
+                if (split && L < NWAVE) {
+                    for (int64_t l = 0; l < L; ++l) {
+                        const int64_t k0 = (R * wv) / NWAVE, k1 = (R * (wv + 1)) / NWAVE;
+                        const double2* a = A + l * d * R + static_cast<int64_t>(16 * m) * R + k0;
+                        const double2* b = B + l * d * R + static_cast<int64_t>(16 * n) * R + k0;
+                        ztile<true>(acc, a, R, 1, mv, b, 1, R, nv, k1 - k0);
+                    }
+                }
             }
         }
-        // reduce within the tpo-sized group (inside one wave)
-        for (int s = tpo / 2; s > 0; s >>= 1) {
-            acc.x += __shfl_xor(acc.x, s, 64);
-            acc.y += __shfl_xor(acc.y, s, 64);
-        }
-        if (o < nout && sub == 0) {
-            const double2 v = cmul(alpha, acc);
-            out[o] = accumulate ? cadd(out[o], v) : v;
+        if (!split) {
+            if (t < tiles)
+                ztile_store(acc, out + static_cast<int64_t>(16 * m) * d + 16 * n, d, 1, mv, nv, alpha, accumulate);
+        } else {
+            // sum the four partial tiles in LDS (reuses the Krylov scratch E[0..7])
+            double* red = reinterpret_cast<double*>(&sh.E[0][0]);
+            const int lane = threadIdx.x & 63;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                red[(wv * 64 + lane) * 8 + j] = acc.re[j];
+                red[(wv * 64 + lane) * 8 + 4 + j] = acc.im[j];
+            }
+            __syncthreads();
+            if (wv == 0) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    double re = 0.0, im = 0.0;
+                    for (int w = 0; w < NWAVE; ++w) {
+                        re += red[(w * 64 + lane) * 8 + j];
+                        im += red[(w * 64 + lane) * 8 + 4 + j];
+                    }
+                    acc.re[j] = re;
+                    acc.im[j] = im;
+                }
+                ztile_store(acc, out + static_cast<int64_t>(16 * m) * d + 16 * n, d, 1, mv, nv, alpha, accumulate);
+            }
+            __syncthreads();
         }
     }
     __syncthreads();
 }
 
 // out = A^H B for A, B: rows x r
-__device__ void wg_adjoint_mul(const double2* A, const double2* B, int64_t rows, int r, double2* out, double2 alpha,
-                               bool accumulate) {
-    wg_outer_contract(A, B, rows, r, 1, out, alpha, accumulate);
+__device__ void wg_adjoint_mul(Shared& sh, const double2* A, const double2* B, int64_t rows, int r, double2* out,
+                               double2 alpha, bool accumulate) {
+    wg_outer_contract(sh, A, B, rows, r, 1, out, alpha, accumulate);
 }
 
 // ---------------------------------------------------------------- primitive leaf operators
 // y = op x, x/y: dim x ncols row-major; kind: 0 dense, 1 diag, 2 csr
+// y (=|+=) alpha * op x
 __device__ void wg_prim_apply(const DK& k, const double2* base, const int64_t* ints, const double2* x, double2* y,
-                              int ncols) {
+                              int ncols, double2 alpha = make_double2(1.0, 0.0), bool accumulate = false) {
     const int d = k.dim;
     const double2* mat = base + k.mat;
     const int64_t total = static_cast<int64_t>(d) * ncols;
@@ -226,7 +293,8 @@ __device__ void wg_prim_apply(const DK& k, const double2* base, const int64_t* i
             for (int64_t q = rowptr[i]; q < rowptr[i + 1]; ++q)
                 acc = cadd(acc, cmul(mat[q], x[colind[q] * ncols + c]));
         }
-        y[idx] = acc;
+        acc = cmul(alpha, acc);
+        y[idx] = accumulate ? cadd(y[idx], acc) : acc;
     }
     __syncthreads();
 }

@@ -1,0 +1,85 @@
+// Complex128 GEMM tiles on the FP64 matrix cores of gfx950.
+//
+// One wave computes a 16x16 complex tile with V_MFMA_F64_16X16X4F64: a complex
+// multiply-accumulate over k is four real MFMAs (re*re - im*im into the real
+// accumulator, re*im + im*re into the imaginary one); no 3M trick, so rounding matches a
+// plain complex product sum.  Operand layout of the instruction (one double per lane):
+//   A (16 x 4): lane l holds A[l % 16][l / 16];  B (4 x 16): lane l holds B[l / 16][l % 16];
+//   D (16 x 16): lane l holds D[l / 16 + 4 * j][l % 16] in accumulator element j
+//   (measured on gfx950 with a lane-id probe).
+// Operands come straight from L2/L1 (node tensors are too large for LDS); loads for four
+// k-steps are issued before their MFMAs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace ttnd {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+struct ZAcc {
+    d4 re, im;
+};
+
+__device__ __forceinline__ void zacc_zero(ZAcc& a) {
+    a.re = d4{0.0, 0.0, 0.0, 0.0};
+    a.im = d4{0.0, 0.0, 0.0, 0.0};
+}
+
+__device__ __forceinline__ void zmfma(ZAcc& acc, double2 a, double2 b) {
+    acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, acc.re, 0, 0, 0);
+    acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, acc.re, 0, 0, 0);
+    acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, acc.im, 0, 0, 0);
+    acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, acc.im, 0, 0, 0);
+}
+
+// acc += op(A) B over k in [0, K): A(i, k) at a[i*sar + k*sak] (conjugated if CONJA),
+// B(k, j) at b[k*sbk + j*sbc]; rows >= mv and columns >= nv read as zero.
+template <bool CONJA>
+__device__ __forceinline__ void ztile(ZAcc& acc, const double2* __restrict__ a, int64_t sar, int64_t sak, int mv,
+                                      const double2* __restrict__ b, int64_t sbk, int64_t sbc, int nv, int64_t K) {
+    const int lane = threadIdx.x & 63, i = lane & 15, kq = lane >> 4;
+    const bool arow = i < mv, bcol = i < nv;
+    const double2 z = make_double2(0.0, 0.0);
+    for (int64_t k0 = 0; k0 < K; k0 += 16) {
+        double2 av[4], bv[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int64_t k = k0 + 4 * s + kq;
+            const bool ok = k < K;
+            av[s] = (arow && ok) ? a[i * sar + k * sak] : z;
+            bv[s] = (bcol && ok) ? b[k * sbk + i * sbc] : z;
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            if (k0 + 4 * s < K) {  // wave-uniform
+                double2 x = av[s];
+                if (CONJA) x.y = -x.y;
+                zmfma(acc, x, bv[s]);
+            }
+        }
+    }
+}
+
+// C(row, col) (=|+=) alpha * acc for the tile's valid rows/cols; C(row, col) at c[row*scr + col*scc]
+__device__ __forceinline__ void ztile_store(const ZAcc& acc, double2* c, int64_t scr, int64_t scc, int mv, int nv,
+                                            double2 alpha, bool accumulate) {
+    const int lane = threadIdx.x & 63, col = lane & 15, rb = lane >> 4;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int row = rb + 4 * j;
+        if (row < mv && col < nv) {
+            const double re = acc.re[j], im = acc.im[j];
+            double2 v = make_double2(alpha.x * re - alpha.y * im, alpha.x * im + alpha.y * re);
+            double2* p = c + row * scr + col * scc;
+            if (accumulate) {
+                const double2 o = *p;
+                v.x += o.x;
+                v.y += o.y;
+            }
+            *p = v;
+        }
+    }
+}
+
+}  // namespace ttnd
