@@ -1,0 +1,216 @@
+"""Benchmark of the TDVP time step on one MI355X.
+
+Default workload: the batched configuration -- 64 uneven spin-boson instances (Ohmic baths,
+alpha in [0.05, 1], wc in {1, 2, 5}, beta in {1, 10, inf}, 32-128 modes, boson dims 8-20,
+bond dims 16-48, dt 0.05, complex128) advanced together by the persistent sweep kernel.
+
+One "step" is one TDVP time step of every instance in the batch.  The timed region covers
+exactly --steps batched steps after --warmup untimed ones (the first step of a run carries
+the reference's prepare pass and six-sub-step ladder, so it is always part of the warmup).
+`value` is instance-steps per second of the whole batch.  Besides it the JSON line carries:
+  roofline      algorithmic FP64 flops of the timed steps (H applications counted on the
+                device, per-node cost from the compiled term tables) / kernel time, against
+                the MI355X FP64 matrix peak (78.6 TFLOP/s)
+  plain_gpu     the same instances stepped one at a time (a sample of them)
+  cpu_baseline  the oracle (C++ CPU restatement, 1 thread) on a bounded sample, scaled to
+                the whole batch by flop count
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "TDVP steps/s + FP64-MFMA roofline %, batched vs plain GPU vs CPU, <sz> err<=1e-10"
+PEAK_FP64_TFLOPS = 78.6
+
+
+def make(lib, inst):
+    from ttns_lib_amd.capi import SIGMA, Integrator, sb_problem
+
+    tree, sop = sb_problem(lib, inst.model)
+    ig = Integrator(lib, tree, sop, dt=inst.dt, krylov_dim=inst.krylov_dim, krylov_tol=1e-12)
+    ig.set_sb_initial_state(inst.model)
+    ig.set_observable(0, SIGMA["z"])
+    return ig
+
+
+def workload(name):
+    from ttns_lib_amd import workloads as W
+
+    if name == "c3":
+        return W.c3_instances(), ("64 uneven spin-boson instances, Ohmic alpha U[0.05,1], wc {1,2,5}, "
+                                  "beta {1,10,inf}, 32-128 modes, boson 8-20, bond 16-48, dt 0.05")
+    if name == "c0":
+        return [W.c0_instance()], "16 modes, boson 10, bond 8, Ohmic alpha 0.1 wc 5 T=0, dt 0.05"
+    if name == "c0x64":
+        insts = []
+        for s in range(64):
+            i = W.c0_instance()
+            i.model.seed = s
+            insts.append(i)
+        return insts, "64 copies of the 16-mode C0 instance (different random unoccupied functions)"
+    if name == "c1":
+        return [W.c1_instance()], "Debye alpha 0.5 wc 2 beta 10, 128 modes, boson 16, bond 32 (fixed), dt 0.025"
+    if name == "c4":
+        return [W.c4_instance()], "Ohmic alpha 0.2 wc 5 T=0, 64 modes, boson 12, bond 24, dt 0.05"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(insts, flops_per_step, budget_s):
+    """Time the oracle (1 thread) on the cheapest instances within a time budget; scale the
+    measured CPU flop rate to the whole batch."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from ttns_lib_amd.capi import Integrator, Library, sb_problem
+
+    so = os.path.join(ROOT, "oracle", "_build", "libttn_oracle.so")
+    if not os.path.exists(so):
+        return None
+    orc = Library(so, "orc_")
+    order = np.argsort(flops_per_step)
+    used, t_cpu, f_cpu, steps = [], 0.0, 0.0, 0
+    t_start = time.perf_counter()
+    for i in order:
+        inst = insts[i]
+        tree, sop = sb_problem(orc, inst.model)
+        ig = Integrator(orc, tree, sop, dt=inst.dt, krylov_dim=inst.krylov_dim, krylov_tol=1e-12)
+        ig.set_sb_initial_state(inst.model)
+        ig.step(1)  # first step (ladder) untimed
+        t0 = time.perf_counter()
+        ig.step(1)
+        dt = time.perf_counter() - t0
+        used.append(int(i))
+        t_cpu += dt
+        f_cpu += flops_per_step[i]
+        steps += 1
+        if time.perf_counter() - t_start > budget_s:
+            break
+    rate = f_cpu / t_cpu  # flop/s of the CPU restatement
+    total_time = float(np.sum(flops_per_step)) / rate  # one batched step on one core
+    return {
+        "value": len(insts) / total_time,
+        "unit": "instance-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"oracle, 1 thread, one regular step of instances {used} (after their first step); "
+                  f"measured {steps / t_cpu:.4g} steps/s = {rate / 1e9:.3g} GFLOP/s, scaled to the batch by "
+                  f"per-instance flop count",
+        "cpu_model": _cpu_model(),
+        "nproc": os.cpu_count(),
+    }
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c3")
+    ap.add_argument("--plain-sample", type=int, default=4, help="instances stepped alone for the plain GPU rate")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle time for the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the batched sweep "
+              "shards across GPUs by instance, which is not part of this build", file=sys.stderr)
+        raise SystemExit(2)
+    warm = max(1, args.warmup)
+
+    import ttns_lib_amd
+    from ttns_lib_amd.capi import Batch
+
+    lib = ttns_lib_amd.load_library()
+    insts, desc = workload(args.workload)
+    print(f"[bench] workload {args.workload}: {len(insts)} instances", file=sys.stderr, flush=True)
+    igs = [make(lib, i) for i in insts]
+    batch = Batch(lib, igs)
+    t0 = time.perf_counter()
+    batch.step(warm)
+    print(f"[bench] warmup {warm} step(s) {time.perf_counter() - t0:.2f}s", file=sys.stderr, flush=True)
+    f0 = np.array([ig.flops() for ig in igs])
+    t0 = time.perf_counter()
+    batch.step(args.steps)
+    wall = time.perf_counter() - t0
+    kernel_ms = batch.last_step_ms()
+    f1 = np.array([ig.flops() for ig in igs])
+    flops = float(np.sum(f1 - f0))
+    per_inst_flops_step = (f1 - f0) / args.steps
+    value = len(insts) * args.steps / wall
+    achieved = flops / (kernel_ms / 1e3) / 1e12
+    print(f"[bench] timed {args.steps} step(s): wall {wall:.3f}s kernel {kernel_ms / 1e3:.3f}s "
+          f"{flops / 1e9:.1f} GFLOP", file=sys.stderr, flush=True)
+    sz = [float(ig.trajectory()[-1, 0]) for ig in igs]
+    norm_dev = max(abs(float(ig.trajectory()[-1, 1]) - 1.0) for ig in igs)
+
+    # plain per-instance path on a sample spread over the flop range
+    plain = None
+    if args.plain_sample > 0 and len(insts) > 1:
+        order = np.argsort(per_inst_flops_step)
+        pick = sorted(set(int(order[int(round(q * (len(order) - 1)))])
+                          for q in np.linspace(0, 1, args.plain_sample)))
+        t_sum, f_sum = 0.0, 0.0
+        for i in pick:
+            ig = make(lib, insts[i])
+            ig.step(warm)
+            a = ig.flops()
+            t1 = time.perf_counter()
+            ig.step(1)
+            t_sum += time.perf_counter() - t1
+            f_sum += ig.flops() - a
+            ig.destroy()
+        # time of a whole batched step if every instance ran alone, scaled by flop count
+        est = t_sum * float(np.sum(per_inst_flops_step)) / f_sum
+        plain = {"value": len(insts) / est, "unit": "instance-steps/s",
+                 "sample": f"instances {pick} stepped alone (1 regular step each), scaled by flop count",
+                 "batched_over_plain": value / (len(insts) / est)}
+    elif len(insts) == 1:
+        plain = {"value": value, "unit": "instance-steps/s", "sample": "single instance: batched == plain"}
+
+    cpu = None if args.no_cpu else cpu_baseline(insts, per_inst_flops_step, args.cpu_budget)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "instance-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": warm,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (frozen bath discretisations, random orthonormal unoccupied functions)",
+        "config": {"workload": f"{args.workload}: {desc}", "instances": len(insts), "krylov_dim": 12,
+                   "krylov_tol": 1e-12},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "ttnd::tdvp_run_kernel", "flops_timed": flops},
+        "plain_gpu": plain,
+        "cpu_baseline": cpu,
+        "check": {"sigma_z_last": sz[:4], "max_norm_deviation": norm_dev},
+    }
+    if cpu:
+        out["vs_cpu"] = value / cpu["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

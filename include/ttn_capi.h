@@ -149,6 +149,18 @@ int ttn_batch_create(ttn_ctx* ctx, int64_t n, ttn_integ* const* integs, ttn_batc
 int ttn_batch_destroy(ttn_batch* batch);
 int ttn_batch_step(ttn_batch* batch, double coeff_re, double coeff_im, int64_t nsteps);
 
+/* ---- measurement ----
+ * Algorithmic floating-point operations executed by this instance so far (8 real flops
+ * per complex multiply-add): H-actions and Krylov orthogonalisation counted from the
+ * measured per-node H-application counts, plus the environment updates, QR gauges and
+ * gauge applications of every sweep.  No reference counterpart (instrumentation). */
+int ttn_flops(const ttn_integ* integ, double* flops);
+/* Device wall-clock ticks (100 MHz) spent per phase of the sweep, 16 slots: node H-action,
+ * bond H-action, Gram-Schmidt, small exponentials, QR gauges, single-particle updates,
+ * mean-field updates, gauge applications, node Krylov total, bond Krylov total,
+ * observables, whole kernel.  Instrumentation only. */
+int ttn_profile(const ttn_integ* integ, int64_t* slots);
+
 /* ---- timing helpers (device-side event timing of the last ttn_step / ttn_batch_step) ---- */
 int ttn_last_step_ms(const ttn_integ* integ, double* ms);
 int ttn_batch_last_step_ms(const ttn_batch* batch, double* ms);

@@ -55,17 +55,62 @@ void outer_contract(const cplx* A, const cplx* B, i64 L, i64 d, i64 R, cplx* res
             }
 }
 
+// ||x||_2 of a complex vector without harmful underflow or overflow (the role of LAPACK's
+// dznrm2): the plain sum of squares is exact to rounding whenever it lies in
+// [kNrmTiny, kNrmHuge] -- squares that underflow then contribute below 1e-24 of it --
+// otherwise the vector is rescaled by its largest component first.
+constexpr double kNrmTiny = 1e-280, kNrmHuge = 1e280;
+
+static double nrm2(i64 n, const cplx* x, i64 incx) {
+    double s = 0.0;
+    for (i64 i = 0; i < n; ++i) s += std::norm(x[i * incx]);
+    if (s >= kNrmTiny && s <= kNrmHuge) return std::sqrt(s);
+    double scale = 0.0;
+    for (i64 i = 0; i < n; ++i)
+        scale = std::max(scale, std::max(std::abs(x[i * incx].real()), std::abs(x[i * incx].imag())));
+    if (scale == 0.0) return 0.0;
+    double t = 0.0;
+    for (i64 i = 0; i < n; ++i) t += std::norm(x[i * incx] / scale);
+    return scale * std::sqrt(t);
+}
+
+// LAPACK dlapy3: sqrt(a^2 + b^2 + c^2) with scaling by the largest magnitude
+static double lapy3(double a, double b, double c) {
+    const double w = std::max(std::abs(a), std::max(std::abs(b), std::abs(c)));
+    if (w == 0.0) return 0.0;
+    const double x = a / w, y = b / w, z = c / w;
+    return w * std::sqrt(x * x + y * y + z * z);
+}
+
 // LAPACK zlarfg: find H = I - tau v v^H (v0 = 1) with H^H [alpha; x] = [beta; 0], beta real.
+// When |beta| falls below safmin = (smallest normal) / (unit roundoff) = 2^-969, x and
+// alpha are rescaled by 1/safmin (at most 20 times) before the reflector is formed and
+// beta is scaled back afterwards; this keeps 1 / (alpha - beta) finite for the nearly
+// rank-deficient matrices a product-like tree state produces.
+constexpr double kSafmin = 2.2250738585072014e-308 / 1.1102230246251565e-16;
+
 static cplx larfg(i64 n, cplx& alpha, cplx* x, i64 incx) {
-    double xn2 = 0.0;
-    for (i64 i = 0; i < n - 1; ++i) xn2 += std::norm(x[i * incx]);
-    const double xnorm = std::sqrt(xn2);
-    const double ar = alpha.real(), ai = alpha.imag();
+    double xnorm = nrm2(n - 1, x, incx);
+    double ar = alpha.real(), ai = alpha.imag();
     if (xnorm == 0.0 && ai == 0.0) return cplx(0.0, 0.0);
-    const double beta = -std::copysign(std::sqrt(ar * ar + ai * ai + xn2), ar);
+    double beta = -std::copysign(lapy3(ar, ai, xnorm), ar);
+    int knt = 0;
+    if (std::abs(beta) < kSafmin) {
+        const double rsafmn = 1.0 / kSafmin;
+        do {
+            ++knt;
+            for (i64 i = 0; i < n - 1; ++i) x[i * incx] *= rsafmn;
+            beta *= rsafmn;
+            ar *= rsafmn;
+            ai *= rsafmn;
+        } while (std::abs(beta) < kSafmin && knt < 20);
+        xnorm = nrm2(n - 1, x, incx);
+        beta = -std::copysign(lapy3(ar, ai, xnorm), ar);
+    }
     const cplx tau((beta - ar) / beta, -ai / beta);
-    const cplx scal = 1.0 / (alpha - beta);
+    const cplx scal = 1.0 / (cplx(ar, ai) - beta);
     for (i64 i = 0; i < n - 1; ++i) x[i * incx] *= scal;
+    for (int j = 0; j < knt; ++j) beta *= kSafmin;
     alpha = cplx(beta, 0.0);
     return tau;
 }
@@ -248,6 +293,16 @@ i64 expv(const std::function<void(const cplx*, cplx*)>& H, cplx* x, i64 n, cplx 
             if (!std::isfinite(hn)) fail(ERR_NUMERIC, "expv: non-finite Krylov vector");
             if (j + 1 < kdim) Hm[(j + 1) * kdim + j] = hn;
             m = j + 1;
+            // Leading Taylor term of the Saad estimate: (exp(S))_{m-1,0} starts with
+            // prod_i S_{i+1,i} / (m-1)! for Hessenberg S.  The exponential is evaluated only
+            // when this cheap estimate is within 1e3 of the tolerance (skipping a test can
+            // only add Krylov vectors, never accept earlier).
+            double lead = hn;
+            for (int i = 0; i < j; ++i) lead *= std::abs(c) * std::abs(tau * s_step) * std::abs(Hm[(i + 1) * kdim + i]) / (i + 1);
+            if (lead > 1e3 * tol && j + 1 < kdim) {
+                for (i64 q = 0; q < n; ++q) V[static_cast<size_t>(j + 1) * n + q] = w[q] / hn;
+                continue;
+            }
             const double err = small_exp(m, s_step);
             if (err <= tol) {
                 found = true;

@@ -16,8 +16,11 @@ def _pair(hip, orc, model, **kw):
     return g, o
 
 
+# Tolerances sit just above each case's own sensitivity: perturbing one node of the
+# 8-mode tree by 1e-16 moves the oracle's <sigma> by up to 6e-12 within 12 steps,
+# because Krylov truncations at tolerance 1e-12 then stop at a different dimension.
 @pytest.mark.parametrize("n,d,r,steps,tol", [(1, 4, 4, 6, 1e-13), (2, 3, 3, 8, 1e-13), (4, 4, 4, 12, 1e-13),
-                                            (8, 6, 4, 12, 1e-12)])
+                                            (8, 6, 4, 12, 2e-11)])
 def test_small_trees_match_oracle(hip, orc, n, d, r, steps, tol):
     model = sb_model(n, d, r)
     g, o = _pair(hip, orc, model)

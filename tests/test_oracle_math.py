@@ -138,6 +138,37 @@ def test_qr_rank_deficient_completion_is_deterministic(orc):
     assert np.allclose(Q[:, 1], np.eye(6)[1])
 
 
+def test_qr_graded_columns_stay_finite(orc):
+    # columns scaled from 1 down to 1e-300: squares of the trailing columns underflow, so
+    # the reflectors need the scaled norm and zlarfg's 1/safmin rescaling.  Householder QR
+    # is columnwise backward stable, and LAPACK (numpy) gives the |R_jj| to compare with.
+    rng = np.random.default_rng(11)
+    m, n = 40, 16
+    G = rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))
+    A = G * np.logspace(0, -300, n)[None, :]
+    Q, R = orc_qr(orc, A)
+    assert np.isfinite(Q).all() and np.isfinite(R).all()
+    assert np.abs(Q.conj().T @ Q - np.eye(n)).max() < 1e-13
+    s = np.abs(A).max(axis=0)  # column scales (numpy's norm would underflow too)
+    colerr = np.linalg.norm((Q @ R - A) / s, axis=0) / np.linalg.norm(A / s, axis=0)
+    assert colerr.max() < 1e-13
+    ref = np.abs(np.diag(np.linalg.qr(A, mode="r")))
+    assert np.allclose(np.abs(np.diag(R)), ref, rtol=1e-10, atol=0)
+
+
+def test_qr_column_with_underflowing_squares(orc):
+    # every square in column 1 underflows to zero: the plain formula gives beta = -0 and
+    # tau = inf; the scaled norm and safmin rescaling give an exact reflector
+    A = np.zeros((3, 2), dtype=complex)
+    A[:, 0] = [1, 0, 0]
+    A[:, 1] = [0, 1e-170 * (1 + 1j), 2e-170]
+    Q, R = orc_qr(orc, A)
+    assert np.isfinite(Q).all() and np.isfinite(R).all()
+    assert np.abs(Q.conj().T @ Q - np.eye(2)).max() < 1e-15
+    assert abs(abs(R[1, 1]) - np.sqrt(6) * 1e-170) < 1e-15 * 1e-170 * 4
+    assert np.abs(Q @ R - A)[:, 1].max() < 1e-185
+
+
 def test_qr_wide_matrix_is_zero_padded(orc):
     rng = np.random.default_rng(2)
     A = rng.standard_normal((2, 5)) + 1j * rng.standard_normal((2, 5))

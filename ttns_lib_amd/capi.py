@@ -94,6 +94,8 @@ _SIGS = {
     "batch_destroy": [vp],
     "batch_step": [vp, C.c_double, C.c_double, i64],
     "last_step_ms": [vp, dblp],
+    "flops": [vp, dblp],
+    "profile": [vp, i64p],
     "batch_last_step_ms": [vp, dblp],
 }
 _RESTYPE = {"last_error": C.c_char_p}
@@ -339,6 +341,30 @@ class Integrator:
         out = C.c_double()
         self.lib._call("last_step_ms", self.h, C.byref(out))
         return out.value
+
+    def flops(self) -> float:
+        out = C.c_double()
+        self.lib._call("flops", self.h, C.byref(out))
+        return out.value
+
+    PROF_NAMES = ["heff", "hbond", "orth", "expm", "qr", "spf", "mf", "gauge", "expv_node", "expv_bond", "obs",
+                  "total"]
+
+    def profile(self) -> dict:
+        out = np.zeros(16, dtype=np.int64)
+        self.lib._call("profile", self.h, _ip(out))
+        return {k: out[i] / 100.0 for i, k in enumerate(self.PROF_NAMES)}  # microseconds
+
+    def set_observable(self, mode: int, op):
+        self._obs = _cplx_in(op)
+        self.lib._call("set_observable", self.h, mode, _dp(self._obs))
+
+    def trajectory(self) -> np.ndarray:
+        n = i64(0)
+        self.lib._call("trajectory", self.h, C.byref(n), None)
+        out = np.zeros(2 * n.value)
+        self.lib._call("trajectory", self.h, C.byref(n), _dp(out))
+        return out.reshape(-1, 2)
 
     def destroy(self):
         if self.h:

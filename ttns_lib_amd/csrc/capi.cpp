@@ -14,6 +14,7 @@
 
 #include "../../include/ttn_capi.h"
 #include "device/desc.hpp"
+#include "host/flops.hpp"
 #include "host/marshal.hpp"
 #include "host/tables.hpp"
 
@@ -80,6 +81,11 @@ struct ttn_integ {
     DevBuf<int> status;
     DevBuf<double> traj;
     DevBuf<double> scratch;
+    DevBuf<int64_t> node_stats;
+    DevBuf<int64_t> prof;
+    FlopModel fm;
+    int64_t sweep_pairs = 0;   // forward+backward sweeps run (with or without propagation)
+    int64_t prepares = 0;
     int64_t arena_size = 0;
     std::vector<int64_t> node_A;
     int64_t obs_op = -1;
@@ -322,9 +328,14 @@ void build_instance(ttn_integ* in) {
               "upload tables");
     in->counters.alloc(2);
     hip_check(hipMemsetAsync(in->counters.p, 0, 2 * sizeof(int64_t), in->ctx->stream), "memset counters");
-    in->status.alloc(1);
-    hip_check(hipMemsetAsync(in->status.p, 0, sizeof(int), in->ctx->stream), "memset status");
+    in->status.alloc(2);
+    hip_check(hipMemsetAsync(in->status.p, 0, 2 * sizeof(int), in->ctx->stream), "memset status");
     in->scratch.alloc(4);
+    in->node_stats.alloc(4 * n);
+    hip_check(hipMemsetAsync(in->node_stats.p, 0, 4 * n * sizeof(int64_t), in->ctx->stream), "memset stats");
+    in->fm = flop_model(t, ops, in->sop);
+    in->prof.alloc(ttnd::PROF_NSLOTS);
+    hip_check(hipMemsetAsync(in->prof.p, 0, ttnd::PROF_NSLOTS * sizeof(int64_t), in->ctx->stream), "memset prof");
 
     char* tb = in->tables.p;
     I.base = in->arena.p;
@@ -343,6 +354,8 @@ void build_instance(ttn_integ* in) {
     I.obs_leaf = -1;
     I.traj = nullptr;
     I.counters = in->counters.p;
+    I.node_stats = in->node_stats.p;
+    I.prof = in->prof.p;
     I.status = in->status.p;
     in->d_inst.alloc(1);
     hip_check(hipMemcpyAsync(in->d_inst.p, &I, sizeof(I), hipMemcpyHostToDevice, in->ctx->stream), "upload inst");
@@ -385,15 +398,36 @@ void prepare_traj(ttn_integ* in, int64_t nsteps) {
     in->inst.obs_leaf = in->obs_leaf;
 }
 
+void account(ttn_integ* in, const ttnd::RunArgs& a) {
+    if (a.prepare) {
+        ++in->prepares;
+        ++in->sweep_pairs;
+    }
+    in->sweep_pairs += a.nladder + a.nregular;
+}
+
+// Message for a failure record of the sweep kernel (reason | bond << 4 | node << 5).  The
+// record is sticky: an instance whose sweep failed keeps failing until it is recreated.
+std::string failure_message(const int st[2]) {
+    static const char* reasons[] = {"", "the vector to propagate has a non-finite norm",
+                                    "a Krylov basis vector has a non-finite norm",
+                                    "no Krylov sub-step met the tolerance"};
+    const int reason = st[0] & 15, bond = (st[0] >> 4) & 1, node = st[0] >> 5;
+    std::string m = "TDVP sweep failed at step " + std::to_string(st[1]) + ", " +
+                    (bond ? "bond propagation above node " : "node propagation of node ") + std::to_string(node) +
+                    ": " + (reason < 4 ? reasons[reason] : "unknown reason");
+    return m;
+}
+
 void finish(ttn_integ* in, int64_t nsteps) {
-    int st = 0;
-    hip_check(hipMemcpy(&st, in->status.p, sizeof(int), hipMemcpyDeviceToHost), "status");
+    int st[2] = {0, 0};
+    hip_check(hipMemcpy(st, in->status.p, sizeof(st), hipMemcpyDeviceToHost), "status");
     if (in->obs_leaf >= 0 && nsteps > 0) {
         std::vector<double> buf(2 * nsteps);
         hip_check(hipMemcpy(buf.data(), in->traj.p, buf.size() * sizeof(double), hipMemcpyDeviceToHost), "traj");
         in->trajectory.insert(in->trajectory.end(), buf.begin(), buf.end());
     }
-    check(st == 0, ERR_NUMERIC, "non-finite values or an unconverged Krylov step during the TDVP sweep");
+    if (st[0] != 0) fail(ERR_NUMERIC, failure_message(st));
     in->nsteps += nsteps;
 }
 
@@ -583,6 +617,7 @@ int ttn_step(ttn_integ* in, double cr, double ci, int64_t nsteps) {
         float ms = 0.f;
         hip_check(hipEventElapsedTime(&ms, in->ctx->ev0, in->ctx->ev1), "elapsed");
         in->last_ms = ms;
+        account(in, a);
         finish(in, nsteps);
     });
 }
@@ -697,7 +732,41 @@ int ttn_batch_step(ttn_batch* b, double cr, double ci, int64_t nsteps) {
         float ms = 0.f;
         hip_check(hipEventElapsedTime(&ms, b->ctx->ev0, b->ctx->ev1), "elapsed");
         b->last_ms = ms;
-        for (ttn_integ* in : b->integs) finish(in, nsteps);
+        // every instance is accounted before the first failure is reported
+        std::string err;
+        for (size_t k = 0; k < b->integs.size(); ++k) {
+            ttn_integ* in = b->integs[k];
+            account(in, a);
+            try {
+                finish(in, nsteps);
+            } catch (const Error& e) {
+                if (err.empty()) err = "batch instance " + std::to_string(k) + ": " + e.what();
+            }
+        }
+        if (!err.empty()) fail(ERR_NUMERIC, err);
+    });
+}
+
+int ttn_flops(const ttn_integ* in, double* flops) {
+    return guard(in->ctx, [&] {
+        const int n = in->topo.size();
+        std::vector<int64_t> st(4 * n);
+        hip_check(hipMemcpy(st.data(), in->node_stats.p, st.size() * sizeof(int64_t), hipMemcpyDeviceToHost),
+                  "stats");
+        const FlopModel& f = in->fm;
+        double tot = static_cast<double>(in->sweep_pairs) * f.sweep_pair + static_cast<double>(in->prepares) * f.prepare_spf;
+        for (int i = 0; i < n; ++i) {
+            tot += static_cast<double>(st[4 * i]) * f.heff_node[i] + 32.0 * f.node_size[i] * static_cast<double>(st[4 * i + 1]);
+            tot += static_cast<double>(st[4 * i + 2]) * f.heff_bond[i] +
+                   32.0 * f.bond_size[i] * static_cast<double>(st[4 * i + 3]);
+        }
+        *flops = tot;
+    });
+}
+
+int ttn_profile(const ttn_integ* in, int64_t* slots) {
+    return guard(in->ctx, [&] {
+        hip_check(hipMemcpy(slots, in->prof.p, ttnd::PROF_NSLOTS * sizeof(int64_t), hipMemcpyDeviceToHost), "prof");
     });
 }
 

@@ -16,6 +16,10 @@ constexpr int MAXC = 4;        // max children of a branch node on the device pa
 constexpr int MAXK = 16;       // max Krylov dimension on the device path
 constexpr int NWORK = 6;       // generic work buffers, each max node size
 
+// device wall-clock phase slots (100 MHz ticks, accumulated by thread 0)
+enum ProfSlot { PROF_HEFF = 0, PROF_HBOND, PROF_ORTH, PROF_EXPM, PROF_QR, PROF_SPF, PROF_MF, PROF_GAUGE,
+                PROF_EXPV_NODE, PROF_EXPV_BOND, PROF_OBS, PROF_TOTAL, PROF_NSLOTS = 16 };
+
 struct DNode {
     int parent, child_id, nchild, leaf;  // leaf = primitive mode index, -1 for branch nodes
     int nmodes, hrank, dimen, size;      // size = dimen * hrank
@@ -57,6 +61,15 @@ struct DMf {
     int k0, nk;            // sibling spf chain (DK, branch form, applied to the parent tensor)
 };
 
+// Failure record written by the sweep kernel: reason | bond << 4 | node << 5, where bond
+// is 1 for the bond (R) propagation behind `node` and 0 for the node propagation.
+enum FailReason {
+    FAIL_NONE = 0,
+    FAIL_INPUT_NONFINITE = 1,   // norm of the vector to propagate is not finite
+    FAIL_KRYLOV_NONFINITE = 2,  // a Krylov basis norm is not finite
+    FAIL_KRYLOV_STEP = 3,       // no sub-step size met the Krylov tolerance
+};
+
 struct DInst {
     double2* base;                  // arena
     const DNode* nodes;
@@ -77,7 +90,10 @@ struct DInst {
     int pad1;
     double* traj;                  // per-step <O>, <1> (2 doubles per step) or null
     int64_t* counters;             // [0] H applications (node updates), [1] steps done
-    int* status;                   // 0 ok, 1 numeric
+    int64_t* node_stats;           // per node: node-update H applications, their Krylov sizes
+                                   // summed, bond-update applications (by operator node), sizes
+    int* status;                   // [0]: failure record (0 = ok, see FailReason), [1]: step index
+    int64_t* prof;                 // per-phase device wall-clock ticks (PROF_* slots), or null
 };
 
 struct RunArgs {

@@ -23,8 +23,21 @@ struct Ctx {
     // failures are detected on workgroup-uniform values, so every thread takes the same
     // branch; the LDS flag is read only after a barrier
     __device__ bool failed() const { return sh.fail != 0; }
-    __device__ void fail() const { sh.fail = 1; }
+    // where: node whose propagation is running, bond: 1 for its bond matrix
+    int where = 0, bond = 0;
+    __device__ void fail(int reason) const { sh.fail = reason | (bond << 4) | (where << 5); }
 };
+
+__device__ __forceinline__ uint64_t prof_now(const Ctx& c) { return c.I.prof ? wall_clock64() : 0; }
+__device__ __forceinline__ void prof_add(const Ctx& c, int slot, uint64_t t0) {
+    if (c.I.prof && threadIdx.x == 0) c.I.prof[slot] += static_cast<int64_t>(wall_clock64() - t0);
+}
+#define PROFILE(slot, stmt)                  \
+    {                                        \
+        const uint64_t _pt0 = prof_now(c);   \
+        stmt;                                \
+        prof_add(c, slot, _pt0);             \
+    }
 
 // (L, R) of the rank-3 view of mode `mode` of a node
 __device__ __forceinline__ void mode_view(const DNode& nd, int mode, int64_t& L, int64_t& R) {
@@ -272,7 +285,7 @@ __device__ void apply_to_parent(Ctx& c, int n) {
 // Pade(13) small exponential (same algorithm as the oracle's expv).  Returns the number
 // of H applications.  H(v, w) may use W1..W4; the Krylov vector w is W0.
 template <class HF>
-__device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, double tau) {
+__device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, double tau, int64_t& sumj) {
     if (tau == 0.0 || n == 0) return 0;
     const int kdim = c.I.kdim;
     const double tol = c.I.tol;
@@ -287,7 +300,7 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
         s_step = fmin(s_step, 1.0 - s_done);
         const double beta = sqrt(block_norm2(sh, x, n));
         if (!isfinite(beta)) {
-            c.fail();
+            c.fail(FAIL_INPUT_NONFINITE);
             return napply;
         }
         if (beta == 0.0) return napply;
@@ -304,12 +317,14 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
                 sh.S[e] = cmul(f, sh.Hm[i * MAXK + j]);
             }
             __syncthreads();
-            sm_expm(sh, sh.S, sh.F, mm);
+            if ((threadIdx.x >> 6) == 0) wave_expm(sh, sh.S, sh.F, mm);
+            __syncthreads();
             return hn * cabsv(sh.F[(mm - 1) * mm]);
         };
         for (int j = 0; j < kdim; ++j) {
             H(V + j * ld, w);
             ++napply;
+            sumj += j + 1;
             // classical Gram-Schmidt applied twice (CGS2): the second pass's inner
             // products accumulate while the first correction is written, the norm while
             // the second one is written (three sweeps over the basis instead of five)
@@ -318,12 +333,16 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
             const double* h2 = sh.hb[1];
 #pragma unroll
             for (int i = 0; i < 2 * MAXK; ++i) p[i] = 0.0;
-            for (int64_t q = threadIdx.x; q < n; q += NT) {
-                const double2 wq = w[q];
+            // two strides per iteration keep 2(j+2) loads in flight per thread
+            for (int64_t q0 = threadIdx.x; q0 < n; q0 += 2 * NT) {
+                const int64_t q1 = q0 + NT;
+                const bool ok1 = q1 < n;
+                const double2 w0 = w[q0], w1 = ok1 ? w[q1] : cz();
 #pragma unroll
                 for (int i = 0; i < MAXK; ++i) {
                     if (i <= j) {
-                        const double2 pr = cmulc(V[i * ld + q], wq);
+                        const double2 a0 = V[i * ld + q0], a1 = ok1 ? V[i * ld + q1] : cz();
+                        const double2 pr = cadd(cmulc(a0, w0), cmulc(a1, w1));
                         p[2 * i] += pr.x;
                         p[2 * i + 1] += pr.y;
                     }
@@ -336,12 +355,11 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
                 double2 vi[MAXK];
                 double2 wq = w[q];
 #pragma unroll
-                for (int i = 0; i < MAXK; ++i) {
-                    if (i <= j) {
-                        vi[i] = V[i * ld + q];
-                        wq = csub(wq, cmul(cmk(h1[2 * i], h1[2 * i + 1]), vi[i]));
-                    }
-                }
+                for (int i = 0; i < MAXK; ++i)
+                    if (i <= j) vi[i] = V[i * ld + q];
+#pragma unroll
+                for (int i = 0; i < MAXK; ++i)
+                    if (i <= j) wq = csub(wq, cmul(cmk(h1[2 * i], h1[2 * i + 1]), vi[i]));
                 w[q] = wq;
 #pragma unroll
                 for (int i = 0; i < MAXK; ++i) {
@@ -354,13 +372,22 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
             }
             block_sum_lds<2 * MAXK>(sh, p, sh.hb[1]);
             double nn[1] = {0.0}, nno[1];
-            for (int64_t q = threadIdx.x; q < n; q += NT) {
-                double2 wq = w[q];
+            for (int64_t q0 = threadIdx.x; q0 < n; q0 += 2 * NT) {
+                const int64_t q1 = q0 + NT;
+                const bool ok1 = q1 < n;
+                double2 w0 = w[q0], w1 = ok1 ? w[q1] : cz();
 #pragma unroll
-                for (int i = 0; i < MAXK; ++i)
-                    if (i <= j) wq = csub(wq, cmul(cmk(h2[2 * i], h2[2 * i + 1]), V[i * ld + q]));
-                w[q] = wq;
-                nn[0] += cabs2(wq);
+                for (int i = 0; i < MAXK; ++i) {
+                    if (i <= j) {
+                        const double2 h = cmk(h2[2 * i], h2[2 * i + 1]);
+                        const double2 a0 = V[i * ld + q0], a1 = ok1 ? V[i * ld + q1] : cz();
+                        w0 = csub(w0, cmul(h, a0));
+                        w1 = csub(w1, cmul(h, a1));
+                    }
+                }
+                w[q0] = w0;
+                if (ok1) w[q1] = w1;
+                nn[0] += cabs2(w0) + cabs2(w1);
             }
             block_sum<1>(sh, nn, nno);
             if (threadIdx.x <= j) {
@@ -371,13 +398,23 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
             __syncthreads();
             hn = sqrt(nno[0]);
             if (!isfinite(hn)) {
-                c.fail();
+                c.fail(FAIL_KRYLOV_NONFINITE);
                 return napply;
             }
             if (j + 1 < kdim && threadIdx.x == 0) sh.Hm[(j + 1) * MAXK + j] = cmk(hn, 0.0);
             __syncthreads();
             m = j + 1;
-            const double err = small_exp(m, s_step);
+            // leading Taylor term of the Saad estimate (prod of subdiagonal entries / j!);
+            // the exponential is evaluated only when it is within 1e3 of the tolerance
+            double lead = hn;
+            for (int i = 0; i < j; ++i)
+                lead *= cabsv(coeff) * fabs(tau * s_step) * cabsv(sh.Hm[(i + 1) * MAXK + i]) / (i + 1);
+            if (lead > 1e3 * tol && j + 1 < kdim) {
+                wg_scale_to(V + (j + 1) * ld, cmk(1.0 / hn, 0.0), w, n);
+                continue;
+            }
+            double err;
+            PROFILE(PROF_EXPM, err = small_exp(m, s_step));
             if (err <= tol) {
                 found = true;
                 break;
@@ -389,13 +426,20 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
             if (small_exp(m, s_step) <= tol) found = true;
         }
         if (!found) {
-            c.fail();
+            c.fail(FAIL_KRYLOV_STEP);
             return napply;
         }
-        for (int64_t q = threadIdx.x; q < n; q += NT) {
-            double2 s = cz();
-            for (int i = 0; i < m; ++i) s = cadd(s, cmul(sh.F[i * m], V[i * ld + q]));
-            x[q] = cscale(s, beta);
+        for (int64_t q0 = threadIdx.x; q0 < n; q0 += 2 * NT) {
+            const int64_t q1 = q0 + NT;
+            const bool ok1 = q1 < n;
+            double2 s0 = cz(), s1 = cz();
+            for (int i = 0; i < m; ++i) {
+                const double2 f = sh.F[i * m];
+                s0 = cadd(s0, cmul(f, V[i * ld + q0]));
+                if (ok1) s1 = cadd(s1, cmul(f, V[i * ld + q1]));
+            }
+            x[q0] = cscale(s0, beta);
+            if (ok1) x[q1] = cscale(s1, beta);
         }
         __syncthreads();
         s_done += s_step;
@@ -412,16 +456,35 @@ struct SweepState {
 
 __device__ void node_update(Ctx& c, SweepState& st, int n) {
     const DNode& nd = c.node(n);
-    st.nh += expv(
-        c, [&](const double2* v, double2* res) { heff_node(c, n, v, res); }, c.base + nd.A, nd.size, st.coeff,
-        st.sub_dt / 2.0);
+    int64_t sumj = 0;
+    c.where = n;
+    c.bond = 0;
+    const uint64_t pt0 = prof_now(c);
+    const int64_t na = expv(
+        c, [&](const double2* v, double2* res) { PROFILE(PROF_HEFF, heff_node(c, n, v, res)); }, c.base + nd.A, nd.size, st.coeff,
+        st.sub_dt / 2.0, sumj);
+    prof_add(c, PROF_EXPV_NODE, pt0);
+    st.nh += na;
+    if (threadIdx.x == 0) {
+        c.I.node_stats[4 * n] += na;
+        c.I.node_stats[4 * n + 1] += sumj;
+    }
 }
 
 __device__ void bond_update(Ctx& c, SweepState& st, int n, int ops) {
     const int r = c.node(ops).hrank;
-    expv(
-        c, [&](const double2* v, double2* res) { heff_bond(c, ops, v, res); }, c.base + c.node(n).R,
-        static_cast<int64_t>(r) * r, st.coeff, -st.sub_dt / 2.0);
+    int64_t sumj = 0;
+    c.where = ops;
+    c.bond = 1;
+    const uint64_t pt0 = prof_now(c);
+    const int64_t na = expv(
+        c, [&](const double2* v, double2* res) { PROFILE(PROF_HBOND, heff_bond(c, ops, v, res)); }, c.base + c.node(n).R,
+        static_cast<int64_t>(r) * r, st.coeff, -st.sub_dt / 2.0, sumj);
+    prof_add(c, PROF_EXPV_BOND, pt0);
+    if (threadIdx.x == 0) {
+        c.I.node_stats[4 * ops + 2] += na;
+        c.I.node_stats[4 * ops + 3] += sumj;
+    }
 }
 
 __device__ void forward(Ctx& c, SweepState& st, bool evolve, int* visits) {
@@ -436,18 +499,18 @@ __device__ void forward(Ctx& c, SweepState& st, bool evolve, int* visits) {
         if (threadIdx.x == 0) visits[id] = tv + 1;
         __syncthreads();
         if (tv != nd.nmodes) {
-            if (tv == 0 && nd.parent >= 0) apply_from_parent(c, id);
+            if (tv == 0 && nd.parent >= 0) PROFILE(PROF_GAUGE, apply_from_parent(c, id));
             if (nd.leaf < 0) {
-                r2l(c, id, tv);
-                update_mf(c, nd.children[tv]);
+                PROFILE(PROF_QR, r2l(c, id, tv));
+                PROFILE(PROF_MF, update_mf(c, nd.children[tv]));
             }
         } else {
             if (evolve) node_update(c, st, id);
             if (nd.parent >= 0) {
-                l2r(c, id);
-                update_spf(c, id);
+                PROFILE(PROF_QR, l2r(c, id));
+                PROFILE(PROF_SPF, update_spf(c, id));
                 if (evolve) bond_update(c, st, id, id);
-                apply_to_parent(c, id);
+                PROFILE(PROF_GAUGE, apply_to_parent(c, id));
             }
         }
     }
@@ -469,18 +532,18 @@ __device__ void backward(Ctx& c, SweepState& st, bool evolve, int* visits) {
             if (tv == 0) {
                 if (nd.parent >= 0) {
                     if (evolve) bond_update(c, st, nd.parent, id);
-                    apply_from_parent(c, id);
+                    PROFILE(PROF_GAUGE, apply_from_parent(c, id));
                 }
                 if (evolve) node_update(c, st, id);
             }
             if (nd.leaf < 0) {
-                r2l(c, id, mode);
-                update_mf(c, nd.children[mode]);
+                PROFILE(PROF_QR, r2l(c, id, mode));
+                PROFILE(PROF_MF, update_mf(c, nd.children[mode]));
             }
         } else if (nd.parent >= 0) {
-            l2r(c, id);
-            apply_to_parent(c, id);
-            update_spf(c, id);
+            PROFILE(PROF_QR, l2r(c, id));
+            PROFILE(PROF_GAUGE, apply_to_parent(c, id));
+            PROFILE(PROF_SPF, update_spf(c, id));
         }
     }
 }
@@ -538,6 +601,7 @@ __global__ __launch_bounds__(NT) void tdvp_run_kernel(const DInst* insts, RunArg
     if (threadIdx.x == 0) sh.fail = 0;
     __syncthreads();
     Ctx c(I, sh);
+    const uint64_t prof_t0 = prof_now(c);
     SweepState st;
     st.coeff = a.coeff;
     st.nh = 0;
@@ -565,7 +629,11 @@ __global__ __launch_bounds__(NT) void tdvp_run_kernel(const DInst* insts, RunArg
     __syncthreads();
     if (threadIdx.x == 0) {
         I.counters[0] += st.nh;
-        if (sh.fail) *I.status = 1;
+        if (sh.fail) {
+            I.status[0] = sh.fail;
+            I.status[1] = static_cast<int>(step_index);
+        }
+        if (I.prof) I.prof[PROF_TOTAL] += static_cast<int64_t>(wall_clock64() - prof_t0);
     }
 }
 

@@ -101,30 +101,79 @@ __device__ void block_sum_lds(Shared& sh, double (&v)[K], double* dst) {
     __syncthreads();
 }
 
+// maximum over the workgroup; every thread receives it
+__device__ double block_max(Shared& sh, double v) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    if (lane == 0) sh.red[wv] = v;
+    __syncthreads();
+    double m = sh.red[0];
+#pragma unroll
+    for (int w = 1; w < NWAVE; ++w) m = fmax(m, sh.red[w]);
+    __syncthreads();
+    return m;
+}
+
 __device__ double block_norm2(Shared& sh, const double2* x, int64_t n) {
     double p[1] = {0.0}, o[1];
-    for (int64_t i = threadIdx.x; i < n; i += NT) p[0] += cabs2(x[i]);
+    for (int64_t b = threadIdx.x; b < n; b += 4 * NT) {
+        double2 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = (b + u * NT < n) ? x[b + u * NT] : cz();
+#pragma unroll
+        for (int u = 0; u < 4; ++u) p[0] += cabs2(v[u]);
+    }
     block_sum<1>(sh, p, o);
     return o[0];
 }
 
 // ---------------------------------------------------------------- elementwise
+// One workgroup has a single wave per SIMD, so latency is hidden by issuing the loads
+// of UNR consecutive strides before using any of them.
+constexpr int UNR = 4;
+
 __device__ void wg_zero(double2* y, int64_t n) {
     for (int64_t i = threadIdx.x; i < n; i += NT) y[i] = cz();
     __syncthreads();
 }
-__device__ void wg_copy(double2* y, const double2* x, int64_t n) {
-    for (int64_t i = threadIdx.x; i < n; i += NT) y[i] = x[i];
+__device__ void wg_copy(double2* __restrict__ y, const double2* __restrict__ x, int64_t n) {
+    for (int64_t b = threadIdx.x; b < n; b += UNR * NT) {
+        double2 v[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) v[u] = (b + u * NT < n) ? x[b + u * NT] : cz();
+#pragma unroll
+        for (int u = 0; u < UNR; ++u)
+            if (b + u * NT < n) y[b + u * NT] = v[u];
+    }
     __syncthreads();
 }
 // y += a * x
-__device__ void wg_axpy(double2* y, double2 a, const double2* x, int64_t n) {
-    for (int64_t i = threadIdx.x; i < n; i += NT) y[i] = cadd(y[i], cmul(a, x[i]));
+__device__ void wg_axpy(double2* __restrict__ y, double2 a, const double2* __restrict__ x, int64_t n) {
+    for (int64_t b = threadIdx.x; b < n; b += UNR * NT) {
+        double2 v[UNR], o[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const bool ok = b + u * NT < n;
+            v[u] = ok ? x[b + u * NT] : cz();
+            o[u] = ok ? y[b + u * NT] : cz();
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u)
+            if (b + u * NT < n) y[b + u * NT] = cadd(o[u], cmul(a, v[u]));
+    }
     __syncthreads();
 }
 // y = a * x
-__device__ void wg_scale_to(double2* y, double2 a, const double2* x, int64_t n) {
-    for (int64_t i = threadIdx.x; i < n; i += NT) y[i] = cmul(a, x[i]);
+__device__ void wg_scale_to(double2* __restrict__ y, double2 a, const double2* __restrict__ x, int64_t n) {
+    for (int64_t b = threadIdx.x; b < n; b += UNR * NT) {
+        double2 v[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) v[u] = (b + u * NT < n) ? x[b + u * NT] : cz();
+#pragma unroll
+        for (int u = 0; u < UNR; ++u)
+            if (b + u * NT < n) y[b + u * NT] = cmul(a, v[u]);
+    }
     __syncthreads();
 }
 
@@ -299,6 +348,43 @@ __device__ void wg_prim_apply(const DK& k, const double2* base, const int64_t* i
     __syncthreads();
 }
 
+// ---------------------------------------------------------------- safe norms (LAPACK dznrm2/dlapy3 roles)
+// The plain sum of squares is exact to rounding whenever it lies in [kNrmTiny, kNrmHuge];
+// otherwise the column is rescaled by its largest component first.
+constexpr double kNrmTiny = 1e-280, kNrmHuge = 1e280;
+// safmin = (smallest normal) / (unit roundoff) = 2^-969, as in zlarfg
+constexpr double kSafmin = 2.2250738585072014e-308 / 1.1102230246251565e-16;
+
+// ||A(i+1:m, i)||_2 of a row-major m x n matrix
+__device__ double wg_col_nrm2(Shared& sh, const double2* A, int i, int m, int n) {
+    double p[1] = {0.0}, o[1];
+    for (int rr = i + 1 + threadIdx.x; rr < m; rr += NT) p[0] += cabs2(A[static_cast<int64_t>(rr) * n + i]);
+    block_sum<1>(sh, p, o);
+    if (o[0] >= kNrmTiny && o[0] <= kNrmHuge) return sqrt(o[0]);
+    double mx = 0.0;
+    for (int rr = i + 1 + threadIdx.x; rr < m; rr += NT) {
+        const double2 v = A[static_cast<int64_t>(rr) * n + i];
+        mx = fmax(mx, fmax(fabs(v.x), fabs(v.y)));
+    }
+    const double scale = block_max(sh, mx);
+    if (scale == 0.0) return 0.0;
+    p[0] = 0.0;
+    for (int rr = i + 1 + threadIdx.x; rr < m; rr += NT) {
+        const double2 v = A[static_cast<int64_t>(rr) * n + i];
+        p[0] += cabs2(make_double2(v.x / scale, v.y / scale));
+    }
+    block_sum<1>(sh, p, o);
+    return scale * sqrt(o[0]);
+}
+
+// sqrt(a^2 + b^2 + c^2) scaled by the largest magnitude
+__device__ __forceinline__ double lapy3(double a, double b, double c) {
+    const double w = fmax(fabs(a), fmax(fabs(b), fabs(c)));
+    if (w == 0.0) return 0.0;
+    const double x = a / w, y = b / w, z = c / w;
+    return w * sqrt(x * x + y * y + z * z);
+}
+
 // ---------------------------------------------------------------- Householder QR
 // A (m x n, row-major, destroyed: holds the reflectors) = Q (m x n) R (n x n).
 // LAPACK zgeqr2/zlarfg/zung2r conventions; for m < n, Q gets zero columns and R zero
@@ -307,18 +393,34 @@ __device__ void wg_householder_qr(Shared& sh, double2* A, int m, int n, double2*
     const int k = m < n ? m : n;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int i = 0; i < k; ++i) {
-        // |x|^2 of the column below the diagonal
-        double p[1] = {0.0}, o[1];
-        for (int rr = i + 1 + threadIdx.x; rr < m; rr += NT) p[0] += cabs2(A[static_cast<int64_t>(rr) * n + i]);
-        block_sum<1>(sh, p, o);
-        const double xn2 = o[0];
+        // reflector of column i (zlarfg): beta from a scaled norm; tiny columns are
+        // rescaled by 1/safmin until |beta| >= safmin
+        double xnorm = wg_col_nrm2(sh, A, i, m, n);
         const double2 alpha = A[static_cast<int64_t>(i) * n + i];
+        double ar = alpha.x, ai = alpha.y;
         double2 tau = cz(), scal = cz();
-        double beta = alpha.x;
-        if (!(xn2 == 0.0 && alpha.y == 0.0)) {
-            beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2), alpha.x);
-            tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
-            scal = cdiv(make_double2(1.0, 0.0), make_double2(alpha.x - beta, alpha.y));
+        double beta = ar;
+        const bool refl = !(xnorm == 0.0 && ai == 0.0);
+        int knt = 0;
+        if (refl) {
+            beta = -copysign(lapy3(ar, ai, xnorm), ar);
+            if (fabs(beta) < kSafmin) {
+                do {
+                    ++knt;
+                    for (int rr = i + 1 + threadIdx.x; rr < m; rr += NT) {
+                        double2& v = A[static_cast<int64_t>(rr) * n + i];
+                        v = cscale(v, 1.0 / kSafmin);
+                    }
+                    beta *= 1.0 / kSafmin;
+                    ar *= 1.0 / kSafmin;
+                    ai *= 1.0 / kSafmin;
+                } while (fabs(beta) < kSafmin && knt < 20);
+                __syncthreads();
+                xnorm = wg_col_nrm2(sh, A, i, m, n);
+                beta = -copysign(lapy3(ar, ai, xnorm), ar);
+            }
+            tau = make_double2((beta - ar) / beta, -ai / beta);
+            scal = cdiv(make_double2(1.0, 0.0), make_double2(ar - beta, ai));
         }
         if (threadIdx.x == 0) sh.qtau[i] = tau;
         if (tau.x != 0.0 || tau.y != 0.0) {
@@ -349,8 +451,10 @@ __device__ void wg_householder_qr(Shared& sh, double2* A, int m, int n, double2*
             }
         }
         __syncthreads();
-        if (threadIdx.x == 0) A[static_cast<int64_t>(i) * n + i] = make_double2(beta, 0.0 * alpha.y);
-        if (threadIdx.x == 0 && tau.x == 0.0 && tau.y == 0.0) A[static_cast<int64_t>(i) * n + i] = alpha;
+        if (refl) {
+            for (int j = 0; j < knt; ++j) beta *= kSafmin;
+            if (threadIdx.x == 0) A[static_cast<int64_t>(i) * n + i] = make_double2(beta, 0.0);
+        }
         __syncthreads();
     }
     // R
@@ -508,6 +612,148 @@ __device__ void sm_expm(Shared& sh, const double2* S, double2* Eout, int n) {
     }
     if (t < nn) Eout[t] = cur[t];
     __syncthreads();
+}
+
+
+// ---------------------------------------------------------------- small dense expm on one wave
+// The same Pade(13) scaling-and-squaring algorithm as sm_expm, executed by a single
+// wavefront: the 64 lanes run in lockstep, so LDS hand-offs need only a wavefront-scope
+// fence instead of a workgroup barrier.  Called by wave 0 only.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ void wv_matmul(const double2* A, const double2* B, double2* C, int n, int lane) {
+    double2 acc[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int e = lane + 64 * u;
+        double2 s = cz();
+        if (e < n * n) {
+            const int i = e / n, j = e % n;
+            for (int q = 0; q < n; ++q) s = cadd(s, cmul(A[i * n + q], B[q * n + j]));
+        }
+        acc[u] = s;
+    }
+    wave_sync();
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int e = lane + 64 * u;
+        if (e < n * n) C[e] = acc[u];
+    }
+    wave_sync();
+}
+
+// max over the wave with the lowest lane winning ties
+__device__ __forceinline__ void wave_argmax(double& v, int& idx) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(v, o, 64);
+        const int oi = __shfl_xor(idx, o, 64);
+        if (ov > v || (ov == v && oi < idx)) {
+            v = ov;
+            idx = oi;
+        }
+    }
+}
+
+__device__ void wave_expm(Shared& sh, const double2* S, double2* Eout, int n) {
+    const double b[14] = {64764752532480000.0, 32382376266240000.0, 7771770303897600.0, 1187353796428800.0,
+                          129060195264000.0,   10559470521600.0,   670442572800.0,     33522128640.0,
+                          1323241920.0,        40840800.0,         960960.0,           16380.0,
+                          182.0,               1.0};
+    const double theta13 = 5.371920351148152;
+    const int lane = threadIdx.x & 63, nn = n * n;
+    double2 *A = sh.E[0], *A2 = sh.E[1], *A4 = sh.E[2], *A6 = sh.E[3], *T1 = sh.E[4], *T2 = sh.E[5], *U = sh.E[6],
+            *V = sh.E[7];
+    // 1-norm: lane j sums column j, then a wave max
+    double colsum = 0.0;
+    if (lane < n)
+        for (int i = 0; i < n; ++i) colsum += cabsv(S[i * n + lane]);
+    int dummy = lane;
+    wave_argmax(colsum, dummy);
+    const double norm1 = colsum;
+    int sc = 0;
+    if (norm1 > theta13) sc = static_cast<int>(ceil(log2(norm1 / theta13)));
+    const double scale = ldexp(1.0, -sc);
+    for (int e = lane; e < nn; e += 64) A[e] = cscale(S[e], scale);
+    wave_sync();
+    wv_matmul(A, A, A2, n, lane);
+    wv_matmul(A2, A2, A4, n, lane);
+    wv_matmul(A4, A2, A6, n, lane);
+    for (int e = lane; e < nn; e += 64)
+        T1[e] = cadd(cadd(cscale(A6[e], b[13]), cscale(A4[e], b[11])), cscale(A2[e], b[9]));
+    wave_sync();
+    wv_matmul(A6, T1, T2, n, lane);
+    for (int e = lane; e < nn; e += 64) {
+        double2 v = cadd(T2[e], cadd(cadd(cscale(A6[e], b[7]), cscale(A4[e], b[5])), cscale(A2[e], b[3])));
+        if (e / n == e % n) v.x += b[1];
+        T2[e] = v;
+    }
+    wave_sync();
+    wv_matmul(A, T2, U, n, lane);
+    for (int e = lane; e < nn; e += 64)
+        T1[e] = cadd(cadd(cscale(A6[e], b[12]), cscale(A4[e], b[10])), cscale(A2[e], b[8]));
+    wave_sync();
+    wv_matmul(A6, T1, V, n, lane);
+    for (int e = lane; e < nn; e += 64) {
+        double2 v = cadd(V[e], cadd(cadd(cscale(A6[e], b[6]), cscale(A4[e], b[4])), cscale(A2[e], b[2])));
+        if (e / n == e % n) v.x += b[0];
+        V[e] = v;
+    }
+    wave_sync();
+    double2 *P = A, *X = T1;
+    for (int e = lane; e < nn; e += 64) {
+        const double2 u = U[e], v = V[e];
+        P[e] = csub(v, u);
+        X[e] = cadd(v, u);
+    }
+    wave_sync();
+    for (int kk = 0; kk < n; ++kk) {
+        // partial pivoting: lanes kk..n-1 take |P(i, kk)|, first maximum wins
+        double best = (lane >= kk && lane < n) ? cabsv(P[lane * n + kk]) : -1.0;
+        int piv = lane;
+        wave_argmax(best, piv);
+        if (piv != kk && lane < n) {
+            double2 a = P[kk * n + lane];
+            P[kk * n + lane] = P[piv * n + lane];
+            P[piv * n + lane] = a;
+            a = X[kk * n + lane];
+            X[kk * n + lane] = X[piv * n + lane];
+            X[piv * n + lane] = a;
+        }
+        wave_sync();
+        if (lane > kk && lane < n) sh.lu_f[lane] = cdiv(P[lane * n + kk], P[kk * n + kk]);
+        wave_sync();
+        for (int e = lane; e < nn; e += 64) {
+            const int i = e / n, j = e % n;
+            if (i > kk) {
+                const double2 f = sh.lu_f[i];
+                if (j >= kk) P[i * n + j] = csub(P[i * n + j], cmul(f, P[kk * n + j]));
+                X[i * n + j] = csub(X[i * n + j], cmul(f, X[kk * n + j]));
+            }
+        }
+        wave_sync();
+    }
+    for (int kk = n - 1; kk >= 0; --kk) {
+        if (lane < n) {
+            double2 s = X[kk * n + lane];
+            for (int i = kk + 1; i < n; ++i) s = csub(s, cmul(P[kk * n + i], X[i * n + lane]));
+            X[kk * n + lane] = cdiv(s, P[kk * n + kk]);
+        }
+        wave_sync();
+    }
+    double2* cur = X;
+    double2* nxt = T2;
+    for (int s = 0; s < sc; ++s) {
+        wv_matmul(cur, cur, nxt, n, lane);
+        double2* tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+    }
+    for (int e = lane; e < nn; e += 64) Eout[e] = cur[e];
+    wave_sync();
 }
 
 }  // namespace ttnd

@@ -7,8 +7,8 @@
 //   A (16 x 4): lane l holds A[l % 16][l / 16];  B (4 x 16): lane l holds B[l / 16][l % 16];
 //   D (16 x 16): lane l holds D[l / 16 + 4 * j][l % 16] in accumulator element j
 //   (measured on gfx950 with a lane-id probe).
-// Operands come straight from L2/L1 (node tensors are too large for LDS); loads for four
-// k-steps are issued before their MFMAs.
+// Operands come straight from L2/L1 (node tensors are too large for LDS); the loads of up
+// to sixteen k-steps are issued before their MFMAs.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -41,18 +41,22 @@ __device__ __forceinline__ void ztile(ZAcc& acc, const double2* __restrict__ a, 
     const int lane = threadIdx.x & 63, i = lane & 15, kq = lane >> 4;
     const bool arow = i < mv, bcol = i < nv;
     const double2 z = make_double2(0.0, 0.0);
-    for (int64_t k0 = 0; k0 < K; k0 += 16) {
-        double2 av[4], bv[4];
+    // up to 8 k-steps (k <= 32) of operands are loaded before the first MFMA
+    for (int64_t k0 = 0; k0 < K; k0 += 32) {
+        const int nks = static_cast<int>((K - k0 + 3) / 4 < 8 ? (K - k0 + 3) / 4 : 8);  // wave-uniform
+        double2 av[8], bv[8];
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int64_t k = k0 + 4 * s + kq;
-            const bool ok = k < K;
-            av[s] = (arow && ok) ? a[i * sar + k * sak] : z;
-            bv[s] = (bcol && ok) ? b[k * sbk + i * sbc] : z;
+        for (int s = 0; s < 8; ++s) {
+            if (s < nks) {
+                const int64_t k = k0 + 4 * s + kq;
+                const bool ok = k < K;
+                av[s] = (arow && ok) ? a[i * sar + k * sak] : z;
+                bv[s] = (bcol && ok) ? b[k * sbk + i * sbc] : z;
+            }
         }
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            if (k0 + 4 * s < K) {  // wave-uniform
+        for (int s = 0; s < 8; ++s) {
+            if (s < nks) {
                 double2 x = av[s];
                 if (CONJA) x.y = -x.y;
                 zmfma(acc, x, bv[s]);
