@@ -161,6 +161,15 @@ int ttn_flops(const ttn_integ* integ, double* flops);
  * observables, whole kernel.  Instrumentation only. */
 int ttn_profile(const ttn_integ* integ, int64_t* slots);
 
+/* ---- workgroup teams (no reference counterpart: GPU execution tuning) ----
+ * An instance is advanced by a team of workgroups on one XCD.  Operations on node
+ * tensors of at least team_min complex entries are spread over the team; smaller ones run
+ * on its first workgroup.  teams_enabled = 0 runs every instance on one workgroup.
+ * Results do not depend on these settings beyond rounding. */
+int ttn_ctx_set_tuning(ttn_ctx* ctx, int64_t team_min, int teams_enabled);
+/* team size chosen for every instance by the last ttn_batch_step (sizes[n]) */
+int ttn_batch_teams(const ttn_batch* batch, int64_t* sizes);
+
 /* ---- timing helpers (device-side event timing of the last ttn_step / ttn_batch_step) ---- */
 int ttn_last_step_ms(const ttn_integ* integ, double* ms);
 int ttn_batch_last_step_ms(const ttn_batch* batch, double* ms);

@@ -97,6 +97,8 @@ _SIGS = {
     "flops": [vp, dblp],
     "profile": [vp, i64p],
     "batch_last_step_ms": [vp, dblp],
+    "ctx_set_tuning": [vp, i64, C.c_int],
+    "batch_teams": [vp, i64p],
 }
 _RESTYPE = {"last_error": C.c_char_p}
 
@@ -138,6 +140,12 @@ class Library:
 
     def has(self, name: str) -> bool:
         return name in self.fn
+
+    def set_tuning(self, team_min: int = 2048, teams: bool = True):
+        """Workgroup-team knobs of the GPU library: operations on node tensors of at least
+        team_min entries are spread over an instance's team; teams=False runs every
+        instance on one workgroup."""
+        self._call("ctx_set_tuning", self.ctx, int(team_min), int(bool(teams)))
 
     def error(self) -> str:
         msg = self.fn["last_error"](self.ctx)
@@ -387,6 +395,12 @@ class Batch:
         out = C.c_double()
         self.lib._call("batch_last_step_ms", self.h, C.byref(out))
         return out.value
+
+    def teams(self) -> np.ndarray:
+        """team size of every instance in the last batched step"""
+        out = np.zeros(len(self.integs), dtype=np.int64)
+        self.lib._call("batch_teams", self.h, _ip(out))
+        return out
 
     def destroy(self):
         if self.h:

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -18,9 +19,13 @@
 #include "host/marshal.hpp"
 #include "host/tables.hpp"
 
-extern "C" hipError_t ttnd_launch_run(const ttnd::DInst* d_insts, int n, const ttnd::RunArgs* a, hipStream_t s);
+extern "C" hipError_t ttnd_launch_run(const ttnd::DInst* d_insts, int n, const ttnd::RunArgs* a,
+                                      const ttnd::TeamPlan* d_plan, ttnd::TeamWords* d_words, unsigned* d_fill,
+                                      int grid, hipStream_t s);
 extern "C" hipError_t ttnd_launch_expect(const ttnd::DInst* d_inst, int leaf, int64_t op_off, double* d_out,
                                          hipStream_t s);
+extern "C" hipError_t ttnd_run_occupancy(int* blocks_per_cu);
+extern "C" hipError_t ttnd_launch_census(int* d_out, int n, hipStream_t s);
 
 using namespace ttnh;
 
@@ -57,6 +62,15 @@ struct ttn_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
+    // team launches: every workgroup of a grid of ncu * occupancy is resident at once and
+    // the dispatcher spreads them evenly over nxcd XCDs (xcd_cap each), checked by a
+    // placement census when the context is created
+    bool teams_ok = false;
+    int grid = 0, nxcd = 0, xcd_cap = 0;
+    int64_t team_min = 2048;      // smaller operations run on a team's leader alone
+    DevBuf<unsigned> team_words;  // XCD slot counters (MAX_XCD lines) + TeamWords per instance
+    int64_t team_words_n = 0;     // instances the buffer holds
+    DevBuf<ttnd::TeamPlan> plan;
 };
 struct ttn_tree {
     ttn_ctx* ctx;
@@ -102,6 +116,8 @@ struct ttn_batch {
     std::vector<ttn_integ*> integs;
     DevBuf<ttnd::DInst> d_insts;
     double last_ms = 0.0;
+    std::vector<double> cost;      // per-instance flops per sweep pair (model, then measured)
+    std::vector<int> team;         // team sizes of the last launch
 };
 
 namespace {
@@ -297,6 +313,9 @@ void build_instance(ttn_integ* in) {
     for (int k = 0; k < ttnd::NWORK; ++k) I.work[k] = alloc(maxsize);
     I.krylov = alloc((kdim + 1) * maxsize);
     I.maxsize = maxsize;
+    // team scratch: partial sums (doubles, two buffers) and partial products per member
+    I.team_part = alloc(ttnd::TEAM_MAX * ttnd::PART_K);
+    I.team_tiles = alloc(static_cast<int64_t>(ttnd::TEAM_MAX) * ttnd::TILE_MAX);
     in->arena_size = align16(off);
 
     // device allocations
@@ -379,6 +398,7 @@ void make_args(const ttn_integ* in, double cr, double ci, int64_t nsteps, ttnd::
     a.coeff = make_double2(cr, ci);
     a.dt = in->p.dt;
     a.step0 = 0;
+    a.team_min = in->ctx->team_min;
     if (in->nsteps == 0) {
         a.prepare = 1;
         a.nladder = ladder(in->p.dt, a.ladder_dt);
@@ -409,13 +429,18 @@ void account(ttn_integ* in, const ttnd::RunArgs& a) {
 // Message for a failure record of the sweep kernel (reason | bond << 4 | node << 5).  The
 // record is sticky: an instance whose sweep failed keeps failing until it is recreated.
 std::string failure_message(const int st[2]) {
-    static const char* reasons[] = {"", "the vector to propagate has a non-finite norm",
+    static const char* reasons[] = {"",
+                                    "the vector to propagate has a non-finite norm",
                                     "a Krylov basis vector has a non-finite norm",
-                                    "no Krylov sub-step met the tolerance"};
+                                    "no Krylov sub-step met the tolerance",
+                                    "a team barrier timed out",
+                                    "the team's workgroups did not assemble"};
     const int reason = st[0] & 15, bond = (st[0] >> 4) & 1, node = st[0] >> 5;
+    if (reason == ttnd::FAIL_TEAM_TIMEOUT || reason == ttnd::FAIL_TEAM_FORM)
+        return std::string("TDVP sweep failed at step ") + std::to_string(st[1]) + ": " + reasons[reason];
     std::string m = "TDVP sweep failed at step " + std::to_string(st[1]) + ", " +
                     (bond ? "bond propagation above node " : "node propagation of node ") + std::to_string(node) +
-                    ": " + (reason < 4 ? reasons[reason] : "unknown reason");
+                    ": " + (reason < 6 ? reasons[reason] : "unknown reason");
     return m;
 }
 
@@ -429,6 +454,209 @@ void finish(ttn_integ* in, int64_t nsteps) {
     }
     if (st[0] != 0) fail(ERR_NUMERIC, failure_message(st));
     in->nsteps += nsteps;
+}
+
+// ---------------------------------------------------------------- teams
+// Placement census: which XCD each workgroup of a resident grid lands on.  Teams are
+// enabled only when every workgroup of the grid is resident at once and the grid spreads
+// evenly over at most MAX_XCD XCDs.
+void team_census(ttn_ctx* c) {
+    hipDeviceProp_t prop;
+    hip_check(hipGetDeviceProperties(&prop, c->device), "hipGetDeviceProperties");
+    int occ = 0;
+    hip_check(ttnd_run_occupancy(&occ), "occupancy");
+    c->teams_ok = false;
+    if (occ < 1 || prop.multiProcessorCount < 2) return;
+    const int grid = prop.multiProcessorCount * occ;
+    DevBuf<int> ids;
+    ids.alloc(grid);
+    hip_check(ttnd_launch_census(ids.p, grid, c->stream), "census");
+    std::vector<int> h(grid);
+    hip_check(hipMemcpyAsync(h.data(), ids.p, grid * sizeof(int), hipMemcpyDeviceToHost, c->stream), "census");
+    hip_check(hipStreamSynchronize(c->stream), "sync");
+    int nx = 0;
+    for (int x : h) {
+        if (x < 0 || x >= ttnd::MAX_XCD) return;
+        nx = std::max(nx, x + 1);
+    }
+    std::vector<int> cnt(nx, 0);
+    for (int x : h) ++cnt[x];
+    for (int x = 0; x < nx; ++x)
+        if (cnt[x] != grid / nx || grid % nx != 0) return;
+    c->grid = grid;
+    c->nxcd = nx;
+    c->xcd_cap = std::min(grid / nx, ttnd::XCD_SLOTS);
+    c->teams_ok = c->xcd_cap >= 2;
+}
+
+// largest team the context can place (one XCD, TEAM_MAX members)
+int max_team(const ttn_ctx* c) { return c->teams_ok ? std::min(c->xcd_cap, ttnd::TEAM_MAX) : 1; }
+
+// First-fit-decreasing placement of the teams on the XCDs.  Returns false when they do
+// not fit.
+bool make_plan(const ttn_ctx* c, const std::vector<int>& size, ttnd::TeamPlan& plan) {
+    std::memset(&plan, 0, sizeof(plan));
+    plan.nxcd = c->nxcd;
+    std::vector<int> order(size.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = static_cast<int>(i);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return size[a] > size[b]; });
+    for (int i : order) {
+        int x = 0;
+        while (x < c->nxcd && plan.count[x] + size[i] > c->xcd_cap) ++x;
+        if (x == c->nxcd) return false;
+        for (int r = 0; r < size[i]; ++r) plan.slot[x][plan.count[x] + r] = ttnd::TeamSlot{i, r, size[i], 0};
+        plan.count[x] += size[i];
+    }
+    return true;
+}
+
+// Launch the sweep kernel over n instances with the given team sizes (size 1 everywhere
+// runs one workgroup per instance without a plan).  A team that did not assemble leaves
+// its instance untouched with FAIL_TEAM_FORM; such instances are run again one workgroup
+// each, and teams are switched off for the context.
+void launch(ttn_ctx* c, ttnd::DInst* d_insts, const std::vector<ttnd::DInst>& insts, const ttnd::RunArgs& a,
+            std::vector<int> size) {
+    const int n = static_cast<int>(insts.size());
+    hipStream_t s = c->stream;
+    int maxsize = 1;
+    for (int& t : size) {
+        t = std::max(1, std::min(t, max_team(c)));
+        maxsize = std::max(maxsize, t);
+    }
+    ttnd::TeamPlan plan;
+    const bool teams = c->teams_ok && maxsize > 1 && make_plan(c, size, plan);
+    hip_check(hipEventRecord(c->ev0, s), "event");
+    if (!teams) {
+        hip_check(ttnd_launch_run(d_insts, n, &a, nullptr, nullptr, nullptr, 0, s), "launch tdvp_run_kernel");
+    } else {
+        const int64_t head = 32 * ttnd::MAX_XCD;  // slot counters, one 128-byte line each
+        const int64_t words = static_cast<int64_t>(sizeof(ttnd::TeamWords) / sizeof(unsigned));
+        if (c->team_words_n < n) {
+            c->team_words.alloc(head + words * n);
+            c->team_words_n = n;
+        }
+        if (c->plan.p == nullptr) c->plan.alloc(1);
+        hip_check(hipMemsetAsync(c->team_words.p, 0, (head + words * n) * sizeof(unsigned), s), "memset team words");
+        hip_check(hipMemcpyAsync(c->plan.p, &plan, sizeof(plan), hipMemcpyHostToDevice, s), "plan");
+        hip_check(ttnd_launch_run(d_insts, n, &a, c->plan.p, reinterpret_cast<ttnd::TeamWords*>(c->team_words.p + head),
+                                  c->team_words.p, c->grid, s),
+                  "launch tdvp_run_kernel");
+    }
+    hip_check(hipEventRecord(c->ev1, s), "event");
+    hip_check(hipEventSynchronize(c->ev1), "sync");
+    if (!teams) return;
+    // instances whose team did not assemble: run them again, one workgroup each
+    std::vector<ttnd::DInst> again;
+    for (const ttnd::DInst& I : insts) {
+        int st = 0;
+        hip_check(hipMemcpy(&st, I.status, sizeof(int), hipMemcpyDeviceToHost), "status");
+        if ((st & 15) == ttnd::FAIL_TEAM_FORM) {
+            hip_check(hipMemset(I.status, 0, 2 * sizeof(int)), "reset status");
+            again.push_back(I);
+        }
+    }
+    if (again.empty()) return;
+    c->teams_ok = false;
+    DevBuf<ttnd::DInst> d_again;
+    d_again.alloc(again.size());
+    hip_check(hipMemcpyAsync(d_again.p, again.data(), again.size() * sizeof(ttnd::DInst), hipMemcpyHostToDevice, s),
+              "insts");
+    hip_check(ttnd_launch_run(d_again.p, static_cast<int>(again.size()), &a, nullptr, nullptr, nullptr, 0, s),
+              "launch tdvp_run_kernel");
+    hip_check(hipEventRecord(c->ev1, s), "event");
+    hip_check(hipEventSynchronize(c->ev1), "sync");
+}
+
+// Algorithmic flops executed by an instance so far (the ttn_flops model)
+double measured_flops(const ttn_integ* in) {
+    const int n = in->topo.size();
+    std::vector<int64_t> st(4 * n);
+    hip_check(hipMemcpy(st.data(), in->node_stats.p, st.size() * sizeof(int64_t), hipMemcpyDeviceToHost), "stats");
+    const FlopModel& f = in->fm;
+    double tot = static_cast<double>(in->sweep_pairs) * f.sweep_pair + static_cast<double>(in->prepares) * f.prepare_spf;
+    for (int i = 0; i < n; ++i) {
+        tot += static_cast<double>(st[4 * i]) * f.heff_node[i] + 32.0 * f.node_size[i] * static_cast<double>(st[4 * i + 1]);
+        tot += static_cast<double>(st[4 * i + 2]) * f.heff_bond[i] + 32.0 * f.bond_size[i] * static_cast<double>(st[4 * i + 3]);
+    }
+    return tot;
+}
+
+// Team size of an instance stepped on its own: one workgroup per 2048 entries of its
+// largest node tensor, at most one XCD.
+int auto_team(const ttn_integ* in) {
+    if (in->p.team_size > 0) return static_cast<int>(in->p.team_size);
+    const int64_t t = in->inst.maxsize / 2048;
+    return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(t, max_team(in->ctx))));
+}
+
+// Static cost model of one sweep pair (flops), used before anything is measured: every
+// propagation taken as six H applications with an average Krylov size of three.
+double static_cost(const ttn_integ* in) {
+    const FlopModel& f = in->fm;
+    double c = f.sweep_pair;
+    for (size_t i = 0; i < f.heff_node.size(); ++i)
+        c += 6.0 * (f.heff_node[i] + 3.0 * 32.0 * f.node_size[i]) + 6.0 * (f.heff_bond[i] + 3.0 * 32.0 * f.bond_size[i]);
+    return c;
+}
+
+// Share of an instance's work in operations below team_min, which its leader runs alone.
+double serial_share(const ttn_integ* in) {
+    const FlopModel& f = in->fm;
+    double all = 0.0, small = 0.0;
+    for (size_t i = 0; i < f.heff_node.size(); ++i) {
+        const double wn = f.heff_node[i] + 3.0 * 32.0 * f.node_size[i];
+        const double wb = f.heff_bond[i] + 3.0 * 32.0 * f.bond_size[i];
+        all += wn + wb;
+        if (f.node_size[i] < in->ctx->team_min) small += wn;
+        if (f.bond_size[i] < in->ctx->team_min) small += wb;
+    }
+    return all > 0.0 ? small / all : 1.0;
+}
+
+// Team sizes of a batch (the autotuner): starting from one workgroup per instance, the
+// next workgroup always goes to the instance with the longest predicted time
+// cost * (s + (1 - s) / t), s its serial share, until the XCDs are full; sizes given
+// explicitly by an integrator's team_size are kept.
+std::vector<int> batch_teams(const ttn_batch* b) {
+    const ttn_ctx* c = b->ctx;
+    const size_t n = b->integs.size();
+    std::vector<int> t(n, 1);
+    if (!c->teams_ok) return t;
+    std::vector<double> s(n);
+    std::vector<bool> fixed(n, false);
+    int used = 0;
+    for (size_t i = 0; i < n; ++i) {
+        s[i] = serial_share(b->integs[i]);
+        if (b->integs[i]->p.team_size > 0) {
+            t[i] = static_cast<int>(std::min<int64_t>(b->integs[i]->p.team_size, max_team(c)));
+            fixed[i] = true;
+        }
+        used += t[i];
+    }
+    const int cap = c->nxcd * c->xcd_cap;
+    auto time = [&](size_t i, int ti) { return b->cost[i] * (s[i] + (1.0 - s[i]) / ti); };
+    // an instance takes another workgroup only while that cuts its predicted time by at
+    // least 3% (every member adds barrier traffic)
+    auto grows = [&](size_t i) { return time(i, t[i] + 1) < 0.97 * time(i, t[i]); };
+    while (used < cap) {
+        size_t best = n;
+        for (size_t i = 0; i < n; ++i)
+            if (!fixed[i] && t[i] < max_team(c) && grows(i) && (best == n || time(i, t[i]) > time(best, t[best])))
+                best = i;
+        if (best == n) break;
+        ++t[best];
+        ++used;
+    }
+    // keep the placement feasible: shrink the largest team until first-fit succeeds
+    ttnd::TeamPlan plan;
+    while (!make_plan(c, t, plan)) {
+        size_t big = 0;
+        for (size_t i = 1; i < n; ++i)
+            if (t[i] > t[big]) big = i;
+        if (t[big] == 1) break;
+        --t[big];
+    }
+    return t;
 }
 
 }  // namespace
@@ -452,6 +680,12 @@ int ttn_ctx_create(int device, ttn_ctx** out) {
         hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipEventCreate(&c->ev0), "hipEventCreate");
         hip_check(hipEventCreate(&c->ev1), "hipEventCreate");
+        team_census(c.get());
+        // tuning knobs: TTN_TEAM_MIN (elements below which an operation runs on a team's
+        // leader alone), TTN_TEAMS=0 switches teams off
+        if (const char* e = std::getenv("TTN_TEAM_MIN")) c->team_min = std::max<int64_t>(1, std::atoll(e));
+        if (const char* e = std::getenv("TTN_TEAMS"))
+            if (std::atoi(e) == 0) c->teams_ok = false;
     });
     if (rc != TTN_OK) {
         static std::string last;
@@ -610,10 +844,7 @@ int ttn_step(ttn_integ* in, double cr, double ci, int64_t nsteps) {
         prepare_traj(in, nsteps);
         hipStream_t s = in->ctx->stream;
         hip_check(hipMemcpyAsync(in->d_inst.p, &in->inst, sizeof(in->inst), hipMemcpyHostToDevice, s), "inst");
-        hip_check(hipEventRecord(in->ctx->ev0, s), "event");
-        hip_check(ttnd_launch_run(in->d_inst.p, 1, &a, s), "launch tdvp_run_kernel");
-        hip_check(hipEventRecord(in->ctx->ev1, s), "event");
-        hip_check(hipEventSynchronize(in->ctx->ev1), "sync");
+        launch(in->ctx, in->d_inst.p, {in->inst}, a, {auto_team(in)});
         float ms = 0.f;
         hip_check(hipEventElapsedTime(&ms, in->ctx->ev0, in->ctx->ev1), "elapsed");
         in->last_ms = ms;
@@ -725,18 +956,23 @@ int ttn_batch_step(ttn_batch* b, double cr, double ci, int64_t nsteps) {
         hip_check(hipMemcpyAsync(b->d_insts.p, insts.data(), insts.size() * sizeof(ttnd::DInst), hipMemcpyHostToDevice,
                                  s),
                   "insts");
-        hip_check(hipEventRecord(b->ctx->ev0, s), "event");
-        hip_check(ttnd_launch_run(b->d_insts.p, static_cast<int>(insts.size()), &a, s), "launch tdvp_run_kernel");
-        hip_check(hipEventRecord(b->ctx->ev1, s), "event");
-        hip_check(hipEventSynchronize(b->ctx->ev1), "sync");
+        if (b->cost.empty())
+            for (ttn_integ* in : b->integs) b->cost.push_back(static_cost(in));
+        b->team = batch_teams(b);
+        std::vector<double> f0(b->integs.size());
+        for (size_t k = 0; k < b->integs.size(); ++k) f0[k] = measured_flops(b->integs[k]);
+        launch(b->ctx, b->d_insts.p, insts, a, b->team);
         float ms = 0.f;
         hip_check(hipEventElapsedTime(&ms, b->ctx->ev0, b->ctx->ev1), "elapsed");
         b->last_ms = ms;
-        // every instance is accounted before the first failure is reported
+        // every instance is accounted before the first failure is reported; the measured
+        // flops per sweep pair become the autotuner's costs for the next call
         std::string err;
+        const double pairs = static_cast<double>(a.prepare + a.nladder + a.nregular);
         for (size_t k = 0; k < b->integs.size(); ++k) {
             ttn_integ* in = b->integs[k];
             account(in, a);
+            b->cost[k] = std::max(1.0, (measured_flops(in) - f0[k]) / pairs);
             try {
                 finish(in, nsteps);
             } catch (const Error& e) {
@@ -748,26 +984,29 @@ int ttn_batch_step(ttn_batch* b, double cr, double ci, int64_t nsteps) {
 }
 
 int ttn_flops(const ttn_integ* in, double* flops) {
-    return guard(in->ctx, [&] {
-        const int n = in->topo.size();
-        std::vector<int64_t> st(4 * n);
-        hip_check(hipMemcpy(st.data(), in->node_stats.p, st.size() * sizeof(int64_t), hipMemcpyDeviceToHost),
-                  "stats");
-        const FlopModel& f = in->fm;
-        double tot = static_cast<double>(in->sweep_pairs) * f.sweep_pair + static_cast<double>(in->prepares) * f.prepare_spf;
-        for (int i = 0; i < n; ++i) {
-            tot += static_cast<double>(st[4 * i]) * f.heff_node[i] + 32.0 * f.node_size[i] * static_cast<double>(st[4 * i + 1]);
-            tot += static_cast<double>(st[4 * i + 2]) * f.heff_bond[i] +
-                   32.0 * f.bond_size[i] * static_cast<double>(st[4 * i + 3]);
-        }
-        *flops = tot;
-    });
+    return guard(in->ctx, [&] { *flops = measured_flops(in); });
 }
 
 int ttn_profile(const ttn_integ* in, int64_t* slots) {
     return guard(in->ctx, [&] {
         hip_check(hipMemcpy(slots, in->prof.p, ttnd::PROF_NSLOTS * sizeof(int64_t), hipMemcpyDeviceToHost), "prof");
     });
+}
+
+int ttn_ctx_set_tuning(ttn_ctx* c, int64_t team_min, int teams_enabled) {
+    return guard(c, [&] {
+        check(team_min >= 1, ERR_INVALID, "team_min must be positive");
+        c->team_min = team_min;
+        if (!teams_enabled)
+            c->teams_ok = false;
+        else
+            team_census(c);
+    });
+}
+
+int ttn_batch_teams(const ttn_batch* b, int64_t* sizes) {
+    for (size_t k = 0; k < b->integs.size(); ++k) sizes[k] = k < b->team.size() ? b->team[k] : 1;
+    return TTN_OK;
 }
 
 int ttn_last_step_ms(const ttn_integ* in, double* ms) {

@@ -68,6 +68,8 @@ enum FailReason {
     FAIL_INPUT_NONFINITE = 1,   // norm of the vector to propagate is not finite
     FAIL_KRYLOV_NONFINITE = 2,  // a Krylov basis norm is not finite
     FAIL_KRYLOV_STEP = 3,       // no sub-step size met the Krylov tolerance
+    FAIL_TEAM_TIMEOUT = 4,      // a team barrier did not complete in time
+    FAIL_TEAM_FORM = 5,         // the team's workgroups did not assemble (state untouched)
 };
 
 struct DInst {
@@ -94,6 +96,41 @@ struct DInst {
                                    // summed, bond-update applications (by operator node), sizes
     int* status;                   // [0]: failure record (0 = ok, see FailReason), [1]: step index
     int64_t* prof;                 // per-phase device wall-clock ticks (PROF_* slots), or null
+    int64_t team_part;             // team partial sums (2 x TEAM_MAX x PART_K doubles)
+    int64_t team_tiles;            // team partial products (TEAM_MAX x TILE_MAX)
+};
+
+// ---------------------------------------------------------------- teams
+constexpr int TEAM_MAX = 32;              // members per team (the CUs of one XCD)
+constexpr int PART_K = 160;               // doubles per member and buffer of team partial sums
+constexpr int TILE_MAX = 64 * 64;         // partial product (hrank x hrank) per member
+constexpr uint64_t TEAM_TIMEOUT = 200000000ull;   // 2 s of 100 MHz ticks at one barrier
+
+// Team words of one team in global memory (zeroed before every launch): barrier arrival
+// count and completed generations on separate 128-byte lines, the team-wide failure
+// record, and an abort flag for a team that did not assemble.
+struct TeamWords {
+    unsigned count;
+    unsigned pad0[31];
+    unsigned gen;
+    unsigned pad1[31];
+    int fail;
+    int aborted;
+    unsigned pad2[30];
+};
+
+// Placement of teams on the XCDs for one launch: a workgroup reads the XCD it runs on,
+// takes the next free slot of that XCD, and becomes member `rank` of the team of
+// instance `inst`.  Workgroups beyond an XCD's slot count exit at once.
+constexpr int MAX_XCD = 16;
+constexpr int XCD_SLOTS = 64;
+struct TeamSlot {
+    int inst, rank, size, pad;
+};
+struct TeamPlan {
+    int nxcd, pad;
+    int count[MAX_XCD];
+    TeamSlot slot[MAX_XCD][XCD_SLOTS];
 };
 
 struct RunArgs {
@@ -104,6 +141,7 @@ struct RunArgs {
     int nregular;                  // regular steps after the ladder
     double dt;
     int64_t step0;                 // index of the first step of this call (trajectory slot)
+    int64_t team_min;              // operations on fewer elements run on a team's leader alone
 };
 
 }  // namespace ttnd

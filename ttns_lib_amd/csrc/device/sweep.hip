@@ -1,11 +1,13 @@
 // Persistent TDVP sweep kernel for gfx950.
 //
-// One workgroup owns one TTNS instance and runs whole integrator calls on the device:
-// prepare_evolution, the first-step ladder and any number of regular steps, each step a
-// forward and a backward projector-splitting sweep (reference
+// A team of workgroups (one XCD) owns one TTNS instance and runs whole integrator calls
+// on the device: prepare_evolution, the first-step ladder and any number of regular
+// steps, each step a forward and a backward projector-splitting sweep (reference
 // tdvp_core/tdvp_algorithm_engine.hpp:207-501) with the Krylov propagators, environment
 // updates and QR gauge moves in between.  No host round trip happens inside a call.  A
-// batch is a grid of such workgroups, one per instance.
+// batch is one launch in which every instance has its own team.  Operations on large
+// node tensors are spread over the whole team; small ones run on the team's leader while
+// the other members wait at the next team barrier.
 #include <hip/hip_runtime.h>
 
 #include "desc.hpp"
@@ -16,21 +18,48 @@ namespace ttnd {
 struct Ctx {
     const DInst& I;
     Shared& sh;
+    Team* T;
     double2* base;
-    __device__ Ctx(const DInst& inst, Shared& s) : I(inst), sh(s), base(inst.base) {}
-    __device__ double2* W(int k) const { return base + I.work[k]; }
-    __device__ const DNode& node(int n) const { return I.nodes[n]; }
-    // failures are detected on workgroup-uniform values, so every thread takes the same
-    // branch; the LDS flag is read only after a barrier
-    __device__ bool failed() const { return sh.fail != 0; }
+    int64_t team_min;
     // where: node whose propagation is running, bond: 1 for its bond matrix
     int where = 0, bond = 0;
-    __device__ void fail(int reason) const { sh.fail = reason | (bond << 4) | (where << 5); }
+    __device__ Ctx(const DInst& inst, Shared& s, Team* t, int64_t tmin)
+        : I(inst), sh(s), T(t), base(inst.base), team_min(tmin) {}
+    __device__ double2* W(int k) const { return base + I.work[k]; }
+    __device__ const DNode& node(int n) const { return I.nodes[n]; }
+    // failures are detected on team-uniform values, so every thread of every member takes
+    // the same branch; the LDS flag is read only after a barrier
+    __device__ bool failed() const { return sh.fail != 0; }
+    __device__ void fail(int reason) const { T->fail(reason | (bond << 4) | (where << 5)); }
+    // thread 0 of the team leader writes statistics and results
+    __device__ bool writer() const { return T->rank == 0 && threadIdx.x == 0; }
+    // operations on at least team_min elements are spread over the team
+    __device__ bool spread(int64_t elems) const { return T->size > 1 && elems >= team_min; }
 };
+
+// Runs f on the team's leader alone (as a team of one) unless the operation is large
+// enough to spread; the other members wait at the barrier that follows.  A failure the
+// leader records reaches the others through the team word.
+template <class F>
+__device__ void on_team(Ctx& c, int64_t elems, F&& f) {
+    if (c.T->size == 1 || c.spread(elems)) {
+        f(c);
+        return;
+    }
+    if (c.T->rank == 0) {
+        Team s = c.T->single();
+        Ctx cs = c;
+        cs.T = &s;
+        f(cs);
+        if (threadIdx.x == 0 && c.sh.fail != 0)
+            __hip_atomic_store(&c.T->w->fail, c.sh.fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    c.T->sync();
+}
 
 __device__ __forceinline__ uint64_t prof_now(const Ctx& c) { return c.I.prof ? wall_clock64() : 0; }
 __device__ __forceinline__ void prof_add(const Ctx& c, int slot, uint64_t t0) {
-    if (c.I.prof && threadIdx.x == 0) c.I.prof[slot] += static_cast<int64_t>(wall_clock64() - t0);
+    if (c.I.prof && c.writer()) c.I.prof[slot] += static_cast<int64_t>(wall_clock64() - t0);
 }
 #define PROFILE(slot, stmt)                  \
     {                                        \
@@ -51,7 +80,7 @@ __device__ __forceinline__ void mode_view(const DNode& nd, int mode, int64_t& L,
 // chain of mode contractions with spf matrices; the last contraction lands in out.
 __device__ void kpo(Ctx& c, const DNode& nd, int k0, int nk, const double2* v, double2* out, double2* tmp) {
     if (nk == 0) {
-        wg_copy(out, v, nd.size);
+        t_copy(*c.T, out, v, nd.size);
         return;
     }
     const double2* src = v;
@@ -60,26 +89,25 @@ __device__ void kpo(Ctx& c, const DNode& nd, int k0, int nk, const double2* v, d
         const DK& k = c.I.ks[k0 + q];
         int64_t L, R;
         mode_view(nd, k.nu, L, R);
-        wg_mode_contract(c.base + k.mat, L, nd.dims[k.nu], R, src, dst, cmk(1.0, 0.0), false);
+        t_mode_contract(*c.T, c.base + k.mat, L, nd.dims[k.nu], R, src, dst, cmk(1.0, 0.0), false);
         src = dst;
     }
 }
 
 // coefficient_evolution_{leaf,branch} (projector_splitting_evolution_functions.hpp:65-258)
-// uses W1..W4
 // out (=|+=) alpha * K v for one spf sub-term: a leaf's primitive operator or a chain of
 // mode contractions (intermediates in t1/t2, the last one lands in out with the epilogue)
 __device__ void subterm_into(Ctx& c, const DNode& nd, const DSub& s, const double2* v, double2* out, double2 alpha,
                              bool accumulate, double2* t1, double2* t2) {
     if (s.nk == 0) {
         if (accumulate)
-            wg_axpy(out, alpha, v, nd.size);
+            t_axpy(*c.T, out, alpha, v, nd.size);
         else
-            wg_scale_to(out, alpha, v, nd.size);
+            t_scale_to(*c.T, out, alpha, v, nd.size);
         return;
     }
     if (nd.leaf >= 0) {
-        wg_prim_apply(c.I.ks[s.k0], c.base, c.I.ints, v, out, nd.hrank, alpha, accumulate);
+        t_prim_apply(*c.T, c.I.ks[s.k0], c.base, c.I.ints, v, out, nd.hrank, alpha, accumulate);
         return;
     }
     const double2* src = v;
@@ -89,8 +117,8 @@ __device__ void subterm_into(Ctx& c, const DNode& nd, const DSub& s, const doubl
         const DK& k = c.I.ks[s.k0 + q];
         int64_t L, R;
         mode_view(nd, k.nu, L, R);
-        wg_mode_contract(c.base + k.mat, L, nd.dims[k.nu], R, src, dst, last ? alpha : cmk(1.0, 0.0),
-                         last && accumulate);
+        t_mode_contract(*c.T, c.base + k.mat, L, nd.dims[k.nu], R, src, dst, last ? alpha : cmk(1.0, 0.0),
+                        last && accumulate);
         src = dst;
     }
 }
@@ -109,11 +137,11 @@ __device__ void heff_node(Ctx& c, int n, const double2* v, double2* res) {
             const double2 coeff = cmk(tm.ire, tm.iim);
             if (tm.id_mf) {
                 if (first)
-                    wg_scale_to(res, coeff, v, nd.size);
+                    t_scale_to(*c.T, res, coeff, v, nd.size);
                 else
-                    wg_axpy(res, coeff, v, nd.size);
+                    t_axpy(*c.T, res, coeff, v, nd.size);
             } else {
-                wg_mul_transposed(v, D, r, c.base + tm.mf, res, coeff, !first);
+                t_mul_transposed(*c.T, v, D, r, c.base + tm.mf, res, coeff, !first);
             }
             first = false;
             continue;
@@ -129,11 +157,11 @@ __device__ void heff_node(Ctx& c, int n, const double2* v, double2* res) {
             }
         }
         if (!tm.id_mf) {
-            wg_mul_transposed(t3, D, r, c.base + tm.mf, res, cmk(1.0, 0.0), !first);
+            t_mul_transposed(*c.T, t3, D, r, c.base + tm.mf, res, cmk(1.0, 0.0), !first);
             first = false;
         }
     }
-    if (first) wg_zero(res, nd.size);
+    if (first) t_zero(*c.T, res, nd.size);
 }
 
 // full_hamiltonian_action (projector_splitting_evolution_functions.hpp:7-62); uses W1, W2
@@ -141,25 +169,25 @@ __device__ void heff_bond(Ctx& c, int n, const double2* v, double2* res) {
     const DNode& nd = c.node(n);
     const int r = nd.hrank;
     const int64_t sz = static_cast<int64_t>(r) * r;
-    double2 *t1 = c.W(1), *t2 = c.W(2);
-    wg_zero(res, sz);
+    double2* t1 = c.W(1);
+    t_zero(*c.T, res, sz);
     for (int ti = 0; ti < nd.nterms; ++ti) {
         const DTerm& tm = c.I.terms[nd.term0 + ti];
         const double2* x = v;
         if (!tm.id_spf) {
-            wg_mode_contract(c.base + tm.spf, 1, r, r, v, t1, cmk(1.0, 0.0), false);
+            t_mode_contract(*c.T, c.base + tm.spf, 1, r, r, v, t1, cmk(1.0, 0.0), false);
             x = t1;
         }
         const double2 coeff = cmk(tm.cre, tm.cim);
         if (!tm.id_mf)
-            wg_mul_transposed(x, r, r, c.base + tm.mf, res, coeff, true);
+            t_mul_transposed(*c.T, x, r, r, c.base + tm.mf, res, coeff, true);
         else
-            wg_axpy(res, coeff, x, sz);
+            t_axpy(*c.T, res, coeff, x, sz);
     }
 }
 
 // single_particle_operator_engine::evaluate (single_particle_operator.hpp:26-103); uses W1, W2
-__device__ void update_spf(Ctx& c, int n) {
+__device__ void update_spf_body(Ctx& c, int n) {
     const DNode& nd = c.node(n);
     const int r = nd.hrank;
     const int64_t D = nd.dimen;
@@ -172,22 +200,24 @@ __device__ void update_spf(Ctx& c, int n) {
         for (int i = 0; i < tm.nsub; ++i) {
             const DSub& s = c.I.subs[tm.sub0 + i];
             if (nd.leaf >= 0 && s.nk > 0)
-                wg_prim_apply(c.I.ks[s.k0], c.base, c.I.ints, a, HA, r);
+                t_prim_apply(*c.T, c.I.ks[s.k0], c.base, c.I.ints, a, HA, r);
             else
                 kpo(c, nd, s.k0, s.nk, a, HA, tmp);
             if (tm.nsub == 1)
-                wg_adjoint_mul(c.sh, a, HA, D, r, out, cmk(1.0, 0.0), false);
+                t_adjoint_mul(*c.T, a, HA, D, r, out, cmk(1.0, 0.0), false);
             else
-                wg_adjoint_mul(c.sh, a, HA, D, r, out, cmk(s.are, s.aim), i > 0);
+                t_adjoint_mul(*c.T, a, HA, D, r, out, cmk(s.are, s.aim), i > 0);
         }
     }
+}
+__device__ void update_spf(Ctx& c, int n) {
+    on_team(c, c.node(n).size, [&](Ctx& cc) { update_spf_body(cc, n); });
 }
 
 // mean_field_operator_engine::evaluate (mean_field_operator.hpp:37-135) for child n;
 // uses W1, W2, W3
-__device__ void update_mf(Ctx& c, int n) {
+__device__ void update_mf_body(Ctx& c, int n) {
     const DNode& nd = c.node(n);
-    if (nd.parent < 0) return;
     const DNode& pn = c.node(nd.parent);
     const int mode = nd.child_id;
     int64_t L, R;
@@ -209,20 +239,26 @@ __device__ void update_mf(Ctx& c, int n) {
                 HA = t1;
             }
             if (m.pmf >= 0) {
-                wg_mul_transposed(HA, pD, pr, c.base + m.pmf, t3, cmk(1.0, 0.0), false);
+                t_mul_transposed(*c.T, HA, pD, pr, c.base + m.pmf, t3, cmk(1.0, 0.0), false);
                 HA = t3;
             }
             if (tm.nmf == 1)
-                wg_outer_contract(c.sh, Ap, HA, L, d, R, out, cmk(1.0, 0.0), false);
+                t_outer_contract(*c.T, Ap, HA, L, d, R, out, cmk(1.0, 0.0), false);
             else
-                wg_outer_contract(c.sh, Ap, HA, L, d, R, out, cmk(m.are, m.aim), it > 0);
+                t_outer_contract(*c.T, Ap, HA, L, d, R, out, cmk(m.are, m.aim), it > 0);
         }
     }
+}
+__device__ void update_mf(Ctx& c, int n) {
+    const DNode& nd = c.node(n);
+    if (nd.parent < 0) return;
+    on_team(c, c.node(nd.parent).size, [&](Ctx& cc) { update_mf_body(cc, n); });
 }
 
 // ---------------------------------------------------------------- gauge moves
 // root_to_leaf_decomposition_engine::evaluate + apply_to_node (root_to_leaf_decomposition.hpp:112-204)
-__device__ void r2l(Ctx& c, int n, int mode) {
+__device__ void r2l_body(Ctx& c, int n, int mode) {
+    Team& T = *c.T;
     const DNode& nd = c.node(n);
     const int64_t sz = nd.size;
     int64_t id1 = 1;
@@ -232,79 +268,92 @@ __device__ void r2l(Ctx& c, int n, int mode) {
     const int d1 = static_cast<int>(sz / d2);
     double2* A = c.base + nd.A;
     double2 *U = c.W(5), *Q = c.W(1), *Rm = c.W(2);
-    for (int64_t e = threadIdx.x; e < sz; e += NT) {
+    for (int64_t e = T.tid(); e < sz; e += T.nthr()) {
         const int64_t i = e / id2, j = e % id2;
-        U[j * id1 + i] = A[e];
+        U[j * id1 + i] = ldc(A + e);
     }
-    __syncthreads();
-    wg_householder_qr(c.sh, U, d1, d2, Q, Rm);
-    for (int64_t e = threadIdx.x; e < sz; e += NT) {
+    T.sync();
+    t_householder_qr(T, U, d1, d2, Q, Rm);
+    for (int64_t e = T.tid(); e < sz; e += T.nthr()) {
         const int64_t i = e / id2, j = e % id2;
-        A[e] = Q[j * id1 + i];
+        A[e] = ldc(Q + j * id1 + i);
     }
     double2* Rn = c.base + nd.R;
-    for (int64_t e = threadIdx.x; e < static_cast<int64_t>(d2) * d2; e += NT) {
+    for (int64_t e = T.tid(); e < static_cast<int64_t>(d2) * d2; e += T.nthr()) {
         const int i = static_cast<int>(e / d2), j = static_cast<int>(e % d2);
-        Rn[e] = Rm[static_cast<int64_t>(j) * d2 + i];
+        Rn[e] = ldc(Rm + static_cast<int64_t>(j) * d2 + i);
     }
-    __syncthreads();
+    T.sync();
+}
+__device__ void r2l(Ctx& c, int n, int mode) {
+    on_team(c, c.node(n).size, [&](Ctx& cc) { r2l_body(cc, n, mode); });
 }
 
 // leaf_to_root_decomposition_engine::evaluate + apply_to_node (leaf_to_root_decomposition.hpp:56-101)
 __device__ void l2r(Ctx& c, int n) {
-    const DNode& nd = c.node(n);
-    double2* A = c.base + nd.A;
-    double2* U = c.W(5);
-    wg_copy(U, A, nd.size);
-    wg_householder_qr(c.sh, U, nd.dimen, nd.hrank, A, c.base + nd.R);
+    on_team(c, c.node(n).size, [&](Ctx& cc) {
+        const DNode& nd = cc.node(n);
+        double2* A = cc.base + nd.A;
+        double2* U = cc.W(5);
+        t_copy(*cc.T, U, A, nd.size);
+        t_householder_qr(*cc.T, U, nd.dimen, nd.hrank, A, cc.base + nd.R);
+    });
 }
 
 // root_to_leaf_decomposition_engine::apply_from_parent (:208-233): A <- A * R_parent
 __device__ void apply_from_parent(Ctx& c, int n) {
-    const DNode& nd = c.node(n);
-    double2* A = c.base + nd.A;
-    double2* T = c.W(5);
-    wg_mul(A, nd.dimen, nd.hrank, c.base + c.node(nd.parent).R, T);
-    wg_copy(A, T, nd.size);
+    on_team(c, c.node(n).size, [&](Ctx& cc) {
+        const DNode& nd = cc.node(n);
+        double2* A = cc.base + nd.A;
+        double2* Tm = cc.W(5);
+        t_mul(*cc.T, A, nd.dimen, nd.hrank, cc.base + cc.node(nd.parent).R, Tm);
+        t_copy(*cc.T, A, Tm, nd.size);
+    });
 }
 
 // leaf_to_root_decomposition_engine::apply_to_parent (leaf_to_root_decomposition.hpp:105-138)
 __device__ void apply_to_parent(Ctx& c, int n) {
-    const DNode& nd = c.node(n);
-    const DNode& pn = c.node(nd.parent);
-    int64_t L, R;
-    mode_view(pn, nd.child_id, L, R);
-    double2* T = c.W(5);
-    double2* Ap = c.base + pn.A;
-    wg_mode_contract(c.base + nd.R, L, pn.dims[nd.child_id], R, Ap, T, cmk(1.0, 0.0), false);
-    wg_copy(Ap, T, pn.size);
+    on_team(c, c.node(c.node(n).parent).size, [&](Ctx& cc) {
+        const DNode& nd = cc.node(n);
+        const DNode& pn = cc.node(nd.parent);
+        int64_t L, R;
+        mode_view(pn, nd.child_id, L, R);
+        double2* Tm = cc.W(5);
+        double2* Ap = cc.base + pn.A;
+        t_mode_contract(*cc.T, cc.base + nd.R, L, pn.dims[nd.child_id], R, Ap, Tm, cmk(1.0, 0.0), false);
+        t_copy(*cc.T, Ap, Tm, pn.size);
+    });
 }
 
 // ---------------------------------------------------------------- Krylov propagator
 // x <- exp(coeff * tau * H) x: Arnoldi with CGS2, Saad error estimate, step halving,
 // Pade(13) small exponential (same algorithm as the oracle's expv).  Returns the number
-// of H applications.  H(v, w) may use W1..W4; the Krylov vector w is W0.
+// of H applications.  H(v, w) may use W1..W4; the Krylov vector w is W0.  Every member
+// owns the same vector entries in all three Gram-Schmidt sweeps, so only the reductions
+// and the new basis vector need team barriers.
 template <class HF>
 __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, double tau, int64_t& sumj) {
     if (tau == 0.0 || n == 0) return 0;
+    Team& T = *c.T;
     const int kdim = c.I.kdim;
     const double tol = c.I.tol;
     Shared& sh = c.sh;
     double2* V = c.base + c.I.krylov;
     const int64_t ld = c.I.maxsize;
     double2* w = c.W(0);
+    const int64_t q00 = T.tid(), st = T.nthr();
     int64_t napply = 0;
     double s_done = 0.0, s_step = 1.0;
     int guard = 0;
     while (s_done < 1.0 && guard++ < 4096) {
         s_step = fmin(s_step, 1.0 - s_done);
-        const double beta = sqrt(block_norm2(sh, x, n));
+        const double beta = sqrt(team_norm2(T, x, n));
         if (!isfinite(beta)) {
             c.fail(FAIL_INPUT_NONFINITE);
             return napply;
         }
         if (beta == 0.0) return napply;
-        wg_scale_to(V, cmk(1.0 / beta, 0.0), x, n);
+        t_scale_to(T, V, cmk(1.0 / beta, 0.0), x, n);
         for (int e = threadIdx.x; e < (MAXK + 1) * MAXK; e += NT) sh.Hm[e] = cz();
         __syncthreads();
         int m = 0;
@@ -334,29 +383,29 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
 #pragma unroll
             for (int i = 0; i < 2 * MAXK; ++i) p[i] = 0.0;
             // two strides per iteration keep 2(j+2) loads in flight per thread
-            for (int64_t q0 = threadIdx.x; q0 < n; q0 += 2 * NT) {
-                const int64_t q1 = q0 + NT;
+            for (int64_t q0 = q00; q0 < n; q0 += 2 * st) {
+                const int64_t q1 = q0 + st;
                 const bool ok1 = q1 < n;
-                const double2 w0 = w[q0], w1 = ok1 ? w[q1] : cz();
+                const double2 w0 = ldc(w + q0), w1 = ok1 ? ldc(w + q1) : cz();
 #pragma unroll
                 for (int i = 0; i < MAXK; ++i) {
                     if (i <= j) {
-                        const double2 a0 = V[i * ld + q0], a1 = ok1 ? V[i * ld + q1] : cz();
+                        const double2 a0 = ldc(V + i * ld + q0), a1 = ok1 ? ldc(V + i * ld + q1) : cz();
                         const double2 pr = cadd(cmulc(a0, w0), cmulc(a1, w1));
                         p[2 * i] += pr.x;
                         p[2 * i + 1] += pr.y;
                     }
                 }
             }
-            block_sum_lds<2 * MAXK>(sh, p, sh.hb[0]);
+            team_sum_lds<2 * MAXK>(T, p, sh.hb[0]);
 #pragma unroll
             for (int i = 0; i < 2 * MAXK; ++i) p[i] = 0.0;
-            for (int64_t q = threadIdx.x; q < n; q += NT) {
+            for (int64_t q = q00; q < n; q += st) {
                 double2 vi[MAXK];
-                double2 wq = w[q];
+                double2 wq = ldc(w + q);
 #pragma unroll
                 for (int i = 0; i < MAXK; ++i)
-                    if (i <= j) vi[i] = V[i * ld + q];
+                    if (i <= j) vi[i] = ldc(V + i * ld + q);
 #pragma unroll
                 for (int i = 0; i < MAXK; ++i)
                     if (i <= j) wq = csub(wq, cmul(cmk(h1[2 * i], h1[2 * i + 1]), vi[i]));
@@ -370,17 +419,17 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
                     }
                 }
             }
-            block_sum_lds<2 * MAXK>(sh, p, sh.hb[1]);
+            team_sum_lds<2 * MAXK>(T, p, sh.hb[1]);
             double nn[1] = {0.0}, nno[1];
-            for (int64_t q0 = threadIdx.x; q0 < n; q0 += 2 * NT) {
-                const int64_t q1 = q0 + NT;
+            for (int64_t q0 = q00; q0 < n; q0 += 2 * st) {
+                const int64_t q1 = q0 + st;
                 const bool ok1 = q1 < n;
-                double2 w0 = w[q0], w1 = ok1 ? w[q1] : cz();
+                double2 w0 = ldc(w + q0), w1 = ok1 ? ldc(w + q1) : cz();
 #pragma unroll
                 for (int i = 0; i < MAXK; ++i) {
                     if (i <= j) {
                         const double2 h = cmk(h2[2 * i], h2[2 * i + 1]);
-                        const double2 a0 = V[i * ld + q0], a1 = ok1 ? V[i * ld + q1] : cz();
+                        const double2 a0 = ldc(V + i * ld + q0), a1 = ok1 ? ldc(V + i * ld + q1) : cz();
                         w0 = csub(w0, cmul(h, a0));
                         w1 = csub(w1, cmul(h, a1));
                     }
@@ -389,7 +438,7 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
                 if (ok1) w[q1] = w1;
                 nn[0] += cabs2(w0) + cabs2(w1);
             }
-            block_sum<1>(sh, nn, nno);
+            team_sum<1>(T, nn, nno);
             if (threadIdx.x <= j) {
                 const int i = threadIdx.x;
                 sh.Hm[i * MAXK + j] =
@@ -410,7 +459,7 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
             for (int i = 0; i < j; ++i)
                 lead *= cabsv(coeff) * fabs(tau * s_step) * cabsv(sh.Hm[(i + 1) * MAXK + i]) / (i + 1);
             if (lead > 1e3 * tol && j + 1 < kdim) {
-                wg_scale_to(V + (j + 1) * ld, cmk(1.0 / hn, 0.0), w, n);
+                t_scale_to(T, V + (j + 1) * ld, cmk(1.0 / hn, 0.0), w, n);
                 continue;
             }
             double err;
@@ -419,7 +468,7 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
                 found = true;
                 break;
             }
-            if (j + 1 < kdim) wg_scale_to(V + (j + 1) * ld, cmk(1.0 / hn, 0.0), w, n);
+            if (j + 1 < kdim) t_scale_to(T, V + (j + 1) * ld, cmk(1.0 / hn, 0.0), w, n);
         }
         for (int k = 0; !found && k < 60; ++k) {
             s_step *= 0.5;
@@ -429,19 +478,19 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
             c.fail(FAIL_KRYLOV_STEP);
             return napply;
         }
-        for (int64_t q0 = threadIdx.x; q0 < n; q0 += 2 * NT) {
-            const int64_t q1 = q0 + NT;
+        for (int64_t q0 = q00; q0 < n; q0 += 2 * st) {
+            const int64_t q1 = q0 + st;
             const bool ok1 = q1 < n;
             double2 s0 = cz(), s1 = cz();
             for (int i = 0; i < m; ++i) {
                 const double2 f = sh.F[i * m];
-                s0 = cadd(s0, cmul(f, V[i * ld + q0]));
-                if (ok1) s1 = cadd(s1, cmul(f, V[i * ld + q1]));
+                s0 = cadd(s0, cmul(f, ldc(V + i * ld + q0)));
+                if (ok1) s1 = cadd(s1, cmul(f, ldc(V + i * ld + q1)));
             }
             x[q0] = cscale(s0, beta);
             if (ok1) x[q1] = cscale(s1, beta);
         }
-        __syncthreads();
+        T.sync();
         s_done += s_step;
     }
     return napply;
@@ -456,16 +505,18 @@ struct SweepState {
 
 __device__ void node_update(Ctx& c, SweepState& st, int n) {
     const DNode& nd = c.node(n);
-    int64_t sumj = 0;
     c.where = n;
     c.bond = 0;
+    int64_t na = 0, sumj = 0;
     const uint64_t pt0 = prof_now(c);
-    const int64_t na = expv(
-        c, [&](const double2* v, double2* res) { PROFILE(PROF_HEFF, heff_node(c, n, v, res)); }, c.base + nd.A, nd.size, st.coeff,
-        st.sub_dt / 2.0, sumj);
+    on_team(c, nd.size, [&](Ctx& cc) {
+        na = expv(
+            cc, [&](const double2* v, double2* res) { PROFILE(PROF_HEFF, heff_node(cc, n, v, res)); },
+            cc.base + nd.A, nd.size, st.coeff, st.sub_dt / 2.0, sumj);
+    });
     prof_add(c, PROF_EXPV_NODE, pt0);
     st.nh += na;
-    if (threadIdx.x == 0) {
+    if (c.writer()) {
         c.I.node_stats[4 * n] += na;
         c.I.node_stats[4 * n + 1] += sumj;
     }
@@ -473,15 +524,17 @@ __device__ void node_update(Ctx& c, SweepState& st, int n) {
 
 __device__ void bond_update(Ctx& c, SweepState& st, int n, int ops) {
     const int r = c.node(ops).hrank;
-    int64_t sumj = 0;
     c.where = ops;
     c.bond = 1;
+    int64_t na = 0, sumj = 0;
     const uint64_t pt0 = prof_now(c);
-    const int64_t na = expv(
-        c, [&](const double2* v, double2* res) { PROFILE(PROF_HBOND, heff_bond(c, ops, v, res)); }, c.base + c.node(n).R,
-        static_cast<int64_t>(r) * r, st.coeff, -st.sub_dt / 2.0, sumj);
+    on_team(c, static_cast<int64_t>(r) * r, [&](Ctx& cc) {
+        na = expv(
+            cc, [&](const double2* v, double2* res) { PROFILE(PROF_HBOND, heff_bond(cc, ops, v, res)); },
+            cc.base + cc.node(n).R, static_cast<int64_t>(r) * r, st.coeff, -st.sub_dt / 2.0, sumj);
+    });
     prof_add(c, PROF_EXPV_BOND, pt0);
-    if (threadIdx.x == 0) {
+    if (c.writer()) {
         c.I.node_stats[4 * ops + 2] += na;
         c.I.node_stats[4 * ops + 3] += sumj;
     }
@@ -551,8 +604,9 @@ __device__ void backward(Ctx& c, SweepState& st, bool evolve, int* visits) {
 // <psi|O|psi> for O on the leaf node `leaf` (path contraction to the root, the other
 // subtrees orthonormal as after every sweep) and <psi|psi> = |A_root|^2; uses W1..W4
 __device__ void path_expect(Ctx& c, int leaf, int64_t op_off, double2& val, double& nrm) {
+    Team& T = *c.T;
     const DNode& ln = c.node(leaf);
-    double2 *HA = c.W(1), *M = c.W(2), *T = c.W(3), *M2 = c.W(4);
+    double2 *HA = c.W(1), *M = c.W(2), *Tm = c.W(3), *M2 = c.W(4);
     DK k;
     k.nu = ln.leaf;
     k.kind = 0;
@@ -560,24 +614,24 @@ __device__ void path_expect(Ctx& c, int leaf, int64_t op_off, double2& val, doub
     k.nnz = 0;
     k.mat = op_off;
     k.idx = 0;
-    wg_prim_apply(k, c.base, c.I.ints, c.base + ln.A, HA, ln.hrank);
-    wg_adjoint_mul(c.sh, c.base + ln.A, HA, ln.dimen, ln.hrank, M, cmk(1.0, 0.0), false);
+    t_prim_apply(T, k, c.base, c.I.ints, c.base + ln.A, HA, ln.hrank);
+    t_adjoint_mul(T, c.base + ln.A, HA, ln.dimen, ln.hrank, M, cmk(1.0, 0.0), false);
     int n = leaf;
     while (c.node(n).parent >= 0) {
         const DNode& nd = c.node(n);
         const DNode& pn = c.node(nd.parent);
         int64_t L, R;
         mode_view(pn, nd.child_id, L, R);
-        wg_mode_contract(M, L, pn.dims[nd.child_id], R, c.base + pn.A, T, cmk(1.0, 0.0), false);
-        wg_adjoint_mul(c.sh, c.base + pn.A, T, pn.dimen, pn.hrank, M2, cmk(1.0, 0.0), false);
+        t_mode_contract(T, M, L, pn.dims[nd.child_id], R, c.base + pn.A, Tm, cmk(1.0, 0.0), false);
+        t_adjoint_mul(T, c.base + pn.A, Tm, pn.dimen, pn.hrank, M2, cmk(1.0, 0.0), false);
         double2* tmp = M;
         M = M2;
         M2 = tmp;
         n = nd.parent;
     }
-    val = M[0];
+    val = ldc(M);
     const DNode& root = c.node(0);
-    nrm = block_norm2(c.sh, c.base + root.A, root.size);
+    nrm = team_norm2(T, c.base + root.A, root.size);
 }
 
 __device__ void step_done(Ctx& c, int64_t step_index) {
@@ -585,22 +639,83 @@ __device__ void step_done(Ctx& c, int64_t step_index) {
         double2 v;
         double nrm;
         path_expect(c, c.I.obs_leaf, c.I.obs_op, v, nrm);
-        if (threadIdx.x == 0) {
+        if (c.writer()) {
             c.I.traj[2 * step_index] = v.x;
             c.I.traj[2 * step_index + 1] = nrm;
         }
     }
-    if (threadIdx.x == 0) c.I.counters[1] += 1;
+    if (c.writer()) c.I.counters[1] += 1;
     __syncthreads();
 }
 
-__global__ __launch_bounds__(NT) void tdvp_run_kernel(const DInst* insts, RunArgs a) {
+__device__ __forceinline__ int xcd_id() {
+    // XCC_ID hardware register, bits [3:0]
+    return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20);
+}
+
+// plan == null: workgroup b runs instance b alone.  Otherwise every workgroup takes the
+// next slot of the XCD it runs on (xcd_fill: one counter per 128-byte line) and joins
+// the team that slot names; words[i] are the team words of instance i.
+__global__ __launch_bounds__(NT) void tdvp_run_kernel(const DInst* insts, RunArgs a, const TeamPlan* plan,
+                                                      TeamWords* words, unsigned* xcd_fill) {
     __shared__ Shared sh;
     __shared__ int visits[1024];
-    const DInst& I = insts[blockIdx.x];
+    __shared__ int slot_info[3];
+    if (plan == nullptr) {
+        if (threadIdx.x == 0) {
+            slot_info[0] = blockIdx.x;
+            slot_info[1] = 0;
+            slot_info[2] = 1;
+        }
+    } else if (threadIdx.x == 0) {
+        const int xcc = xcd_id();
+        int inst = -1, rank = 0, size = 1;
+        if (xcc < plan->nxcd) {
+            const unsigned s =
+                __hip_atomic_fetch_add(&xcd_fill[32 * xcc], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (s < static_cast<unsigned>(plan->count[xcc])) {
+                const TeamSlot& t = plan->slot[xcc][s];
+                inst = t.inst;
+                rank = t.rank;
+                size = t.size;
+            }
+        }
+        slot_info[0] = inst;
+        slot_info[1] = rank;
+        slot_info[2] = size;
+    }
     if (threadIdx.x == 0) sh.fail = 0;
     __syncthreads();
-    Ctx c(I, sh);
+    const int inst = slot_info[0], rank = slot_info[1], size = slot_info[2];
+    if (inst < 0) return;
+    const DInst& I = insts[inst];
+    Team T;
+    T.sh = &sh;
+    T.w = size > 1 ? &words[inst] : nullptr;
+    T.rank = rank;
+    T.size = size;
+    T.gen = 0;
+    T.pbuf = 0;
+    T.part = reinterpret_cast<double*>(I.base + I.team_part);
+    T.tiles = I.base + I.team_tiles;
+    if (size > 1) {
+        // roll call: nothing is touched unless the whole team has assembled
+        T.sync();
+        if (threadIdx.x == 0) {
+            if (sh.fail != 0)
+                __hip_atomic_store(&T.w->aborted, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            else if (__hip_atomic_load(&T.w->aborted, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+                sh.fail = FAIL_TEAM_FORM;
+            if (sh.fail != 0) {
+                sh.fail = FAIL_TEAM_FORM;
+                __hip_atomic_store(I.status, static_cast<int>(FAIL_TEAM_FORM), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        __syncthreads();
+        if (sh.fail != 0) return;
+    }
+    Ctx c(I, sh, &T, a.team_min);
     const uint64_t prof_t0 = prof_now(c);
     SweepState st;
     st.coeff = a.coeff;
@@ -628,12 +743,15 @@ __global__ __launch_bounds__(NT) void tdvp_run_kernel(const DInst* insts, RunArg
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        I.counters[0] += st.nh;
-        if (sh.fail) {
-            I.status[0] = sh.fail;
+        // the first member to report a failure records it
+        int expected = 0;
+        if (sh.fail != 0 && __hip_atomic_compare_exchange_strong(I.status, &expected, sh.fail, __ATOMIC_RELAXED,
+                                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
             I.status[1] = static_cast<int>(step_index);
+        if (rank == 0) {
+            I.counters[0] += st.nh;
+            if (I.prof) I.prof[PROF_TOTAL] += static_cast<int64_t>(wall_clock64() - prof_t0);
         }
-        if (I.prof) I.prof[PROF_TOTAL] += static_cast<int64_t>(wall_clock64() - prof_t0);
     }
 }
 
@@ -642,7 +760,16 @@ __global__ __launch_bounds__(NT) void expect_kernel(const DInst* inst, int leaf,
     __shared__ Shared sh;
     if (threadIdx.x == 0) sh.fail = 0;
     __syncthreads();
-    Ctx c(*inst, sh);
+    Team T;
+    T.sh = &sh;
+    T.w = nullptr;
+    T.rank = 0;
+    T.size = 1;
+    T.gen = 0;
+    T.pbuf = 0;
+    T.part = nullptr;
+    T.tiles = nullptr;
+    Ctx c(*inst, sh, &T, 0);
     double2 v;
     double nrm;
     path_expect(c, leaf, op_off, v, nrm);
@@ -656,13 +783,38 @@ __global__ __launch_bounds__(NT) void expect_kernel(const DInst* inst, int leaf,
 }  // namespace ttnd
 
 // ---------------------------------------------------------------- host launchers
-extern "C" hipError_t ttnd_launch_run(const ttnd::DInst* d_insts, int n, const ttnd::RunArgs* a, hipStream_t s) {
-    hipLaunchKernelGGL(ttnd::tdvp_run_kernel, dim3(n), dim3(ttnd::NT), 0, s, d_insts, *a);
+// Team launches start `grid` workgroups (one per CU); plan == null runs n instances one
+// workgroup each.
+extern "C" hipError_t ttnd_launch_run(const ttnd::DInst* d_insts, int n, const ttnd::RunArgs* a,
+                                      const ttnd::TeamPlan* d_plan, ttnd::TeamWords* d_words, unsigned* d_fill,
+                                      int grid, hipStream_t s) {
+    if (d_plan == nullptr)
+        hipLaunchKernelGGL(ttnd::tdvp_run_kernel, dim3(n), dim3(ttnd::NT), 0, s, d_insts, *a, d_plan, d_words, d_fill);
+    else
+        hipLaunchKernelGGL(ttnd::tdvp_run_kernel, dim3(grid), dim3(ttnd::NT), 0, s, d_insts, *a, d_plan, d_words,
+                           d_fill);
     return hipGetLastError();
 }
 
 extern "C" hipError_t ttnd_launch_expect(const ttnd::DInst* d_inst, int leaf, int64_t op_off, double* d_out,
                                          hipStream_t s) {
     hipLaunchKernelGGL(ttnd::expect_kernel, dim3(1), dim3(ttnd::NT), 0, s, d_inst, leaf, op_off, d_out);
+    return hipGetLastError();
+}
+
+// Workgroups of tdvp_run_kernel that fit on one CU at once (the team launch needs every
+// workgroup resident).
+extern "C" hipError_t ttnd_run_occupancy(int* blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, ttnd::tdvp_run_kernel, ttnd::NT, 0);
+}
+
+// XCD of every workgroup of an n-workgroup launch (placement census)
+namespace ttnd {
+__global__ __launch_bounds__(NT) void xcd_census_kernel(int* out) {
+    if (threadIdx.x == 0) out[blockIdx.x] = xcd_id();
+}
+}  // namespace ttnd
+extern "C" hipError_t ttnd_launch_census(int* d_out, int n, hipStream_t s) {
+    hipLaunchKernelGGL(ttnd::xcd_census_kernel, dim3(n), dim3(ttnd::NT), 0, s, d_out);
     return hipGetLastError();
 }

@@ -1,9 +1,16 @@
-// Workgroup-level complex128 primitives for the persistent TDVP sweep.
+// Team-level complex128 primitives for the persistent TDVP sweep.
 //
-// Every function is executed by all NT threads of one workgroup on data in HBM/L2
-// (node tensors, work buffers) or LDS (small Krylov matrices).  Functions that write
-// results other threads read end with __syncthreads(); callers never need their own
-// barrier between two primitives.
+// A team is a set of workgroups that cooperate on one TTNS instance.  All members sit on
+// one XCD, so they share its L2: a member publishes its global stores by waiting for
+// them to complete (the vector L1 is write-through) and reaches the others through an
+// atomic counter in L2; data other members wrote is read with coherent loads (ldc) that
+// bypass the L1.  A team of one is a single workgroup and its barrier is __syncthreads().
+//
+// Every primitive is executed by all threads of all members and spreads rows, or 16x16
+// output tiles, over them.  Primitives whose results other threads read end with a team
+// barrier, so callers need none of their own between two primitives.  Reductions are
+// summed member by member in rank order, so every member obtains bit-identical totals
+// and all members take the same control-flow decisions.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -41,19 +48,18 @@ __device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
     return make_double2((a.x * r + a.y) / den, (a.y * r - a.x) / den);
 }
 
-// ---------------------------------------------------------------- reductions
+// ---------------------------------------------------------------- workgroup state
 struct Shared {
     double red[NWAVE * 2 * (MAXK + 1)];
-    double2 bc[4];      // broadcast slots
-    double2 qtau[64];   // Householder tau (up to 64 columns)
-    double2 qw[64];     // Householder trailing dot products
+    double tsum[2 * MAXK];   // team totals of block reductions
+    double2 qtau[64];        // Householder tau (up to 64 columns)
+    double2 qw[64];          // Householder trailing dot products
     // Krylov small matrices
     double2 Hm[(MAXK + 1) * MAXK];
     double2 S[MAXK * MAXK], F[MAXK * MAXK];
     double2 E[8][MAXK * MAXK];
     double2 lu_f[MAXK];
-    int ipiv;
-    int fail;           // set (uniformly) when a numerical failure is detected
+    int fail;                // failure record, identical on all members after a barrier
     double hb[2][2 * MAXK];  // Gram-Schmidt coefficients of the two CGS passes
 };
 
@@ -115,16 +121,142 @@ __device__ double block_max(Shared& sh, double v) {
     return m;
 }
 
-__device__ double block_norm2(Shared& sh, const double2* x, int64_t n) {
+// ---------------------------------------------------------------- teams
+struct Team {
+    Shared* sh;
+    TeamWords* w;       // null for a team of one
+    int rank, size;
+    unsigned gen;       // barriers completed by this member
+    int pbuf;           // which of the two partial-sum buffers the next reduction uses
+    double* part;       // 2 x TEAM_MAX x PART_K doubles
+    double2* tiles;     // TEAM_MAX x TILE_MAX partial products
+
+    __device__ int64_t tid() const { return static_cast<int64_t>(rank) * NT + threadIdx.x; }
+    __device__ int64_t nthr() const { return static_cast<int64_t>(size) * NT; }
+    __device__ int64_t gwave() const { return static_cast<int64_t>(rank) * NWAVE + (threadIdx.x >> 6); }
+    __device__ int64_t nwave() const { return static_cast<int64_t>(size) * NWAVE; }
+    // the leader alone, as a team of one
+    __device__ Team single() const {
+        Team s = *this;
+        s.w = nullptr;
+        s.rank = 0;
+        s.size = 1;
+        return s;
+    }
+
+    // Team barrier.  Release: each wave waits for its global stores to complete (they are
+    // then in the shared L2).  Arrival on the L2 counter; the last arrival publishes the
+    // generation.  No acquire step is needed: shared data is only read with coherent
+    // (L1-bypassing) loads, see ldc().
+    // A barrier that does not complete within TEAM_TIMEOUT records FAIL_TEAM_TIMEOUT and
+    // returns, so a broken team ends instead of hanging.  The team-wide failure record is
+    // copied into every member's LDS flag.  A member that has timed out no longer waits,
+    // so a broken team drains instead of timing out barrier after barrier.
+    __device__ void sync() {
+        if (size == 1 || sh->fail == FAIL_TEAM_TIMEOUT) {
+            __syncthreads();
+            return;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned target = gen + 1;
+            const unsigned old = __hip_atomic_fetch_add(&w->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (old + 1 == target * static_cast<unsigned>(size)) {
+                __hip_atomic_store(&w->gen, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                const uint64_t t0 = wall_clock64();
+                while (__hip_atomic_load(&w->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (wall_clock64() - t0 > TEAM_TIMEOUT) {
+                        sh->fail = FAIL_TEAM_TIMEOUT;
+                        break;
+                    }
+                }
+            }
+            const int f = __hip_atomic_load(&w->fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (f != 0 && sh->fail == 0) sh->fail = f;
+        }
+        ++gen;
+        __syncthreads();
+    }
+
+    // record a failure on this member and in the team word (read by the others at their
+    // next barrier)
+    __device__ void fail(int code) {
+        sh->fail = code;
+        if (w != nullptr && threadIdx.x == 0)
+            __hip_atomic_store(&w->fail, code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+};
+
+// Every member holds `count` partial values in the LDS array v (complete: the caller has
+// passed a workgroup barrier); afterwards v holds the team totals, summed in rank order.
+__device__ void team_reduce_lds(Team& T, double* v, int count) {
+    if (T.size == 1) return;
+    double* mine = T.part + (static_cast<int64_t>(T.pbuf) * TEAM_MAX + T.rank) * PART_K;
+    for (int e = threadIdx.x; e < count; e += NT) mine[e] = v[e];
+    T.sync();
+    const double* all = T.part + static_cast<int64_t>(T.pbuf) * TEAM_MAX * PART_K;
+    for (int e = threadIdx.x; e < count; e += NT) {
+        double s = 0.0;
+        for (int r = 0; r < T.size; ++r) s += ldc(all + r * PART_K + e);
+        v[e] = s;
+    }
+    __syncthreads();
+    T.pbuf ^= 1;
+}
+
+// team sum of K doubles; every thread receives the totals in out[]
+template <int K>
+__device__ void team_sum(Team& T, double (&v)[K], double (&out)[K]) {
+    if (T.size == 1) {
+        block_sum<K>(*T.sh, v, out);
+        return;
+    }
+    block_sum_lds<K>(*T.sh, v, T.sh->tsum);
+    team_reduce_lds(T, T.sh->tsum, K);
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = T.sh->tsum[k];
+    __syncthreads();
+}
+
+// team sum of K doubles, totals left in the LDS array dst[K]
+template <int K>
+__device__ void team_sum_lds(Team& T, double (&v)[K], double* dst) {
+    block_sum_lds<K>(*T.sh, v, dst);
+    team_reduce_lds(T, dst, K);
+}
+
+__device__ double team_max(Team& T, double v) {
+    double m = block_max(*T.sh, v);
+    if (T.size == 1) return m;
+    if (threadIdx.x == 0) T.sh->tsum[0] = m;
+    __syncthreads();
+    // maxima are exact, so a second reduction buffer is not needed for determinism
+    double* mine = T.part + (static_cast<int64_t>(T.pbuf) * TEAM_MAX + T.rank) * PART_K;
+    if (threadIdx.x == 0) mine[0] = m;
+    T.sync();
+    const double* all = T.part + static_cast<int64_t>(T.pbuf) * TEAM_MAX * PART_K;
+    m = ldc(all);
+    for (int r = 1; r < T.size; ++r) m = fmax(m, ldc(all + r * PART_K));
+    __syncthreads();
+    T.pbuf ^= 1;
+    return m;
+}
+
+__device__ double team_norm2(Team& T, const double2* x, int64_t n) {
     double p[1] = {0.0}, o[1];
-    for (int64_t b = threadIdx.x; b < n; b += 4 * NT) {
+    const int64_t st = T.nthr();
+    for (int64_t b = T.tid(); b < n; b += 4 * st) {
         double2 v[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = (b + u * NT < n) ? x[b + u * NT] : cz();
+        for (int u = 0; u < 4; ++u) v[u] = (b + u * st < n) ? ldc(x + b + u * st) : cz();
 #pragma unroll
         for (int u = 0; u < 4; ++u) p[0] += cabs2(v[u]);
     }
-    block_sum<1>(sh, p, o);
+    team_sum<1>(T, p, o);
     return o[0];
 }
 
@@ -133,61 +265,63 @@ __device__ double block_norm2(Shared& sh, const double2* x, int64_t n) {
 // of UNR consecutive strides before using any of them.
 constexpr int UNR = 4;
 
-__device__ void wg_zero(double2* y, int64_t n) {
-    for (int64_t i = threadIdx.x; i < n; i += NT) y[i] = cz();
-    __syncthreads();
+__device__ void t_zero(Team& T, double2* y, int64_t n) {
+    for (int64_t i = T.tid(); i < n; i += T.nthr()) y[i] = cz();
+    T.sync();
 }
-__device__ void wg_copy(double2* __restrict__ y, const double2* __restrict__ x, int64_t n) {
-    for (int64_t b = threadIdx.x; b < n; b += UNR * NT) {
+__device__ void t_copy(Team& T, double2* __restrict__ y, const double2* __restrict__ x, int64_t n) {
+    const int64_t st = T.nthr();
+    for (int64_t b = T.tid(); b < n; b += UNR * st) {
         double2 v[UNR];
 #pragma unroll
-        for (int u = 0; u < UNR; ++u) v[u] = (b + u * NT < n) ? x[b + u * NT] : cz();
+        for (int u = 0; u < UNR; ++u) v[u] = (b + u * st < n) ? ldc(x + b + u * st) : cz();
 #pragma unroll
         for (int u = 0; u < UNR; ++u)
-            if (b + u * NT < n) y[b + u * NT] = v[u];
+            if (b + u * st < n) y[b + u * st] = v[u];
     }
-    __syncthreads();
+    T.sync();
 }
 // y += a * x
-__device__ void wg_axpy(double2* __restrict__ y, double2 a, const double2* __restrict__ x, int64_t n) {
-    for (int64_t b = threadIdx.x; b < n; b += UNR * NT) {
+__device__ void t_axpy(Team& T, double2* __restrict__ y, double2 a, const double2* __restrict__ x, int64_t n) {
+    const int64_t st = T.nthr();
+    for (int64_t b = T.tid(); b < n; b += UNR * st) {
         double2 v[UNR], o[UNR];
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
-            const bool ok = b + u * NT < n;
-            v[u] = ok ? x[b + u * NT] : cz();
-            o[u] = ok ? y[b + u * NT] : cz();
+            const bool ok = b + u * st < n;
+            v[u] = ok ? ldc(x + b + u * st) : cz();
+            o[u] = ok ? ldc(y + b + u * st) : cz();
         }
 #pragma unroll
         for (int u = 0; u < UNR; ++u)
-            if (b + u * NT < n) y[b + u * NT] = cadd(o[u], cmul(a, v[u]));
+            if (b + u * st < n) y[b + u * st] = cadd(o[u], cmul(a, v[u]));
     }
-    __syncthreads();
+    T.sync();
 }
 // y = a * x
-__device__ void wg_scale_to(double2* __restrict__ y, double2 a, const double2* __restrict__ x, int64_t n) {
-    for (int64_t b = threadIdx.x; b < n; b += UNR * NT) {
+__device__ void t_scale_to(Team& T, double2* __restrict__ y, double2 a, const double2* __restrict__ x, int64_t n) {
+    const int64_t st = T.nthr();
+    for (int64_t b = T.tid(); b < n; b += UNR * st) {
         double2 v[UNR];
 #pragma unroll
-        for (int u = 0; u < UNR; ++u) v[u] = (b + u * NT < n) ? x[b + u * NT] : cz();
+        for (int u = 0; u < UNR; ++u) v[u] = (b + u * st < n) ? ldc(x + b + u * st) : cz();
 #pragma unroll
         for (int u = 0; u < UNR; ++u)
-            if (b + u * NT < n) y[b + u * NT] = cmul(a, v[u]);
+            if (b + u * st < n) y[b + u * st] = cmul(a, v[u]);
     }
-    __syncthreads();
+    T.sync();
 }
 
 // ---------------------------------------------------------------- contractions (FP64 MFMA)
 // dst(l,i,r) (=|+=) alpha * sum_j op(i,j) src(l,j,r) on the (L, d, R) view.
 // Reference: contract(op, 1, A3, 1) in kronecker_product_operator_helper.hpp:292-303.
 // One 16x16 tile of (i, r) per wave work item; op rows come from L1.
-__device__ void wg_mode_contract(const double2* op, int64_t L, int d, int64_t R, const double2* src, double2* dst,
-                                 double2 alpha, bool accumulate) {
-    const int wv = threadIdx.x >> 6;
+__device__ void t_mode_contract(Team& T, const double2* op, int64_t L, int d, int64_t R, const double2* src,
+                                double2* dst, double2 alpha, bool accumulate) {
     const int mt = (d + 15) >> 4;
     const int64_t nt = (R + 15) >> 4;
     const int64_t items = L * mt * nt;
-    for (int64_t it = wv; it < items; it += NWAVE) {
+    for (int64_t it = T.gwave(); it < items; it += T.nwave()) {
         const int64_t n = it % nt;
         const int m = static_cast<int>((it / nt) % mt);
         const int64_t l = it / (nt * mt);
@@ -200,17 +334,16 @@ __device__ void wg_mode_contract(const double2* op, int64_t L, int d, int64_t R,
         ztile_store(acc, dst + l * d * R + static_cast<int64_t>(16 * m) * R + 16 * n, R, 1, mv, nv, alpha,
                     accumulate);
     }
-    __syncthreads();
+    T.sync();
 }
 
 // dst (=|+=) alpha * X M^T, X: rows x r, M: r x r  (v * trans(mf))
-__device__ void wg_mul_transposed(const double2* X, int64_t rows, int r, const double2* M, double2* dst,
-                                  double2 alpha, bool accumulate) {
-    const int wv = threadIdx.x >> 6;
+__device__ void t_mul_transposed(Team& T, const double2* X, int64_t rows, int r, const double2* M, double2* dst,
+                                 double2 alpha, bool accumulate) {
     const int64_t mt = (rows + 15) >> 4;
     const int nt = (r + 15) >> 4;
     const int64_t items = mt * nt;
-    for (int64_t it = wv; it < items; it += NWAVE) {
+    for (int64_t it = T.gwave(); it < items; it += T.nwave()) {
         const int n = static_cast<int>(it % nt);
         const int64_t m = it / nt;
         ZAcc acc;
@@ -220,16 +353,15 @@ __device__ void wg_mul_transposed(const double2* X, int64_t rows, int r, const d
         ztile<false>(acc, X + 16 * m * r, r, 1, mv, M + static_cast<int64_t>(16 * n) * r, 1, r, nv, r);
         ztile_store(acc, dst + 16 * m * r + 16 * n, r, 1, mv, nv, alpha, accumulate);
     }
-    __syncthreads();
+    T.sync();
 }
 
 // dst = X M, X: rows x r, M: r x r
-__device__ void wg_mul(const double2* X, int64_t rows, int r, const double2* M, double2* dst) {
-    const int wv = threadIdx.x >> 6;
+__device__ void t_mul(Team& T, const double2* X, int64_t rows, int r, const double2* M, double2* dst) {
     const int64_t mt = (rows + 15) >> 4;
     const int nt = (r + 15) >> 4;
     const int64_t items = mt * nt;
-    for (int64_t it = wv; it < items; it += NWAVE) {
+    for (int64_t it = T.gwave(); it < items; it += T.nwave()) {
         const int n = static_cast<int>(it % nt);
         const int64_t m = it / nt;
         ZAcc acc;
@@ -239,14 +371,15 @@ __device__ void wg_mul(const double2* X, int64_t rows, int r, const double2* M, 
         ztile<false>(acc, X + 16 * m * r, r, 1, mv, M + 16 * n, r, 1, nv, r);
         ztile_store(acc, dst + 16 * m * r + 16 * n, r, 1, mv, nv, make_double2(1.0, 0.0), false);
     }
-    __syncthreads();
+    T.sync();
 }
 
-// out(i,j) (=|+=) alpha * sum_{l,r} conj(A(l,i,r)) B(l,j,r) on the (L, d, R) view: a
-// long-K product.  With several output tiles the waves take whole tiles; with a single
-// tile (d <= 16) they split K and the partial tiles are summed through LDS.
+// out(i,j) (=|+=) alpha * sum_{l,r} conj(A(l,i,r)) B(l,j,r) on the (L, d, R) view with
+// element (l, i, r) at (l*d + i)*ldr + r, by one workgroup: a long-K product.  With
+// several output tiles the waves take whole tiles; with a single tile (d <= 16) they
+// split K and the partial tiles are summed through LDS.  out is d x d.
 __device__ void wg_outer_contract(Shared& sh, const double2* A, const double2* B, int64_t L, int d, int64_t R,
-                                  double2* out, double2 alpha, bool accumulate) {
+                                  int64_t ldr, double2* out, double2 alpha, bool accumulate) {
     const int wv = threadIdx.x >> 6;
     const int mt = (d + 15) >> 4;
     const int tiles = mt * mt;
@@ -259,27 +392,23 @@ __device__ void wg_outer_contract(Shared& sh, const double2* A, const double2* B
         const int mv = t < tiles ? min(16, d - 16 * m) : 0;
         const int nv = t < tiles ? min(16, d - 16 * n) : 0;
         if (t < tiles) {
-            if (R == 1) {
+            if (R == 1 && ldr == 1) {
                 // element (l, i) at l*d + i: K runs over l
                 const int64_t k0 = split ? (L * wv) / NWAVE : 0, k1 = split ? (L * (wv + 1)) / NWAVE : L;
                 ztile<true>(acc, A + k0 * d + 16 * m, 1, d, mv, B + k0 * d + 16 * n, d, 1, nv, k1 - k0);
-            } else {
+            } else if (!split || L >= NWAVE) {
                 for (int64_t l = split ? wv : 0; l < L; l += split ? NWAVE : 1) {
-                    const double2* a = A + l * d * R + static_cast<int64_t>(16 * m) * R;
-                    const double2* b = B + l * d * R + static_cast<int64_t>(16 * n) * R;
-                    if (split && L < NWAVE) {
-                        // too few slices: split each slice's K range instead (handled below)
-                        break;
-                    }
-                    ztile<true>(acc, a, R, 1, mv, b, 1, R, nv, R);
+                    const double2* a = A + (l * d + 16 * m) * ldr;
+                    const double2* b = B + (l * d + 16 * n) * ldr;
+                    ztile<true>(acc, a, ldr, 1, mv, b, 1, ldr, nv, R);
                 }
-                if (split && L < NWAVE) {
-                    for (int64_t l = 0; l < L; ++l) {
-                        const int64_t k0 = (R * wv) / NWAVE, k1 = (R * (wv + 1)) / NWAVE;
-                        const double2* a = A + l * d * R + static_cast<int64_t>(16 * m) * R + k0;
-                        const double2* b = B + l * d * R + static_cast<int64_t>(16 * n) * R + k0;
-                        ztile<true>(acc, a, R, 1, mv, b, 1, R, nv, k1 - k0);
-                    }
+            } else {
+                // too few slices: split each slice's K range over the waves instead
+                const int64_t k0 = (R * wv) / NWAVE, k1 = (R * (wv + 1)) / NWAVE;
+                for (int64_t l = 0; l < L; ++l) {
+                    const double2* a = A + (l * d + 16 * m) * ldr + k0;
+                    const double2* b = B + (l * d + 16 * n) * ldr + k0;
+                    ztile<true>(acc, a, ldr, 1, mv, b, 1, ldr, nv, k1 - k0);
                 }
             }
         }
@@ -315,37 +444,65 @@ __device__ void wg_outer_contract(Shared& sh, const double2* A, const double2* B
     __syncthreads();
 }
 
+// Team form: the K range (slices l, or the r range when there are fewer slices than
+// members) is split over the members, each member writes its partial d x d product to
+// its tile buffer, and the partials are summed in rank order.
+__device__ void t_outer_contract(Team& T, const double2* A, const double2* B, int64_t L, int d, int64_t R,
+                                 double2* out, double2 alpha, bool accumulate) {
+    if (T.size == 1) {
+        wg_outer_contract(*T.sh, A, B, L, d, R, R, out, alpha, accumulate);
+        return;
+    }
+    double2* mine = T.tiles + static_cast<int64_t>(T.rank) * TILE_MAX;
+    if (L >= T.size) {
+        const int64_t l0 = L * T.rank / T.size, l1 = L * (T.rank + 1) / T.size;
+        wg_outer_contract(*T.sh, A + l0 * d * R, B + l0 * d * R, l1 - l0, d, R, R, mine, make_double2(1.0, 0.0),
+                          false);
+    } else {
+        const int64_t r0 = R * T.rank / T.size, r1 = R * (T.rank + 1) / T.size;
+        wg_outer_contract(*T.sh, A + r0, B + r0, L, d, r1 - r0, R, mine, make_double2(1.0, 0.0), false);
+    }
+    T.sync();
+    const int64_t dd = static_cast<int64_t>(d) * d;
+    for (int64_t e = T.tid(); e < dd; e += T.nthr()) {
+        double2 s = cz();
+        for (int r = 0; r < T.size; ++r) s = cadd(s, ldc(T.tiles + static_cast<int64_t>(r) * TILE_MAX + e));
+        const double2 v = cmul(alpha, s);
+        out[e] = accumulate ? cadd(ldc(out + e), v) : v;
+    }
+    T.sync();
+}
+
 // out = A^H B for A, B: rows x r
-__device__ void wg_adjoint_mul(Shared& sh, const double2* A, const double2* B, int64_t rows, int r, double2* out,
-                               double2 alpha, bool accumulate) {
-    wg_outer_contract(sh, A, B, rows, r, 1, out, alpha, accumulate);
+__device__ void t_adjoint_mul(Team& T, const double2* A, const double2* B, int64_t rows, int r, double2* out,
+                              double2 alpha, bool accumulate) {
+    t_outer_contract(T, A, B, rows, r, 1, out, alpha, accumulate);
 }
 
 // ---------------------------------------------------------------- primitive leaf operators
-// y = op x, x/y: dim x ncols row-major; kind: 0 dense, 1 diag, 2 csr
-// y (=|+=) alpha * op x
-__device__ void wg_prim_apply(const DK& k, const double2* base, const int64_t* ints, const double2* x, double2* y,
-                              int ncols, double2 alpha = make_double2(1.0, 0.0), bool accumulate = false) {
+// y (=|+=) alpha * op x, x/y: dim x ncols row-major; kind: 0 dense, 1 diag, 2 csr
+__device__ void t_prim_apply(Team& T, const DK& k, const double2* base, const int64_t* ints, const double2* x,
+                             double2* y, int ncols, double2 alpha = make_double2(1.0, 0.0), bool accumulate = false) {
     const int d = k.dim;
     const double2* mat = base + k.mat;
     const int64_t total = static_cast<int64_t>(d) * ncols;
-    for (int64_t idx = threadIdx.x; idx < total; idx += NT) {
+    for (int64_t idx = T.tid(); idx < total; idx += T.nthr()) {
         const int i = static_cast<int>(idx / ncols), c = static_cast<int>(idx % ncols);
         double2 acc = cz();
         if (k.kind == 0) {
-            for (int j = 0; j < d; ++j) acc = cadd(acc, cmul(mat[i * d + j], x[static_cast<int64_t>(j) * ncols + c]));
+            for (int j = 0; j < d; ++j) acc = cadd(acc, cmul(mat[i * d + j], ldc(x + static_cast<int64_t>(j) * ncols + c)));
         } else if (k.kind == 1) {
-            acc = cmul(mat[i], x[idx]);
+            acc = cmul(mat[i], ldc(x + idx));
         } else {
             const int64_t* rowptr = ints + k.idx;
             const int64_t* colind = rowptr + d + 1;
             for (int64_t q = rowptr[i]; q < rowptr[i + 1]; ++q)
-                acc = cadd(acc, cmul(mat[q], x[colind[q] * ncols + c]));
+                acc = cadd(acc, cmul(mat[q], ldc(x + colind[q] * ncols + c)));
         }
         acc = cmul(alpha, acc);
-        y[idx] = accumulate ? cadd(y[idx], acc) : acc;
+        y[idx] = accumulate ? cadd(ldc(y + idx), acc) : acc;
     }
-    __syncthreads();
+    T.sync();
 }
 
 // ---------------------------------------------------------------- safe norms (LAPACK dznrm2/dlapy3 roles)
@@ -355,25 +512,25 @@ constexpr double kNrmTiny = 1e-280, kNrmHuge = 1e280;
 // safmin = (smallest normal) / (unit roundoff) = 2^-969, as in zlarfg
 constexpr double kSafmin = 2.2250738585072014e-308 / 1.1102230246251565e-16;
 
-// ||A(i+1:m, i)||_2 of a row-major m x n matrix
-__device__ double wg_col_nrm2(Shared& sh, const double2* A, int i, int m, int n) {
+// ||A(lo:hi, i)||_2 over the rows this member owns, totalled over the team
+__device__ double t_col_nrm2(Team& T, const double2* A, int i, int lo, int hi, int n) {
     double p[1] = {0.0}, o[1];
-    for (int rr = i + 1 + threadIdx.x; rr < m; rr += NT) p[0] += cabs2(A[static_cast<int64_t>(rr) * n + i]);
-    block_sum<1>(sh, p, o);
+    for (int rr = lo + threadIdx.x; rr < hi; rr += NT) p[0] += cabs2(ldc(A + static_cast<int64_t>(rr) * n + i));
+    team_sum<1>(T, p, o);
     if (o[0] >= kNrmTiny && o[0] <= kNrmHuge) return sqrt(o[0]);
     double mx = 0.0;
-    for (int rr = i + 1 + threadIdx.x; rr < m; rr += NT) {
-        const double2 v = A[static_cast<int64_t>(rr) * n + i];
+    for (int rr = lo + threadIdx.x; rr < hi; rr += NT) {
+        const double2 v = ldc(A + static_cast<int64_t>(rr) * n + i);
         mx = fmax(mx, fmax(fabs(v.x), fabs(v.y)));
     }
-    const double scale = block_max(sh, mx);
+    const double scale = team_max(T, mx);
     if (scale == 0.0) return 0.0;
     p[0] = 0.0;
-    for (int rr = i + 1 + threadIdx.x; rr < m; rr += NT) {
-        const double2 v = A[static_cast<int64_t>(rr) * n + i];
+    for (int rr = lo + threadIdx.x; rr < hi; rr += NT) {
+        const double2 v = ldc(A + static_cast<int64_t>(rr) * n + i);
         p[0] += cabs2(make_double2(v.x / scale, v.y / scale));
     }
-    block_sum<1>(sh, p, o);
+    team_sum<1>(T, p, o);
     return scale * sqrt(o[0]);
 }
 
@@ -388,15 +545,22 @@ __device__ __forceinline__ double lapy3(double a, double b, double c) {
 // ---------------------------------------------------------------- Householder QR
 // A (m x n, row-major, destroyed: holds the reflectors) = Q (m x n) R (n x n).
 // LAPACK zgeqr2/zlarfg/zung2r conventions; for m < n, Q gets zero columns and R zero
-// rows beyond m (decomposition_engine.hpp:124-158).
-__device__ void wg_householder_qr(Shared& sh, double2* A, int m, int n, double2* Q, double2* Rout) {
+// rows beyond m (decomposition_engine.hpp:124-158).  Each member owns a fixed block of
+// rows for the whole factorisation, so a column costs two team reductions (its norm and
+// the trailing dot products) and forming Q one per reflector.
+__device__ void t_householder_qr(Team& T, double2* A, int m, int n, double2* Q, double2* Rout) {
+    Shared& sh = *T.sh;
     const int k = m < n ? m : n;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int r0 = static_cast<int>(static_cast<int64_t>(m) * T.rank / T.size);
+    const int r1 = static_cast<int>(static_cast<int64_t>(m) * (T.rank + 1) / T.size);
     for (int i = 0; i < k; ++i) {
+        const int lo = r0 > i + 1 ? r0 : i + 1;  // own rows strictly below the diagonal
+        const bool own_i = i >= r0 && i < r1;
         // reflector of column i (zlarfg): beta from a scaled norm; tiny columns are
         // rescaled by 1/safmin until |beta| >= safmin
-        double xnorm = wg_col_nrm2(sh, A, i, m, n);
-        const double2 alpha = A[static_cast<int64_t>(i) * n + i];
+        double xnorm = t_col_nrm2(T, A, i, lo, r1, n);
+        const double2 alpha = ldc(A + static_cast<int64_t>(i) * n + i);  // published by the barrier above
         double ar = alpha.x, ai = alpha.y;
         double2 tau = cz(), scal = cz();
         double beta = ar;
@@ -407,16 +571,16 @@ __device__ void wg_householder_qr(Shared& sh, double2* A, int m, int n, double2*
             if (fabs(beta) < kSafmin) {
                 do {
                     ++knt;
-                    for (int rr = i + 1 + threadIdx.x; rr < m; rr += NT) {
-                        double2& v = A[static_cast<int64_t>(rr) * n + i];
-                        v = cscale(v, 1.0 / kSafmin);
+                    for (int rr = lo + threadIdx.x; rr < r1; rr += NT) {
+                        double2* v = A + static_cast<int64_t>(rr) * n + i;
+                        *v = cscale(ldc(v), 1.0 / kSafmin);
                     }
                     beta *= 1.0 / kSafmin;
                     ar *= 1.0 / kSafmin;
                     ai *= 1.0 / kSafmin;
                 } while (fabs(beta) < kSafmin && knt < 20);
                 __syncthreads();
-                xnorm = wg_col_nrm2(sh, A, i, m, n);
+                xnorm = t_col_nrm2(T, A, i, lo, r1, n);
                 beta = -copysign(lapy3(ar, ai, xnorm), ar);
             }
             tau = make_double2((beta - ar) / beta, -ai / beta);
@@ -424,199 +588,85 @@ __device__ void wg_householder_qr(Shared& sh, double2* A, int m, int n, double2*
         }
         if (threadIdx.x == 0) sh.qtau[i] = tau;
         if (tau.x != 0.0 || tau.y != 0.0) {
-            for (int rr = i + 1 + threadIdx.x; rr < m; rr += NT) {
-                double2& v = A[static_cast<int64_t>(rr) * n + i];
-                v = cmul(v, scal);
+            for (int rr = lo + threadIdx.x; rr < r1; rr += NT) {
+                double2* v = A + static_cast<int64_t>(rr) * n + i;
+                *v = cmul(ldc(v), scal);
             }
             __syncthreads();
-            // w_c = v^H A(i:m, c) with v0 = 1, for c > i; one wave per column
+            // w_c = v^H A(i:m, c) with v_i = 1, for c > i: one wave per column over the
+            // member's rows, then a team reduction
             for (int c = i + 1 + wv; c < n; c += NWAVE) {
                 double2 acc = cz();
-                for (int rr = i + 1 + lane; rr < m; rr += 64)
-                    acc = cadd(acc, cmulc(A[static_cast<int64_t>(rr) * n + i], A[static_cast<int64_t>(rr) * n + c]));
+                for (int rr = lo + lane; rr < r1; rr += 64)
+                    acc = cadd(acc, cmulc(ldc(A + static_cast<int64_t>(rr) * n + i), ldc(A + static_cast<int64_t>(rr) * n + c)));
                 acc.x = wave_sum(acc.x);
                 acc.y = wave_sum(acc.y);
-                if (lane == 0) sh.qw[c] = cadd(acc, A[static_cast<int64_t>(i) * n + c]);
+                if (lane == 0) sh.qw[c] = own_i ? cadd(acc, ldc(A + static_cast<int64_t>(i) * n + c)) : acc;
             }
             __syncthreads();
+            team_reduce_lds(T, reinterpret_cast<double*>(&sh.qw[i + 1]), 2 * (n - i - 1));
             const double2 tc = make_double2(tau.x, -tau.y);
             const int ncol = n - i - 1;
-            const int64_t tot = static_cast<int64_t>(m - i) * ncol;
+            const int rs = r0 > i ? r0 : i;
+            const int64_t tot = static_cast<int64_t>(r1 > rs ? r1 - rs : 0) * ncol;
             for (int64_t e = threadIdx.x; e < tot; e += NT) {
-                const int rr = i + static_cast<int>(e / ncol);
+                const int rr = rs + static_cast<int>(e / ncol);
                 const int c = i + 1 + static_cast<int>(e % ncol);
-                const double2 v = rr == i ? make_double2(1.0, 0.0) : A[static_cast<int64_t>(rr) * n + i];
-                double2& a = A[static_cast<int64_t>(rr) * n + c];
-                a = csub(a, cmul(tc, cmul(v, sh.qw[c])));
+                const double2 v = rr == i ? make_double2(1.0, 0.0) : ldc(A + static_cast<int64_t>(rr) * n + i);
+                double2* a = A + static_cast<int64_t>(rr) * n + c;
+                *a = csub(ldc(a), cmul(tc, cmul(v, sh.qw[c])));
             }
         }
         __syncthreads();
-        if (refl) {
+        if (refl && own_i) {
             for (int j = 0; j < knt; ++j) beta *= kSafmin;
             if (threadIdx.x == 0) A[static_cast<int64_t>(i) * n + i] = make_double2(beta, 0.0);
         }
         __syncthreads();
     }
+    T.sync();
     // R
-    for (int64_t e = threadIdx.x; e < static_cast<int64_t>(n) * n; e += NT) {
+    for (int64_t e = T.tid(); e < static_cast<int64_t>(n) * n; e += T.nthr()) {
         const int i = static_cast<int>(e / n), j = static_cast<int>(e % n);
-        Rout[e] = (i < k && j >= i) ? A[static_cast<int64_t>(i) * n + j] : cz();
+        Rout[e] = (i < k && j >= i) ? ldc(A + static_cast<int64_t>(i) * n + j) : cz();
     }
-    // Q = H_0 ... H_{k-1} [I_k; 0]
-    for (int64_t e = threadIdx.x; e < static_cast<int64_t>(m) * n; e += NT) {
-        const int i = static_cast<int>(e / n), j = static_cast<int>(e % n);
-        Q[e] = (i == j && j < k) ? make_double2(1.0, 0.0) : cz();
+    // Q = H_0 ... H_{k-1} [I_k; 0] on the member's rows
+    for (int64_t e = threadIdx.x; e < static_cast<int64_t>(r1 - r0) * n; e += NT) {
+        const int i = r0 + static_cast<int>(e / n), j = static_cast<int>(e % n);
+        Q[static_cast<int64_t>(i) * n + j] = (i == j && j < k) ? make_double2(1.0, 0.0) : cz();
     }
     __syncthreads();
     for (int i = k - 1; i >= 0; --i) {
         const double2 tau = sh.qtau[i];
         if (tau.x == 0.0 && tau.y == 0.0) continue;
+        const int lo = r0 > i + 1 ? r0 : i + 1;
+        const bool own_i = i >= r0 && i < r1;
         for (int c = wv; c < k; c += NWAVE) {
             double2 acc = cz();
-            for (int rr = i + 1 + lane; rr < m; rr += 64)
-                acc = cadd(acc, cmulc(A[static_cast<int64_t>(rr) * n + i], Q[static_cast<int64_t>(rr) * n + c]));
+            for (int rr = lo + lane; rr < r1; rr += 64)
+                acc = cadd(acc, cmulc(ldc(A + static_cast<int64_t>(rr) * n + i), ldc(Q + static_cast<int64_t>(rr) * n + c)));
             acc.x = wave_sum(acc.x);
             acc.y = wave_sum(acc.y);
-            if (lane == 0) sh.qw[c] = cmul(tau, cadd(acc, Q[static_cast<int64_t>(i) * n + c]));
+            if (lane == 0) sh.qw[c] = own_i ? cadd(acc, ldc(Q + static_cast<int64_t>(i) * n + c)) : acc;
         }
         __syncthreads();
-        const int64_t tot = static_cast<int64_t>(m - i) * k;
+        team_reduce_lds(T, reinterpret_cast<double*>(&sh.qw[0]), 2 * k);
+        const int rs = r0 > i ? r0 : i;
+        const int64_t tot = static_cast<int64_t>(r1 > rs ? r1 - rs : 0) * k;
         for (int64_t e = threadIdx.x; e < tot; e += NT) {
-            const int rr = i + static_cast<int>(e / k);
+            const int rr = rs + static_cast<int>(e / k);
             const int c = static_cast<int>(e % k);
-            const double2 v = rr == i ? make_double2(1.0, 0.0) : A[static_cast<int64_t>(rr) * n + i];
-            double2& q = Q[static_cast<int64_t>(rr) * n + c];
-            q = csub(q, cmul(v, sh.qw[c]));
+            const double2 v = rr == i ? make_double2(1.0, 0.0) : ldc(A + static_cast<int64_t>(rr) * n + i);
+            double2* q = Q + static_cast<int64_t>(rr) * n + c;
+            *q = csub(ldc(q), cmul(v, cmul(tau, sh.qw[c])));
         }
         __syncthreads();
     }
+    T.sync();
 }
-
-// ---------------------------------------------------------------- small dense expm (LDS)
-// C = A B for n x n matrices in LDS, n*n <= NT
-__device__ void sm_matmul(const double2* A, const double2* B, double2* C, int n) {
-    const int t = threadIdx.x;
-    double2 acc = cz();
-    if (t < n * n) {
-        const int i = t / n, j = t % n;
-        for (int q = 0; q < n; ++q) acc = cadd(acc, cmul(A[i * n + q], B[q * n + j]));
-    }
-    __syncthreads();
-    if (t < n * n) C[t] = acc;
-    __syncthreads();
-}
-
-// E = exp(S), Pade(13) scaling and squaring (Higham 2005); S, E in LDS; uses sh.E[*]
-__device__ void sm_expm(Shared& sh, const double2* S, double2* Eout, int n) {
-    const double b[14] = {64764752532480000.0, 32382376266240000.0, 7771770303897600.0, 1187353796428800.0,
-                          129060195264000.0,   10559470521600.0,   670442572800.0,     33522128640.0,
-                          1323241920.0,        40840800.0,         960960.0,           16380.0,
-                          182.0,               1.0};
-    const double theta13 = 5.371920351148152;
-    const int t = threadIdx.x, nn = n * n;
-    double2 *A = sh.E[0], *A2 = sh.E[1], *A4 = sh.E[2], *A6 = sh.E[3], *T1 = sh.E[4], *T2 = sh.E[5], *U = sh.E[6],
-            *V = sh.E[7];
-    // 1-norm and scaling (every thread computes the same value from LDS)
-    double norm1 = 0.0;
-    for (int j = 0; j < n; ++j) {
-        double s = 0.0;
-        for (int i = 0; i < n; ++i) s += cabsv(S[i * n + j]);
-        norm1 = fmax(norm1, s);
-    }
-    int sc = 0;
-    if (norm1 > theta13) sc = static_cast<int>(ceil(log2(norm1 / theta13)));
-    const double scale = ldexp(1.0, -sc);
-    if (t < nn) A[t] = cscale(S[t], scale);
-    __syncthreads();
-    sm_matmul(A, A, A2, n);
-    sm_matmul(A2, A2, A4, n);
-    sm_matmul(A4, A2, A6, n);
-    if (t < nn) T1[t] = cadd(cadd(cscale(A6[t], b[13]), cscale(A4[t], b[11])), cscale(A2[t], b[9]));
-    __syncthreads();
-    sm_matmul(A6, T1, T2, n);
-    if (t < nn) {
-        double2 v = cadd(T2[t], cadd(cadd(cscale(A6[t], b[7]), cscale(A4[t], b[5])), cscale(A2[t], b[3])));
-        if (t / n == t % n) v.x += b[1];
-        T2[t] = v;
-    }
-    __syncthreads();
-    sm_matmul(A, T2, U, n);
-    if (t < nn) T1[t] = cadd(cadd(cscale(A6[t], b[12]), cscale(A4[t], b[10])), cscale(A2[t], b[8]));
-    __syncthreads();
-    sm_matmul(A6, T1, V, n);
-    if (t < nn) {
-        double2 v = cadd(V[t], cadd(cadd(cscale(A6[t], b[6]), cscale(A4[t], b[4])), cscale(A2[t], b[2])));
-        if (t / n == t % n) v.x += b[0];
-        V[t] = v;
-    }
-    __syncthreads();
-    // P = V - U (into A), X = V + U (into T1)
-    double2 *P = A, *X = T1;
-    if (t < nn) {
-        const double2 u = U[t], v = V[t];
-        P[t] = csub(v, u);
-        X[t] = cadd(v, u);
-    }
-    __syncthreads();
-    for (int kk = 0; kk < n; ++kk) {
-        if (t == 0) {
-            int piv = kk;
-            double best = cabsv(P[kk * n + kk]);
-            for (int i = kk + 1; i < n; ++i) {
-                const double a = cabsv(P[i * n + kk]);
-                if (a > best) {
-                    best = a;
-                    piv = i;
-                }
-            }
-            sh.ipiv = piv;
-        }
-        __syncthreads();
-        const int piv = sh.ipiv;
-        if (piv != kk && t < n) {
-            double2 a = P[kk * n + t];
-            P[kk * n + t] = P[piv * n + t];
-            P[piv * n + t] = a;
-            a = X[kk * n + t];
-            X[kk * n + t] = X[piv * n + t];
-            X[piv * n + t] = a;
-        }
-        __syncthreads();
-        if (t > kk && t < n) sh.lu_f[t] = cdiv(P[t * n + kk], P[kk * n + kk]);
-        __syncthreads();
-        if (t < nn) {
-            const int i = t / n, j = t % n;
-            if (i > kk) {
-                const double2 f = sh.lu_f[i];
-                if (j >= kk) P[i * n + j] = csub(P[i * n + j], cmul(f, P[kk * n + j]));
-                X[i * n + j] = csub(X[i * n + j], cmul(f, X[kk * n + j]));
-            }
-        }
-        __syncthreads();
-    }
-    for (int kk = n - 1; kk >= 0; --kk) {
-        if (t < n) {
-            double2 s = X[kk * n + t];
-            for (int i = kk + 1; i < n; ++i) s = csub(s, cmul(P[kk * n + i], X[i * n + t]));
-            X[kk * n + t] = cdiv(s, P[kk * n + kk]);
-        }
-        __syncthreads();
-    }
-    double2* cur = X;
-    double2* nxt = T2;
-    for (int s = 0; s < sc; ++s) {
-        sm_matmul(cur, cur, nxt, n);
-        double2* tmp = cur;
-        cur = nxt;
-        nxt = tmp;
-    }
-    if (t < nn) Eout[t] = cur[t];
-    __syncthreads();
-}
-
 
 // ---------------------------------------------------------------- small dense expm on one wave
-// The same Pade(13) scaling-and-squaring algorithm as sm_expm, executed by a single
+// Pade(13) scaling and squaring (Higham 2005, the oracle's expm) executed by a single
 // wavefront: the 64 lanes run in lockstep, so LDS hand-offs need only a wavefront-scope
 // fence instead of a workgroup barrier.  Called by wave 0 only.
 __device__ __forceinline__ void wave_sync() {

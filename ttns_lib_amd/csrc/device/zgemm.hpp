@@ -7,8 +7,8 @@
 //   A (16 x 4): lane l holds A[l % 16][l / 16];  B (4 x 16): lane l holds B[l / 16][l % 16];
 //   D (16 x 16): lane l holds D[l / 16 + 4 * j][l % 16] in accumulator element j
 //   (measured on gfx950 with a lane-id probe).
-// Operands come straight from L2/L1 (node tensors are too large for LDS); the loads of up
-// to sixteen k-steps are issued before their MFMAs.
+// Operands come straight from L2 (node tensors are too large for LDS); the loads of up
+// to eight k-steps are issued before their MFMAs.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -16,6 +16,18 @@
 namespace ttnd {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+
+// Coherent loads.  Workgroups of a team run on different CUs of one XCD and exchange
+// data through the XCD's L2; the per-CU vector L1 is not kept coherent with it, and
+// invalidating it costs about 6 us on gfx950.  Every load of data another workgroup may
+// have written therefore uses device scope, which misses in the L1 and reads the L2.
+__device__ __forceinline__ double ldc(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double2 ldc(const double2* p) {
+    const double* q = reinterpret_cast<const double*>(p);
+    return make_double2(ldc(q), ldc(q + 1));
+}
 
 struct ZAcc {
     d4 re, im;
@@ -50,8 +62,8 @@ __device__ __forceinline__ void ztile(ZAcc& acc, const double2* __restrict__ a, 
             if (s < nks) {
                 const int64_t k = k0 + 4 * s + kq;
                 const bool ok = k < K;
-                av[s] = (arow && ok) ? a[i * sar + k * sak] : z;
-                bv[s] = (bcol && ok) ? b[k * sbk + i * sbc] : z;
+                av[s] = (arow && ok) ? ldc(a + i * sar + k * sak) : z;
+                bv[s] = (bcol && ok) ? ldc(b + k * sbk + i * sbc) : z;
             }
         }
 #pragma unroll
@@ -77,7 +89,7 @@ __device__ __forceinline__ void ztile_store(const ZAcc& acc, double2* c, int64_t
             double2 v = make_double2(alpha.x * re - alpha.y * im, alpha.x * im + alpha.y * re);
             double2* p = c + row * scr + col * scc;
             if (accumulate) {
-                const double2 o = *p;
+                const double2 o = ldc(p);
                 v.x += o.x;
                 v.y += o.y;
             }
