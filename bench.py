@@ -107,6 +107,22 @@ def cpu_baseline(insts, flops_per_step, budget_s):
     }
 
 
+def report(insts, igs, teams, dflops, p0, steps):
+    """Per-instance device phase times (ttn_profile, seconds per step), slowest first."""
+    rows = []
+    for k, ig in enumerate(igs):
+        p1 = ig.profile()
+        d = {n: (p1[n] - p0[k][n]) / 1e6 / steps for n in p1}
+        rows.append((d["total"], k, d))
+    rows.sort(reverse=True)
+    for tot, k, d in rows:
+        m = insts[k].model
+        gf = dflops[k] / steps / 1e9
+        print(f"[report] inst {k:2d} modes {len(m.wk):3d} d {m.boson_dim[0]:2d} r {m.nspf:2d} team {teams[k]:2d} "
+              f"{gf:8.1f} GFLOP {tot:6.2f} s {gf / max(tot, 1e-9):6.1f} GFLOP/s | "
+              + " ".join(f"{n} {v:.2f}" for n, v in d.items() if n != "total"), file=sys.stderr, flush=True)
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -127,6 +143,8 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=4, help="instances stepped alone for the plain GPU rate")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle time for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--report", action="store_true",
+                    help="print each instance's team size, flops and device phase times of the timed steps to stderr")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the batched sweep "
@@ -146,6 +164,7 @@ def main():
     batch.step(warm)
     print(f"[bench] warmup {warm} step(s) {time.perf_counter() - t0:.2f}s", file=sys.stderr, flush=True)
     f0 = np.array([ig.flops() for ig in igs])
+    p0 = [ig.profile() for ig in igs] if args.report else None
     t0 = time.perf_counter()
     batch.step(args.steps)
     wall = time.perf_counter() - t0
@@ -157,6 +176,8 @@ def main():
     achieved = flops / (kernel_ms / 1e3) / 1e12
     print(f"[bench] timed {args.steps} step(s): wall {wall:.3f}s kernel {kernel_ms / 1e3:.3f}s "
           f"{flops / 1e9:.1f} GFLOP", file=sys.stderr, flush=True)
+    if args.report:
+        report(insts, igs, batch.teams(), f1 - f0, p0, args.steps)
     sz = [float(ig.trajectory()[-1, 0]) for ig in igs]
     norm_dev = max(abs(float(ig.trajectory()[-1, 1]) - 1.0) for ig in igs)
 
