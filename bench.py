@@ -138,7 +138,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c3")
     ap.add_argument("--plain-sample", type=int, default=4, help="instances stepped alone for the plain GPU rate")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle time for the CPU baseline")

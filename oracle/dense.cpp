@@ -243,15 +243,24 @@ void expm(const cplx* M, int n, cplx* E) {
     for (int i = 0; i < nn; ++i) E[i] = X[i];
 }
 
+// Sub-step factor from the error model err(s) ~ s^m: 0.9 (tol / err)^(1/m), clamped to
+// [lo, hi]; err == 0 (invariant Krylov space) gives hi.
+double step_factor(double tol, double err, int m, double lo, double hi) {
+    if (!(err > 0.0)) return hi;
+    const double f = 0.9 * std::pow(tol / err, 1.0 / m);
+    return std::min(hi, std::max(lo, f));
+}
+
 i64 expv(const std::function<void(const cplx*, cplx*)>& H, cplx* x, i64 n, cplx c, double tau, int kdim,
          double tol) {
     if (tau == 0.0 || n == 0) return 0;
     i64 napply = 0;
     std::vector<cplx> V(static_cast<size_t>(kdim + 1) * n), w(n);
     std::vector<cplx> Hm(static_cast<size_t>(kdim + 1) * kdim), S, F, h(kdim);
-    double s_done = 0.0, s_step = 1.0;
+    // s_next: fraction of tau predicted for the next sub-step (starts with the whole step)
+    double s_done = 0.0, s_next = 1.0;
     while (s_done < 1.0) {
-        s_step = std::min(s_step, 1.0 - s_done);
+        double s_step = std::min(s_next, 1.0 - s_done);
         double beta = 0.0;
         for (i64 i = 0; i < n; ++i) beta += std::norm(x[i]);
         beta = std::sqrt(beta);
@@ -260,7 +269,7 @@ i64 expv(const std::function<void(const cplx*, cplx*)>& H, cplx* x, i64 n, cplx 
         for (i64 i = 0; i < n; ++i) V[i] = x[i] / beta;
         std::fill(Hm.begin(), Hm.end(), cplx(0.0, 0.0));
         int m = 0;
-        double hn = 0.0;
+        double hn = 0.0, err = 0.0;
         bool found = false;
         auto small_exp = [&](int mm, double s) {
             S.assign(static_cast<size_t>(mm) * mm, cplx(0.0, 0.0));
@@ -303,7 +312,7 @@ i64 expv(const std::function<void(const cplx*, cplx*)>& H, cplx* x, i64 n, cplx 
                 for (i64 q = 0; q < n; ++q) V[static_cast<size_t>(j + 1) * n + q] = w[q] / hn;
                 continue;
             }
-            const double err = small_exp(m, s_step);
+            err = small_exp(m, s_step);
             if (err <= tol) {
                 found = true;
                 break;
@@ -311,9 +320,18 @@ i64 expv(const std::function<void(const cplx*, cplx*)>& H, cplx* x, i64 n, cplx 
             if (j + 1 < kdim)
                 for (i64 q = 0; q < n; ++q) V[static_cast<size_t>(j + 1) * n + q] = w[q] / hn;
         }
+        // The full space does not reach the tolerance at s_step: shrink the sub-step with the
+        // error model err(s) ~ s^m of the Saad estimate (Expokit's step-size rule, Sidje 1998,
+        // safety factor 0.9), then fall back to halving.
+        for (int k = 0; !found && k < 8; ++k) {
+            s_step *= step_factor(tol, err, m, 0.0625, 0.9);
+            err = small_exp(m, s_step);
+            found = err <= tol;
+        }
         for (int k = 0; !found && k < 60; ++k) {
             s_step *= 0.5;
-            if (small_exp(m, s_step) <= tol) found = true;
+            err = small_exp(m, s_step);
+            found = err <= tol;
         }
         if (!found) fail(ERR_NUMERIC, "expv: Krylov step did not reach the tolerance");
         for (i64 q = 0; q < n; ++q) {
@@ -322,6 +340,8 @@ i64 expv(const std::function<void(const cplx*, cplx*)>& H, cplx* x, i64 n, cplx 
             x[q] = beta * s;
         }
         s_done += s_step;
+        // the same model predicts the next sub-step, growing by at most a factor 2
+        s_next = s_step * step_factor(tol, err, m, 1.0, 2.0);
     }
     return napply;
 }

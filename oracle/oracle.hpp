@@ -23,9 +23,10 @@
 // krylov_integrator and llindoy/linalg's SVD, no pinned versions) and are restated
 // from published methods, identically to the HIP path:
 //   * expv: Arnoldi with classical Gram-Schmidt applied twice (CGS2), the Saad (1992)
-//     a-posteriori error estimate, step halving when the Krylov space does not reach
-//     the tolerance (as in Sidje's Expokit, 1998), and the small exponential by
-//     Pade(13) scaling and squaring (Higham 2005);
+//     a-posteriori error estimate, sub-steps sized by Expokit's rule (Sidje 1998: the
+//     model err(s) ~ s^m with safety factor 0.9, growth by at most 2 per sub-step, halving
+//     as the last resort), and the small exponential by Pade(13) scaling and squaring
+//     (Higham 2005);
 //   * gauge factorisations: Householder QR with LAPACK zgeqrf/zlarfg conventions.
 // Parity of those two against the reference's own binaries is unpinned (the
 // reference ships no tests and cannot be built here); they are pinned against

@@ -73,3 +73,17 @@ def test_autotuner_gives_large_instances_teams(hip):
     t = B.teams()
     assert t[0] == 1 and t[1] > 1
     assert t.sum() <= 256
+
+
+@pytest.mark.parametrize("team", [2, 8])
+def test_spread_team_conserves_norm_on_medium_tree(spread, orc, team):
+    """Every operation of a 16-mode, bond-16 tree spread over the team (bond matrices and
+    leaves included): the propagations stay unitary to rounding and track the oracle."""
+    m = sb_model(16, 10, 16)
+    _, _, g = make_integrator(spread, m, team_size=team)
+    _, _, o = make_integrator(orc, m)
+    for _ in range(3):
+        g.step(1)
+        o.step(1)
+        assert abs(g.norm2() - 1.0) < 1e-12
+        assert abs(g.expect(0, SIGMA["z"]) - o.expect(0, SIGMA["z"])) < 1e-10

@@ -116,7 +116,8 @@ struct ttn_batch {
     std::vector<ttn_integ*> integs;
     DevBuf<ttnd::DInst> d_insts;
     double last_ms = 0.0;
-    std::vector<double> cost;      // per-instance flops per sweep pair (model, then measured)
+    std::vector<double> cost;      // per-instance cost of one sweep pair on one workgroup: modelled
+                                   // flops before the first launch, then measured device time
     std::vector<int> team;         // team sizes of the last launch
 };
 
@@ -581,6 +582,13 @@ double measured_flops(const ttn_integ* in) {
     return tot;
 }
 
+// Device wall-clock ticks the instance's team leader has spent in sweep launches
+int64_t device_ticks(const ttn_integ* in) {
+    int64_t v = 0;
+    hip_check(hipMemcpy(&v, in->prof.p + ttnd::PROF_TOTAL, sizeof(int64_t), hipMemcpyDeviceToHost), "prof");
+    return v;
+}
+
 // Team size of an instance stepped on its own: one workgroup per 2048 entries of its
 // largest node tensor, at most one XCD.
 int auto_team(const ttn_integ* in) {
@@ -959,20 +967,23 @@ int ttn_batch_step(ttn_batch* b, double cr, double ci, int64_t nsteps) {
         if (b->cost.empty())
             for (ttn_integ* in : b->integs) b->cost.push_back(static_cost(in));
         b->team = batch_teams(b);
-        std::vector<double> f0(b->integs.size());
-        for (size_t k = 0; k < b->integs.size(); ++k) f0[k] = measured_flops(b->integs[k]);
+        std::vector<int64_t> t0(b->integs.size());
+        for (size_t k = 0; k < b->integs.size(); ++k) t0[k] = device_ticks(b->integs[k]);
         launch(b->ctx, b->d_insts.p, insts, a, b->team);
         float ms = 0.f;
         hip_check(hipEventElapsedTime(&ms, b->ctx->ev0, b->ctx->ev1), "elapsed");
         b->last_ms = ms;
         // every instance is accounted before the first failure is reported; the measured
-        // flops per sweep pair become the autotuner's costs for the next call
+        // device time per sweep pair, brought back to one workgroup with the same team
+        // model, becomes the autotuner's cost for the next call
         std::string err;
         const double pairs = static_cast<double>(a.prepare + a.nladder + a.nregular);
         for (size_t k = 0; k < b->integs.size(); ++k) {
             ttn_integ* in = b->integs[k];
             account(in, a);
-            b->cost[k] = std::max(1.0, (measured_flops(in) - f0[k]) / pairs);
+            const double s = serial_share(in), tk = static_cast<double>(std::max(1, b->team[k]));
+            const double ticks = static_cast<double>(device_ticks(in) - t0[k]);
+            b->cost[k] = std::max(1.0, ticks / pairs / (s + (1.0 - s) / tk));
             try {
                 finish(in, nsteps);
             } catch (const Error& e) {

@@ -326,8 +326,16 @@ __device__ void apply_to_parent(Ctx& c, int n) {
 }
 
 // ---------------------------------------------------------------- Krylov propagator
-// x <- exp(coeff * tau * H) x: Arnoldi with CGS2, Saad error estimate, step halving,
-// Pade(13) small exponential (same algorithm as the oracle's expv).  Returns the number
+// Sub-step factor from the error model err(s) ~ s^m: 0.9 (tol / err)^(1/m), clamped to
+// [lo, hi]; err == 0 (invariant Krylov space) gives hi.
+__device__ __forceinline__ double step_factor(double tol, double err, int m, double lo, double hi) {
+    if (!(err > 0.0)) return hi;
+    const double f = 0.9 * pow(tol / err, 1.0 / m);
+    return fmin(hi, fmax(lo, f));
+}
+
+// x <- exp(coeff * tau * H) x: Arnoldi with CGS2, Saad error estimate, Expokit sub-step
+// sizes, Pade(13) small exponential (same algorithm as the oracle's expv).  Returns the number
 // of H applications.  H(v, w) may use W1..W4; the Krylov vector w is W0.  Every member
 // owns the same vector entries in all three Gram-Schmidt sweeps, so only the reductions
 // and the new basis vector need team barriers.
@@ -343,10 +351,11 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
     double2* w = c.W(0);
     const int64_t q00 = T.tid(), st = T.nthr();
     int64_t napply = 0;
-    double s_done = 0.0, s_step = 1.0;
+    // s_next: fraction of tau predicted for the next sub-step (starts with the whole step)
+    double s_done = 0.0, s_next = 1.0;
     int guard = 0;
     while (s_done < 1.0 && guard++ < 4096) {
-        s_step = fmin(s_step, 1.0 - s_done);
+        double s_step = fmin(s_next, 1.0 - s_done);
         const double beta = sqrt(team_norm2(T, x, n));
         if (!isfinite(beta)) {
             c.fail(FAIL_INPUT_NONFINITE);
@@ -357,7 +366,7 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
         for (int e = threadIdx.x; e < (MAXK + 1) * MAXK; e += NT) sh.Hm[e] = cz();
         __syncthreads();
         int m = 0;
-        double hn = 0.0;
+        double hn = 0.0, err = 0.0;
         bool found = false;
         auto small_exp = [&](int mm, double s) -> double {
             const double2 f = cscale(coeff, tau * s);
@@ -462,7 +471,6 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
                 t_scale_to(T, V + (j + 1) * ld, cmk(1.0 / hn, 0.0), w, n);
                 continue;
             }
-            double err;
             PROFILE(PROF_EXPM, err = small_exp(m, s_step));
             if (err <= tol) {
                 found = true;
@@ -470,9 +478,17 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
             }
             if (j + 1 < kdim) t_scale_to(T, V + (j + 1) * ld, cmk(1.0 / hn, 0.0), w, n);
         }
+        // the full space misses the tolerance at s_step: Expokit's step-size rule on the
+        // error model err(s) ~ s^m, then halving (the oracle's expv, step for step)
+        for (int k = 0; !found && k < 8; ++k) {
+            s_step *= step_factor(tol, err, m, 0.0625, 0.9);
+            PROFILE(PROF_EXPM, err = small_exp(m, s_step));
+            found = err <= tol;
+        }
         for (int k = 0; !found && k < 60; ++k) {
             s_step *= 0.5;
-            if (small_exp(m, s_step) <= tol) found = true;
+            PROFILE(PROF_EXPM, err = small_exp(m, s_step));
+            found = err <= tol;
         }
         if (!found) {
             c.fail(FAIL_KRYLOV_STEP);
@@ -492,6 +508,7 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
         }
         T.sync();
         s_done += s_step;
+        s_next = s_step * step_factor(tol, err, m, 1.0, 2.0);
     }
     return napply;
 }
