@@ -134,6 +134,71 @@ def _cpu_model():
     return "unknown"
 
 
+def bench_c2(args):
+    """Contraction microbench: one 64 x 64 x 64 node, 40 product terms of random Hermitian 64 x 64
+    operators (N(0,1) re/im, (X + X^H)/2, seed 0) on child 0, child 1 and the mean field; the
+    H-action, the mean-field update of child 0 and a Krylov propagation exp(-i tau H) with
+    Krylov dimension 16, on the whole GPU (ttn_node_*).  One "step" is one H-action."""
+    import ttns_lib_amd
+    from ttns_lib_amd.capi import Node
+
+    lib = ttns_lib_amd.load_library()
+    rng = np.random.default_rng(0)
+    d, nt = 64, 40
+
+    def herm(n, scale=1.0):
+        X = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        return scale * 0.5 * (X + X.conj().T)
+
+    op0 = np.array([herm(d) for _ in range(nt)])
+    op1 = np.array([herm(d) for _ in range(nt)])
+    mf = np.array([herm(d) for _ in range(nt)])
+    A = rng.standard_normal((d * d, d)) + 1j * rng.standard_normal((d * d, d))
+    node = Node(lib, op0, op1, mf)
+    node.set(A)
+    node.heff_timed(max(1, args.warmup))
+    ms, fl = node.heff_timed(args.steps)
+    heff_ms = ms / args.steps
+    node.mf(0, reps=1)
+    node.mf(0, reps=args.steps)
+    mf_ms, mf_fl = node.last()
+    # Krylov propagation with dimension 16: operators scaled by 1/d, tau raised until the
+    # propagation uses the whole 16-vector space
+    nodek = Node(lib, op0 / d, op1 / d, mf / d)
+    A1 = A / np.linalg.norm(A)
+    for tau in (0.05, 0.1, 0.2, 0.4, 0.8):
+        nodek.set(A1)
+        na = nodek.expv(tau, krylov_dim=16, tol=1e-12)
+        kms, kfl = nodek.last()
+        if na >= 16:
+            break
+    achieved = fl / (ms / 1e3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": 1e3 / heff_ms,
+        "unit": "H-actions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": heff_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded random Hermitian operators and node tensor)",
+        "config": {"workload": "c2: node 64x64x64, 40 product terms of random Hermitian 64x64 operators",
+                   "terms": nt},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "ttnd::zgemm_kernel (three GEMM stages of the H-action)",
+                     "flops_per_heff": fl / args.steps},
+        "mean_field": {"ms": mf_ms / args.steps, "tflops": mf_fl / (mf_ms / 1e3) / 1e12},
+        "krylov16": {"ms": kms, "tau": tau, "h_applications": na, "tflops": kfl / (kms / 1e3) / 1e12,
+                     "ms_per_iteration": kms / max(na, 1)},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +216,9 @@ def main():
               "shards across GPUs by instance, which is not part of this build", file=sys.stderr)
         raise SystemExit(2)
     warm = max(1, args.warmup)
+    if args.workload == "c2":
+        bench_c2(args)
+        return
 
     import ttns_lib_amd
     from ttns_lib_amd.capi import Batch

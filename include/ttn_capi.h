@@ -170,6 +170,36 @@ int ttn_ctx_set_tuning(ttn_ctx* ctx, int64_t team_min, int teams_enabled);
 /* team size chosen for every instance by the last ttn_batch_step (sizes[n]) */
 int ttn_batch_teams(const ttn_batch* batch, int64_t* sizes);
 
+/* ---- one large branch node on the whole GPU ----
+ * A branch node with two children of dimensions d0, d1 and bond dimension r; its tensor A is
+ * (d0 d1) x r, row-major, child 0 slowest.  Its effective Hamiltonian is given by the
+ * environments a reference operator node holds, as nterms product terms
+ *     H A = sum_t coeff_t (O0_t (x) O1_t) A MF_t^T
+ * (O0_t: d0 x d0 single-particle matrix of child 0, O1_t: d1 x d1 of child 1, MF_t: r x r
+ * mean field; arrays term-major, each matrix row-major, complex interleaved).  The tensor
+ * stays resident in HBM between calls.
+ *   ttn_node_heff     res = H A, reps times; coefficient_evolution_branch::operator()
+ *                     (tdvp_core/projector_splitting_evolution_functions.hpp:166-258)
+ *   ttn_node_mf       mf_t of child c for every term t: sum over all other indices of
+ *                     conj(A) [(other child's O) A MF_t^T]; mean_field_operator_engine::evaluate
+ *                     (tdvp_core/mean_field_operator.hpp:37-135); out: nterms x dc x dc
+ *   ttn_node_expv     A <- exp(coeff tau H) A by the library's Krylov propagator; the
+ *                     node update of projector_splitting_engine::update_node_tensor
+ *                     (tdvp_core/projector_splitting_engine.hpp:117-156)
+ *   ttn_node_last     device time (HIP events) and algorithmic flops of the last call */
+typedef struct ttn_node ttn_node;
+int ttn_node_create(ttn_ctx* ctx, int64_t d0, int64_t d1, int64_t r, int64_t nterms, const double* coeff,
+                    const double* op0, const double* op1, const double* mf, ttn_node** out);
+int ttn_node_destroy(ttn_node* node);
+int ttn_node_set(ttn_node* node, const double* A);
+int ttn_node_get(ttn_node* node, double* A);
+int ttn_node_get_result(ttn_node* node, double* res);
+int ttn_node_heff(ttn_node* node, int64_t reps);
+int ttn_node_mf(ttn_node* node, int64_t child, int64_t reps, double* out);
+int ttn_node_expv(ttn_node* node, double coeff_re, double coeff_im, double tau, int64_t krylov_dim, double tol,
+                  int64_t* napply);
+int ttn_node_last(const ttn_node* node, double* ms, double* flops);
+
 /* ---- timing helpers (device-side event timing of the last ttn_step / ttn_batch_step) ---- */
 int ttn_last_step_ms(const ttn_integ* integ, double* ms);
 int ttn_batch_last_step_ms(const ttn_batch* batch, double* ms);

@@ -266,6 +266,16 @@ int orc_env_set(orc_integ* in, int64_t node, int64_t term, int32_t which_mf, con
         for (size_t i = 0; i < m.size(); ++i) m[i] = cget(src, static_cast<i64>(i));
     });
 }
+int orc_node_expv(orc_integ* in, int64_t node, double cr, double ci, double tau, int64_t kdim, double tol,
+                  int64_t* napply) {
+    return guard(in->ctx, [&] {
+        const int n = static_cast<int>(node);
+        auto& A = in->eng->A(n);
+        const i64 na = orc::expv([&](const cplx* v, cplx* res) { in->eng->heff_node(n, v, res); }, A.data(),
+                                 static_cast<i64>(A.size()), cplx(cr, ci), tau, static_cast<int>(kdim), tol);
+        if (napply) *napply = na;
+    });
+}
 int orc_expv(int64_t n, const double* Hd, double* x, double cr, double ci, double tau, int64_t kdim, double tol,
              int64_t* napply) {
     return guard(nullptr, [&] {

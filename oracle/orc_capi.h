@@ -54,6 +54,8 @@ int orc_update_spf(orc_integ* integ, int64_t node);
 int orc_update_mf(orc_integ* integ, int64_t node);
 int orc_env_get(orc_integ* integ, int64_t node, int64_t term, int32_t which_mf, double* out);
 int orc_env_set(orc_integ* integ, int64_t node, int64_t term, int32_t which_mf, const double* in);
+int orc_node_expv(orc_integ* integ, int64_t node, double coeff_re, double coeff_im, double tau, int64_t kdim,
+                  double tol, int64_t* napply);
 int orc_expv(int64_t n, const double* Hdense, double* x, double c_re, double c_im, double tau, int64_t kdim,
              double tol, int64_t* napply);
 int orc_expm(int64_t n, const double* M, double* E);

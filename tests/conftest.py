@@ -50,4 +50,5 @@ ORACLE_EXTRA = {
     "expv": [_C.c_int64, _d, _d, _C.c_double, _C.c_double, _C.c_double, _C.c_int64, _C.c_double, _i],
     "expm": [_C.c_int64, _d, _d],
     "qr": [_C.c_int64, _C.c_int64, _d, _d, _d],
+    "node_expv": [_C.c_void_p, _C.c_int64, _C.c_double, _C.c_double, _C.c_double, _C.c_int64, _C.c_double, _i],
 }

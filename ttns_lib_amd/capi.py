@@ -99,6 +99,15 @@ _SIGS = {
     "batch_last_step_ms": [vp, dblp],
     "ctx_set_tuning": [vp, i64, C.c_int],
     "batch_teams": [vp, i64p],
+    "node_create": [vp, i64, i64, i64, i64, dblp, dblp, dblp, dblp, C.POINTER(vp)],
+    "node_destroy": [vp],
+    "node_set": [vp, dblp],
+    "node_get": [vp, dblp],
+    "node_get_result": [vp, dblp],
+    "node_heff": [vp, i64],
+    "node_mf": [vp, i64, i64, dblp],
+    "node_expv": [vp, C.c_double, C.c_double, C.c_double, i64, C.c_double, i64p],
+    "node_last": [vp, dblp, dblp],
 }
 _RESTYPE = {"last_error": C.c_char_p}
 
@@ -405,6 +414,63 @@ class Batch:
     def destroy(self):
         if self.h:
             self.lib.fn["batch_destroy"](self.h)
+            self.h = vp()
+
+
+class Node:
+    """One large branch node on the whole GPU (ttn_node_*): children of dimensions d0, d1,
+    bond r, effective Hamiltonian sum_t coeff_t (op0_t (x) op1_t) A mf_t^T."""
+
+    def __init__(self, lib: Library, op0, op1, mf, coeff=None):
+        op0, op1, mf = (np.ascontiguousarray(np.asarray(x, dtype=complex)) for x in (op0, op1, mf))
+        self.nt, self.d0 = op0.shape[0], op0.shape[1]
+        self.d1, self.r = op1.shape[1], mf.shape[1]
+        coeff = np.ones(self.nt, dtype=complex) if coeff is None else np.asarray(coeff, dtype=complex)
+        self.lib = lib
+        self.h = vp()
+        bufs = [_cplx_in(coeff), _cplx_in(op0), _cplx_in(op1), _cplx_in(mf)]
+        lib._call("node_create", lib.ctx, self.d0, self.d1, self.r, self.nt, *[_dp(b) for b in bufs],
+                  C.byref(self.h))
+        self.shape = (self.d0 * self.d1, self.r)
+
+    def set(self, a):
+        a = np.asarray(a, dtype=complex).reshape(self.shape)
+        self.lib._call("node_set", self.h, _dp(_cplx_in(a)))
+
+    def get(self):
+        buf = np.zeros(2 * self.shape[0] * self.shape[1])
+        self.lib._call("node_get", self.h, _dp(buf))
+        return buf.view(np.complex128).reshape(self.shape)
+
+    def heff(self, reps: int = 1):
+        self.lib._call("node_heff", self.h, reps)
+        buf = np.zeros(2 * self.shape[0] * self.shape[1])
+        self.lib._call("node_get_result", self.h, _dp(buf))
+        return buf.view(np.complex128).reshape(self.shape)
+
+    def heff_timed(self, reps: int):
+        self.lib._call("node_heff", self.h, reps)
+        return self.last()
+
+    def mf(self, child: int, reps: int = 1):
+        dc = self.d0 if child == 0 else self.d1
+        buf = np.zeros(2 * self.nt * dc * dc)
+        self.lib._call("node_mf", self.h, child, reps, _dp(buf))
+        return buf.view(np.complex128).reshape(self.nt, dc, dc)
+
+    def expv(self, tau: float, krylov_dim: int = 16, tol: float = 1e-12, coeff: complex = -1j):
+        na = i64(0)
+        self.lib._call("node_expv", self.h, coeff.real, coeff.imag, tau, krylov_dim, tol, C.byref(na))
+        return na.value
+
+    def last(self):
+        ms, fl = C.c_double(), C.c_double()
+        self.lib._call("node_last", self.h, C.byref(ms), C.byref(fl))
+        return ms.value, fl.value
+
+    def destroy(self):
+        if self.h:
+            self.lib.fn["node_destroy"](self.h)
             self.h = vp()
 
 

@@ -15,6 +15,7 @@
 
 #include "../../include/ttn_capi.h"
 #include "device/desc.hpp"
+#include "runtime.hpp"
 #include "host/flops.hpp"
 #include "host/marshal.hpp"
 #include "host/tables.hpp"
@@ -28,50 +29,8 @@ extern "C" hipError_t ttnd_run_occupancy(int* blocks_per_cu);
 extern "C" hipError_t ttnd_launch_census(int* d_out, int n, hipStream_t s);
 
 using namespace ttnh;
+using namespace ttnrt;
 
-namespace {
-
-void hip_check(hipError_t e, const char* what) {
-    if (e != hipSuccess) fail(ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-template <class T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    void alloc(size_t count) {
-        release();
-        if (count == 0) count = 1;
-        hip_check(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)), "hipMalloc");
-        n = count;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    ~DevBuf() { release(); }
-};
-
-int64_t align16(int64_t x) { return (x + 15) & ~int64_t(15); }
-
-}  // namespace
-
-struct ttn_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::string err;
-    // team launches: every workgroup of a grid of ncu * occupancy is resident at once and
-    // the dispatcher spreads them evenly over nxcd XCDs (xcd_cap each), checked by a
-    // placement census when the context is created
-    bool teams_ok = false;
-    int grid = 0, nxcd = 0, xcd_cap = 0;
-    int64_t team_min = 2048;      // smaller operations run on a team's leader alone
-    DevBuf<unsigned> team_words;  // XCD slot counters (MAX_XCD lines) + TeamWords per instance
-    int64_t team_words_n = 0;     // instances the buffer holds
-    DevBuf<ttnd::TeamPlan> plan;
-};
 struct ttn_tree {
     ttn_ctx* ctx;
     Topology topo;
@@ -122,20 +81,6 @@ struct ttn_batch {
 };
 
 namespace {
-
-template <class F>
-int guard(ttn_ctx* ctx, F&& f) {
-    try {
-        f();
-        return TTN_OK;
-    } catch (const Error& e) {
-        if (ctx) ctx->err = e.what();
-        return e.code;
-    } catch (const std::exception& e) {
-        if (ctx) ctx->err = e.what();
-        return TTN_ERR_INVALID;
-    }
-}
 
 // Flatten topology + compiled tables into device descriptors and lay out the arena.
 void build_instance(ttn_integ* in) {

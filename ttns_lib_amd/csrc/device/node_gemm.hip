@@ -1,0 +1,389 @@
+// Full-GPU kernels for one large branch node: batched complex128 GEMMs on the FP64 matrix
+// cores, used by the node-level entry points (node.cpp) that apply the effective
+// Hamiltonian, build mean fields and run the Krylov propagator of a single node tensor.
+//
+// zgemm_kernel: C[b] (M x N) = alpha op(A[b]) (M x K) B[b] (K x N) (+ C), b = b1 * nb0 + b0.
+//   * 64 x 64 output tile per workgroup (256 threads, 4 wave64, each wave a 32 x 32 block of
+//     four 16 x 16 MFMA tiles), K in chunks of 16 staged through LDS with re and im planes
+//     kept apart so every MFMA operand is one LDS read;
+//   * the next chunk's global loads are issued before the current chunk's MFMAs;
+//   * K may be blocked: k = kblk * kb + kin, with a separate stride per block for A and B, so
+//     a sum of products over terms (each term a contiguous K block) is one GEMM;
+//   * split K: gridDim.z > 1 writes partial products to a workspace that zreduce_kernel sums
+//     in a fixed order (results do not depend on scheduling).
+// V_MFMA_F64_16X16X4F64 operand layout (one double per lane): A (16 x 4): lane l holds
+// A[l % 16][l / 16]; B (4 x 16): lane l holds B[l / 16][l % 16]; D (16 x 16): lane l holds
+// D[l / 16 + 4 j][l % 16] in accumulator element j.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "node_gemm.hpp"
+
+namespace ttnd {
+
+namespace {
+
+constexpr int GT = 256;     // threads
+constexpr int BM = 64, BN = 64, BK = 16;
+constexpr int LDP = 80;     // padded LDS row (doubles): rows kq and kq+1 fall 32 banks apart
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+struct Acc {
+    d4 re, im;
+};
+
+__device__ __forceinline__ void mfma_c(Acc& acc, double ar, double ai, double br, double bi) {
+    acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, acc.re, 0, 0, 0);
+    acc.re = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, acc.re, 0, 0, 0);
+    acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, acc.im, 0, 0, 0);
+    acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, acc.im, 0, 0, 0);
+}
+
+struct Panels {
+    double ar[BK][LDP], ai[BK][LDP];  // A(row, k) at [k][row]
+    double br[BK][LDP], bi[BK][LDP];  // B(k, col) at [k][col]
+};
+
+// Global -> register staging of one K chunk: four A and four B elements per thread.
+struct Stage {
+    double2 a[4], b[4];
+};
+
+__device__ __forceinline__ int64_t a_off(const GemmDesc& g, int row, int k) {
+    const int kb = k / g.kb, ki = k - kb * g.kb;
+    return static_cast<int64_t>(row) * g.a_r + static_cast<int64_t>(ki) * g.a_k + static_cast<int64_t>(kb) * g.a_kblk;
+}
+__device__ __forceinline__ int64_t b_off(const GemmDesc& g, int k, int col) {
+    const int kb = k / g.kb, ki = k - kb * g.kb;
+    return static_cast<int64_t>(ki) * g.b_k + static_cast<int64_t>(col) * g.b_c + static_cast<int64_t>(kb) * g.b_kblk;
+}
+
+// element e (0..3) of this thread in the A panel: (row, k) within the tile
+__device__ __forceinline__ void a_slot(bool kfast, int t, int e, int& row, int& k) {
+    if (kfast) {
+        k = t & 15;
+        row = (t >> 4) + 16 * e;
+    } else {
+        row = t & 63;
+        k = (t >> 6) + 4 * e;
+    }
+}
+__device__ __forceinline__ void b_slot(bool cfast, int t, int e, int& k, int& col) {
+    if (cfast) {
+        col = t & 63;
+        k = (t >> 6) + 4 * e;
+    } else {
+        k = t & 15;
+        col = (t >> 4) + 16 * e;
+    }
+}
+
+__device__ __forceinline__ void load_stage(const GemmDesc& g, const double2* A, const double2* B, int row0, int col0,
+                                           int k0, int kend, bool kfast, bool cfast, Stage& s) {
+    const int t = threadIdx.x;
+    const double2 z = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        int row, k;
+        a_slot(kfast, t, e, row, k);
+        const int gr = row0 + row, gk = k0 + k;
+        s.a[e] = (gr < g.M && gk < kend) ? A[a_off(g, gr, gk)] : z;
+        int kk, col;
+        b_slot(cfast, t, e, kk, col);
+        const int gk2 = k0 + kk, gc = col0 + col;
+        s.b[e] = (gc < g.N && gk2 < kend) ? B[b_off(g, gk2, gc)] : z;
+    }
+}
+
+__device__ __forceinline__ void store_stage(Panels& p, bool kfast, bool cfast, bool conja, const Stage& s) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        int row, k;
+        a_slot(kfast, t, e, row, k);
+        p.ar[k][row] = s.a[e].x;
+        p.ai[k][row] = conja ? -s.a[e].y : s.a[e].y;
+        int kk, col;
+        b_slot(cfast, t, e, kk, col);
+        p.br[kk][col] = s.b[e].x;
+        p.bi[kk][col] = s.b[e].y;
+    }
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(GT) void zgemm_kernel(GemmDesc g) {
+    __shared__ Panels P;
+    const int tiles_n = (g.N + BN - 1) / BN;
+    const int tile = blockIdx.x;
+    const int row0 = (tile / tiles_n) * BM, col0 = (tile % tiles_n) * BN;
+    const int b = blockIdx.y;
+    const int b1 = b / g.nb0, b0 = b - b1 * g.nb0;
+    const double2* A = g.A + b0 * g.a_b0 + b1 * g.a_b1;
+    const double2* B = g.B + b0 * g.b_b0 + b1 * g.b_b1;
+    // split K: this workgroup's k range
+    const int nsplit = gridDim.z, sp = blockIdx.z;
+    const int kchunks = (g.K + BK - 1) / BK;
+    const int c0 = static_cast<int>(static_cast<int64_t>(kchunks) * sp / nsplit);
+    const int c1 = static_cast<int>(static_cast<int64_t>(kchunks) * (sp + 1) / nsplit);
+    const bool kfast = g.a_k == 1;
+    const bool cfast = g.b_c == 1;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int wr = wv >> 1, wc = wv & 1;
+    const int li = lane & 15, kq = lane >> 4;
+
+    Acc acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            acc[i][j].re = d4{0.0, 0.0, 0.0, 0.0};
+            acc[i][j].im = d4{0.0, 0.0, 0.0, 0.0};
+        }
+    Stage st;
+    if (c0 < c1) load_stage(g, A, B, row0, col0, c0 * BK, g.K, kfast, cfast, st);
+    for (int c = c0; c < c1; ++c) {
+        __syncthreads();
+        store_stage(P, kfast, cfast, g.conja != 0, st);
+        __syncthreads();
+        if (c + 1 < c1) load_stage(g, A, B, row0, col0, (c + 1) * BK, g.K, kfast, cfast, st);
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 4) {
+            double ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                ar[i] = P.ar[ks + kq][wr * 32 + i * 16 + li];
+                ai[i] = P.ai[ks + kq][wr * 32 + i * 16 + li];
+                br[i] = P.br[ks + kq][wc * 32 + i * 16 + li];
+                bi[i] = P.bi[ks + kq][wc * 32 + i * 16 + li];
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) mfma_c(acc[i][j], ar[i], ai[i], br[j], bi[j]);
+        }
+    }
+    // epilogue: workspace (split K) or C
+    double2* C = g.C + b0 * g.c_b0 + b1 * g.c_b1;
+    if (nsplit > 1) C = g.W + sp * g.w_split + static_cast<int64_t>(b) * g.w_batch;
+    const int64_t cr = nsplit > 1 ? g.N : g.c_r;
+    const int64_t cc = nsplit > 1 ? 1 : g.c_c;
+    const bool direct = nsplit == 1;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = row0 + wr * 32 + i * 16 + kq + 4 * q;
+                const int col = col0 + wc * 32 + j * 16 + li;
+                if (row < g.M && col < g.N) {
+                    double2 v = make_double2(acc[i][j].re[q], acc[i][j].im[q]);
+                    double2* p = C + row * cr + col * cc;
+                    if (direct) {
+                        v = make_double2(g.alpha.x * v.x - g.alpha.y * v.y, g.alpha.x * v.y + g.alpha.y * v.x);
+                        if (g.accumulate) {
+                            const double2 o = *p;
+                            v.x += o.x;
+                            v.y += o.y;
+                        }
+                    }
+                    *p = v;
+                }
+            }
+}
+
+// C[b](row, col) (=|+=) alpha * sum_s W[s][b](row, col), s in order
+__global__ __launch_bounds__(GT) void zreduce_kernel(GemmDesc g, int nsplit) {
+    const int64_t per = static_cast<int64_t>(g.M) * g.N;
+    const int64_t total = per * g.nb0 * g.nb1;
+    for (int64_t e = blockIdx.x * static_cast<int64_t>(GT) + threadIdx.x; e < total;
+         e += static_cast<int64_t>(gridDim.x) * GT) {
+        const int64_t b = e / per, rc = e - b * per;
+        const int row = static_cast<int>(rc / g.N), col = static_cast<int>(rc - static_cast<int64_t>(row) * g.N);
+        double sx = 0.0, sy = 0.0;
+        for (int s = 0; s < nsplit; ++s) {
+            const double2 w = g.W[s * g.w_split + b * g.w_batch + rc];
+            sx += w.x;
+            sy += w.y;
+        }
+        const int b1 = static_cast<int>(b / g.nb0), b0 = static_cast<int>(b - static_cast<int64_t>(b1) * g.nb0);
+        double2* p = g.C + b0 * g.c_b0 + b1 * g.c_b1 + row * g.c_r + col * g.c_c;
+        double2 v = make_double2(g.alpha.x * sx - g.alpha.y * sy, g.alpha.x * sy + g.alpha.y * sx);
+        if (g.accumulate) {
+            const double2 o = *p;
+            v.x += o.x;
+            v.y += o.y;
+        }
+        *p = v;
+    }
+}
+
+// ---------------------------------------------------------------- Krylov vector kernels
+// partial[blk][i] = sum over this block's entries of conj(V_i) w, i < m (two doubles each)
+__global__ __launch_bounds__(GT) void zdots_kernel(const double2* V, int64_t ld, int m, const double2* w, int64_t n,
+                                                   double* partial) {
+    __shared__ double red[GT / 64][2 * MAXV];
+    double acc[2 * MAXV];
+#pragma unroll
+    for (int i = 0; i < 2 * MAXV; ++i) acc[i] = 0.0;
+    for (int64_t q = blockIdx.x * static_cast<int64_t>(GT) + threadIdx.x; q < n;
+         q += static_cast<int64_t>(gridDim.x) * GT) {
+        const double2 x = w[q];
+#pragma unroll
+        for (int i = 0; i < MAXV; ++i) {
+            if (i < m) {
+                const double2 v = V[i * ld + q];
+                acc[2 * i] += v.x * x.x + v.y * x.y;
+                acc[2 * i + 1] += v.x * x.y - v.y * x.x;
+            }
+        }
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < 2 * MAXV; ++i) {
+        double s = acc[i];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (lane == 0) red[wv][i] = s;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 2 * m; i += GT) {
+        double s = 0.0;
+        for (int w2 = 0; w2 < GT / 64; ++w2) s += red[w2][i];
+        partial[blockIdx.x * 2 * MAXV + i] = s;
+    }
+}
+
+// h[i] = sum over blocks of partial[blk][i] (fixed order), one thread per entry
+__global__ void zdots_finish_kernel(const double* partial, int nblk, int m, double* h) {
+    const int i = threadIdx.x;
+    if (i < 2 * m) {
+        double s = 0.0;
+        for (int b = 0; b < nblk; ++b) s += partial[b * 2 * MAXV + i];
+        h[i] = s;
+    }
+}
+
+// w -= sum_i h[i] V_i ; then partial norms of w per block
+__global__ __launch_bounds__(GT) void zorth_kernel(const double2* V, int64_t ld, int m, const double* h, double2* w,
+                                                   int64_t n, double* partial) {
+    __shared__ double red[GT / 64];
+    double nn = 0.0;
+    for (int64_t q = blockIdx.x * static_cast<int64_t>(GT) + threadIdx.x; q < n;
+         q += static_cast<int64_t>(gridDim.x) * GT) {
+        double2 x = w[q];
+        for (int i = 0; i < m; ++i) {
+            const double2 v = V[i * ld + q];
+            const double hr = h[2 * i], hi = h[2 * i + 1];
+            x.x -= hr * v.x - hi * v.y;
+            x.y -= hr * v.y + hi * v.x;
+        }
+        w[q] = x;
+        nn += x.x * x.x + x.y * x.y;
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) nn += __shfl_xor(nn, o, 64);
+    if (lane == 0) red[wv] = nn;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w2 = 0; w2 < GT / 64; ++w2) s += red[w2];
+        partial[blockIdx.x] = s;
+    }
+}
+
+// y = a x (scale) or y = sum_i c[i] V_i scaled by beta (combine)
+__global__ __launch_bounds__(GT) void zscale_kernel(double2* y, const double2* x, double ar, double ai, int64_t n) {
+    for (int64_t q = blockIdx.x * static_cast<int64_t>(GT) + threadIdx.x; q < n;
+         q += static_cast<int64_t>(gridDim.x) * GT) {
+        const double2 v = x[q];
+        y[q] = make_double2(ar * v.x - ai * v.y, ar * v.y + ai * v.x);
+    }
+}
+__global__ __launch_bounds__(GT) void zcombine_kernel(double2* y, const double2* V, int64_t ld, int m, const double* c,
+                                                      double beta, int64_t n) {
+    for (int64_t q = blockIdx.x * static_cast<int64_t>(GT) + threadIdx.x; q < n;
+         q += static_cast<int64_t>(gridDim.x) * GT) {
+        double sx = 0.0, sy = 0.0;
+        for (int i = 0; i < m; ++i) {
+            const double2 v = V[i * ld + q];
+            const double cr = c[2 * i], ci = c[2 * i + 1];
+            sx += cr * v.x - ci * v.y;
+            sy += cr * v.y + ci * v.x;
+        }
+        y[q] = make_double2(beta * sx, beta * sy);
+    }
+}
+// partial norms of x per block
+__global__ __launch_bounds__(GT) void znorm_kernel(const double2* x, int64_t n, double* partial) {
+    __shared__ double red[GT / 64];
+    double nn = 0.0;
+    for (int64_t q = blockIdx.x * static_cast<int64_t>(GT) + threadIdx.x; q < n;
+         q += static_cast<int64_t>(gridDim.x) * GT) {
+        const double2 v = x[q];
+        nn += v.x * v.x + v.y * v.y;
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) nn += __shfl_xor(nn, o, 64);
+    if (lane == 0) red[wv] = nn;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w2 = 0; w2 < GT / 64; ++w2) s += red[w2];
+        partial[blockIdx.x] = s;
+    }
+}
+
+}  // namespace ttnd
+
+// ---------------------------------------------------------------- launchers
+extern "C" hipError_t ttnd_zgemm(const ttnd::GemmDesc* g, int nsplit, hipStream_t s) {
+    using namespace ttnd;
+    const int tiles = ((g->M + BM - 1) / BM) * ((g->N + BN - 1) / BN);
+    const int batch = g->nb0 * g->nb1;
+    hipLaunchKernelGGL(zgemm_kernel, dim3(tiles, batch, nsplit), dim3(GT), 0, s, *g);
+    if (nsplit > 1) {
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        const int64_t total = static_cast<int64_t>(g->M) * g->N * batch;
+        const int blocks = static_cast<int>(total / GT + 1 < 4096 ? total / GT + 1 : 4096);
+        hipLaunchKernelGGL(zreduce_kernel, dim3(blocks), dim3(GT), 0, s, *g, nsplit);
+    }
+    return hipGetLastError();
+}
+
+extern "C" hipError_t ttnd_zdots(const double2* V, int64_t ld, int m, const double2* w, int64_t n, double* partial,
+                                 int nblk, double* h, hipStream_t s) {
+    using namespace ttnd;
+    hipLaunchKernelGGL(zdots_kernel, dim3(nblk), dim3(GT), 0, s, V, ld, m, w, n, partial);
+    hipLaunchKernelGGL(zdots_finish_kernel, dim3(1), dim3(64), 0, s, partial, nblk, m, h);
+    return hipGetLastError();
+}
+extern "C" hipError_t ttnd_zorth(const double2* V, int64_t ld, int m, const double* h, double2* w, int64_t n,
+                                 double* partial, int nblk, hipStream_t s) {
+    using namespace ttnd;
+    hipLaunchKernelGGL(zorth_kernel, dim3(nblk), dim3(GT), 0, s, V, ld, m, h, w, n, partial);
+    return hipGetLastError();
+}
+extern "C" hipError_t ttnd_zscale(double2* y, const double2* x, double ar, double ai, int64_t n, int nblk,
+                                  hipStream_t s) {
+    using namespace ttnd;
+    hipLaunchKernelGGL(zscale_kernel, dim3(nblk), dim3(GT), 0, s, y, x, ar, ai, n);
+    return hipGetLastError();
+}
+extern "C" hipError_t ttnd_zcombine(double2* y, const double2* V, int64_t ld, int m, const double* c, double beta,
+                                    int64_t n, int nblk, hipStream_t s) {
+    using namespace ttnd;
+    hipLaunchKernelGGL(zcombine_kernel, dim3(nblk), dim3(GT), 0, s, y, V, ld, m, c, beta, n);
+    return hipGetLastError();
+}
+extern "C" hipError_t ttnd_znorm(const double2* x, int64_t n, double* partial, int nblk, hipStream_t s) {
+    using namespace ttnd;
+    hipLaunchKernelGGL(znorm_kernel, dim3(nblk), dim3(GT), 0, s, x, n, partial);
+    return hipGetLastError();
+}

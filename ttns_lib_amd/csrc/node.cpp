@@ -1,0 +1,476 @@
+// Node-level C ABI (include/ttn_capi.h, ttn_node_*): one large branch node tensor on the
+// whole GPU.  The effective Hamiltonian is given as product terms coeff_t (O0_t (x) O1_t)
+// . MF_t^T over the node's two child modes and its bond (the environments an operator node
+// of the reference holds), and is applied with the batched FP64-MFMA GEMMs of
+// device/node_gemm.hip in three stages:
+//   Y_t = O1_t x_1 A           batch (t, i0):  d1 x r = (d1 x d1)(d1 x r)
+//   Z_t = O0_t x_0 Y_t         batch t:        d0 x (d1 r) = (d0 x d0)(d0 x d1 r)
+//   out = sum_t Z_t (coeff_t MF_t^T)           one GEMM with K = nterms * r, split over K
+// Mean fields of child c and the Krylov propagator reuse the same GEMMs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <memory>
+#include <vector>
+
+#include "device/node_gemm.hpp"
+#include "host/smallexp.hpp"
+#include "runtime.hpp"
+
+extern "C" hipError_t ttnd_zgemm(const ttnd::GemmDesc* g, int nsplit, hipStream_t s);
+extern "C" hipError_t ttnd_zdots(const double2* V, int64_t ld, int m, const double2* w, int64_t n, double* partial,
+                                 int nblk, double* h, hipStream_t s);
+extern "C" hipError_t ttnd_zorth(const double2* V, int64_t ld, int m, const double* h, double2* w, int64_t n,
+                                 double* partial, int nblk, hipStream_t s);
+extern "C" hipError_t ttnd_zscale(double2* y, const double2* x, double ar, double ai, int64_t n, int nblk,
+                                  hipStream_t s);
+extern "C" hipError_t ttnd_zcombine(double2* y, const double2* V, int64_t ld, int m, const double* c, double beta,
+                                    int64_t n, int nblk, hipStream_t s);
+extern "C" hipError_t ttnd_znorm(const double2* x, int64_t n, double* partial, int nblk, hipStream_t s);
+
+using namespace ttnh;
+using namespace ttnrt;
+
+struct ttn_node {
+    ttn_ctx* ctx;
+    int64_t d0, d1, r, nt;
+    int64_t D, size;
+    DevBuf<double2> op0, op1, mf, mft;  // operators; mft_t = coeff_t MF_t^T stacked (nt r x r)
+    DevBuf<double2> A, out, Y, Z, ws, V, w;
+    DevBuf<double> part, h;
+    int nblk = 1024;                  // blocks of the Krylov vector kernels
+    int64_t kcap = 0;                 // Krylov vectors allocated
+    double last_ms = 0.0;
+    double last_flops = 0.0;
+};
+
+namespace {
+
+constexpr int TILE = 64;     // GEMM output tile (node_gemm.hip)
+constexpr int KCHUNK = 16;   // GEMM K chunk
+constexpr int TARGET_WG = 512;
+
+int tiles(int64_t M, int64_t N) { return static_cast<int>(((M + TILE - 1) / TILE) * ((N + TILE - 1) / TILE)); }
+
+// K split that gives the GEMM about TARGET_WG workgroups
+int split_for(int64_t M, int64_t N, int64_t batch, int64_t K) {
+    const int64_t wg = static_cast<int64_t>(tiles(M, N)) * batch;
+    if (wg >= TARGET_WG / 2) return 1;
+    const int64_t want = (TARGET_WG + wg - 1) / wg;
+    const int64_t chunks = (K + KCHUNK - 1) / KCHUNK;
+    return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(want, chunks)));
+}
+
+ttnd::GemmDesc desc0() {
+    ttnd::GemmDesc g{};
+    g.nb0 = g.nb1 = 1;
+    g.alpha = make_double2(1.0, 0.0);
+    return g;
+}
+
+void run(ttn_node* nd, ttnd::GemmDesc g) {
+    if (g.kb <= 0) g.kb = g.K;
+    const int ns = split_for(g.M, g.N, static_cast<int64_t>(g.nb0) * g.nb1, g.K);
+    if (ns > 1) {
+        const int64_t need = static_cast<int64_t>(ns) * g.M * g.N * g.nb0 * g.nb1;
+        check(need <= static_cast<int64_t>(nd->ws.n), ERR_INVALID, "node workspace too small for the K split");
+        g.W = nd->ws.p;
+        g.w_batch = static_cast<int64_t>(g.M) * g.N;
+        g.w_split = g.w_batch * g.nb0 * g.nb1;
+    }
+    hip_check(ttnd_zgemm(&g, ns, nd->ctx->stream), "zgemm");
+}
+
+// Y_t = O1_t x_1 src for every term
+void stage_mode1(ttn_node* nd, const double2* src) {
+    ttnd::GemmDesc g = desc0();
+    g.M = static_cast<int>(nd->d1);
+    g.N = static_cast<int>(nd->r);
+    g.K = static_cast<int>(nd->d1);
+    g.nb0 = static_cast<int>(nd->d0);
+    g.nb1 = static_cast<int>(nd->nt);
+    g.A = nd->op1.p;
+    g.a_b1 = nd->d1 * nd->d1;
+    g.a_r = nd->d1;
+    g.a_k = 1;
+    g.B = src;
+    g.b_b0 = nd->d1 * nd->r;
+    g.b_k = nd->r;
+    g.b_c = 1;
+    g.C = nd->Y.p;
+    g.c_b0 = nd->d1 * nd->r;
+    g.c_b1 = nd->size;
+    g.c_r = nd->r;
+    g.c_c = 1;
+    run(nd, g);
+}
+
+// dst_t = O0_t x_0 src_t (src batch stride src_b: size for Y, 0 for a single tensor)
+void stage_mode0(ttn_node* nd, const double2* src, int64_t src_b, double2* dst) {
+    ttnd::GemmDesc g = desc0();
+    g.M = static_cast<int>(nd->d0);
+    g.N = static_cast<int>(nd->d1 * nd->r);
+    g.K = static_cast<int>(nd->d0);
+    g.nb0 = static_cast<int>(nd->nt);
+    g.A = nd->op0.p;
+    g.a_b0 = nd->d0 * nd->d0;
+    g.a_r = nd->d0;
+    g.a_k = 1;
+    g.B = src;
+    g.b_b0 = src_b;
+    g.b_k = nd->d1 * nd->r;
+    g.b_c = 1;
+    g.C = dst;
+    g.c_b0 = nd->size;
+    g.c_r = nd->d1 * nd->r;
+    g.c_c = 1;
+    run(nd, g);
+}
+
+// res = sum_t Z_t (coeff_t MF_t^T): K = nt r, blocked by term
+void stage_mf_sum(ttn_node* nd, double2* res) {
+    ttnd::GemmDesc g = desc0();
+    g.M = static_cast<int>(nd->D);
+    g.N = static_cast<int>(nd->r);
+    g.K = static_cast<int>(nd->nt * nd->r);
+    g.kb = static_cast<int>(nd->r);
+    g.A = nd->Z.p;
+    g.a_r = nd->r;
+    g.a_k = 1;
+    g.a_kblk = nd->size;
+    g.B = nd->mft.p;
+    g.b_k = nd->r;
+    g.b_kblk = nd->r * nd->r;
+    g.b_c = 1;
+    g.C = res;
+    g.c_r = nd->r;
+    g.c_c = 1;
+    run(nd, g);
+}
+
+void heff(ttn_node* nd, const double2* v, double2* res) {
+    stage_mode1(nd, v);
+    stage_mode0(nd, nd->Y.p, nd->size, nd->Z.p);
+    stage_mf_sum(nd, res);
+}
+
+double heff_flops(const ttn_node* nd) {
+    return 8.0 * static_cast<double>(nd->nt) * static_cast<double>(nd->size) *
+           static_cast<double>(nd->d0 + nd->d1 + nd->r);
+}
+
+void ensure_krylov(ttn_node* nd, int64_t kdim) {
+    if (nd->kcap >= kdim + 1) return;
+    nd->V.alloc(static_cast<size_t>((kdim + 1) * nd->size));
+    nd->kcap = kdim + 1;
+}
+
+// host sum (fixed order) of nblk partial norms
+double sum_partials(ttn_node* nd, int nblk) {
+    std::vector<double> p(nblk);
+    hip_check(hipMemcpyAsync(p.data(), nd->part.p, nblk * sizeof(double), hipMemcpyDeviceToHost, nd->ctx->stream),
+              "partials");
+    hip_check(hipStreamSynchronize(nd->ctx->stream), "sync");
+    double s = 0.0;
+    for (double x : p) s += x;
+    return s;
+}
+
+double step_factor(double tol, double err, int m, double lo, double hi) {
+    if (!(err > 0.0)) return hi;
+    const double f = 0.9 * std::pow(tol / err, 1.0 / m);
+    return std::min(hi, std::max(lo, f));
+}
+
+// A <- exp(coeff tau H) A: Arnoldi with CGS2, Saad estimate, Expokit sub-step sizes, Pade(13)
+// small exponential; the same algorithm, step for step, as the sweep kernel's expv.
+int64_t node_expv(ttn_node* nd, cplx coeff, double tau, int kdim, double tol) {
+    if (tau == 0.0) return 0;
+    ensure_krylov(nd, kdim);
+    hipStream_t s = nd->ctx->stream;
+    const int64_t n = nd->size, ld = nd->size;
+    const int nb = nd->nblk;
+    double2* V = nd->V.p;
+    double2* w = nd->w.p;
+    int64_t napply = 0;
+    double s_done = 0.0, s_next = 1.0;
+    std::vector<cplx> Hm(static_cast<size_t>((kdim + 1) * kdim)), S, F;
+    std::vector<double> hv(4 * ttnd::MAXV);
+    int guard_iter = 0;
+    while (s_done < 1.0 && guard_iter++ < 4096) {
+        double s_step = std::min(s_next, 1.0 - s_done);
+        hip_check(ttnd_znorm(nd->A.p, n, nd->part.p, nb, s), "norm");
+        const double beta = std::sqrt(sum_partials(nd, nb));
+        check(std::isfinite(beta), ERR_NUMERIC, "node expv: non-finite vector");
+        if (beta == 0.0) return napply;
+        hip_check(ttnd_zscale(V, nd->A.p, 1.0 / beta, 0.0, n, nb, s), "scale");
+        std::fill(Hm.begin(), Hm.end(), cplx(0.0, 0.0));
+        int m = 0;
+        double hn = 0.0, err = 0.0;
+        bool found = false;
+        auto small_exp = [&](int mm, double sf) {
+            S.assign(static_cast<size_t>(mm) * mm, cplx(0.0, 0.0));
+            F.assign(static_cast<size_t>(mm) * mm, cplx(0.0, 0.0));
+            const cplx f = coeff * (tau * sf);
+            for (int i = 0; i < mm; ++i)
+                for (int j = 0; j < mm; ++j) S[i * mm + j] = f * Hm[i * kdim + j];
+            small_expm(S.data(), mm, F.data());
+            return hn * std::abs(F[(mm - 1) * mm]);
+        };
+        for (int j = 0; j < kdim; ++j) {
+            heff(nd, V + j * ld, w);
+            ++napply;
+            for (int pass = 0; pass < 2; ++pass) {
+                hip_check(ttnd_zdots(V, ld, j + 1, w, n, nd->part.p, nb, nd->h.p, s), "dots");
+                hip_check(ttnd_zorth(V, ld, j + 1, nd->h.p, w, n, nd->part.p, nb, s), "orth");
+                hip_check(hipMemcpyAsync(hv.data(), nd->h.p, 2 * (j + 1) * sizeof(double), hipMemcpyDeviceToHost, s),
+                          "h");
+                const double nn = sum_partials(nd, nb);  // synchronises the stream
+                for (int i = 0; i <= j; ++i) Hm[i * kdim + j] += cplx(hv[2 * i], hv[2 * i + 1]);
+                hn = std::sqrt(nn);
+            }
+            check(std::isfinite(hn), ERR_NUMERIC, "node expv: non-finite Krylov vector");
+            if (j + 1 < kdim) Hm[(j + 1) * kdim + j] = hn;
+            m = j + 1;
+            double lead = hn;
+            for (int i = 0; i < j; ++i)
+                lead *= std::abs(coeff) * std::abs(tau * s_step) * std::abs(Hm[(i + 1) * kdim + i]) / (i + 1);
+            if (lead > 1e3 * tol && j + 1 < kdim) {
+                hip_check(ttnd_zscale(V + (j + 1) * ld, w, 1.0 / hn, 0.0, n, nb, s), "scale");
+                continue;
+            }
+            err = small_exp(m, s_step);
+            if (err <= tol) {
+                found = true;
+                break;
+            }
+            if (j + 1 < kdim) hip_check(ttnd_zscale(V + (j + 1) * ld, w, 1.0 / hn, 0.0, n, nb, s), "scale");
+        }
+        for (int k = 0; !found && k < 8; ++k) {
+            s_step *= step_factor(tol, err, m, 0.0625, 0.9);
+            err = small_exp(m, s_step);
+            found = err <= tol;
+        }
+        for (int k = 0; !found && k < 60; ++k) {
+            s_step *= 0.5;
+            err = small_exp(m, s_step);
+            found = err <= tol;
+        }
+        check(found, ERR_NUMERIC, "node expv: Krylov step did not reach the tolerance");
+        for (int i = 0; i < m; ++i) {
+            hv[2 * i] = F[i * m].real();
+            hv[2 * i + 1] = F[i * m].imag();
+        }
+        hip_check(hipMemcpyAsync(nd->h.p, hv.data(), 2 * m * sizeof(double), hipMemcpyHostToDevice, s), "coef");
+        hip_check(ttnd_zcombine(nd->A.p, V, ld, m, nd->h.p, beta, n, nb, s), "combine");
+        s_done += s_step;
+        s_next = s_step * step_factor(tol, err, m, 1.0, 2.0);
+    }
+    return napply;
+}
+
+void upload(const double* src, DevBuf<double2>& dst, int64_t count, hipStream_t s) {
+    dst.alloc(static_cast<size_t>(count));
+    hip_check(hipMemcpyAsync(dst.p, src, count * sizeof(double2), hipMemcpyHostToDevice, s), "upload");
+}
+
+struct Timer {
+    ttn_node* nd;
+    explicit Timer(ttn_node* n) : nd(n) { hip_check(hipEventRecord(nd->ctx->ev0, nd->ctx->stream), "event"); }
+    void stop() {
+        hip_check(hipEventRecord(nd->ctx->ev1, nd->ctx->stream), "event");
+        hip_check(hipEventSynchronize(nd->ctx->ev1), "sync");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, nd->ctx->ev0, nd->ctx->ev1), "elapsed");
+        nd->last_ms = ms;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int ttn_node_create(ttn_ctx* ctx, int64_t d0, int64_t d1, int64_t r, int64_t nterms, const double* coeff,
+                    const double* op0, const double* op1, const double* mf, ttn_node** out) {
+    return guard(ctx, [&] {
+        check(ctx != nullptr && out != nullptr, ERR_INVALID, "null argument");
+        check(d0 >= 1 && d1 >= 1 && r >= 1 && nterms >= 1, ERR_INVALID, "node dimensions must be positive");
+        check(op0 && op1 && mf && coeff, ERR_INVALID, "null operator array");
+        check(d0 * d1 * r <= (int64_t(1) << 26), ERR_INVALID, "node tensor too large");
+        auto nd = std::make_unique<ttn_node>();
+        nd->ctx = ctx;
+        nd->d0 = d0;
+        nd->d1 = d1;
+        nd->r = r;
+        nd->nt = nterms;
+        nd->D = d0 * d1;
+        nd->size = d0 * d1 * r;
+        hipStream_t s = ctx->stream;
+        upload(op0, nd->op0, nterms * d0 * d0, s);
+        upload(op1, nd->op1, nterms * d1 * d1, s);
+        upload(mf, nd->mf, nterms * r * r, s);
+        // coeff_t MF_t^T, stacked over terms: (p, j) of term t = coeff_t MF_t(j, p)
+        std::vector<double2> mft(static_cast<size_t>(nterms * r * r));
+        for (int64_t t = 0; t < nterms; ++t) {
+            const cplx c(coeff[2 * t], coeff[2 * t + 1]);
+            for (int64_t p = 0; p < r; ++p)
+                for (int64_t j = 0; j < r; ++j) {
+                    const int64_t src = 2 * (t * r * r + j * r + p);
+                    const cplx v = c * cplx(mf[src], mf[src + 1]);
+                    mft[t * r * r + p * r + j] = make_double2(v.real(), v.imag());
+                }
+        }
+        nd->mft.alloc(mft.size());
+        hip_check(hipMemcpyAsync(nd->mft.p, mft.data(), mft.size() * sizeof(double2), hipMemcpyHostToDevice, s),
+                  "upload");
+        nd->A.alloc(nd->size);
+        nd->out.alloc(nd->size);
+        nd->w.alloc(nd->size);
+        nd->Y.alloc(static_cast<size_t>(nterms * nd->size));
+        nd->Z.alloc(static_cast<size_t>(nterms * nd->size));
+        // workspace of the K splits: the largest need over every GEMM the node runs
+        // (M, N, batch, K of: mode-1 and mode-0 contractions, the mean-field sum, the
+        // per-term mean-field products and both mean-field contractions)
+        const int64_t shapes[][4] = {{d1, r, d0 * nterms, d1},        {d0, d1 * r, nterms, d0},
+                                     {nd->D, r, 1, nterms * r},       {nd->D, r, nterms, r},
+                                     {d0, d0, nterms, nd->size / d0}, {d1, d1, nterms, nd->size / d1}};
+        int64_t ws = 1;
+        for (const auto& g : shapes) {
+            const int ns = split_for(g[0], g[1], g[2], g[3]);
+            if (ns > 1) ws = std::max<int64_t>(ws, static_cast<int64_t>(ns) * g[0] * g[1] * g[2]);
+        }
+        nd->ws.alloc(static_cast<size_t>(ws));
+        nd->part.alloc(static_cast<size_t>(nd->nblk) * 2 * ttnd::MAXV);
+        nd->h.alloc(4 * ttnd::MAXV);
+        hip_check(hipMemsetAsync(nd->A.p, 0, nd->size * sizeof(double2), s), "memset");
+        hip_check(hipStreamSynchronize(s), "sync");
+        *out = nd.release();
+    });
+}
+
+int ttn_node_destroy(ttn_node* nd) {
+    delete nd;
+    return TTN_OK;
+}
+
+int ttn_node_set(ttn_node* nd, const double* A) {
+    return guard(nd->ctx, [&] {
+        hip_check(hipMemcpyAsync(nd->A.p, A, nd->size * sizeof(double2), hipMemcpyHostToDevice, nd->ctx->stream),
+                  "set");
+        hip_check(hipStreamSynchronize(nd->ctx->stream), "sync");
+    });
+}
+
+int ttn_node_get(ttn_node* nd, double* A) {
+    return guard(nd->ctx, [&] {
+        hip_check(hipMemcpy(A, nd->A.p, nd->size * sizeof(double2), hipMemcpyDeviceToHost), "get");
+    });
+}
+
+int ttn_node_get_result(ttn_node* nd, double* out) {
+    return guard(nd->ctx, [&] {
+        hip_check(hipMemcpy(out, nd->out.p, nd->size * sizeof(double2), hipMemcpyDeviceToHost), "get");
+    });
+}
+
+int ttn_node_heff(ttn_node* nd, int64_t reps) {
+    return guard(nd->ctx, [&] {
+        check(reps >= 1, ERR_INVALID, "reps must be positive");
+        Timer tm(nd);
+        for (int64_t k = 0; k < reps; ++k) heff(nd, nd->A.p, nd->out.p);
+        tm.stop();
+        nd->last_flops = heff_flops(nd) * static_cast<double>(reps);
+    });
+}
+
+int ttn_node_mf(ttn_node* nd, int64_t child, int64_t reps, double* out) {
+    return guard(nd->ctx, [&] {
+        check(child == 0 || child == 1, ERR_INVALID, "child must be 0 or 1");
+        check(reps >= 1, ERR_INVALID, "reps must be positive");
+        const int64_t dc = child == 0 ? nd->d0 : nd->d1;
+        DevBuf<double2> res;
+        res.alloc(static_cast<size_t>(nd->nt * dc * dc));
+        Timer tm(nd);
+        for (int64_t k = 0; k < reps; ++k) {
+            // the other child's operator on A, then MF_t^T: Z_t
+            if (child == 0)
+                stage_mode1(nd, nd->A.p);
+            else
+                stage_mode0(nd, nd->A.p, 0, nd->Y.p);
+            ttnd::GemmDesc g = desc0();
+            g.M = static_cast<int>(nd->D);
+            g.N = static_cast<int>(nd->r);
+            g.K = static_cast<int>(nd->r);
+            g.nb0 = static_cast<int>(nd->nt);
+            g.A = nd->Y.p;
+            g.a_b0 = nd->size;
+            g.a_r = nd->r;
+            g.a_k = 1;
+            g.B = nd->mf.p;  // B(p, j) = MF_t(j, p)
+            g.b_b0 = nd->r * nd->r;
+            g.b_k = 1;
+            g.b_c = nd->r;
+            g.C = nd->Z.p;
+            g.c_b0 = nd->size;
+            g.c_r = nd->r;
+            g.c_c = 1;
+            run(nd, g);
+            // mf_t(i, j) = sum over the other indices of conj(A)(.., i, ..) Z_t(.., j, ..)
+            ttnd::GemmDesc m = desc0();
+            m.M = m.N = static_cast<int>(dc);
+            m.nb0 = static_cast<int>(nd->nt);
+            m.conja = 1;
+            m.A = nd->A.p;
+            m.B = nd->Z.p;
+            m.b_b0 = nd->size;
+            if (child == 0) {
+                m.K = static_cast<int>(nd->d1 * nd->r);
+                m.a_r = nd->d1 * nd->r;
+                m.a_k = 1;
+                m.b_k = 1;
+                m.b_c = nd->d1 * nd->r;
+            } else {
+                m.K = static_cast<int>(nd->d0 * nd->r);
+                m.kb = static_cast<int>(nd->r);
+                m.a_r = nd->r;
+                m.a_k = 1;
+                m.a_kblk = nd->d1 * nd->r;
+                m.b_k = 1;
+                m.b_c = nd->r;
+                m.b_kblk = nd->d1 * nd->r;
+            }
+            m.C = res.p;
+            m.c_b0 = dc * dc;
+            m.c_r = dc;
+            m.c_c = 1;
+            run(nd, m);
+        }
+        tm.stop();
+        nd->last_flops = static_cast<double>(reps) * 8.0 * static_cast<double>(nd->nt) *
+                         static_cast<double>(nd->size) * static_cast<double>((child == 0 ? nd->d1 : nd->d0) + nd->r + dc);
+        if (out)
+            hip_check(hipMemcpy(out, res.p, nd->nt * dc * dc * sizeof(double2), hipMemcpyDeviceToHost), "mf");
+    });
+}
+
+int ttn_node_expv(ttn_node* nd, double cr, double ci, double tau, int64_t kdim, double tol, int64_t* napply) {
+    return guard(nd->ctx, [&] {
+        check(kdim >= 1 && kdim < ttnd::MAXV, ERR_INVALID, "krylov_dim must be in [1, 31]");
+        check(tol > 0.0, ERR_INVALID, "tolerance must be positive");
+        Timer tm(nd);
+        const int64_t na = node_expv(nd, cplx(cr, ci), tau, static_cast<int>(kdim), tol);
+        tm.stop();
+        if (napply) *napply = na;
+        nd->last_flops = static_cast<double>(na) * heff_flops(nd);
+    });
+}
+
+int ttn_node_last(const ttn_node* nd, double* ms, double* flops) {
+    if (ms) *ms = nd->last_ms;
+    if (flops) *flops = nd->last_flops;
+    return TTN_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,78 @@
+// Host-side runtime shared by the C-ABI translation units (capi.cpp, node.cpp): device
+// buffers, HIP error checks, the context (one per GPU, one HIP stream) and the exception
+// guard that turns failures into status codes plus a message (ttn_last_error).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <exception>
+#include <string>
+
+#include "../../include/ttn_capi.h"
+#include "device/desc.hpp"
+#include "host/common.hpp"
+
+namespace ttnrt {
+
+using ttnh::ERR_DEVICE;
+using ttnh::fail;
+
+inline void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) fail(ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        hip_check(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)), "hipMalloc");
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+inline int64_t align16(int64_t x) { return (x + 15) & ~int64_t(15); }
+
+}  // namespace ttnrt
+
+struct ttn_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    // team launches: every workgroup of a grid of ncu * occupancy is resident at once and
+    // the dispatcher spreads them evenly over nxcd XCDs (xcd_cap each), checked by a
+    // placement census when the context is created
+    bool teams_ok = false;
+    int grid = 0, nxcd = 0, xcd_cap = 0;
+    int64_t team_min = 2048;      // smaller operations run on a team's leader alone
+    ttnrt::DevBuf<unsigned> team_words;  // XCD slot counters (MAX_XCD lines) + TeamWords per instance
+    int64_t team_words_n = 0;     // instances the buffer holds
+    ttnrt::DevBuf<ttnd::TeamPlan> plan;
+};
+
+namespace ttnrt {
+
+template <class F>
+int guard(ttn_ctx* ctx, F&& f) {
+    try {
+        f();
+        return TTN_OK;
+    } catch (const ttnh::Error& e) {
+        if (ctx) ctx->err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->err = e.what();
+        return TTN_ERR_INVALID;
+    }
+}
+
+}  // namespace ttnrt
