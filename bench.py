@@ -30,6 +30,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "TDVP steps/s + FP64-MFMA roofline %, batched vs plain GPU vs CPU, <sz> err<=1e-10"
 PEAK_FP64_TFLOPS = 78.6
+# HBM bytes moved per unit of work, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes on
+# MI355X (FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0: the factors that make the counters
+# read the 1 GiB device copy of `bench.py --workload copy` as 1 GiB each way); the commands and
+# raw counts are in profiles/hbm_traffic_counters.md.  Not measured by this script.
+TRAFFIC_PER_BATCH_STEP = 5.73e12   # the 64-instance batch, one regular TDVP step
+TRAFFIC_PER_NODE_HEFF = 8.45e8     # 64 x 64 x 64 node, 40 terms, one H-action (all GEMM stages)
 
 
 def make(lib, inst):
@@ -189,7 +195,7 @@ def bench_c2(args):
         "config": {"workload": "c2: node 64x64x64, 40 product terms of random Hermitian 64x64 operators",
                    "terms": nt},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": TRAFFIC_PER_NODE_HEFF,
                      "kernel": "ttnd::zgemm_kernel (three GEMM stages of the H-action)",
                      "flops_per_heff": fl / args.steps},
         "mean_field": {"ms": mf_ms / args.steps, "tflops": mf_fl / (mf_ms / 1e3) / 1e12},
@@ -197,6 +203,24 @@ def bench_c2(args):
                      "ms_per_iteration": kms / max(na, 1)},
     }
     print(json.dumps(out))
+
+
+def bench_copy(args):
+    """Calibration for HBM traffic counters: a device-to-device copy of 1 GiB (read 1 GiB,
+    write 1 GiB per repetition) by torch's elementwise copy kernel."""
+    import torch
+
+    n = 1 << 27  # 2^27 float64 = 1 GiB
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    print(json.dumps({"workload": "copy", "bytes_read_per_copy": 8 * n, "bytes_written_per_copy": 8 * n,
+                      "copies": args.steps, "GBps": 2 * 8 * n * args.steps / dt / 1e9}))
 
 
 def main():
@@ -218,6 +242,9 @@ def main():
     warm = max(1, args.warmup)
     if args.workload == "c2":
         bench_c2(args)
+        return
+    if args.workload == "copy":
+        bench_copy(args)
         return
 
     import ttns_lib_amd
@@ -290,7 +317,8 @@ def main():
         "config": {"workload": f"{args.workload}: {desc}", "instances": len(insts), "krylov_dim": 12,
                    "krylov_tol": 1e-12},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": TRAFFIC_PER_BATCH_STEP * args.steps if args.workload == "c3" else None,
                      "kernel": "ttnd::tdvp_run_kernel", "flops_timed": flops},
         "plain_gpu": plain,
         "cpu_baseline": cpu,

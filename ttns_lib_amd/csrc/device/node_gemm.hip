@@ -253,18 +253,24 @@ __global__ __launch_bounds__(GT) void zdots_kernel(const double2* V, int64_t ld,
     for (int i = threadIdx.x; i < 2 * m; i += GT) {
         double s = 0.0;
         for (int w2 = 0; w2 < GT / 64; ++w2) s += red[w2][i];
-        partial[blockIdx.x * 2 * MAXV + i] = s;
+        partial[static_cast<int64_t>(i) * gridDim.x + blockIdx.x] = s;  // entry-major
     }
 }
 
-// h[i] = sum over blocks of partial[blk][i] (fixed order), one thread per entry
-__global__ void zdots_finish_kernel(const double* partial, int nblk, int m, double* h) {
-    const int i = threadIdx.x;
-    if (i < 2 * m) {
-        double s = 0.0;
-        for (int b = 0; b < nblk; ++b) s += partial[b * 2 * MAXV + i];
-        h[i] = s;
+// h[i] = sum over blocks of partial[i][blk]: one workgroup per entry, each thread a strided
+// share in block order, then a fixed-shape tree (the same sums every run)
+__global__ __launch_bounds__(GT) void zdots_finish_kernel(const double* partial, int nblk, double* h) {
+    __shared__ double red[GT];
+    const int i = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += GT) s += partial[static_cast<int64_t>(i) * nblk + b];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = GT / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) h[i] = red[0];
 }
 
 // w -= sum_i h[i] V_i ; then partial norms of w per block
@@ -361,7 +367,7 @@ extern "C" hipError_t ttnd_zdots(const double2* V, int64_t ld, int m, const doub
                                  int nblk, double* h, hipStream_t s) {
     using namespace ttnd;
     hipLaunchKernelGGL(zdots_kernel, dim3(nblk), dim3(GT), 0, s, V, ld, m, w, n, partial);
-    hipLaunchKernelGGL(zdots_finish_kernel, dim3(1), dim3(64), 0, s, partial, nblk, m, h);
+    hipLaunchKernelGGL(zdots_finish_kernel, dim3(2 * m), dim3(GT), 0, s, partial, nblk, h);
     return hipGetLastError();
 }
 extern "C" hipError_t ttnd_zorth(const double2* V, int64_t ld, int m, const double* h, double2* w, int64_t n,
