@@ -148,6 +148,11 @@ int ttn_sb_initial_state(ttn_integ* integ, const ttn_sb_params* p);
 int ttn_batch_create(ttn_ctx* ctx, int64_t n, ttn_integ* const* integs, ttn_batch** out);
 int ttn_batch_destroy(ttn_batch* batch);
 int ttn_batch_step(ttn_batch* batch, double coeff_re, double coeff_im, int64_t nsteps);
+/* Each batched step is one hipGraph launch (a fill kernel clearing the team words, then the
+ * persistent sweep), captured on first use and replayed while the launch arguments and the
+ * team plan are unchanged; enabled = 0 launches directly.  No reference counterpart. */
+int ttn_batch_set_graph(ttn_batch* batch, int enabled);
+int ttn_batch_graph_stats(const ttn_batch* batch, int64_t* captures, int64_t* replays);
 
 /* ---- measurement ----
  * Algorithmic floating-point operations executed by this instance so far (8 real flops

@@ -99,6 +99,8 @@ _SIGS = {
     "batch_last_step_ms": [vp, dblp],
     "ctx_set_tuning": [vp, i64, C.c_int],
     "batch_teams": [vp, i64p],
+    "batch_set_graph": [vp, C.c_int],
+    "batch_graph_stats": [vp, i64p, i64p],
     "node_create": [vp, i64, i64, i64, i64, dblp, dblp, dblp, dblp, C.POINTER(vp)],
     "node_destroy": [vp],
     "node_set": [vp, dblp],
@@ -404,6 +406,14 @@ class Batch:
         out = C.c_double()
         self.lib._call("batch_last_step_ms", self.h, C.byref(out))
         return out.value
+
+    def set_graph(self, enabled: bool):
+        self.lib._call("batch_set_graph", self.h, int(bool(enabled)))
+
+    def graph_stats(self):
+        c, r = i64(0), i64(0)
+        self.lib._call("batch_graph_stats", self.h, C.byref(c), C.byref(r))
+        return c.value, r.value
 
     def teams(self) -> np.ndarray:
         """team size of every instance in the last batched step"""

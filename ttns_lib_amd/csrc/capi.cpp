@@ -27,6 +27,7 @@ extern "C" hipError_t ttnd_launch_expect(const ttnd::DInst* d_inst, int leaf, in
                                          hipStream_t s);
 extern "C" hipError_t ttnd_run_occupancy(int* blocks_per_cu);
 extern "C" hipError_t ttnd_launch_census(int* d_out, int n, hipStream_t s);
+extern "C" hipError_t ttnd_launch_zero_u32(unsigned* p, int64_t n, hipStream_t s);
 
 using namespace ttnh;
 using namespace ttnrt;
@@ -70,8 +71,29 @@ struct ttn_integ {
     std::vector<double> trajectory;  // (re <O>, <1>) per completed step
 };
 
+// A captured sweep launch (zero fill of the team words + tdvp_run_kernel), replayed while
+// the launch arguments and the team plan stay the same.
+struct SweepGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ttnd::RunArgs args{};
+    ttnd::TeamPlan plan{};
+    bool teams = false;
+    int n = 0;
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    ~SweepGraph() { reset(); }
+};
+
 struct ttn_batch {
     ttn_ctx* ctx;
+    bool use_graph = true;         // replay captured launches (ttn_batch_set_graph)
+    SweepGraph graph;
+    int64_t graph_replays = 0, graph_captures = 0;
     std::vector<ttn_integ*> integs;
     DevBuf<ttnd::DInst> d_insts;
     double last_ms = 0.0;
@@ -459,9 +481,12 @@ bool make_plan(const ttn_ctx* c, const std::vector<int>& size, ttnd::TeamPlan& p
 // Launch the sweep kernel over n instances with the given team sizes (size 1 everywhere
 // runs one workgroup per instance without a plan).  A team that did not assemble leaves
 // its instance untouched with FAIL_TEAM_FORM; such instances are run again one workgroup
-// each, and teams are switched off for the context.
+// each, and teams are switched off for the context.  With a graph cache (gb) the zero fill
+// of the team words and the kernel launch are captured once and replayed as long as the
+// launch arguments and the team plan are unchanged (the instance descriptors and the plan
+// are uploaded before, outside the graph).
 void launch(ttn_ctx* c, ttnd::DInst* d_insts, const std::vector<ttnd::DInst>& insts, const ttnd::RunArgs& a,
-            std::vector<int> size) {
+            std::vector<int> size, ttn_batch* gb = nullptr) {
     const int n = static_cast<int>(insts.size());
     hipStream_t s = c->stream;
     int maxsize = 1;
@@ -469,24 +494,53 @@ void launch(ttn_ctx* c, ttnd::DInst* d_insts, const std::vector<ttnd::DInst>& in
         t = std::max(1, std::min(t, max_team(c)));
         maxsize = std::max(maxsize, t);
     }
-    ttnd::TeamPlan plan;
+    ttnd::TeamPlan plan{};
     const bool teams = c->teams_ok && maxsize > 1 && make_plan(c, size, plan);
-    hip_check(hipEventRecord(c->ev0, s), "event");
-    if (!teams) {
-        hip_check(ttnd_launch_run(d_insts, n, &a, nullptr, nullptr, nullptr, 0, s), "launch tdvp_run_kernel");
-    } else {
-        const int64_t head = 32 * ttnd::MAX_XCD;  // slot counters, one 128-byte line each
-        const int64_t words = static_cast<int64_t>(sizeof(ttnd::TeamWords) / sizeof(unsigned));
+    const int64_t head = 32 * ttnd::MAX_XCD;  // slot counters, one 128-byte line each
+    const int64_t words = static_cast<int64_t>(sizeof(ttnd::TeamWords) / sizeof(unsigned));
+    if (teams) {
         if (c->team_words_n < n) {
+            if (gb) gb->graph.reset();  // the captured launch points at the old buffer
             c->team_words.alloc(head + words * n);
             c->team_words_n = n;
         }
         if (c->plan.p == nullptr) c->plan.alloc(1);
-        hip_check(hipMemsetAsync(c->team_words.p, 0, (head + words * n) * sizeof(unsigned), s), "memset team words");
         hip_check(hipMemcpyAsync(c->plan.p, &plan, sizeof(plan), hipMemcpyHostToDevice, s), "plan");
+    }
+    // the work itself: a fill kernel zeroes the team words (no memset node, so the sequence
+    // can live in a graph), then the sweep
+    auto enqueue = [&] {
+        if (!teams) {
+            hip_check(ttnd_launch_run(d_insts, n, &a, nullptr, nullptr, nullptr, 0, s), "launch tdvp_run_kernel");
+            return;
+        }
+        hip_check(ttnd_launch_zero_u32(c->team_words.p, head + words * n, s), "zero team words");
         hip_check(ttnd_launch_run(d_insts, n, &a, c->plan.p, reinterpret_cast<ttnd::TeamWords*>(c->team_words.p + head),
                                   c->team_words.p, c->grid, s),
                   "launch tdvp_run_kernel");
+    };
+    if (gb && gb->use_graph) {
+        SweepGraph& g = gb->graph;
+        const bool same = g.exec && g.n == n && g.teams == teams && std::memcmp(&g.args, &a, sizeof(a)) == 0 &&
+                          (!teams || std::memcmp(&g.plan, &plan, sizeof(plan)) == 0);
+        if (!same) {
+            g.reset();
+            hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+            enqueue();
+            hip_check(hipStreamEndCapture(s, &g.graph), "end capture");
+            hip_check(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "instantiate graph");
+            g.args = a;
+            g.plan = plan;
+            g.teams = teams;
+            g.n = n;
+            ++gb->graph_captures;
+        }
+        hip_check(hipEventRecord(c->ev0, s), "event");
+        hip_check(hipGraphLaunch(g.exec, s), "graph launch");
+        ++gb->graph_replays;
+    } else {
+        hip_check(hipEventRecord(c->ev0, s), "event");
+        enqueue();
     }
     hip_check(hipEventRecord(c->ev1, s), "event");
     hip_check(hipEventSynchronize(c->ev1), "sync");
@@ -601,7 +655,7 @@ std::vector<int> batch_teams(const ttn_batch* b) {
         ++used;
     }
     // keep the placement feasible: shrink the largest team until first-fit succeeds
-    ttnd::TeamPlan plan;
+    ttnd::TeamPlan plan{};
     while (!make_plan(c, t, plan)) {
         size_t big = 0;
         for (size_t i = 1; i < n; ++i)
@@ -914,7 +968,7 @@ int ttn_batch_step(ttn_batch* b, double cr, double ci, int64_t nsteps) {
         b->team = batch_teams(b);
         std::vector<int64_t> t0(b->integs.size());
         for (size_t k = 0; k < b->integs.size(); ++k) t0[k] = device_ticks(b->integs[k]);
-        launch(b->ctx, b->d_insts.p, insts, a, b->team);
+        launch(b->ctx, b->d_insts.p, insts, a, b->team, b);
         float ms = 0.f;
         hip_check(hipEventElapsedTime(&ms, b->ctx->ev0, b->ctx->ev1), "elapsed");
         b->last_ms = ms;
@@ -937,6 +991,19 @@ int ttn_batch_step(ttn_batch* b, double cr, double ci, int64_t nsteps) {
         }
         if (!err.empty()) fail(ERR_NUMERIC, err);
     });
+}
+
+int ttn_batch_set_graph(ttn_batch* b, int enabled) {
+    return guard(b->ctx, [&] {
+        b->use_graph = enabled != 0;
+        if (!b->use_graph) b->graph.reset();
+    });
+}
+
+int ttn_batch_graph_stats(const ttn_batch* b, int64_t* captures, int64_t* replays) {
+    if (captures) *captures = b->graph_captures;
+    if (replays) *replays = b->graph_replays;
+    return TTN_OK;
 }
 
 int ttn_flops(const ttn_integ* in, double* flops) {

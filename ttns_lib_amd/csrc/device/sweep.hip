@@ -825,6 +825,20 @@ extern "C" hipError_t ttnd_run_occupancy(int* blocks_per_cu) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, ttnd::tdvp_run_kernel, ttnd::NT, 0);
 }
 
+// p[0 .. n) = 0: clears the team words before a team launch (a kernel, not a memset, so the
+// sequence can be captured in a graph and replayed)
+namespace ttnd {
+__global__ __launch_bounds__(256) void zero_u32_kernel(unsigned* p, int64_t n) {
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * 256) p[i] = 0u;
+}
+}  // namespace ttnd
+extern "C" hipError_t ttnd_launch_zero_u32(unsigned* p, int64_t n, hipStream_t s) {
+    const int64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(ttnd::zero_u32_kernel, dim3(static_cast<unsigned>(blocks < 1024 ? blocks : 1024)), dim3(256), 0,
+                       s, p, n);
+    return hipGetLastError();
+}
+
 // XCD of every workgroup of an n-workgroup launch (placement census)
 namespace ttnd {
 __global__ __launch_bounds__(NT) void xcd_census_kernel(int* out) {
