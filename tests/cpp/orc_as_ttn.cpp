@@ -1,0 +1,43 @@
+// Test infrastructure: the ttn_* entry points that include/ttns_amd.hpp calls, forwarded
+// to the oracle's orc_* C ABI, so the C++ drop-in driver also runs on the CPU with the
+// oracle as its backend (the fake backend of the test strategy).  Never part of the product.
+#include "../../oracle/orc_capi.h"
+#include "../../include/ttn_capi.h"
+
+#define O(T) reinterpret_cast<orc_##T*>
+#define OC(T) reinterpret_cast<const orc_##T*>
+
+extern "C" {
+const char* ttn_last_error(const ttn_ctx* c) { return orc_last_error(OC(ctx)(c)); }
+int ttn_ctx_create(int device, ttn_ctx** out) { return orc_ctx_create(device, reinterpret_cast<orc_ctx**>(out)); }
+int ttn_ctx_destroy(ttn_ctx* c) { return orc_ctx_destroy(O(ctx)(c)); }
+int ttn_tree_create(ttn_ctx* c, int64_t n, const int64_t* parent, const int64_t* hrank, const int64_t* cap,
+                    const int64_t* leaf, ttn_tree** out) {
+    return orc_tree_create(O(ctx)(c), n, parent, hrank, cap, leaf, reinterpret_cast<orc_tree**>(out));
+}
+int ttn_tree_destroy(ttn_tree* t) { return orc_tree_destroy(O(tree)(t)); }
+int ttn_sop_create(ttn_ctx* c, int64_t nmodes, const int64_t* dims, int64_t nterms, const double* coeff, int64_t nops,
+                   const int32_t* mode, const int32_t* kind, const int64_t* coff, const double* cdata,
+                   const int64_t* ioff, const int64_t* idata, const int64_t* roff, const int64_t* r, ttn_sop** out) {
+    return orc_sop_create(O(ctx)(c), nmodes, dims, nterms, coeff, nops, mode, kind, coff, cdata, ioff, idata, roff, r,
+                          reinterpret_cast<orc_sop**>(out));
+}
+int ttn_sop_destroy(ttn_sop* s) { return orc_sop_destroy(O(sop)(s)); }
+int ttn_integrator_create(ttn_ctx* c, const ttn_tree* t, const ttn_sop* s, const ttn_integ_params* p,
+                          ttn_integ** out) {
+    return orc_integrator_create(O(ctx)(c), OC(tree)(t), OC(sop)(s), p, reinterpret_cast<orc_integ**>(out));
+}
+int ttn_integrator_destroy(ttn_integ* i) { return orc_integrator_destroy(O(integ)(i)); }
+int ttn_state_set(ttn_integ* i, int64_t node, const double* d, int64_t r, int64_t c) {
+    return orc_state_set(O(integ)(i), node, d, r, c);
+}
+int ttn_state_get(ttn_integ* i, int64_t node, double* d, int64_t r, int64_t c) {
+    return orc_state_get(O(integ)(i), node, d, r, c);
+}
+int ttn_step(ttn_integ* i, double cr, double ci, int64_t n) { return orc_step(O(integ)(i), cr, ci, n); }
+int ttn_time(const ttn_integ* i, double* t) { return orc_time(OC(integ)(i), t); }
+int ttn_counters(const ttn_integ* i, int64_t* nh, int64_t* ns) { return orc_counters(OC(integ)(i), nh, ns); }
+int ttn_expect(ttn_integ* i, int64_t mode, const double* op, double* out) {
+    return orc_expect(O(integ)(i), mode, op, out);
+}
+}

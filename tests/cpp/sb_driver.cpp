@@ -1,0 +1,169 @@
+// Spin-boson driver written against the reference's C++ API (include/ttns_amd.hpp), the
+// way the reference's own driver sets up and runs a trajectory (src/spin_boson.cpp:447-649
+// for the tree, operator binding and initial state, :41-107 for the step loop).  Used by
+// the tests as the drop-in check: its <sigma_z(t)> must equal the oracle's on the same model.
+//
+// usage: sb_driver nmodes boson_dim nspf nsteps dt alpha wc seed krylov_dim
+// Bath: w_k = wc * 4 k / n (k = 1..n), g_k = sqrt(J(w_k) dw / pi), J(w) = pi/2 alpha w e^{-w/wc}.
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <iostream>
+#include <random>
+#include <vector>
+
+#include "ttns_amd.hpp"
+
+using namespace ttns;
+using complex_type = std::complex<double>;
+using backend_type = blas_backend;
+
+int main(int argc, char** argv) {
+    if (argc < 10) {
+        std::cerr << "usage: sb_driver nmodes boson_dim nspf nsteps dt alpha wc seed krylov_dim\n";
+        return 2;
+    }
+    const size_type nblocks = std::strtoul(argv[1], nullptr, 10);
+    const size_type nmax = std::strtoul(argv[2], nullptr, 10);
+    const size_type nspf = std::strtoul(argv[3], nullptr, 10), nspf_lower = nspf;
+    const int nsteps = std::atoi(argv[4]);
+    const double dt = std::atof(argv[5]), alpha = std::atof(argv[6]), wc = std::atof(argv[7]);
+    const unsigned seed = static_cast<unsigned>(std::strtoul(argv[8], nullptr, 10));
+    const size_type krylov_dim = std::strtoul(argv[9], nullptr, 10);
+    try {
+        const size_type nhilb = 2;
+        std::vector<double> wk(nblocks), gk(nblocks);
+        for (size_type k = 0; k < nblocks; ++k) wk[k] = wc * 4.0 * static_cast<double>(k + 1) / nblocks;
+        const double dw = nblocks > 1 ? wk[1] - wk[0] : wk[0];
+        for (size_type k = 0; k < nblocks; ++k) {
+            const double J = 0.5 * std::acos(-1.0) * alpha * wk[k] * std::exp(-wk[k] / wc);
+            gk[k] = std::sqrt(J * dw / std::acos(-1.0));
+        }
+        std::vector<size_type> mode_dimensions(nblocks + 1, nmax), nmodes(nblocks, nmax);
+        mode_dimensions[0] = nhilb;
+
+        // topology: root -> [system node -> primitive], [bath node -> balanced binary subtree]
+        auto build = [&](size_type top, size_type bottom) {
+            ntree<size_type> t{};
+            t.insert(1);
+            t().insert(mode_dimensions[0]);
+            t()[0].insert(mode_dimensions[0]);
+            t().insert(mode_dimensions[0]);
+            const size_type nlevels = static_cast<size_type>(std::log2(nmodes.size()));
+            ntree_builder<size_type>::htucker_subtree(t()[1], nmodes, 2, [top, bottom, nlevels](size_type l) {
+                size_type ret = 0;
+                if (l >= nlevels)
+                    ret = bottom;
+                else if (l == 0)
+                    ret = top;
+                else {
+                    const double rmax = std::log2(top), rmin = std::log2(bottom);
+                    ret = static_cast<size_type>(std::pow(2.0, ((nlevels - l) * (rmax - rmin)) / nlevels + rmin));
+                }
+                return ret;
+            });
+            ntree_builder<size_type>::sanitise_tree(t);
+            return t;
+        };
+        ntree<size_type> topology = build(nspf, nspf_lower), capacity = build(nspf, nspf_lower);
+        httensor<complex_type, backend_type> A(topology, capacity);
+
+        // Hamiltonian: Hsys on mode 0, Scoup on mode 0 for terms 2+2i, H0_i and Hc_i on mode i+1
+        matrix<complex_type> Hsys(2, 2), Scoup(2, 2), Sz(2, 2);
+        Hsys(0, 1) = Hsys(1, 0) = 1.0;
+        Scoup(0, 0) = 1.0;
+        Scoup(1, 1) = -1.0;
+        Sz = Scoup;
+        std::vector<size_type> terms_per_mode(nblocks + 1, 2);
+        sop_operator<complex_type, backend_type> H(1 + 2 * nblocks, mode_dimensions, terms_per_mode);
+        H.bind(ops::dense_matrix_operator<complex_type, backend_type>{Hsys}, {0}, 0);
+        std::vector<size_type> sigma_z_indices(nblocks);
+        for (size_type i = 0; i < nblocks; ++i) sigma_z_indices[i] = 1 + 2 * i + 1;
+        H.bind(ops::dense_matrix_operator<complex_type, backend_type>{Scoup}, sigma_z_indices, 0);
+        for (size_type i = 0; i < nblocks; ++i) {
+            diagonal_matrix<complex_type> H0;
+            for (size_type n = 0; n < nmax; ++n) H0.diag.push_back(wk[i] * static_cast<double>(n));
+            H.bind(ops::diagonal_matrix_operator<complex_type, backend_type>(H0), {1 + 2 * i}, i + 1);
+            csr_matrix<complex_type> Hc;
+            Hc.n = nmax;
+            Hc.rowptr.push_back(0);
+            for (size_type n = 0; n < nmax; ++n) {
+                if (n > 0) {
+                    Hc.colind.push_back(static_cast<int64_t>(n - 1));
+                    Hc.values.push_back(std::sqrt(static_cast<double>(n)) * gk[i]);
+                }
+                if (n + 1 < nmax) {
+                    Hc.colind.push_back(static_cast<int64_t>(n + 1));
+                    Hc.values.push_back(std::sqrt(static_cast<double>(n) + 1.0) * gk[i]);
+                }
+                Hc.rowptr.push_back(static_cast<int64_t>(Hc.values.size()));
+            }
+            H.bind(ops::sparse_matrix_operator<complex_type, backend_type>(Hc), {1 + 2 * i + 1}, i + 1);
+        }
+
+        // initial state: spin up x vacuum, random orthonormal unoccupied functions
+        std::mt19937 rng(seed);
+        for (auto& c : A) {
+            if (c.is_leaf()) {
+                const size_type D = c().shape(0), r = c().shape(1);
+                matrix<complex_type> ct(r, D);
+                if (c.leaf_index() == 0) {
+                    const complex_type psi0[2] = {1.0, 0.0};
+                    for (size_type j = 0; j < D; ++j) ct(0, j) = psi0[j];
+                } else {
+                    ct(0, 0) = 1.0;
+                }
+                std::uniform_real_distribution<double> dist(0, 2.0 * std::acos(-1.0));
+                std::normal_distribution<double> length_dist(0, 1);
+                for (size_type i = 1; i < r; ++i) {
+                    bool vector_generated = false;
+                    while (!vector_generated) {
+                        for (size_type j = 0; j < D; ++j) {
+                            const double theta = dist(rng);
+                            ct(i, j) = length_dist(rng) * complex_type(std::cos(theta), std::sin(theta));
+                        }
+                        double nrm = 0.0;
+                        for (size_type j = 0; j < D; ++j) nrm += std::norm(ct(i, j));
+                        for (size_type j = 0; j < D; ++j) ct(i, j) /= std::sqrt(nrm);
+                        for (size_type k = 0; k < i; ++k) {
+                            complex_type ov(0.0, 0.0);
+                            for (size_type j = 0; j < D; ++j) ov += std::conj(ct(k, j)) * ct(i, j);
+                            for (size_type j = 0; j < D; ++j) ct(i, j) -= ov * ct(k, j);
+                        }
+                        double nn = 0.0;
+                        for (size_type j = 0; j < D; ++j) nn += std::norm(ct(i, j));
+                        nn = std::sqrt(nn);
+                        if (nn > 1e-12) {
+                            for (size_type j = 0; j < D; ++j) ct(i, j) /= nn;
+                            vector_generated = true;
+                        }
+                    }
+                }
+                for (size_type j = 0; j < D; ++j)
+                    for (size_type i = 0; i < r; ++i) c()(j, i) = ct(i, j);
+            } else if (!c.is_root()) {
+                for (size_type i = 0; i < c().size(0); ++i)
+                    for (size_type j = 0; j < c().size(1); ++j) c()(i, j) = (i == j ? 1.0 : 0.0);
+            } else {
+                for (size_type i = 0; i < c().size(0); ++i)
+                    for (size_type j = 0; j < c().size(1); ++j) c()(i, j) = ((i == j) && (i == 0) ? 1.0 : 0.0);
+            }
+        }
+        A.set_is_orthogonalised();
+
+        fixed_step_projector_splitting_integrator<complex_type, backend_type> tdvp(A, H, krylov_dim, 1e-12, 1);
+        tdvp.dt() = dt;
+        ops::dense_matrix_operator<complex_type, backend_type> mop(Sz);
+        std::printf("%.17g %.17g\n", 0.0, std::real(tdvp.mel()(mop, 0, A)));
+        for (int s = 0; s < nsteps; ++s) {
+            tdvp(A, H, complex_type(0, -1));
+            std::printf("%.17g %.17g %zu\n", tdvp.t(), std::real(tdvp.mel()(mop, 0, A)),
+                        static_cast<size_t>(tdvp.nh_applications()));
+        }
+    } catch (const std::exception& ex) {
+        std::cerr << ex.what() << std::endl;
+        return 1;
+    }
+    return 0;
+}
