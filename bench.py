@@ -113,12 +113,13 @@ def cpu_baseline(insts, flops_per_step, budget_s):
     }
 
 
-def report(insts, igs, teams, dflops, p0, steps):
-    """Per-instance device phase times (ttn_profile, seconds per step), slowest first."""
+def report(insts, igs, teams, dflops, p0, steps, s0=None):
+    """Per-instance device phase times (ttn_profile, seconds per step), slowest first; for the
+    slowest instance also the H applications per node shape (node and bond propagations)."""
     rows = []
     for k, ig in enumerate(igs):
         p1 = ig.profile()
-        d = {n: (p1[n] - p0[k][n]) / 1e6 / steps for n in p1}
+        d = {n: (p1[n] - p0[k][n]) / (1 if n == "n_small" else 1e6) / steps for n in p1}
         rows.append((d["total"], k, d))
     rows.sort(reverse=True)
     for tot, k, d in rows:
@@ -127,6 +128,18 @@ def report(insts, igs, teams, dflops, p0, steps):
         print(f"[report] inst {k:2d} modes {len(m.wk):3d} d {m.boson_dim[0]:2d} r {m.nspf:2d} team {teams[k]:2d} "
               f"{gf:8.1f} GFLOP {tot:6.2f} s {gf / max(tot, 1e-9):6.1f} GFLOP/s | "
               + " ".join(f"{n} {v:.2f}" for n, v in d.items() if n != "total"), file=sys.stderr, flush=True)
+    if s0 is not None:
+        k = rows[0][1]
+        ig = igs[k]
+        st = (ig.node_stats() - s0[k]) / steps
+        groups = {}
+        for n, shp in enumerate(ig.shapes):
+            g = groups.setdefault(shp, np.zeros(5))
+            g += np.array([1, *st[n]])
+        for shp, g in sorted(groups.items(), key=lambda kv: -kv[1][1]):
+            print(f"[report] inst {k} nodes {shp[0]}x{shp[1]} x{int(g[0])}: node H-apps {g[1]:.0f} "
+                  f"(mean Krylov size {g[2] / max(g[1], 1):.1f}), bond H-apps {g[3]:.0f} "
+                  f"(mean size {g[4] / max(g[3], 1):.1f}) per step", file=sys.stderr, flush=True)
 
 
 def _cpu_model():
@@ -260,6 +273,7 @@ def main():
     print(f"[bench] warmup {warm} step(s) {time.perf_counter() - t0:.2f}s", file=sys.stderr, flush=True)
     f0 = np.array([ig.flops() for ig in igs])
     p0 = [ig.profile() for ig in igs] if args.report else None
+    s0 = [ig.node_stats() for ig in igs] if args.report else None
     t0 = time.perf_counter()
     batch.step(args.steps)
     wall = time.perf_counter() - t0
@@ -272,7 +286,7 @@ def main():
     print(f"[bench] timed {args.steps} step(s): wall {wall:.3f}s kernel {kernel_ms / 1e3:.3f}s "
           f"{flops / 1e9:.1f} GFLOP", file=sys.stderr, flush=True)
     if args.report:
-        report(insts, igs, batch.teams(), f1 - f0, p0, args.steps)
+        report(insts, igs, batch.teams(), f1 - f0, p0, args.steps, s0)
     sz = [float(ig.trajectory()[-1, 0]) for ig in igs]
     norm_dev = max(abs(float(ig.trajectory()[-1, 1]) - 1.0) for ig in igs)
 

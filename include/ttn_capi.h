@@ -163,8 +163,14 @@ int ttn_flops(const ttn_integ* integ, double* flops);
 /* Device wall-clock ticks (100 MHz) spent per phase of the sweep, 16 slots: node H-action,
  * bond H-action, Gram-Schmidt, small exponentials, QR gauges, single-particle updates,
  * mean-field updates, gauge applications, node Krylov total, bond Krylov total,
- * observables, whole kernel.  Instrumentation only. */
+ * observables, whole kernel, propagations run by the small-vector propagator; slot 13 counts
+ * those propagations.  Instrumentation only. */
 int ttn_profile(const ttn_integ* integ, int64_t* slots);
+
+/* Per node (out[4 n .. 4 n + 3], n in pre-order): H applications of its node propagations,
+ * their Krylov sizes summed over iterations, H applications of the bond propagations that
+ * use its operators, and their Krylov sizes summed.  Instrumentation only. */
+int ttn_node_stats(const ttn_integ* integ, int64_t* out);
 
 /* ---- workgroup teams (no reference counterpart: GPU execution tuning) ----
  * An instance is advanced by a team of workgroups on one XCD.  Operations on node
@@ -172,6 +178,10 @@ int ttn_profile(const ttn_integ* integ, int64_t* slots);
  * on its first workgroup.  teams_enabled = 0 runs every instance on one workgroup.
  * Results do not depend on these settings beyond rounding. */
 int ttn_ctx_set_tuning(ttn_ctx* ctx, int64_t team_min, int teams_enabled);
+/* Krylov propagations of at most small_max entries (capped at 512) run on one workgroup with the
+ * basis in registers and the operators in LDS; 0 runs every propagation on the general path.
+ * Same algorithm, results equal to rounding.  No reference counterpart (a launch knob). */
+int ttn_ctx_set_small(ttn_ctx* ctx, int64_t small_max);
 /* team size chosen for every instance by the last ttn_batch_step (sizes[n]) */
 int ttn_batch_teams(const ttn_batch* batch, int64_t* sizes);
 

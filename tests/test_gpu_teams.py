@@ -12,9 +12,11 @@ pytestmark = pytest.mark.gpu
 @pytest.fixture(scope="module")
 def spread(hip):
     """A second context on the same GPU whose operations are all spread over the team
-    (team_min = 1), so every team primitive runs even on small trees."""
+    (team_min = 1) and none runs on the small-vector propagator, so every team primitive
+    runs even on small trees."""
     lib = Library(hip.path)
     lib.set_tuning(team_min=1, teams=True)
+    lib.set_small(0)
     yield lib
     lib.close()
 

@@ -98,7 +98,9 @@ _SIGS = {
     "profile": [vp, i64p],
     "batch_last_step_ms": [vp, dblp],
     "ctx_set_tuning": [vp, i64, C.c_int],
+    "ctx_set_small": [vp, i64],
     "batch_teams": [vp, i64p],
+    "node_stats": [vp, i64p],
     "batch_set_graph": [vp, C.c_int],
     "batch_graph_stats": [vp, i64p, i64p],
     "node_create": [vp, i64, i64, i64, i64, dblp, dblp, dblp, dblp, C.POINTER(vp)],
@@ -157,6 +159,11 @@ class Library:
         team_min entries are spread over an instance's team; teams=False runs every
         instance on one workgroup."""
         self._call("ctx_set_tuning", self.ctx, int(team_min), int(bool(teams)))
+
+    def set_small(self, small_max: int = 512):
+        """Krylov propagations of at most small_max entries run on one workgroup with the
+        basis in registers (0: none)."""
+        self._call("ctx_set_small", self.ctx, int(small_max))
 
     def error(self) -> str:
         msg = self.fn["last_error"](self.ctx)
@@ -367,12 +374,22 @@ class Integrator:
         return out.value
 
     PROF_NAMES = ["heff", "hbond", "orth", "expm", "qr", "spf", "mf", "gauge", "expv_node", "expv_bond", "obs",
-                  "total"]
+                  "total", "small"]
 
     def profile(self) -> dict:
+        """Device time per sweep phase in microseconds; "n_small" counts the propagations
+        run by the small-vector propagator (their time is "small")."""
         out = np.zeros(16, dtype=np.int64)
         self.lib._call("profile", self.h, _ip(out))
-        return {k: out[i] / 100.0 for i, k in enumerate(self.PROF_NAMES)}  # microseconds
+        d = {k: out[i] / 100.0 for i, k in enumerate(self.PROF_NAMES)}
+        d["n_small"] = int(out[len(self.PROF_NAMES)])
+        return d
+
+    def node_stats(self) -> np.ndarray:
+        """per node: node H applications, their Krylov sizes summed, bond H applications, sizes"""
+        out = np.zeros(4 * len(self.shapes), dtype=np.int64)
+        self.lib._call("node_stats", self.h, _ip(out))
+        return out.reshape(-1, 4)
 
     def set_observable(self, mode: int, op):
         self._obs = _cplx_in(op)

@@ -367,6 +367,7 @@ void make_args(const ttn_integ* in, double cr, double ci, int64_t nsteps, ttnd::
     a.dt = in->p.dt;
     a.step0 = 0;
     a.team_min = in->ctx->team_min;
+    a.small_max = in->ctx->small_max;
     if (in->nsteps == 0) {
         a.prepare = 1;
         a.nladder = ladder(in->p.dt, a.ladder_dt);
@@ -689,8 +690,10 @@ int ttn_ctx_create(int device, ttn_ctx** out) {
         hip_check(hipEventCreate(&c->ev1), "hipEventCreate");
         team_census(c.get());
         // tuning knobs: TTN_TEAM_MIN (elements below which an operation runs on a team's
-        // leader alone), TTN_TEAMS=0 switches teams off
+        // leader alone), TTN_TEAMS=0 switches teams off, TTN_SMALL_MAX (largest propagation
+        // run in one workgroup's LDS, 0: none)
         if (const char* e = std::getenv("TTN_TEAM_MIN")) c->team_min = std::max<int64_t>(1, std::atoll(e));
+        if (const char* e = std::getenv("TTN_SMALL_MAX")) c->small_max = std::max<int64_t>(0, std::atoll(e));
         if (const char* e = std::getenv("TTN_TEAMS"))
             if (std::atoi(e) == 0) c->teams_ok = false;
     });
@@ -1016,6 +1019,13 @@ int ttn_profile(const ttn_integ* in, int64_t* slots) {
     });
 }
 
+int ttn_node_stats(const ttn_integ* in, int64_t* out) {
+    return guard(in->ctx, [&] {
+        const int n = in->topo.size();
+        hip_check(hipMemcpy(out, in->node_stats.p, 4 * n * sizeof(int64_t), hipMemcpyDeviceToHost), "stats");
+    });
+}
+
 int ttn_ctx_set_tuning(ttn_ctx* c, int64_t team_min, int teams_enabled) {
     return guard(c, [&] {
         check(team_min >= 1, ERR_INVALID, "team_min must be positive");
@@ -1024,6 +1034,13 @@ int ttn_ctx_set_tuning(ttn_ctx* c, int64_t team_min, int teams_enabled) {
             c->teams_ok = false;
         else
             team_census(c);
+    });
+}
+
+int ttn_ctx_set_small(ttn_ctx* c, int64_t small_max) {
+    return guard(c, [&] {
+        check(small_max >= 0, ERR_INVALID, "small_max must not be negative");
+        c->small_max = small_max;
     });
 }
 

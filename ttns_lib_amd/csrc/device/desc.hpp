@@ -18,7 +18,10 @@ constexpr int NWORK = 6;       // generic work buffers, each max node size
 
 // device wall-clock phase slots (100 MHz ticks, accumulated by thread 0)
 enum ProfSlot { PROF_HEFF = 0, PROF_HBOND, PROF_ORTH, PROF_EXPM, PROF_QR, PROF_SPF, PROF_MF, PROF_GAUGE,
-                PROF_EXPV_NODE, PROF_EXPV_BOND, PROF_OBS, PROF_TOTAL, PROF_NSLOTS = 16 };
+                PROF_EXPV_NODE, PROF_EXPV_BOND, PROF_OBS, PROF_TOTAL,
+                PROF_SMALL,    // ticks in propagations run by the small-vector propagator
+                PROF_NSMALL,   // number of those propagations (a count, not ticks)
+                PROF_NSLOTS = 16 };
 
 struct DNode {
     int parent, child_id, nchild, leaf;  // leaf = primitive mode index, -1 for branch nodes
@@ -142,6 +145,7 @@ struct RunArgs {
     double dt;
     int64_t step0;                 // index of the first step of this call (trajectory slot)
     int64_t team_min;              // operations on fewer elements run on a team's leader alone
+    int64_t small_max;             // propagations of at most this many entries run in LDS (0: never)
 };
 
 }  // namespace ttnd

@@ -15,12 +15,16 @@
 
 namespace ttnd {
 
+struct SmallLds;
+
 struct Ctx {
     const DInst& I;
     Shared& sh;
     Team* T;
     double2* base;
     int64_t team_min;
+    SmallLds* small = nullptr;  // LDS of the small-vector propagator
+    int small_max = 0;          // propagations of at most this many entries use it
     // where: node whose propagation is running, bond: 1 for its bond matrix
     int where = 0, bond = 0;
     __device__ Ctx(const DInst& inst, Shared& s, Team* t, int64_t tmin)
@@ -513,6 +517,393 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
     return napply;
 }
 
+// ---------------------------------------------------------------- small propagations
+// A propagation of at most SMALL_N entries (leaf nodes, bond matrices of small bonds, every
+// node of a small tree) runs on one workgroup with the Krylov basis in registers (SMALL_E
+// entries of every basis vector per thread) and the vector, the intermediates and every
+// operator matrix of its H-action in LDS: a Krylov iteration then costs workgroup barriers
+// instead of dependent round trips to L2.  Same algorithm as expv (Arnoldi, CGS2, Saad
+// estimate, Expokit sub-steps, Pade(13)); the H-action mirrors heff_node / heff_bond term by
+// term, with each output entry summed over k in order.
+constexpr int SMALL_E = 2;
+constexpr int SMALL_N = SMALL_E * NT;
+constexpr int SMALL_OPS = 2048;  // LDS operator arena (double2)
+
+struct SmallLds {
+    double2 v[SMALL_N], w[SMALL_N], t1[SMALL_N], t2[SMALL_N], t3[SMALL_N];
+    double2 ops[SMALL_OPS];
+};
+
+__device__ __forceinline__ int prim_count(const DK& k) { return k.kind == 0 ? k.dim * k.dim : (k.kind == 1 ? k.dim : k.nnz); }
+
+// Every operator matrix the H-action reads, in evaluation order: f(arena offset of the
+// matrix, element count).  bond: the bond H-action with node n's operators.
+template <class F>
+__device__ void small_walk(const Ctx& c, int n, bool bond, F&& f) {
+    const DNode& nd = c.node(n);
+    const int rr = nd.hrank * nd.hrank;
+    for (int ti = 0; ti < nd.nterms; ++ti) {
+        const DTerm& tm = c.I.terms[nd.term0 + ti];
+        if (bond) {
+            if (!tm.id_spf) f(tm.spf, rr);
+            if (!tm.id_mf) f(tm.mf, rr);
+            continue;
+        }
+        if (tm.id_spf) {
+            if (!tm.id_mf) f(tm.mf, rr);
+            continue;
+        }
+        for (int i = 0; i < tm.nsub; ++i) {
+            const DSub& sb = c.I.subs[tm.sub0 + i];
+            if (sb.nk == 0) continue;
+            if (nd.leaf >= 0) {
+                const DK& k = c.I.ks[sb.k0];
+                f(k.mat, prim_count(k));
+            } else {
+                for (int q = 0; q < sb.nk; ++q) {
+                    const DK& k = c.I.ks[sb.k0 + q];
+                    f(k.mat, k.dim * k.dim);
+                }
+            }
+        }
+        if (!tm.id_mf) f(tm.mf, rr);
+    }
+}
+
+__device__ bool small_eligible(const Ctx& c, int n, bool bond, int64_t size) {
+    if (c.small == nullptr || size > c.small_max || size > SMALL_N) return false;
+    int total = 0;
+    small_walk(c, n, bond, [&](int64_t, int cnt) { total += cnt; });
+    return total <= SMALL_OPS;
+}
+
+// ---- LDS elementary operations (all threads of the workgroup; each ends with a barrier)
+__device__ __forceinline__ double2 s_out(double2 acc, double2 alpha, double2 old, bool accumulate) {
+    const double2 v = cmul(alpha, acc);
+    return accumulate ? cadd(old, v) : v;
+}
+__device__ void s_axpby(double2* y, double2 a, const double2* x, int n, bool accumulate) {
+    for (int e = threadIdx.x; e < n; e += NT) y[e] = accumulate ? cadd(y[e], cmul(a, x[e])) : cmul(a, x[e]);
+    __syncthreads();
+}
+__device__ void s_zero(double2* y, int n) {
+    for (int e = threadIdx.x; e < n; e += NT) y[e] = cz();
+    __syncthreads();
+}
+// dst(l,i,r) (=|+=) alpha sum_j op(i,j) src(l,j,r)
+__device__ void s_mode_contract(const double2* op, int L, int d, int R, const double2* src, double2* dst, double2 alpha,
+                                bool accumulate) {
+    const int total = L * d * R;
+    for (int e = threadIdx.x; e < total; e += NT) {
+        const int l = e / (d * R), rem = e - l * d * R, i = rem / R, r = rem - i * R;
+        double2 acc = cz();
+        for (int j = 0; j < d; ++j) acc = cadd(acc, cmul(op[i * d + j], src[(l * d + j) * R + r]));
+        dst[e] = s_out(acc, alpha, dst[e], accumulate);
+    }
+    __syncthreads();
+}
+// dst (=|+=) alpha X M^T, X: rows x r
+__device__ void s_mul_t(const double2* X, int rows, int r, const double2* M, double2* dst, double2 alpha,
+                        bool accumulate) {
+    const int total = rows * r;
+    for (int e = threadIdx.x; e < total; e += NT) {
+        const int x = e / r, i = e - x * r;
+        double2 acc = cz();
+        for (int k = 0; k < r; ++k) acc = cadd(acc, cmul(X[x * r + k], M[i * r + k]));
+        dst[e] = s_out(acc, alpha, dst[e], accumulate);
+    }
+    __syncthreads();
+}
+// y (=|+=) alpha op x for a primitive leaf operator (op values in LDS, CSR indices in the table)
+__device__ void s_prim(const DK& k, const double2* op, const int64_t* ints, const double2* x, double2* y, int ncols,
+                       double2 alpha, bool accumulate) {
+    const int d = k.dim, total = d * ncols;
+    for (int e = threadIdx.x; e < total; e += NT) {
+        const int i = e / ncols, cc = e - i * ncols;
+        double2 acc = cz();
+        if (k.kind == 0) {
+            for (int j = 0; j < d; ++j) acc = cadd(acc, cmul(op[i * d + j], x[j * ncols + cc]));
+        } else if (k.kind == 1) {
+            acc = cmul(op[i], x[e]);
+        } else {
+            const int64_t* rowptr = ints + k.idx;
+            const int64_t* colind = rowptr + d + 1;
+            for (int64_t q = rowptr[i]; q < rowptr[i + 1]; ++q) acc = cadd(acc, cmul(op[q], x[colind[q] * ncols + cc]));
+        }
+        y[e] = s_out(acc, alpha, y[e], accumulate);
+    }
+    __syncthreads();
+}
+
+// heff_node on LDS data (v, res in LDS, operators staged in L.ops in walk order)
+__device__ void small_heff_node(const Ctx& c, int n, SmallLds& L, const double2* v, double2* res) {
+    const DNode& nd = c.node(n);
+    const int r = nd.hrank, D = nd.dimen, size = nd.size, rr = r * r;
+    int cur = 0;
+    bool first = true;
+    for (int ti = 0; ti < nd.nterms; ++ti) {
+        const DTerm& tm = c.I.terms[nd.term0 + ti];
+        if (tm.id_spf) {
+            const double2 coeff = cmk(tm.ire, tm.iim);
+            if (tm.id_mf) {
+                s_axpby(res, coeff, v, size, !first);
+            } else {
+                s_mul_t(v, D, r, L.ops + cur, res, coeff, !first);
+                cur += rr;
+            }
+            first = false;
+            continue;
+        }
+        for (int i = 0; i < tm.nsub; ++i) {
+            const DSub& sb = c.I.subs[tm.sub0 + i];
+            const double2 ci = (i == 0) ? (tm.nsub > 1 ? cmk(sb.are, sb.aim) : cmk(tm.cre, tm.cim)) : cmk(sb.are, sb.aim);
+            double2* dst = tm.id_mf ? res : L.t3;
+            const bool acc = tm.id_mf ? !first : i > 0;
+            if (sb.nk == 0) {
+                s_axpby(dst, ci, v, size, acc);
+            } else if (nd.leaf >= 0) {
+                const DK& k = c.I.ks[sb.k0];
+                s_prim(k, L.ops + cur, c.I.ints, v, dst, r, ci, acc);
+                cur += prim_count(k);
+            } else {
+                const double2* src = v;
+                for (int q = 0; q < sb.nk; ++q) {
+                    const bool last = q == sb.nk - 1;
+                    double2* out = last ? dst : (q % 2 == 0 ? L.t1 : L.t2);
+                    const DK& k = c.I.ks[sb.k0 + q];
+                    int64_t LL, RR;
+                    mode_view(nd, k.nu, LL, RR);
+                    s_mode_contract(L.ops + cur, static_cast<int>(LL), nd.dims[k.nu], static_cast<int>(RR), src, out,
+                                    last ? ci : cmk(1.0, 0.0), last && acc);
+                    cur += k.dim * k.dim;
+                    src = out;
+                }
+            }
+            if (tm.id_mf) first = false;
+        }
+        if (!tm.id_mf) {
+            s_mul_t(L.t3, D, r, L.ops + cur, res, cmk(1.0, 0.0), !first);
+            cur += rr;
+            first = false;
+        }
+    }
+    if (first) s_zero(res, size);
+}
+
+// heff_bond on LDS data
+__device__ void small_heff_bond(const Ctx& c, int n, SmallLds& L, const double2* v, double2* res) {
+    const DNode& nd = c.node(n);
+    const int r = nd.hrank, sz = r * r;
+    int cur = 0;
+    s_zero(res, sz);
+    for (int ti = 0; ti < nd.nterms; ++ti) {
+        const DTerm& tm = c.I.terms[nd.term0 + ti];
+        const double2* x = v;
+        if (!tm.id_spf) {
+            s_mode_contract(L.ops + cur, 1, r, r, v, L.t1, cmk(1.0, 0.0), false);
+            cur += sz;
+            x = L.t1;
+        }
+        const double2 coeff = cmk(tm.cre, tm.cim);
+        if (!tm.id_mf) {
+            s_mul_t(x, r, r, L.ops + cur, res, coeff, true);
+            cur += sz;
+        } else {
+            s_axpby(res, coeff, x, sz, true);
+        }
+    }
+}
+
+// x <- exp(coeff tau H) x for a vector of n <= SMALL_N entries on one workgroup (team of one)
+template <int KM>
+__device__ int64_t expv_small(Ctx& c, int n_op, bool bond, double2* x, int n, double2 coeff, double tau, int64_t& sumj) {
+    if (tau == 0.0 || n == 0) return 0;
+    SmallLds& L = *c.small;
+    Shared& sh = c.sh;
+    const int kdim = c.I.kdim;
+    const double tol = c.I.tol;
+    // stage the operators (written by other team members: coherent loads)
+    int cur = 0;
+    small_walk(c, n_op, bond, [&](int64_t off, int cnt) {
+        for (int e = threadIdx.x; e < cnt; e += NT) L.ops[cur + e] = ldc(c.base + off + e);
+        cur += cnt;
+    });
+    double2 xe[SMALL_E];
+#pragma unroll
+    for (int e = 0; e < SMALL_E; ++e) {
+        const int q = threadIdx.x + e * NT;
+        xe[e] = q < n ? ldc(x + q) : cz();
+    }
+    __syncthreads();
+    double2 vb[KM + 1][SMALL_E];
+    int64_t napply = 0;
+    double s_done = 0.0, s_next = 1.0;
+    int guard = 0;
+    bool ok = true;
+    while (s_done < 1.0 && guard++ < 4096) {
+        double s_step = fmin(s_next, 1.0 - s_done);
+        double b2[1] = {0.0}, bo[1];
+#pragma unroll
+        for (int e = 0; e < SMALL_E; ++e) b2[0] += cabs2(xe[e]);
+        block_sum<1>(sh, b2, bo);
+        const double beta = sqrt(bo[0]);
+        if (!isfinite(beta)) {
+            c.fail(FAIL_INPUT_NONFINITE);
+            ok = false;
+            break;
+        }
+        if (beta == 0.0) break;
+#pragma unroll
+        for (int e = 0; e < SMALL_E; ++e) vb[0][e] = cscale(xe[e], 1.0 / beta);
+        for (int e = threadIdx.x; e < (MAXK + 1) * MAXK; e += NT) sh.Hm[e] = cz();
+        __syncthreads();
+        int m = 0;
+        double hn = 0.0, err = 0.0;
+        bool found = false;
+        auto small_exp = [&](int mm, double sf) -> double {
+            const double2 f = cscale(coeff, tau * sf);
+            for (int e = threadIdx.x; e < mm * mm; e += NT) {
+                const int i = e / mm, j = e % mm;
+                sh.S[e] = cmul(f, sh.Hm[i * MAXK + j]);
+            }
+            __syncthreads();
+            if ((threadIdx.x >> 6) == 0) wave_expm(sh, sh.S, sh.F, mm);
+            __syncthreads();
+            return hn * cabsv(sh.F[(mm - 1) * mm]);
+        };
+        for (int j = 0; j < kdim; ++j) {
+            // V_j into LDS for the H-action (static register indexing: unrolled select)
+#pragma unroll
+            for (int i = 0; i <= KM; ++i)
+                if (i == j)
+#pragma unroll
+                    for (int e = 0; e < SMALL_E; ++e) {
+                        const int q = threadIdx.x + e * NT;
+                        if (q < n) L.v[q] = vb[i][e];
+                    }
+            __syncthreads();
+            if (bond) {
+                PROFILE(PROF_HBOND, small_heff_bond(c, n_op, L, L.v, L.w));
+            } else {
+                PROFILE(PROF_HEFF, small_heff_node(c, n_op, L, L.v, L.w));
+            }
+            ++napply;
+            sumj += j + 1;
+            double2 wv[SMALL_E];
+#pragma unroll
+            for (int e = 0; e < SMALL_E; ++e) {
+                const int q = threadIdx.x + e * NT;
+                wv[e] = q < n ? L.w[q] : cz();
+            }
+            // CGS2 in registers, the coefficients through workgroup sums
+#pragma unroll
+            for (int pass = 0; pass < 2; ++pass) {
+                double p[2 * KM];
+#pragma unroll
+                for (int i = 0; i < 2 * KM; ++i) p[i] = 0.0;
+#pragma unroll
+                for (int i = 0; i < KM; ++i)
+                    if (i <= j)
+#pragma unroll
+                        for (int e = 0; e < SMALL_E; ++e) {
+                            const double2 pr = cmulc(vb[i][e], wv[e]);
+                            p[2 * i] += pr.x;
+                            p[2 * i + 1] += pr.y;
+                        }
+                block_sum_lds<2 * KM>(sh, p, sh.hb[pass]);
+                const double* h = sh.hb[pass];
+#pragma unroll
+                for (int i = 0; i < KM; ++i)
+                    if (i <= j)
+#pragma unroll
+                        for (int e = 0; e < SMALL_E; ++e) wv[e] = csub(wv[e], cmul(cmk(h[2 * i], h[2 * i + 1]), vb[i][e]));
+            }
+            double nn[1] = {0.0}, nno[1];
+#pragma unroll
+            for (int e = 0; e < SMALL_E; ++e) nn[0] += cabs2(wv[e]);
+            block_sum<1>(sh, nn, nno);
+            if (threadIdx.x <= j) {
+                const int i = threadIdx.x;
+                sh.Hm[i * MAXK + j] = cadd(cadd(sh.Hm[i * MAXK + j], cmk(sh.hb[0][2 * i], sh.hb[0][2 * i + 1])),
+                                           cmk(sh.hb[1][2 * i], sh.hb[1][2 * i + 1]));
+            }
+            __syncthreads();
+            hn = sqrt(nno[0]);
+            if (!isfinite(hn)) {
+                c.fail(FAIL_KRYLOV_NONFINITE);
+                ok = false;
+                break;
+            }
+            if (j + 1 < kdim && threadIdx.x == 0) sh.Hm[(j + 1) * MAXK + j] = cmk(hn, 0.0);
+            __syncthreads();
+            m = j + 1;
+            double lead = hn;
+            for (int i = 0; i < j; ++i)
+                lead *= cabsv(coeff) * fabs(tau * s_step) * cabsv(sh.Hm[(i + 1) * MAXK + i]) / (i + 1);
+            bool next_vec = j + 1 < kdim;
+            if (!(lead > 1e3 * tol && j + 1 < kdim)) {
+                PROFILE(PROF_EXPM, err = small_exp(m, s_step));
+                if (err <= tol) {
+                    found = true;
+                    next_vec = false;
+                }
+            }
+            if (next_vec) {
+#pragma unroll
+                for (int i = 1; i <= KM; ++i)
+                    if (i == j + 1)
+#pragma unroll
+                        for (int e = 0; e < SMALL_E; ++e) vb[i][e] = cscale(wv[e], 1.0 / hn);
+            }
+            if (found) break;
+        }
+        if (!ok) break;
+        for (int k = 0; !found && k < 8; ++k) {
+            s_step *= step_factor(tol, err, m, 0.0625, 0.9);
+            PROFILE(PROF_EXPM, err = small_exp(m, s_step));
+            found = err <= tol;
+        }
+        for (int k = 0; !found && k < 60; ++k) {
+            s_step *= 0.5;
+            PROFILE(PROF_EXPM, err = small_exp(m, s_step));
+            found = err <= tol;
+        }
+        if (!found) {
+            c.fail(FAIL_KRYLOV_STEP);
+            ok = false;
+            break;
+        }
+#pragma unroll
+        for (int e = 0; e < SMALL_E; ++e) {
+            double2 sacc = cz();
+#pragma unroll
+            for (int i = 0; i < KM; ++i)
+                if (i < m) sacc = cadd(sacc, cmul(sh.F[i * m], vb[i][e]));
+            xe[e] = cscale(sacc, beta);
+        }
+        __syncthreads();
+        s_done += s_step;
+        s_next = s_step * step_factor(tol, err, m, 1.0, 2.0);
+    }
+#pragma unroll
+    for (int e = 0; e < SMALL_E; ++e) {
+        const int q = threadIdx.x + e * NT;
+        if (q < n) x[q] = xe[e];
+    }
+    __syncthreads();
+    return napply;
+}
+
+__device__ void count_small(const Ctx& c, uint64_t t0) {
+    prof_add(c, PROF_SMALL, t0);
+    if (c.I.prof && c.writer()) c.I.prof[PROF_NSMALL] += 1;
+}
+
+__device__ int64_t run_small(Ctx& c, int n_op, bool bond, double2* x, int n, double2 coeff, double tau, int64_t& sumj) {
+    if (c.I.kdim <= 12) return expv_small<12>(c, n_op, bond, x, n, coeff, tau, sumj);
+    return expv_small<MAXK>(c, n_op, bond, x, n, coeff, tau, sumj);
+}
+
 // ---------------------------------------------------------------- sweeps
 struct SweepState {
     double2 coeff;
@@ -526,11 +917,17 @@ __device__ void node_update(Ctx& c, SweepState& st, int n) {
     c.bond = 0;
     int64_t na = 0, sumj = 0;
     const uint64_t pt0 = prof_now(c);
-    on_team(c, nd.size, [&](Ctx& cc) {
-        na = expv(
-            cc, [&](const double2* v, double2* res) { PROFILE(PROF_HEFF, heff_node(cc, n, v, res)); },
-            cc.base + nd.A, nd.size, st.coeff, st.sub_dt / 2.0, sumj);
-    });
+    if (small_eligible(c, n, false, nd.size)) {
+        // one workgroup (the leader, other members wait at the barrier that follows)
+        on_team(c, 0, [&](Ctx& cc) { na = run_small(cc, n, false, cc.base + nd.A, nd.size, st.coeff, st.sub_dt / 2.0, sumj); });
+        count_small(c, pt0);
+    } else {
+        on_team(c, nd.size, [&](Ctx& cc) {
+            na = expv(
+                cc, [&](const double2* v, double2* res) { PROFILE(PROF_HEFF, heff_node(cc, n, v, res)); },
+                cc.base + nd.A, nd.size, st.coeff, st.sub_dt / 2.0, sumj);
+        });
+    }
     prof_add(c, PROF_EXPV_NODE, pt0);
     st.nh += na;
     if (c.writer()) {
@@ -545,11 +942,18 @@ __device__ void bond_update(Ctx& c, SweepState& st, int n, int ops) {
     c.bond = 1;
     int64_t na = 0, sumj = 0;
     const uint64_t pt0 = prof_now(c);
-    on_team(c, static_cast<int64_t>(r) * r, [&](Ctx& cc) {
-        na = expv(
-            cc, [&](const double2* v, double2* res) { PROFILE(PROF_HBOND, heff_bond(cc, ops, v, res)); },
-            cc.base + cc.node(n).R, static_cast<int64_t>(r) * r, st.coeff, -st.sub_dt / 2.0, sumj);
-    });
+    if (small_eligible(c, ops, true, static_cast<int64_t>(r) * r)) {
+        on_team(c, 0, [&](Ctx& cc) {
+            na = run_small(cc, ops, true, cc.base + cc.node(n).R, r * r, st.coeff, -st.sub_dt / 2.0, sumj);
+        });
+        count_small(c, pt0);
+    } else {
+        on_team(c, static_cast<int64_t>(r) * r, [&](Ctx& cc) {
+            na = expv(
+                cc, [&](const double2* v, double2* res) { PROFILE(PROF_HBOND, heff_bond(cc, ops, v, res)); },
+                cc.base + cc.node(n).R, static_cast<int64_t>(r) * r, st.coeff, -st.sub_dt / 2.0, sumj);
+        });
+    }
     prof_add(c, PROF_EXPV_BOND, pt0);
     if (c.writer()) {
         c.I.node_stats[4 * ops + 2] += na;
@@ -676,6 +1080,7 @@ __device__ __forceinline__ int xcd_id() {
 __global__ __launch_bounds__(NT) void tdvp_run_kernel(const DInst* insts, RunArgs a, const TeamPlan* plan,
                                                       TeamWords* words, unsigned* xcd_fill) {
     __shared__ Shared sh;
+    __shared__ SmallLds small;
     __shared__ int visits[1024];
     __shared__ int slot_info[3];
     if (plan == nullptr) {
@@ -733,6 +1138,8 @@ __global__ __launch_bounds__(NT) void tdvp_run_kernel(const DInst* insts, RunArg
         if (sh.fail != 0) return;
     }
     Ctx c(I, sh, &T, a.team_min);
+    c.small = &small;
+    c.small_max = a.small_max;
     const uint64_t prof_t0 = prof_now(c);
     SweepState st;
     st.coeff = a.coeff;
