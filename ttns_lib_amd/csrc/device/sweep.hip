@@ -387,6 +387,7 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
             H(V + j * ld, w);
             ++napply;
             sumj += j + 1;
+            const uint64_t ot0 = prof_now(c);
             // classical Gram-Schmidt applied twice (CGS2): the second pass's inner
             // products accumulate while the first correction is written, the norm while
             // the second one is written (three sweeps over the basis instead of five)
@@ -458,6 +459,7 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
                     cadd(cadd(sh.Hm[i * MAXK + j], cmk(h1[2 * i], h1[2 * i + 1])), cmk(h2[2 * i], h2[2 * i + 1]));
             }
             __syncthreads();
+            prof_add(c, PROF_ORTH, ot0);
             hn = sqrt(nno[0]);
             if (!isfinite(hn)) {
                 c.fail(FAIL_KRYLOV_NONFINITE);
@@ -789,6 +791,7 @@ __device__ int64_t expv_small(Ctx& c, int n_op, bool bond, double2* x, int n, do
             }
             ++napply;
             sumj += j + 1;
+            const uint64_t ot0 = prof_now(c);
             double2 wv[SMALL_E];
 #pragma unroll
             for (int e = 0; e < SMALL_E; ++e) {
@@ -828,6 +831,7 @@ __device__ int64_t expv_small(Ctx& c, int n_op, bool bond, double2* x, int n, do
                                            cmk(sh.hb[1][2 * i], sh.hb[1][2 * i + 1]));
             }
             __syncthreads();
+            prof_add(c, PROF_ORTH, ot0);
             hn = sqrt(nno[0]);
             if (!isfinite(hn)) {
                 c.fail(FAIL_KRYLOV_NONFINITE);

@@ -200,8 +200,17 @@ __device__ void team_reduce_lds(Team& T, double* v, int count) {
     T.sync();
     const double* all = T.part + static_cast<int64_t>(T.pbuf) * TEAM_MAX * PART_K;
     for (int e = threadIdx.x; e < count; e += NT) {
+        // the members' partials are loaded 8 at a time (independent loads in flight) and
+        // added in rank order
         double s = 0.0;
-        for (int r = 0; r < T.size; ++r) s += ldc(all + r * PART_K + e);
+        for (int r0 = 0; r0 < T.size; r0 += 8) {
+            double p[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) p[u] = r0 + u < T.size ? ldc(all + (r0 + u) * PART_K + e) : 0.0;
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (r0 + u < T.size) s += p[u];
+        }
         v[e] = s;
     }
     __syncthreads();
@@ -240,7 +249,13 @@ __device__ double team_max(Team& T, double v) {
     T.sync();
     const double* all = T.part + static_cast<int64_t>(T.pbuf) * TEAM_MAX * PART_K;
     m = ldc(all);
-    for (int r = 1; r < T.size; ++r) m = fmax(m, ldc(all + r * PART_K));
+    for (int r0 = 1; r0 < T.size; r0 += 8) {
+        double p[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) p[u] = r0 + u < T.size ? ldc(all + (r0 + u) * PART_K) : m;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) m = fmax(m, p[u]);
+    }
     __syncthreads();
     T.pbuf ^= 1;
     return m;
@@ -466,7 +481,15 @@ __device__ void t_outer_contract(Team& T, const double2* A, const double2* B, in
     const int64_t dd = static_cast<int64_t>(d) * d;
     for (int64_t e = T.tid(); e < dd; e += T.nthr()) {
         double2 s = cz();
-        for (int r = 0; r < T.size; ++r) s = cadd(s, ldc(T.tiles + static_cast<int64_t>(r) * TILE_MAX + e));
+        for (int r0 = 0; r0 < T.size; r0 += 8) {
+            double2 p[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                p[u] = r0 + u < T.size ? ldc(T.tiles + static_cast<int64_t>(r0 + u) * TILE_MAX + e) : cz();
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (r0 + u < T.size) s = cadd(s, p[u]);
+        }
         const double2 v = cmul(alpha, s);
         out[e] = accumulate ? cadd(ldc(out + e), v) : v;
     }
