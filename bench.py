@@ -218,6 +218,71 @@ def bench_c2(args):
     print(json.dumps(out))
 
 
+def bench_c4grad(args):
+    """Gradient workload: d<sigma_z(T)>/d(g_k, w_k) for the 64-mode Ohmic bath (alpha 0.2,
+    wc 5, T = 0, boson dim 12, bond dim 24, dt 0.05) by central differences, h = 1e-6: the
+    257 trajectories (base and +-h in each of the 128 parameters) advance together in one
+    batched launch per step.  One "step" is one TDVP step of all 257 trajectories; a full
+    gradient at T = 10 is 200 of them."""
+    import ttns_lib_amd
+    from ttns_lib_amd import workloads as W
+    from ttns_lib_amd.capi import SIGMA, Batch, Integrator, sb_problem
+    from ttns_lib_amd.grad import perturbed_models
+
+    lib = ttns_lib_amd.load_library()
+    inst = W.c4_instance()
+    models = perturbed_models(inst.model, 1e-6)
+    keep, igs = [], []
+    for m in models:
+        tree, sop = sb_problem(lib, m)
+        ig = Integrator(lib, tree, sop, dt=inst.dt, krylov_dim=inst.krylov_dim, krylov_tol=1e-14)
+        ig.set_sb_initial_state(m)
+        keep.append((tree, sop))
+        igs.append(ig)
+    batch = Batch(lib, igs)
+    warm = max(1, args.warmup)
+    t0 = time.perf_counter()
+    batch.step(warm)
+    print(f"[bench] c4grad warmup {warm} step(s) {time.perf_counter() - t0:.2f}s", file=sys.stderr, flush=True)
+    f0 = sum(ig.flops() for ig in igs)
+    t0 = time.perf_counter()
+    batch.step(args.steps)
+    wall = time.perf_counter() - t0
+    kernel_ms = batch.last_step_ms()
+    flops = sum(ig.flops() for ig in igs) - f0
+    vals = np.array([ig.expect(0, SIGMA["z"]).real for ig in igs])
+    n = len(inst.model.wk)
+    d = (vals[1::2] - vals[2::2]) / 2e-6
+    achieved = flops / (kernel_ms / 1e3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": len(igs) * args.steps / wall,
+        "unit": "instance-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": warm,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Ohmic bath discretisation, random orthonormal unoccupied functions)",
+        "config": {"workload": "c4grad: central-difference gradient, 64-mode Ohmic alpha 0.2 wc 5 T=0, boson 12, "
+                               "bond 24, dt 0.05, 257 trajectories in one batch", "instances": len(igs),
+                   "krylov_dim": inst.krylov_dim, "krylov_tol": 1e-14},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "kernel": "ttnd::tdvp_run_kernel",
+                     "flops_timed": flops},
+        "gradient": {"T": float(igs[0].time()), "sigma_z": float(vals[0]),
+                     "max_abs_dg": float(np.abs(d[:n]).max()), "max_abs_dw": float(np.abs(d[n:]).max()),
+                     "projected_s_for_T10": 200 * wall / args.steps},
+        "plain_gpu": None,
+        "cpu_baseline": None,
+    }
+    batch.destroy()
+    print(json.dumps(out))
+
+
 def bench_copy(args):
     """Calibration for HBM traffic counters: a device-to-device copy of 1 GiB (read 1 GiB,
     write 1 GiB per repetition) by torch's elementwise copy kernel."""
@@ -258,6 +323,9 @@ def main():
         return
     if args.workload == "copy":
         bench_copy(args)
+        return
+    if args.workload == "c4grad":
+        bench_c4grad(args)
         return
 
     import ttns_lib_amd
