@@ -27,9 +27,9 @@ def closed_form(w, g, t):
     return val, dg, dw
 
 
-def check_closed_form(r):
+def check_closed_form(r, value_tol=1e-12):
     val, dg, dw = closed_form(W, G, NSTEPS * DT)
-    assert abs(r.value - val) < 1e-12
+    assert abs(r.value - val) < value_tol
     # FD truncation is O(h^2) ~ 1e-12; rounding noise / h ~ 1e-8
     assert np.all(np.abs(r.dg - dg) < 1e-7 * np.abs(dg).max())
     assert np.all(np.abs(r.dw - dw) < 1e-7 * np.abs(dg).max())
@@ -41,6 +41,13 @@ def test_fd_gradient_closed_form(orc):
     r = observable_and_gradient(orc, dephasing_model(), NSTEPS, DT, op="x", batched=False)
     assert r.instances == 1 + 4 * len(W) and not r.batched
     check_closed_form(r)
+
+
+def test_gradient_without_base_trajectory(orc):
+    r = observable_and_gradient(orc, dephasing_model(), NSTEPS, DT, op="x", batched=False, include_base=False)
+    assert r.instances == 4 * len(W)
+    # the value is the mean of the g_0 +- h pair: off by h^2/2 d2<O>/dg_0^2 ~ 1e-11
+    check_closed_form(r, value_tol=1e-10)
 
 
 def test_autograd_function(orc):

@@ -42,9 +42,10 @@ def _perturbed(model: SpinBoson, which: str, k: int, delta: float) -> SpinBoson:
     return dataclasses.replace(model, wk=wk, gk=gk)
 
 
-def perturbed_models(model: SpinBoson, h: float):
-    """[model, g_0 + h, g_0 - h, ..., g_{N-1} - h, w_0 + h, ..., w_{N-1} - h]"""
-    models = [model]
+def perturbed_models(model: SpinBoson, h: float, include_base: bool = True):
+    """[model, g_0 + h, g_0 - h, ..., g_{N-1} - h, w_0 + h, ..., w_{N-1} - h] (without the
+    leading unperturbed model when include_base is False)"""
+    models = [model] if include_base else []
     for which in ("g", "w"):
         for k in range(len(model.wk)):
             models.append(_perturbed(model, which, k, +h))
@@ -54,11 +55,14 @@ def perturbed_models(model: SpinBoson, h: float):
 
 def observable_and_gradient(lib: Library, model: SpinBoson, nsteps: int, dt: float, *, op: str = "z",
                             h: float = 1e-6, krylov_dim: int = 12, krylov_tol: float = 1e-14,
-                            batched: Optional[bool] = None) -> GradientResult:
+                            batched: Optional[bool] = None, include_base: bool = True) -> GradientResult:
     """<sigma_op(T)> at T = nsteps * dt and its central-difference gradient in (g_k, w_k).
-    batched=None batches whenever the library has the batch entry points."""
+    batched=None batches whenever the library has the batch entry points.  With
+    include_base=False the unperturbed trajectory is not run and the value is the mean of
+    the g_0 +- h pair (an O(h^2) difference): 4 N trajectories, one per workgroup of the
+    GPU for N = 64."""
     n = len(model.wk)
-    models = perturbed_models(model, h)
+    models = perturbed_models(model, h, include_base)
     keep, igs = [], []
     for m in models:
         tree, sop = sb_problem(lib, m)
@@ -82,8 +86,12 @@ def observable_and_gradient(lib: Library, model: SpinBoson, nsteps: int, dt: flo
     finally:
         for ig in igs:
             ig.destroy()
-    d = (vals[1::2] - vals[2::2]) / (2.0 * h)
-    return GradientResult(float(vals[0]), d[:n].copy(), d[n:].copy(), len(models), bool(batched))
+    if include_base:
+        value, vals = float(vals[0]), vals[1:]
+    else:
+        value = float(0.5 * (vals[0] + vals[1]))
+    d = (vals[0::2] - vals[1::2]) / (2.0 * h)
+    return GradientResult(value, d[:n].copy(), d[n:].copy(), len(models), bool(batched))
 
 
 try:
