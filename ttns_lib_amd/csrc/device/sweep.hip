@@ -379,8 +379,7 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
                 sh.S[e] = cmul(f, sh.Hm[i * MAXK + j]);
             }
             __syncthreads();
-            if ((threadIdx.x >> 6) == 0) wave_expm(sh, sh.S, sh.F, mm);
-            __syncthreads();
+            block_expm(sh, sh.S, sh.F, mm);
             return hn * cabsv(sh.F[(mm - 1) * mm]);
         };
         for (int j = 0; j < kdim; ++j) {
@@ -769,8 +768,7 @@ __device__ int64_t expv_small(Ctx& c, int n_op, bool bond, double2* x, int n, do
                 sh.S[e] = cmul(f, sh.Hm[i * MAXK + j]);
             }
             __syncthreads();
-            if ((threadIdx.x >> 6) == 0) wave_expm(sh, sh.S, sh.F, mm);
-            __syncthreads();
+            block_expm(sh, sh.S, sh.F, mm);
             return hn * cabsv(sh.F[(mm - 1) * mm]);
         };
         for (int j = 0; j < kdim; ++j) {

@@ -61,6 +61,7 @@ struct Shared {
     double2 lu_f[MAXK];
     int fail;                // failure record, identical on all members after a barrier
     double hb[2][2 * MAXK];  // Gram-Schmidt coefficients of the two CGS passes
+    int expm_sc;             // squarings of the current small exponential
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -688,34 +689,12 @@ __device__ void t_householder_qr(Team& T, double2* A, int m, int n, double2* Q, 
     T.sync();
 }
 
-// ---------------------------------------------------------------- small dense expm on one wave
-// Pade(13) scaling and squaring (Higham 2005, the oracle's expm) executed by a single
-// wavefront: the 64 lanes run in lockstep, so LDS hand-offs need only a wavefront-scope
-// fence instead of a workgroup barrier.  Called by wave 0 only.
+// ---------------------------------------------------------------- small dense expm
+// Within one wavefront the 64 lanes run in lockstep, so LDS hand-offs between them need only a
+// wavefront-scope fence instead of a workgroup barrier.
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-}
-
-__device__ __forceinline__ void wv_matmul(const double2* A, const double2* B, double2* C, int n, int lane) {
-    double2 acc[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        const int e = lane + 64 * u;
-        double2 s = cz();
-        if (e < n * n) {
-            const int i = e / n, j = e % n;
-            for (int q = 0; q < n; ++q) s = cadd(s, cmul(A[i * n + q], B[q * n + j]));
-        }
-        acc[u] = s;
-    }
-    wave_sync();
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        const int e = lane + 64 * u;
-        if (e < n * n) C[e] = acc[u];
-    }
-    wave_sync();
 }
 
 // max over the wave with the lowest lane winning ties
@@ -731,102 +710,140 @@ __device__ __forceinline__ void wave_argmax(double& v, int& idx) {
     }
 }
 
-__device__ void wave_expm(Shared& sh, const double2* S, double2* Eout, int n) {
+// one entry of C = A B per thread (n <= 16), summed over q in ascending order as in
+// wv_matmul, with the operands of four q at a time loaded before their products
+__device__ __forceinline__ void blk_matmul(const double2* A, const double2* B, double2* C, int n) {
+    const int e = threadIdx.x;
+    const bool on = e < n * n;
+    double2 s = cz();
+    if (on) {
+        const int i = e / n, j = e % n;
+        int q = 0;
+        for (; q + 4 <= n; q += 4) {
+            double2 a[4], bb[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                a[u] = A[i * n + q + u];
+                bb[u] = B[(q + u) * n + j];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) s = cadd(s, cmul(a[u], bb[u]));
+        }
+        for (; q < n; ++q) s = cadd(s, cmul(A[i * n + q], B[q * n + j]));
+    }
+    __syncthreads();
+    if (on) C[e] = s;
+    __syncthreads();
+}
+
+// Pade(13) scaling and squaring (Higham 2005, the oracle's expm) by the whole workgroup: the
+// matrix products take one entry per thread, the 1-norm and the pivoted LU solve run on wave
+// 0 (lockstep lanes, wavefront fences).  Called by every thread of the workgroup.
+__device__ void block_expm(Shared& sh, const double2* S, double2* Eout, int n) {
     const double b[14] = {64764752532480000.0, 32382376266240000.0, 7771770303897600.0, 1187353796428800.0,
                           129060195264000.0,   10559470521600.0,   670442572800.0,     33522128640.0,
                           1323241920.0,        40840800.0,         960960.0,           16380.0,
                           182.0,               1.0};
     const double theta13 = 5.371920351148152;
     const int lane = threadIdx.x & 63, nn = n * n;
+    const bool w0 = (threadIdx.x >> 6) == 0;
     double2 *A = sh.E[0], *A2 = sh.E[1], *A4 = sh.E[2], *A6 = sh.E[3], *T1 = sh.E[4], *T2 = sh.E[5], *U = sh.E[6],
             *V = sh.E[7];
-    // 1-norm: lane j sums column j, then a wave max
-    double colsum = 0.0;
-    if (lane < n)
-        for (int i = 0; i < n; ++i) colsum += cabsv(S[i * n + lane]);
-    int dummy = lane;
-    wave_argmax(colsum, dummy);
-    const double norm1 = colsum;
-    int sc = 0;
-    if (norm1 > theta13) sc = static_cast<int>(ceil(log2(norm1 / theta13)));
+    if (w0) {
+        // 1-norm: lane j sums column j, then a wave max
+        double colsum = 0.0;
+        if (lane < n)
+            for (int i = 0; i < n; ++i) colsum += cabsv(S[i * n + lane]);
+        int dummy = lane;
+        wave_argmax(colsum, dummy);
+        const double norm1 = colsum;
+        int sc = 0;
+        if (norm1 > theta13) sc = static_cast<int>(ceil(log2(norm1 / theta13)));
+        if (lane == 0) sh.expm_sc = sc;
+    }
+    __syncthreads();
+    const int sc = sh.expm_sc;
     const double scale = ldexp(1.0, -sc);
-    for (int e = lane; e < nn; e += 64) A[e] = cscale(S[e], scale);
-    wave_sync();
-    wv_matmul(A, A, A2, n, lane);
-    wv_matmul(A2, A2, A4, n, lane);
-    wv_matmul(A4, A2, A6, n, lane);
-    for (int e = lane; e < nn; e += 64)
+    for (int e = threadIdx.x; e < nn; e += NT) A[e] = cscale(S[e], scale);
+    __syncthreads();
+    blk_matmul(A, A, A2, n);
+    blk_matmul(A2, A2, A4, n);
+    blk_matmul(A4, A2, A6, n);
+    for (int e = threadIdx.x; e < nn; e += NT)
         T1[e] = cadd(cadd(cscale(A6[e], b[13]), cscale(A4[e], b[11])), cscale(A2[e], b[9]));
-    wave_sync();
-    wv_matmul(A6, T1, T2, n, lane);
-    for (int e = lane; e < nn; e += 64) {
+    __syncthreads();
+    blk_matmul(A6, T1, T2, n);
+    for (int e = threadIdx.x; e < nn; e += NT) {
         double2 v = cadd(T2[e], cadd(cadd(cscale(A6[e], b[7]), cscale(A4[e], b[5])), cscale(A2[e], b[3])));
         if (e / n == e % n) v.x += b[1];
         T2[e] = v;
     }
-    wave_sync();
-    wv_matmul(A, T2, U, n, lane);
-    for (int e = lane; e < nn; e += 64)
+    __syncthreads();
+    blk_matmul(A, T2, U, n);
+    for (int e = threadIdx.x; e < nn; e += NT)
         T1[e] = cadd(cadd(cscale(A6[e], b[12]), cscale(A4[e], b[10])), cscale(A2[e], b[8]));
-    wave_sync();
-    wv_matmul(A6, T1, V, n, lane);
-    for (int e = lane; e < nn; e += 64) {
+    __syncthreads();
+    blk_matmul(A6, T1, V, n);
+    for (int e = threadIdx.x; e < nn; e += NT) {
         double2 v = cadd(V[e], cadd(cadd(cscale(A6[e], b[6]), cscale(A4[e], b[4])), cscale(A2[e], b[2])));
         if (e / n == e % n) v.x += b[0];
         V[e] = v;
     }
-    wave_sync();
+    __syncthreads();
     double2 *P = A, *X = T1;
-    for (int e = lane; e < nn; e += 64) {
+    for (int e = threadIdx.x; e < nn; e += NT) {
         const double2 u = U[e], v = V[e];
         P[e] = csub(v, u);
         X[e] = cadd(v, u);
     }
-    wave_sync();
-    for (int kk = 0; kk < n; ++kk) {
-        // partial pivoting: lanes kk..n-1 take |P(i, kk)|, first maximum wins
-        double best = (lane >= kk && lane < n) ? cabsv(P[lane * n + kk]) : -1.0;
-        int piv = lane;
-        wave_argmax(best, piv);
-        if (piv != kk && lane < n) {
-            double2 a = P[kk * n + lane];
-            P[kk * n + lane] = P[piv * n + lane];
-            P[piv * n + lane] = a;
-            a = X[kk * n + lane];
-            X[kk * n + lane] = X[piv * n + lane];
-            X[piv * n + lane] = a;
-        }
-        wave_sync();
-        if (lane > kk && lane < n) sh.lu_f[lane] = cdiv(P[lane * n + kk], P[kk * n + kk]);
-        wave_sync();
-        for (int e = lane; e < nn; e += 64) {
-            const int i = e / n, j = e % n;
-            if (i > kk) {
-                const double2 f = sh.lu_f[i];
-                if (j >= kk) P[i * n + j] = csub(P[i * n + j], cmul(f, P[kk * n + j]));
-                X[i * n + j] = csub(X[i * n + j], cmul(f, X[kk * n + j]));
+    __syncthreads();
+    if (w0) {
+        for (int kk = 0; kk < n; ++kk) {
+            // partial pivoting: lanes kk..n-1 take |P(i, kk)|, first maximum wins
+            double best = (lane >= kk && lane < n) ? cabsv(P[lane * n + kk]) : -1.0;
+            int piv = lane;
+            wave_argmax(best, piv);
+            if (piv != kk && lane < n) {
+                double2 a = P[kk * n + lane];
+                P[kk * n + lane] = P[piv * n + lane];
+                P[piv * n + lane] = a;
+                a = X[kk * n + lane];
+                X[kk * n + lane] = X[piv * n + lane];
+                X[piv * n + lane] = a;
             }
+            wave_sync();
+            if (lane > kk && lane < n) sh.lu_f[lane] = cdiv(P[lane * n + kk], P[kk * n + kk]);
+            wave_sync();
+            for (int e = lane; e < nn; e += 64) {
+                const int i = e / n, j = e % n;
+                if (i > kk) {
+                    const double2 f = sh.lu_f[i];
+                    if (j >= kk) P[i * n + j] = csub(P[i * n + j], cmul(f, P[kk * n + j]));
+                    X[i * n + j] = csub(X[i * n + j], cmul(f, X[kk * n + j]));
+                }
+            }
+            wave_sync();
         }
-        wave_sync();
-    }
-    for (int kk = n - 1; kk >= 0; --kk) {
-        if (lane < n) {
-            double2 s = X[kk * n + lane];
-            for (int i = kk + 1; i < n; ++i) s = csub(s, cmul(P[kk * n + i], X[i * n + lane]));
-            X[kk * n + lane] = cdiv(s, P[kk * n + kk]);
+        for (int kk = n - 1; kk >= 0; --kk) {
+            if (lane < n) {
+                double2 s = X[kk * n + lane];
+                for (int i = kk + 1; i < n; ++i) s = csub(s, cmul(P[kk * n + i], X[i * n + lane]));
+                X[kk * n + lane] = cdiv(s, P[kk * n + kk]);
+            }
+            wave_sync();
         }
-        wave_sync();
     }
+    __syncthreads();
     double2* cur = X;
     double2* nxt = T2;
     for (int s = 0; s < sc; ++s) {
-        wv_matmul(cur, cur, nxt, n, lane);
+        blk_matmul(cur, cur, nxt, n);
         double2* tmp = cur;
         cur = nxt;
         nxt = tmp;
     }
-    for (int e = lane; e < nn; e += 64) Eout[e] = cur[e];
-    wave_sync();
+    for (int e = threadIdx.x; e < nn; e += NT) Eout[e] = cur[e];
+    __syncthreads();
 }
 
 }  // namespace ttnd
