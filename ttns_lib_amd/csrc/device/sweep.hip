@@ -33,7 +33,7 @@ struct Ctx {
     __device__ const DNode& node(int n) const { return I.nodes[n]; }
     // failures are detected on team-uniform values, so every thread of every member takes
     // the same branch; the LDS flag is read only after a barrier
-    __device__ bool failed() const { return sh.fail != 0; }
+    __device__ bool failed() const { return g_sh.fail != 0; }
     __device__ void fail(int reason) const { T->fail(reason | (bond << 4) | (where << 5)); }
     // thread 0 of the team leader writes statistics and results
     __device__ bool writer() const { return T->rank == 0 && threadIdx.x == 0; }
@@ -55,8 +55,8 @@ __device__ void on_team(Ctx& c, int64_t elems, F&& f) {
         Ctx cs = c;
         cs.T = &s;
         f(cs);
-        if (threadIdx.x == 0 && c.sh.fail != 0)
-            __hip_atomic_store(&c.T->w->fail, c.sh.fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (threadIdx.x == 0 && g_sh.fail != 0)
+            __hip_atomic_store(&c.T->w->fail, g_sh.fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     c.T->sync();
 }
@@ -349,7 +349,6 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
     Team& T = *c.T;
     const int kdim = c.I.kdim;
     const double tol = c.I.tol;
-    Shared& sh = c.sh;
     double2* V = c.base + c.I.krylov;
     const int64_t ld = c.I.maxsize;
     double2* w = c.W(0);
@@ -367,7 +366,7 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
         }
         if (beta == 0.0) return napply;
         t_scale_to(T, V, cmk(1.0 / beta, 0.0), x, n);
-        for (int e = threadIdx.x; e < (MAXK + 1) * MAXK; e += NT) sh.Hm[e] = cz();
+        for (int e = threadIdx.x; e < (MAXK + 1) * MAXK; e += NT) g_sh.Hm[e] = cz();
         __syncthreads();
         int m = 0;
         double hn = 0.0, err = 0.0;
@@ -376,11 +375,11 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
             const double2 f = cscale(coeff, tau * s);
             for (int e = threadIdx.x; e < mm * mm; e += NT) {
                 const int i = e / mm, j = e % mm;
-                sh.S[e] = cmul(f, sh.Hm[i * MAXK + j]);
+                g_sh.S[e] = cmul(f, g_sh.Hm[i * MAXK + j]);
             }
             __syncthreads();
-            block_expm(sh, sh.S, sh.F, mm);
-            return hn * cabsv(sh.F[(mm - 1) * mm]);
+            block_expm(g_sh, g_sh.S, g_sh.F, mm);
+            return hn * cabsv(g_sh.F[(mm - 1) * mm]);
         };
         for (int j = 0; j < kdim; ++j) {
             H(V + j * ld, w);
@@ -391,8 +390,8 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
             // products accumulate while the first correction is written, the norm while
             // the second one is written (three sweeps over the basis instead of five)
             double p[2 * MAXK];
-            const double* h1 = sh.hb[0];
-            const double* h2 = sh.hb[1];
+            const double* h1 = g_sh.hb[0];
+            const double* h2 = g_sh.hb[1];
 #pragma unroll
             for (int i = 0; i < 2 * MAXK; ++i) p[i] = 0.0;
             // two strides per iteration keep 2(j+2) loads in flight per thread
@@ -410,7 +409,7 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
                     }
                 }
             }
-            team_sum_lds<2 * MAXK>(T, p, sh.hb[0]);
+            team_sum_lds<2 * MAXK>(T, p, g_sh.hb[0]);
 #pragma unroll
             for (int i = 0; i < 2 * MAXK; ++i) p[i] = 0.0;
             for (int64_t q = q00; q < n; q += st) {
@@ -432,7 +431,7 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
                     }
                 }
             }
-            team_sum_lds<2 * MAXK>(T, p, sh.hb[1]);
+            team_sum_lds<2 * MAXK>(T, p, g_sh.hb[1]);
             double nn[1] = {0.0}, nno[1];
             for (int64_t q0 = q00; q0 < n; q0 += 2 * st) {
                 const int64_t q1 = q0 + st;
@@ -454,8 +453,8 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
             team_sum<1>(T, nn, nno);
             if (threadIdx.x <= j) {
                 const int i = threadIdx.x;
-                sh.Hm[i * MAXK + j] =
-                    cadd(cadd(sh.Hm[i * MAXK + j], cmk(h1[2 * i], h1[2 * i + 1])), cmk(h2[2 * i], h2[2 * i + 1]));
+                g_sh.Hm[i * MAXK + j] =
+                    cadd(cadd(g_sh.Hm[i * MAXK + j], cmk(h1[2 * i], h1[2 * i + 1])), cmk(h2[2 * i], h2[2 * i + 1]));
             }
             __syncthreads();
             prof_add(c, PROF_ORTH, ot0);
@@ -464,14 +463,14 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
                 c.fail(FAIL_KRYLOV_NONFINITE);
                 return napply;
             }
-            if (j + 1 < kdim && threadIdx.x == 0) sh.Hm[(j + 1) * MAXK + j] = cmk(hn, 0.0);
+            if (j + 1 < kdim && threadIdx.x == 0) g_sh.Hm[(j + 1) * MAXK + j] = cmk(hn, 0.0);
             __syncthreads();
             m = j + 1;
             // leading Taylor term of the Saad estimate (prod of subdiagonal entries / j!);
             // the exponential is evaluated only when it is within 1e3 of the tolerance
             double lead = hn;
             for (int i = 0; i < j; ++i)
-                lead *= cabsv(coeff) * fabs(tau * s_step) * cabsv(sh.Hm[(i + 1) * MAXK + i]) / (i + 1);
+                lead *= cabsv(coeff) * fabs(tau * s_step) * cabsv(g_sh.Hm[(i + 1) * MAXK + i]) / (i + 1);
             if (lead > 1e3 * tol && j + 1 < kdim) {
                 t_scale_to(T, V + (j + 1) * ld, cmk(1.0 / hn, 0.0), w, n);
                 continue;
@@ -504,7 +503,7 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
             const bool ok1 = q1 < n;
             double2 s0 = cz(), s1 = cz();
             for (int i = 0; i < m; ++i) {
-                const double2 f = sh.F[i * m];
+                const double2 f = g_sh.F[i * m];
                 s0 = cadd(s0, cmul(f, ldc(V + i * ld + q0)));
                 if (ok1) s1 = cadd(s1, cmul(f, ldc(V + i * ld + q1)));
             }
@@ -534,6 +533,7 @@ struct SmallLds {
     double2 v[SMALL_N], w[SMALL_N], t1[SMALL_N], t2[SMALL_N], t3[SMALL_N];
     double2 ops[SMALL_OPS];
 };
+__shared__ SmallLds g_small;
 
 __device__ __forceinline__ int prim_count(const DK& k) { return k.kind == 0 ? k.dim * k.dim : (k.kind == 1 ? k.dim : k.nnz); }
 
@@ -583,16 +583,16 @@ __device__ __forceinline__ double2 s_out(double2 acc, double2 alpha, double2 old
     const double2 v = cmul(alpha, acc);
     return accumulate ? cadd(old, v) : v;
 }
-__device__ void s_axpby(double2* y, double2 a, const double2* x, int n, bool accumulate) {
+__device__ __forceinline__ void s_axpby(double2* y, double2 a, const double2* x, int n, bool accumulate) {
     for (int e = threadIdx.x; e < n; e += NT) y[e] = accumulate ? cadd(y[e], cmul(a, x[e])) : cmul(a, x[e]);
     __syncthreads();
 }
-__device__ void s_zero(double2* y, int n) {
+__device__ __forceinline__ void s_zero(double2* y, int n) {
     for (int e = threadIdx.x; e < n; e += NT) y[e] = cz();
     __syncthreads();
 }
 // dst(l,i,r) (=|+=) alpha sum_j op(i,j) src(l,j,r)
-__device__ void s_mode_contract(const double2* op, int L, int d, int R, const double2* src, double2* dst, double2 alpha,
+__device__ __forceinline__ void s_mode_contract(const double2* op, int L, int d, int R, const double2* src, double2* dst, double2 alpha,
                                 bool accumulate) {
     const int total = L * d * R;
     for (int e = threadIdx.x; e < total; e += NT) {
@@ -604,7 +604,7 @@ __device__ void s_mode_contract(const double2* op, int L, int d, int R, const do
     __syncthreads();
 }
 // dst (=|+=) alpha X M^T, X: rows x r
-__device__ void s_mul_t(const double2* X, int rows, int r, const double2* M, double2* dst, double2 alpha,
+__device__ __forceinline__ void s_mul_t(const double2* X, int rows, int r, const double2* M, double2* dst, double2 alpha,
                         bool accumulate) {
     const int total = rows * r;
     for (int e = threadIdx.x; e < total; e += NT) {
@@ -616,7 +616,7 @@ __device__ void s_mul_t(const double2* X, int rows, int r, const double2* M, dou
     __syncthreads();
 }
 // y (=|+=) alpha op x for a primitive leaf operator (op values in LDS, CSR indices in the table)
-__device__ void s_prim(const DK& k, const double2* op, const int64_t* ints, const double2* x, double2* y, int ncols,
+__device__ __forceinline__ void s_prim(const DK& k, const double2* op, const int64_t* ints, const double2* x, double2* y, int ncols,
                        double2 alpha, bool accumulate) {
     const int d = k.dim, total = d * ncols;
     for (int e = threadIdx.x; e < total; e += NT) {
@@ -636,8 +636,11 @@ __device__ void s_prim(const DK& k, const double2* op, const int64_t* ints, cons
     __syncthreads();
 }
 
-// heff_node on LDS data (v, res in LDS, operators staged in L.ops in walk order)
-__device__ void small_heff_node(const Ctx& c, int n, SmallLds& L, const double2* v, double2* res) {
+// heff_node on the LDS vector g_small.v into g_small.w
+__device__ void small_heff_node(const Ctx& c, int n) {
+    SmallLds& L = g_small;
+    const double2* v = L.v;
+    double2* res = L.w;
     const DNode& nd = c.node(n);
     const int r = nd.hrank, D = nd.dimen, size = nd.size, rr = r * r;
     int cur = 0;
@@ -692,7 +695,10 @@ __device__ void small_heff_node(const Ctx& c, int n, SmallLds& L, const double2*
 }
 
 // heff_bond on LDS data
-__device__ void small_heff_bond(const Ctx& c, int n, SmallLds& L, const double2* v, double2* res) {
+__device__ void small_heff_bond(const Ctx& c, int n) {
+    SmallLds& L = g_small;
+    const double2* v = L.v;
+    double2* res = L.w;
     const DNode& nd = c.node(n);
     const int r = nd.hrank, sz = r * r;
     int cur = 0;
@@ -719,8 +725,7 @@ __device__ void small_heff_bond(const Ctx& c, int n, SmallLds& L, const double2*
 template <int KM>
 __device__ int64_t expv_small(Ctx& c, int n_op, bool bond, double2* x, int n, double2 coeff, double tau, int64_t& sumj) {
     if (tau == 0.0 || n == 0) return 0;
-    SmallLds& L = *c.small;
-    Shared& sh = c.sh;
+    SmallLds& L = g_small;
     const int kdim = c.I.kdim;
     const double tol = c.I.tol;
     // stage the operators (written by other team members: coherent loads)
@@ -746,7 +751,7 @@ __device__ int64_t expv_small(Ctx& c, int n_op, bool bond, double2* x, int n, do
         double b2[1] = {0.0}, bo[1];
 #pragma unroll
         for (int e = 0; e < SMALL_E; ++e) b2[0] += cabs2(xe[e]);
-        block_sum<1>(sh, b2, bo);
+        block_sum<1>(g_sh, b2, bo);
         const double beta = sqrt(bo[0]);
         if (!isfinite(beta)) {
             c.fail(FAIL_INPUT_NONFINITE);
@@ -756,7 +761,7 @@ __device__ int64_t expv_small(Ctx& c, int n_op, bool bond, double2* x, int n, do
         if (beta == 0.0) break;
 #pragma unroll
         for (int e = 0; e < SMALL_E; ++e) vb[0][e] = cscale(xe[e], 1.0 / beta);
-        for (int e = threadIdx.x; e < (MAXK + 1) * MAXK; e += NT) sh.Hm[e] = cz();
+        for (int e = threadIdx.x; e < (MAXK + 1) * MAXK; e += NT) g_sh.Hm[e] = cz();
         __syncthreads();
         int m = 0;
         double hn = 0.0, err = 0.0;
@@ -765,11 +770,11 @@ __device__ int64_t expv_small(Ctx& c, int n_op, bool bond, double2* x, int n, do
             const double2 f = cscale(coeff, tau * sf);
             for (int e = threadIdx.x; e < mm * mm; e += NT) {
                 const int i = e / mm, j = e % mm;
-                sh.S[e] = cmul(f, sh.Hm[i * MAXK + j]);
+                g_sh.S[e] = cmul(f, g_sh.Hm[i * MAXK + j]);
             }
             __syncthreads();
-            block_expm(sh, sh.S, sh.F, mm);
-            return hn * cabsv(sh.F[(mm - 1) * mm]);
+            block_expm(g_sh, g_sh.S, g_sh.F, mm);
+            return hn * cabsv(g_sh.F[(mm - 1) * mm]);
         };
         for (int j = 0; j < kdim; ++j) {
             // V_j into LDS for the H-action (static register indexing: unrolled select)
@@ -783,9 +788,9 @@ __device__ int64_t expv_small(Ctx& c, int n_op, bool bond, double2* x, int n, do
                     }
             __syncthreads();
             if (bond) {
-                PROFILE(PROF_HBOND, small_heff_bond(c, n_op, L, L.v, L.w));
+                PROFILE(PROF_HBOND, small_heff_bond(c, n_op));
             } else {
-                PROFILE(PROF_HEFF, small_heff_node(c, n_op, L, L.v, L.w));
+                PROFILE(PROF_HEFF, small_heff_node(c, n_op));
             }
             ++napply;
             sumj += j + 1;
@@ -811,8 +816,8 @@ __device__ int64_t expv_small(Ctx& c, int n_op, bool bond, double2* x, int n, do
                             p[2 * i] += pr.x;
                             p[2 * i + 1] += pr.y;
                         }
-                block_sum_lds<2 * KM>(sh, p, sh.hb[pass]);
-                const double* h = sh.hb[pass];
+                block_sum_lds<2 * KM>(g_sh, p, g_sh.hb[pass]);
+                const double* h = g_sh.hb[pass];
 #pragma unroll
                 for (int i = 0; i < KM; ++i)
                     if (i <= j)
@@ -822,11 +827,11 @@ __device__ int64_t expv_small(Ctx& c, int n_op, bool bond, double2* x, int n, do
             double nn[1] = {0.0}, nno[1];
 #pragma unroll
             for (int e = 0; e < SMALL_E; ++e) nn[0] += cabs2(wv[e]);
-            block_sum<1>(sh, nn, nno);
+            block_sum<1>(g_sh, nn, nno);
             if (threadIdx.x <= j) {
                 const int i = threadIdx.x;
-                sh.Hm[i * MAXK + j] = cadd(cadd(sh.Hm[i * MAXK + j], cmk(sh.hb[0][2 * i], sh.hb[0][2 * i + 1])),
-                                           cmk(sh.hb[1][2 * i], sh.hb[1][2 * i + 1]));
+                g_sh.Hm[i * MAXK + j] = cadd(cadd(g_sh.Hm[i * MAXK + j], cmk(g_sh.hb[0][2 * i], g_sh.hb[0][2 * i + 1])),
+                                           cmk(g_sh.hb[1][2 * i], g_sh.hb[1][2 * i + 1]));
             }
             __syncthreads();
             prof_add(c, PROF_ORTH, ot0);
@@ -836,12 +841,12 @@ __device__ int64_t expv_small(Ctx& c, int n_op, bool bond, double2* x, int n, do
                 ok = false;
                 break;
             }
-            if (j + 1 < kdim && threadIdx.x == 0) sh.Hm[(j + 1) * MAXK + j] = cmk(hn, 0.0);
+            if (j + 1 < kdim && threadIdx.x == 0) g_sh.Hm[(j + 1) * MAXK + j] = cmk(hn, 0.0);
             __syncthreads();
             m = j + 1;
             double lead = hn;
             for (int i = 0; i < j; ++i)
-                lead *= cabsv(coeff) * fabs(tau * s_step) * cabsv(sh.Hm[(i + 1) * MAXK + i]) / (i + 1);
+                lead *= cabsv(coeff) * fabs(tau * s_step) * cabsv(g_sh.Hm[(i + 1) * MAXK + i]) / (i + 1);
             bool next_vec = j + 1 < kdim;
             if (!(lead > 1e3 * tol && j + 1 < kdim)) {
                 PROFILE(PROF_EXPM, err = small_exp(m, s_step));
@@ -880,7 +885,7 @@ __device__ int64_t expv_small(Ctx& c, int n_op, bool bond, double2* x, int n, do
             double2 sacc = cz();
 #pragma unroll
             for (int i = 0; i < KM; ++i)
-                if (i < m) sacc = cadd(sacc, cmul(sh.F[i * m], vb[i][e]));
+                if (i < m) sacc = cadd(sacc, cmul(g_sh.F[i * m], vb[i][e]));
             xe[e] = cscale(sacc, beta);
         }
         __syncthreads();
@@ -1081,8 +1086,6 @@ __device__ __forceinline__ int xcd_id() {
 // the team that slot names; words[i] are the team words of instance i.
 __global__ __launch_bounds__(NT) void tdvp_run_kernel(const DInst* insts, RunArgs a, const TeamPlan* plan,
                                                       TeamWords* words, unsigned* xcd_fill) {
-    __shared__ Shared sh;
-    __shared__ SmallLds small;
     __shared__ int visits[1024];
     __shared__ int slot_info[3];
     if (plan == nullptr) {
@@ -1108,13 +1111,13 @@ __global__ __launch_bounds__(NT) void tdvp_run_kernel(const DInst* insts, RunArg
         slot_info[1] = rank;
         slot_info[2] = size;
     }
-    if (threadIdx.x == 0) sh.fail = 0;
+    if (threadIdx.x == 0) g_sh.fail = 0;
     __syncthreads();
     const int inst = slot_info[0], rank = slot_info[1], size = slot_info[2];
     if (inst < 0) return;
     const DInst& I = insts[inst];
     Team T;
-    T.sh = &sh;
+    T.sh = &g_sh;
     T.w = size > 1 ? &words[inst] : nullptr;
     T.rank = rank;
     T.size = size;
@@ -1126,21 +1129,21 @@ __global__ __launch_bounds__(NT) void tdvp_run_kernel(const DInst* insts, RunArg
         // roll call: nothing is touched unless the whole team has assembled
         T.sync();
         if (threadIdx.x == 0) {
-            if (sh.fail != 0)
+            if (g_sh.fail != 0)
                 __hip_atomic_store(&T.w->aborted, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             else if (__hip_atomic_load(&T.w->aborted, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
-                sh.fail = FAIL_TEAM_FORM;
-            if (sh.fail != 0) {
-                sh.fail = FAIL_TEAM_FORM;
+                g_sh.fail = FAIL_TEAM_FORM;
+            if (g_sh.fail != 0) {
+                g_sh.fail = FAIL_TEAM_FORM;
                 __hip_atomic_store(I.status, static_cast<int>(FAIL_TEAM_FORM), __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
             }
         }
         __syncthreads();
-        if (sh.fail != 0) return;
+        if (g_sh.fail != 0) return;
     }
-    Ctx c(I, sh, &T, a.team_min);
-    c.small = &small;
+    Ctx c(I, g_sh, &T, a.team_min);
+    c.small = &g_small;
     c.small_max = a.small_max;
     const uint64_t prof_t0 = prof_now(c);
     SweepState st;
@@ -1171,7 +1174,7 @@ __global__ __launch_bounds__(NT) void tdvp_run_kernel(const DInst* insts, RunArg
     if (threadIdx.x == 0) {
         // the first member to report a failure records it
         int expected = 0;
-        if (sh.fail != 0 && __hip_atomic_compare_exchange_strong(I.status, &expected, sh.fail, __ATOMIC_RELAXED,
+        if (g_sh.fail != 0 && __hip_atomic_compare_exchange_strong(I.status, &expected, g_sh.fail, __ATOMIC_RELAXED,
                                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
             I.status[1] = static_cast<int>(step_index);
         if (rank == 0) {
@@ -1183,11 +1186,10 @@ __global__ __launch_bounds__(NT) void tdvp_run_kernel(const DInst* insts, RunArg
 
 // <psi|O|psi> on one instance (O stored at obs_op); out: re, im, norm2
 __global__ __launch_bounds__(NT) void expect_kernel(const DInst* inst, int leaf, int64_t op_off, double* out) {
-    __shared__ Shared sh;
-    if (threadIdx.x == 0) sh.fail = 0;
+    if (threadIdx.x == 0) g_sh.fail = 0;
     __syncthreads();
     Team T;
-    T.sh = &sh;
+    T.sh = &g_sh;
     T.w = nullptr;
     T.rank = 0;
     T.size = 1;
@@ -1195,7 +1197,7 @@ __global__ __launch_bounds__(NT) void expect_kernel(const DInst* inst, int leaf,
     T.pbuf = 0;
     T.part = nullptr;
     T.tiles = nullptr;
-    Ctx c(*inst, sh, &T, 0);
+    Ctx c(*inst, g_sh, &T, 0);
     double2 v;
     double nrm;
     path_expect(c, leaf, op_off, v, nrm);

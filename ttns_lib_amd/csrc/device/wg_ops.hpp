@@ -64,6 +64,11 @@ struct Shared {
     int expm_sc;             // squarings of the current small exponential
 };
 
+// The workgroup's shared area.  Declared once at namespace scope and named directly by the
+// routines that use it, so that every access compiles to an LDS instruction (through a
+// reference passed between non-inlined functions it would be a flat access).
+__shared__ Shared g_sh;
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -77,14 +82,14 @@ __device__ void block_sum(Shared& sh, double (&v)[K], double (&out)[K]) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const double s = wave_sum(v[k]);
-        if (lane == 0) sh.red[wv * K + k] = s;
+        if (lane == 0) g_sh.red[wv * K + k] = s;
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         double s = 0.0;
 #pragma unroll
-        for (int w = 0; w < NWAVE; ++w) s += sh.red[w * K + k];
+        for (int w = 0; w < NWAVE; ++w) s += g_sh.red[w * K + k];
         out[k] = s;
     }
     __syncthreads();
@@ -97,12 +102,12 @@ __device__ void block_sum_lds(Shared& sh, double (&v)[K], double* dst) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const double s = wave_sum(v[k]);
-        if (lane == 0) sh.red[wv * K + k] = s;
+        if (lane == 0) g_sh.red[wv * K + k] = s;
     }
     __syncthreads();
     for (int k = threadIdx.x; k < K; k += NT) {
         double s = 0.0;
-        for (int w = 0; w < NWAVE; ++w) s += sh.red[w * K + k];
+        for (int w = 0; w < NWAVE; ++w) s += g_sh.red[w * K + k];
         dst[k] = s;
     }
     __syncthreads();
@@ -113,11 +118,11 @@ __device__ double block_max(Shared& sh, double v) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    if (lane == 0) sh.red[wv] = v;
+    if (lane == 0) g_sh.red[wv] = v;
     __syncthreads();
-    double m = sh.red[0];
+    double m = g_sh.red[0];
 #pragma unroll
-    for (int w = 1; w < NWAVE; ++w) m = fmax(m, sh.red[w]);
+    for (int w = 1; w < NWAVE; ++w) m = fmax(m, g_sh.red[w]);
     __syncthreads();
     return m;
 }
@@ -154,7 +159,7 @@ struct Team {
     // copied into every member's LDS flag.  A member that has timed out no longer waits,
     // so a broken team drains instead of timing out barrier after barrier.
     __device__ void sync() {
-        if (size == 1 || sh->fail == FAIL_TEAM_TIMEOUT) {
+        if (size == 1 || g_sh.fail == FAIL_TEAM_TIMEOUT) {
             __syncthreads();
             return;
         }
@@ -171,13 +176,13 @@ struct Team {
                 while (__hip_atomic_load(&w->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
                     __builtin_amdgcn_s_sleep(1);
                     if (wall_clock64() - t0 > TEAM_TIMEOUT) {
-                        sh->fail = FAIL_TEAM_TIMEOUT;
+                        g_sh.fail = FAIL_TEAM_TIMEOUT;
                         break;
                     }
                 }
             }
             const int f = __hip_atomic_load(&w->fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (f != 0 && sh->fail == 0) sh->fail = f;
+            if (f != 0 && g_sh.fail == 0) g_sh.fail = f;
         }
         ++gen;
         __syncthreads();
@@ -186,7 +191,7 @@ struct Team {
     // record a failure on this member and in the team word (read by the others at their
     // next barrier)
     __device__ void fail(int code) {
-        sh->fail = code;
+        g_sh.fail = code;
         if (w != nullptr && threadIdx.x == 0)
             __hip_atomic_store(&w->fail, code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -222,27 +227,27 @@ __device__ void team_reduce_lds(Team& T, double* v, int count) {
 template <int K>
 __device__ void team_sum(Team& T, double (&v)[K], double (&out)[K]) {
     if (T.size == 1) {
-        block_sum<K>(*T.sh, v, out);
+        block_sum<K>(g_sh, v, out);
         return;
     }
-    block_sum_lds<K>(*T.sh, v, T.sh->tsum);
-    team_reduce_lds(T, T.sh->tsum, K);
+    block_sum_lds<K>(g_sh, v, g_sh.tsum);
+    team_reduce_lds(T, g_sh.tsum, K);
 #pragma unroll
-    for (int k = 0; k < K; ++k) out[k] = T.sh->tsum[k];
+    for (int k = 0; k < K; ++k) out[k] = g_sh.tsum[k];
     __syncthreads();
 }
 
 // team sum of K doubles, totals left in the LDS array dst[K]
 template <int K>
 __device__ void team_sum_lds(Team& T, double (&v)[K], double* dst) {
-    block_sum_lds<K>(*T.sh, v, dst);
+    block_sum_lds<K>(g_sh, v, dst);
     team_reduce_lds(T, dst, K);
 }
 
 __device__ double team_max(Team& T, double v) {
-    double m = block_max(*T.sh, v);
+    double m = block_max(g_sh, v);
     if (T.size == 1) return m;
-    if (threadIdx.x == 0) T.sh->tsum[0] = m;
+    if (threadIdx.x == 0) g_sh.tsum[0] = m;
     __syncthreads();
     // maxima are exact, so a second reduction buffer is not needed for determinism
     double* mine = T.part + (static_cast<int64_t>(T.pbuf) * TEAM_MAX + T.rank) * PART_K;
@@ -433,7 +438,7 @@ __device__ void wg_outer_contract(Shared& sh, const double2* A, const double2* B
                 ztile_store(acc, out + static_cast<int64_t>(16 * m) * d + 16 * n, d, 1, mv, nv, alpha, accumulate);
         } else {
             // sum the four partial tiles in LDS (reuses the Krylov scratch E[0..7])
-            double* red = reinterpret_cast<double*>(&sh.E[0][0]);
+            double* red = reinterpret_cast<double*>(&g_sh.E[0][0]);
             const int lane = threadIdx.x & 63;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
@@ -466,17 +471,17 @@ __device__ void wg_outer_contract(Shared& sh, const double2* A, const double2* B
 __device__ void t_outer_contract(Team& T, const double2* A, const double2* B, int64_t L, int d, int64_t R,
                                  double2* out, double2 alpha, bool accumulate) {
     if (T.size == 1) {
-        wg_outer_contract(*T.sh, A, B, L, d, R, R, out, alpha, accumulate);
+        wg_outer_contract(g_sh, A, B, L, d, R, R, out, alpha, accumulate);
         return;
     }
     double2* mine = T.tiles + static_cast<int64_t>(T.rank) * TILE_MAX;
     if (L >= T.size) {
         const int64_t l0 = L * T.rank / T.size, l1 = L * (T.rank + 1) / T.size;
-        wg_outer_contract(*T.sh, A + l0 * d * R, B + l0 * d * R, l1 - l0, d, R, R, mine, make_double2(1.0, 0.0),
+        wg_outer_contract(g_sh, A + l0 * d * R, B + l0 * d * R, l1 - l0, d, R, R, mine, make_double2(1.0, 0.0),
                           false);
     } else {
         const int64_t r0 = R * T.rank / T.size, r1 = R * (T.rank + 1) / T.size;
-        wg_outer_contract(*T.sh, A + r0, B + r0, L, d, r1 - r0, R, mine, make_double2(1.0, 0.0), false);
+        wg_outer_contract(g_sh, A + r0, B + r0, L, d, r1 - r0, R, mine, make_double2(1.0, 0.0), false);
     }
     T.sync();
     const int64_t dd = static_cast<int64_t>(d) * d;
@@ -573,7 +578,7 @@ __device__ __forceinline__ double lapy3(double a, double b, double c) {
 // rows for the whole factorisation, so a column costs two team reductions (its norm and
 // the trailing dot products) and forming Q one per reflector.
 __device__ void t_householder_qr(Team& T, double2* A, int m, int n, double2* Q, double2* Rout) {
-    Shared& sh = *T.sh;
+    Shared& sh = g_sh;
     const int k = m < n ? m : n;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int r0 = static_cast<int>(static_cast<int64_t>(m) * T.rank / T.size);
@@ -610,7 +615,7 @@ __device__ void t_householder_qr(Team& T, double2* A, int m, int n, double2* Q, 
             tau = make_double2((beta - ar) / beta, -ai / beta);
             scal = cdiv(make_double2(1.0, 0.0), make_double2(ar - beta, ai));
         }
-        if (threadIdx.x == 0) sh.qtau[i] = tau;
+        if (threadIdx.x == 0) g_sh.qtau[i] = tau;
         if (tau.x != 0.0 || tau.y != 0.0) {
             for (int rr = lo + threadIdx.x; rr < r1; rr += NT) {
                 double2* v = A + static_cast<int64_t>(rr) * n + i;
@@ -625,10 +630,10 @@ __device__ void t_householder_qr(Team& T, double2* A, int m, int n, double2* Q, 
                     acc = cadd(acc, cmulc(ldc(A + static_cast<int64_t>(rr) * n + i), ldc(A + static_cast<int64_t>(rr) * n + c)));
                 acc.x = wave_sum(acc.x);
                 acc.y = wave_sum(acc.y);
-                if (lane == 0) sh.qw[c] = own_i ? cadd(acc, ldc(A + static_cast<int64_t>(i) * n + c)) : acc;
+                if (lane == 0) g_sh.qw[c] = own_i ? cadd(acc, ldc(A + static_cast<int64_t>(i) * n + c)) : acc;
             }
             __syncthreads();
-            team_reduce_lds(T, reinterpret_cast<double*>(&sh.qw[i + 1]), 2 * (n - i - 1));
+            team_reduce_lds(T, reinterpret_cast<double*>(&g_sh.qw[i + 1]), 2 * (n - i - 1));
             const double2 tc = make_double2(tau.x, -tau.y);
             const int ncol = n - i - 1;
             const int rs = r0 > i ? r0 : i;
@@ -638,7 +643,7 @@ __device__ void t_householder_qr(Team& T, double2* A, int m, int n, double2* Q, 
                 const int c = i + 1 + static_cast<int>(e % ncol);
                 const double2 v = rr == i ? make_double2(1.0, 0.0) : ldc(A + static_cast<int64_t>(rr) * n + i);
                 double2* a = A + static_cast<int64_t>(rr) * n + c;
-                *a = csub(ldc(a), cmul(tc, cmul(v, sh.qw[c])));
+                *a = csub(ldc(a), cmul(tc, cmul(v, g_sh.qw[c])));
             }
         }
         __syncthreads();
@@ -661,7 +666,7 @@ __device__ void t_householder_qr(Team& T, double2* A, int m, int n, double2* Q, 
     }
     __syncthreads();
     for (int i = k - 1; i >= 0; --i) {
-        const double2 tau = sh.qtau[i];
+        const double2 tau = g_sh.qtau[i];
         if (tau.x == 0.0 && tau.y == 0.0) continue;
         const int lo = r0 > i + 1 ? r0 : i + 1;
         const bool own_i = i >= r0 && i < r1;
@@ -671,10 +676,10 @@ __device__ void t_householder_qr(Team& T, double2* A, int m, int n, double2* Q, 
                 acc = cadd(acc, cmulc(ldc(A + static_cast<int64_t>(rr) * n + i), ldc(Q + static_cast<int64_t>(rr) * n + c)));
             acc.x = wave_sum(acc.x);
             acc.y = wave_sum(acc.y);
-            if (lane == 0) sh.qw[c] = own_i ? cadd(acc, ldc(Q + static_cast<int64_t>(i) * n + c)) : acc;
+            if (lane == 0) g_sh.qw[c] = own_i ? cadd(acc, ldc(Q + static_cast<int64_t>(i) * n + c)) : acc;
         }
         __syncthreads();
-        team_reduce_lds(T, reinterpret_cast<double*>(&sh.qw[0]), 2 * k);
+        team_reduce_lds(T, reinterpret_cast<double*>(&g_sh.qw[0]), 2 * k);
         const int rs = r0 > i ? r0 : i;
         const int64_t tot = static_cast<int64_t>(r1 > rs ? r1 - rs : 0) * k;
         for (int64_t e = threadIdx.x; e < tot; e += NT) {
@@ -682,7 +687,7 @@ __device__ void t_householder_qr(Team& T, double2* A, int m, int n, double2* Q, 
             const int c = static_cast<int>(e % k);
             const double2 v = rr == i ? make_double2(1.0, 0.0) : ldc(A + static_cast<int64_t>(rr) * n + i);
             double2* q = Q + static_cast<int64_t>(rr) * n + c;
-            *q = csub(ldc(q), cmul(v, cmul(tau, sh.qw[c])));
+            *q = csub(ldc(q), cmul(v, cmul(tau, g_sh.qw[c])));
         }
         __syncthreads();
     }
@@ -747,8 +752,8 @@ __device__ void block_expm(Shared& sh, const double2* S, double2* Eout, int n) {
     const double theta13 = 5.371920351148152;
     const int lane = threadIdx.x & 63, nn = n * n;
     const bool w0 = (threadIdx.x >> 6) == 0;
-    double2 *A = sh.E[0], *A2 = sh.E[1], *A4 = sh.E[2], *A6 = sh.E[3], *T1 = sh.E[4], *T2 = sh.E[5], *U = sh.E[6],
-            *V = sh.E[7];
+    double2 *A = g_sh.E[0], *A2 = g_sh.E[1], *A4 = g_sh.E[2], *A6 = g_sh.E[3], *T1 = g_sh.E[4], *T2 = g_sh.E[5], *U = g_sh.E[6],
+            *V = g_sh.E[7];
     if (w0) {
         // 1-norm: lane j sums column j, then a wave max
         double colsum = 0.0;
@@ -759,10 +764,10 @@ __device__ void block_expm(Shared& sh, const double2* S, double2* Eout, int n) {
         const double norm1 = colsum;
         int sc = 0;
         if (norm1 > theta13) sc = static_cast<int>(ceil(log2(norm1 / theta13)));
-        if (lane == 0) sh.expm_sc = sc;
+        if (lane == 0) g_sh.expm_sc = sc;
     }
     __syncthreads();
-    const int sc = sh.expm_sc;
+    const int sc = g_sh.expm_sc;
     const double scale = ldexp(1.0, -sc);
     for (int e = threadIdx.x; e < nn; e += NT) A[e] = cscale(S[e], scale);
     __syncthreads();
@@ -812,12 +817,12 @@ __device__ void block_expm(Shared& sh, const double2* S, double2* Eout, int n) {
                 X[piv * n + lane] = a;
             }
             wave_sync();
-            if (lane > kk && lane < n) sh.lu_f[lane] = cdiv(P[lane * n + kk], P[kk * n + kk]);
+            if (lane > kk && lane < n) g_sh.lu_f[lane] = cdiv(P[lane * n + kk], P[kk * n + kk]);
             wave_sync();
             for (int e = lane; e < nn; e += 64) {
                 const int i = e / n, j = e % n;
                 if (i > kk) {
-                    const double2 f = sh.lu_f[i];
+                    const double2 f = g_sh.lu_f[i];
                     if (j >= kk) P[i * n + j] = csub(P[i * n + j], cmul(f, P[kk * n + j]));
                     X[i * n + j] = csub(X[i * n + j], cmul(f, X[kk * n + j]));
                 }

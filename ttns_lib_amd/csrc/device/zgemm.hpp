@@ -22,7 +22,10 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 // invalidating it costs about 6 us on gfx950.  Every load of data another workgroup may
 // have written therefore uses device scope, which misses in the L1 and reads the L2.
 __device__ __forceinline__ double ldc(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // through the global address space: a global_load (a generic pointer would give a flat
+    // load, which also waits on the LDS counter)
+    return __hip_atomic_load((const __attribute__((address_space(1))) double*)p, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ double2 ldc(const double2* p) {
     const double* q = reinterpret_cast<const double*>(p);
