@@ -592,13 +592,31 @@ __device__ __forceinline__ void s_zero(double2* y, int n) {
     __syncthreads();
 }
 // dst(l,i,r) (=|+=) alpha sum_j op(i,j) src(l,j,r)
+// sum_k a(k) b(k) for k = 0 .. K-1 in that order, the operands of four k loaded ahead
+template <class FA, class FB>
+__device__ __forceinline__ double2 s_dot(FA a, FB b, int K) {
+    double2 acc = cz();
+    int k = 0;
+    for (; k + 4 <= K; k += 4) {
+        double2 x[4], y[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            x[u] = a(k + u);
+            y[u] = b(k + u);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc = cadd(acc, cmul(x[u], y[u]));
+    }
+    for (; k < K; ++k) acc = cadd(acc, cmul(a(k), b(k)));
+    return acc;
+}
 __device__ __forceinline__ void s_mode_contract(const double2* op, int L, int d, int R, const double2* src, double2* dst, double2 alpha,
                                 bool accumulate) {
     const int total = L * d * R;
     for (int e = threadIdx.x; e < total; e += NT) {
         const int l = e / (d * R), rem = e - l * d * R, i = rem / R, r = rem - i * R;
-        double2 acc = cz();
-        for (int j = 0; j < d; ++j) acc = cadd(acc, cmul(op[i * d + j], src[(l * d + j) * R + r]));
+        const double2 acc = s_dot([&](int j) { return op[i * d + j]; },
+                                  [&](int j) { return src[(l * d + j) * R + r]; }, d);
         dst[e] = s_out(acc, alpha, dst[e], accumulate);
     }
     __syncthreads();
@@ -609,8 +627,7 @@ __device__ __forceinline__ void s_mul_t(const double2* X, int rows, int r, const
     const int total = rows * r;
     for (int e = threadIdx.x; e < total; e += NT) {
         const int x = e / r, i = e - x * r;
-        double2 acc = cz();
-        for (int k = 0; k < r; ++k) acc = cadd(acc, cmul(X[x * r + k], M[i * r + k]));
+        const double2 acc = s_dot([&](int k) { return X[x * r + k]; }, [&](int k) { return M[i * r + k]; }, r);
         dst[e] = s_out(acc, alpha, dst[e], accumulate);
     }
     __syncthreads();
@@ -623,7 +640,7 @@ __device__ __forceinline__ void s_prim(const DK& k, const double2* op, const int
         const int i = e / ncols, cc = e - i * ncols;
         double2 acc = cz();
         if (k.kind == 0) {
-            for (int j = 0; j < d; ++j) acc = cadd(acc, cmul(op[i * d + j], x[j * ncols + cc]));
+            acc = s_dot([&](int j) { return op[i * d + j]; }, [&](int j) { return x[j * ncols + cc]; }, d);
         } else if (k.kind == 1) {
             acc = cmul(op[i], x[e]);
         } else {
