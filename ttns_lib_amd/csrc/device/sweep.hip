@@ -19,7 +19,6 @@ struct SmallLds;
 
 struct Ctx {
     const DInst& I;
-    Shared& sh;
     Team* T;
     double2* base;
     int64_t team_min;
@@ -27,8 +26,8 @@ struct Ctx {
     int small_max = 0;          // propagations of at most this many entries use it
     // where: node whose propagation is running, bond: 1 for its bond matrix
     int where = 0, bond = 0;
-    __device__ Ctx(const DInst& inst, Shared& s, Team* t, int64_t tmin)
-        : I(inst), sh(s), T(t), base(inst.base), team_min(tmin) {}
+    __device__ Ctx(const DInst& inst, Team* t, int64_t tmin)
+        : I(inst), T(t), base(inst.base), team_min(tmin) {}
     __device__ double2* W(int k) const { return base + I.work[k]; }
     __device__ const DNode& node(int n) const { return I.nodes[n]; }
     // failures are detected on team-uniform values, so every thread of every member takes
@@ -378,7 +377,7 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
                 g_sh.S[e] = cmul(f, g_sh.Hm[i * MAXK + j]);
             }
             __syncthreads();
-            block_expm(g_sh, g_sh.S, g_sh.F, mm);
+            block_expm(g_sh.S, g_sh.F, mm);
             return hn * cabsv(g_sh.F[(mm - 1) * mm]);
         };
         for (int j = 0; j < kdim; ++j) {
@@ -768,7 +767,7 @@ __device__ int64_t expv_small(Ctx& c, int n_op, bool bond, double2* x, int n, do
         double b2[1] = {0.0}, bo[1];
 #pragma unroll
         for (int e = 0; e < SMALL_E; ++e) b2[0] += cabs2(xe[e]);
-        block_sum<1>(g_sh, b2, bo);
+        block_sum<1>(b2, bo);
         const double beta = sqrt(bo[0]);
         if (!isfinite(beta)) {
             c.fail(FAIL_INPUT_NONFINITE);
@@ -790,7 +789,7 @@ __device__ int64_t expv_small(Ctx& c, int n_op, bool bond, double2* x, int n, do
                 g_sh.S[e] = cmul(f, g_sh.Hm[i * MAXK + j]);
             }
             __syncthreads();
-            block_expm(g_sh, g_sh.S, g_sh.F, mm);
+            block_expm(g_sh.S, g_sh.F, mm);
             return hn * cabsv(g_sh.F[(mm - 1) * mm]);
         };
         for (int j = 0; j < kdim; ++j) {
@@ -833,7 +832,7 @@ __device__ int64_t expv_small(Ctx& c, int n_op, bool bond, double2* x, int n, do
                             p[2 * i] += pr.x;
                             p[2 * i + 1] += pr.y;
                         }
-                block_sum_lds<2 * KM>(g_sh, p, g_sh.hb[pass]);
+                block_sum_lds<2 * KM>(p, g_sh.hb[pass]);
                 const double* h = g_sh.hb[pass];
 #pragma unroll
                 for (int i = 0; i < KM; ++i)
@@ -844,7 +843,7 @@ __device__ int64_t expv_small(Ctx& c, int n_op, bool bond, double2* x, int n, do
             double nn[1] = {0.0}, nno[1];
 #pragma unroll
             for (int e = 0; e < SMALL_E; ++e) nn[0] += cabs2(wv[e]);
-            block_sum<1>(g_sh, nn, nno);
+            block_sum<1>(nn, nno);
             if (threadIdx.x <= j) {
                 const int i = threadIdx.x;
                 g_sh.Hm[i * MAXK + j] = cadd(cadd(g_sh.Hm[i * MAXK + j], cmk(g_sh.hb[0][2 * i], g_sh.hb[0][2 * i + 1])),
@@ -1134,7 +1133,6 @@ __global__ __launch_bounds__(NT) void tdvp_run_kernel(const DInst* insts, RunArg
     if (inst < 0) return;
     const DInst& I = insts[inst];
     Team T;
-    T.sh = &g_sh;
     T.w = size > 1 ? &words[inst] : nullptr;
     T.rank = rank;
     T.size = size;
@@ -1159,7 +1157,7 @@ __global__ __launch_bounds__(NT) void tdvp_run_kernel(const DInst* insts, RunArg
         __syncthreads();
         if (g_sh.fail != 0) return;
     }
-    Ctx c(I, g_sh, &T, a.team_min);
+    Ctx c(I, &T, a.team_min);
     c.small = &g_small;
     c.small_max = a.small_max;
     const uint64_t prof_t0 = prof_now(c);
@@ -1206,7 +1204,6 @@ __global__ __launch_bounds__(NT) void expect_kernel(const DInst* inst, int leaf,
     if (threadIdx.x == 0) g_sh.fail = 0;
     __syncthreads();
     Team T;
-    T.sh = &g_sh;
     T.w = nullptr;
     T.rank = 0;
     T.size = 1;
@@ -1214,7 +1211,7 @@ __global__ __launch_bounds__(NT) void expect_kernel(const DInst* inst, int leaf,
     T.pbuf = 0;
     T.part = nullptr;
     T.tiles = nullptr;
-    Ctx c(*inst, g_sh, &T, 0);
+    Ctx c(*inst, &T, 0);
     double2 v;
     double nrm;
     path_expect(c, leaf, op_off, v, nrm);

@@ -77,7 +77,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // sum of K doubles over the workgroup; every thread receives the totals in out[]
 template <int K>
-__device__ void block_sum(Shared& sh, double (&v)[K], double (&out)[K]) {
+__device__ void block_sum(double (&v)[K], double (&out)[K]) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -97,7 +97,7 @@ __device__ void block_sum(Shared& sh, double (&v)[K], double (&out)[K]) {
 
 // sum of K doubles over the workgroup, totals left in the LDS array dst[K]
 template <int K>
-__device__ void block_sum_lds(Shared& sh, double (&v)[K], double* dst) {
+__device__ void block_sum_lds(double (&v)[K], double* dst) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -114,7 +114,7 @@ __device__ void block_sum_lds(Shared& sh, double (&v)[K], double* dst) {
 }
 
 // maximum over the workgroup; every thread receives it
-__device__ double block_max(Shared& sh, double v) {
+__device__ double block_max(double v) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
@@ -129,7 +129,6 @@ __device__ double block_max(Shared& sh, double v) {
 
 // ---------------------------------------------------------------- teams
 struct Team {
-    Shared* sh;
     TeamWords* w;       // null for a team of one
     int rank, size;
     unsigned gen;       // barriers completed by this member
@@ -227,10 +226,10 @@ __device__ void team_reduce_lds(Team& T, double* v, int count) {
 template <int K>
 __device__ void team_sum(Team& T, double (&v)[K], double (&out)[K]) {
     if (T.size == 1) {
-        block_sum<K>(g_sh, v, out);
+        block_sum<K>(v, out);
         return;
     }
-    block_sum_lds<K>(g_sh, v, g_sh.tsum);
+    block_sum_lds<K>(v, g_sh.tsum);
     team_reduce_lds(T, g_sh.tsum, K);
 #pragma unroll
     for (int k = 0; k < K; ++k) out[k] = g_sh.tsum[k];
@@ -240,12 +239,12 @@ __device__ void team_sum(Team& T, double (&v)[K], double (&out)[K]) {
 // team sum of K doubles, totals left in the LDS array dst[K]
 template <int K>
 __device__ void team_sum_lds(Team& T, double (&v)[K], double* dst) {
-    block_sum_lds<K>(g_sh, v, dst);
+    block_sum_lds<K>(v, dst);
     team_reduce_lds(T, dst, K);
 }
 
 __device__ double team_max(Team& T, double v) {
-    double m = block_max(g_sh, v);
+    double m = block_max(v);
     if (T.size == 1) return m;
     if (threadIdx.x == 0) g_sh.tsum[0] = m;
     __syncthreads();
@@ -399,7 +398,7 @@ __device__ void t_mul(Team& T, const double2* X, int64_t rows, int r, const doub
 // element (l, i, r) at (l*d + i)*ldr + r, by one workgroup: a long-K product.  With
 // several output tiles the waves take whole tiles; with a single tile (d <= 16) they
 // split K and the partial tiles are summed through LDS.  out is d x d.
-__device__ void wg_outer_contract(Shared& sh, const double2* A, const double2* B, int64_t L, int d, int64_t R,
+__device__ void wg_outer_contract(const double2* A, const double2* B, int64_t L, int d, int64_t R,
                                   int64_t ldr, double2* out, double2 alpha, bool accumulate) {
     const int wv = threadIdx.x >> 6;
     const int mt = (d + 15) >> 4;
@@ -471,17 +470,17 @@ __device__ void wg_outer_contract(Shared& sh, const double2* A, const double2* B
 __device__ void t_outer_contract(Team& T, const double2* A, const double2* B, int64_t L, int d, int64_t R,
                                  double2* out, double2 alpha, bool accumulate) {
     if (T.size == 1) {
-        wg_outer_contract(g_sh, A, B, L, d, R, R, out, alpha, accumulate);
+        wg_outer_contract(A, B, L, d, R, R, out, alpha, accumulate);
         return;
     }
     double2* mine = T.tiles + static_cast<int64_t>(T.rank) * TILE_MAX;
     if (L >= T.size) {
         const int64_t l0 = L * T.rank / T.size, l1 = L * (T.rank + 1) / T.size;
-        wg_outer_contract(g_sh, A + l0 * d * R, B + l0 * d * R, l1 - l0, d, R, R, mine, make_double2(1.0, 0.0),
+        wg_outer_contract(A + l0 * d * R, B + l0 * d * R, l1 - l0, d, R, R, mine, make_double2(1.0, 0.0),
                           false);
     } else {
         const int64_t r0 = R * T.rank / T.size, r1 = R * (T.rank + 1) / T.size;
-        wg_outer_contract(g_sh, A + r0, B + r0, L, d, r1 - r0, R, mine, make_double2(1.0, 0.0), false);
+        wg_outer_contract(A + r0, B + r0, L, d, r1 - r0, R, mine, make_double2(1.0, 0.0), false);
     }
     T.sync();
     const int64_t dd = static_cast<int64_t>(d) * d;
@@ -578,7 +577,6 @@ __device__ __forceinline__ double lapy3(double a, double b, double c) {
 // rows for the whole factorisation, so a column costs two team reductions (its norm and
 // the trailing dot products) and forming Q one per reflector.
 __device__ void t_householder_qr(Team& T, double2* A, int m, int n, double2* Q, double2* Rout) {
-    Shared& sh = g_sh;
     const int k = m < n ? m : n;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int r0 = static_cast<int>(static_cast<int64_t>(m) * T.rank / T.size);
@@ -744,7 +742,7 @@ __device__ __forceinline__ void blk_matmul(const double2* A, const double2* B, d
 // Pade(13) scaling and squaring (Higham 2005, the oracle's expm) by the whole workgroup: the
 // matrix products take one entry per thread, the 1-norm and the pivoted LU solve run on wave
 // 0 (lockstep lanes, wavefront fences).  Called by every thread of the workgroup.
-__device__ void block_expm(Shared& sh, const double2* S, double2* Eout, int n) {
+__device__ void block_expm(const double2* S, double2* Eout, int n) {
     const double b[14] = {64764752532480000.0, 32382376266240000.0, 7771770303897600.0, 1187353796428800.0,
                           129060195264000.0,   10559470521600.0,   670442572800.0,     33522128640.0,
                           1323241920.0,        40840800.0,         960960.0,           16380.0,
