@@ -15,7 +15,18 @@
 
 namespace ttnd {
 
-struct SmallLds;
+// LDS of the small-vector propagator (below); between its propagations the same 4608 entries
+// stage the operands of one-workgroup bond products
+constexpr int SMALL_E = 2;
+constexpr int SMALL_N = SMALL_E * NT;
+constexpr int SMALL_OPS = 2048;  // LDS operator arena (double2)
+
+struct SmallLds {
+    double2 v[SMALL_N], w[SMALL_N], t1[SMALL_N], t2[SMALL_N], t3[SMALL_N];
+    double2 ops[SMALL_OPS];
+};
+__shared__ SmallLds g_small;
+constexpr int LDS_STAGE = (5 * SMALL_N + SMALL_OPS) / 2;  // entries per staged operand
 
 struct Ctx {
     const DInst& I;
@@ -167,6 +178,36 @@ __device__ void heff_node(Ctx& c, int n, const double2* v, double2* res) {
     if (first) t_zero(*c.T, res, nd.size);
 }
 
+// dst (=|+=) alpha * P Q on one workgroup, P and Q r x r matrices copied into LDS first, with
+// Q transposed when TRANS (Q^T: the mean-field product x mf^T).  The tiles and their k order
+// are those of t_mode_contract (L = 1) and t_mul_transposed, so the sums are the same.
+template <bool TRANS>
+__device__ void wg_square_product_lds(const double2* P, const double2* Q, int r, double2* dst, double2 alpha,
+                                      bool accumulate) {
+    double2* a = reinterpret_cast<double2*>(&g_small);  // the whole 4608-entry area
+    double2* b = a + LDS_STAGE;
+    const int rr = r * r;
+    for (int e = threadIdx.x; e < rr; e += NT) {
+        a[e] = ldc(P + e);
+        b[e] = ldc(Q + e);
+    }
+    __syncthreads();
+    const int wv = threadIdx.x >> 6;
+    const int mt = (r + 15) >> 4;
+    for (int it = wv; it < mt * mt; it += NWAVE) {
+        const int n = it % mt, m = it / mt;
+        const int mv = min(16, r - 16 * m), nv = min(16, r - 16 * n);
+        ZAcc acc;
+        zacc_zero(acc);
+        if (TRANS)
+            ztile<false>(acc, a + 16 * m * r, r, 1, mv, b + 16 * n * r, 1, r, nv, r, LoadLocal());
+        else
+            ztile<false>(acc, a + 16 * m * r, r, 1, mv, b + 16 * n, r, 1, nv, r, LoadLocal());
+        ztile_store(acc, dst + 16 * m * r + 16 * n, r, 1, mv, nv, alpha, accumulate);
+    }
+    __syncthreads();
+}
+
 // full_hamiltonian_action (projector_splitting_evolution_functions.hpp:7-62); uses W1, W2
 __device__ void heff_bond(Ctx& c, int n, const double2* v, double2* res) {
     const DNode& nd = c.node(n);
@@ -174,6 +215,23 @@ __device__ void heff_bond(Ctx& c, int n, const double2* v, double2* res) {
     const int64_t sz = static_cast<int64_t>(r) * r;
     double2* t1 = c.W(1);
     t_zero(*c.T, res, sz);
+    if (c.T->size == 1 && sz <= LDS_STAGE && c.small != nullptr) {
+        // one workgroup: every product with both operands in LDS
+        for (int ti = 0; ti < nd.nterms; ++ti) {
+            const DTerm& tm = c.I.terms[nd.term0 + ti];
+            const double2* x = v;
+            if (!tm.id_spf) {
+                wg_square_product_lds<false>(c.base + tm.spf, v, r, t1, cmk(1.0, 0.0), false);
+                x = t1;
+            }
+            const double2 coeff = cmk(tm.cre, tm.cim);
+            if (!tm.id_mf)
+                wg_square_product_lds<true>(x, c.base + tm.mf, r, res, coeff, true);
+            else
+                t_axpy(*c.T, res, coeff, x, sz);
+        }
+        return;
+    }
     for (int ti = 0; ti < nd.nterms; ++ti) {
         const DTerm& tm = c.I.terms[nd.term0 + ti];
         const double2* x = v;
@@ -524,15 +582,6 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
 // instead of dependent round trips to L2.  Same algorithm as expv (Arnoldi, CGS2, Saad
 // estimate, Expokit sub-steps, Pade(13)); the H-action mirrors heff_node / heff_bond term by
 // term, with each output entry summed over k in order.
-constexpr int SMALL_E = 2;
-constexpr int SMALL_N = SMALL_E * NT;
-constexpr int SMALL_OPS = 2048;  // LDS operator arena (double2)
-
-struct SmallLds {
-    double2 v[SMALL_N], w[SMALL_N], t1[SMALL_N], t2[SMALL_N], t3[SMALL_N];
-    double2 ops[SMALL_OPS];
-};
-__shared__ SmallLds g_small;
 
 __device__ __forceinline__ int prim_count(const DK& k) { return k.kind == 0 ? k.dim * k.dim : (k.kind == 1 ? k.dim : k.nnz); }
 

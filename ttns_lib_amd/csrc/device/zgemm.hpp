@@ -50,9 +50,17 @@ __device__ __forceinline__ void zmfma(ZAcc& acc, double2 a, double2 b) {
 
 // acc += op(A) B over k in [0, K): A(i, k) at a[i*sar + k*sak] (conjugated if CONJA),
 // B(k, j) at b[k*sbk + j*sbc]; rows >= mv and columns >= nv read as zero.
-template <bool CONJA>
+struct LoadL2 {  // operands in global memory, possibly written by other workgroups
+    __device__ double2 operator()(const double2* p) const { return ldc(p); }
+};
+struct LoadLocal {  // operands in this workgroup's LDS
+    __device__ double2 operator()(const double2* p) const { return *p; }
+};
+
+template <bool CONJA, class LD = LoadL2>
 __device__ __forceinline__ void ztile(ZAcc& acc, const double2* __restrict__ a, int64_t sar, int64_t sak, int mv,
-                                      const double2* __restrict__ b, int64_t sbk, int64_t sbc, int nv, int64_t K) {
+                                      const double2* __restrict__ b, int64_t sbk, int64_t sbc, int nv, int64_t K,
+                                      LD ld = LD()) {
     const int lane = threadIdx.x & 63, i = lane & 15, kq = lane >> 4;
     const bool arow = i < mv, bcol = i < nv;
     const double2 z = make_double2(0.0, 0.0);
@@ -65,8 +73,8 @@ __device__ __forceinline__ void ztile(ZAcc& acc, const double2* __restrict__ a, 
             if (s < nks) {
                 const int64_t k = k0 + 4 * s + kq;
                 const bool ok = k < K;
-                av[s] = (arow && ok) ? ldc(a + i * sar + k * sak) : z;
-                bv[s] = (bcol && ok) ? ldc(b + k * sbk + i * sbc) : z;
+                av[s] = (arow && ok) ? ld(a + i * sar + k * sak) : z;
+                bv[s] = (bcol && ok) ? ld(b + k * sbk + i * sbc) : z;
             }
         }
 #pragma unroll
