@@ -205,6 +205,8 @@ __device__ void wg_square_product_lds(const double2* P, const double2* Q, int r,
             ztile<false>(acc, a + 16 * m * r, r, 1, mv, b + 16 * n, r, 1, nv, r, LoadLocal());
         ztile_store(acc, dst + 16 * m * r + 16 * n, r, 1, mv, nv, alpha, accumulate);
     }
+    // the stores must have landed before other waves read dst with L1-bypassing loads
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
 }
 

@@ -159,6 +159,9 @@ struct Team {
     // so a broken team drains instead of timing out barrier after barrier.
     __device__ void sync() {
         if (size == 1 || g_sh.fail == FAIL_TEAM_TIMEOUT) {
+            // the workgroup barrier alone does not wait for this wave's global stores, and the
+            // other waves read them with loads that bypass the L1
+            __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
             return;
         }
