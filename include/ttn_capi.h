@@ -48,6 +48,10 @@ extern "C" {
 #define TTN_OP_IDENTITY 3
 
 #define TTN_MODE_FIXED_STEP 0
+/* subspace_expansion_projector_splitting_integrator (projector_splitting_integrator.hpp:281-326):
+ * accepted on trees whose bonds are all at capacity, where the reference's expansion returns
+ * without adding functions (subspace_expansion_projector_splitting_engine.hpp:214, :378) and the
+ * step equals the fixed-step one; TTN_ERR_INVALID when a bond is below its capacity. */
 #define TTN_MODE_SUBSPACE_EXPANSION 1
 
 typedef struct ttn_ctx ttn_ctx;

@@ -121,7 +121,9 @@ int orc_sop_compiled_tables(const orc_tree* t, const orc_sop* s, int64_t* len, i
 int orc_integrator_create(orc_ctx* ctx, const orc_tree* t, const orc_sop* s, const ttn_integ_params* p,
                           orc_integ** out) {
     return guard(ctx, [&] {
-        check(p->mode == TTN_MODE_FIXED_STEP, ERR_INVALID, "oracle: only the fixed-step integrator is restated");
+        check(p->mode == TTN_MODE_FIXED_STEP || (p->mode == TTN_MODE_SUBSPACE_EXPANSION && t->topo.at_capacity()),
+              ERR_INVALID, "oracle: the expansion integrator is restated only on trees at capacity (where it is "
+                           "the fixed-step one)");
         check(p->krylov_dim >= 1 && p->krylov_dim <= 64, ERR_INVALID, "krylov_dim must be in [1, 64]");
         orc::Params pp;
         pp.krylov_dim = static_cast<int>(p->krylov_dim);

@@ -1,0 +1,53 @@
+"""The expansion integrator's mode (subspace_expansion_projector_splitting_integrator): accepted
+on trees whose bonds are all at capacity, where the reference's expansion returns without adding
+functions and the step is the fixed-step one; rejected when a bond could grow."""
+import numpy as np
+import pytest
+
+from helpers import make_integrator, sb_model
+from ttns_lib_amd.capi import MODE_FIXED_STEP, MODE_SUBSPACE_EXPANSION, SIGMA, TTNError
+
+
+def _run(lib, model, mode, steps=6):
+    ig = make_integrator(lib, model, mode=mode)[2]
+    out = []
+    for _ in range(steps):
+        ig.step(1)
+        out.append([ig.expect(0, SIGMA[o]) for o in "xyz"] + [ig.norm2()])
+    nodes = [ig.get_node(i) for i in range(len(ig.shapes))]
+    return np.array(out), nodes, ig.counters()[0]
+
+
+def check_at_capacity_equals_fixed_step(lib):
+    m = sb_model(8, 6, 4)
+    a = _run(lib, m, MODE_FIXED_STEP)
+    b = _run(lib, m, MODE_SUBSPACE_EXPANSION)
+    assert np.array_equal(a[0], b[0])
+    assert all(np.array_equal(x, y) for x, y in zip(a[1], b[1]))
+    assert a[2] == b[2]
+
+
+def check_below_capacity_rejected(lib):
+    m = sb_model(8, 6, 4, nspf_cap=6, nspf_lower_cap=6)
+    with pytest.raises(TTNError) as e:
+        make_integrator(lib, m, mode=MODE_SUBSPACE_EXPANSION)
+    assert e.value.code == 3
+    make_integrator(lib, m, mode=MODE_FIXED_STEP)  # the fixed-step integrator runs below capacity
+
+
+def test_oracle_at_capacity_equals_fixed_step(orc):
+    check_at_capacity_equals_fixed_step(orc)
+
+
+def test_oracle_below_capacity_rejected(orc):
+    check_below_capacity_rejected(orc)
+
+
+@pytest.mark.gpu
+def test_gpu_at_capacity_equals_fixed_step(hip):
+    check_at_capacity_equals_fixed_step(hip)
+
+
+@pytest.mark.gpu
+def test_gpu_below_capacity_rejected(hip):
+    check_below_capacity_rejected(hip)

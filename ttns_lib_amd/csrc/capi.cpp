@@ -786,8 +786,14 @@ int ttn_integrator_create(ttn_ctx* ctx, const ttn_tree* t, const ttn_sop* s, con
                           ttn_integ** out) {
     return guard(ctx, [&] {
         check(p != nullptr, ERR_INVALID, "null parameters");
-        check(p->mode == TTN_MODE_FIXED_STEP, ERR_INVALID,
-              "subspace expansion is not available on the device path yet; use TTN_MODE_FIXED_STEP");
+        check(p->mode == TTN_MODE_FIXED_STEP || p->mode == TTN_MODE_SUBSPACE_EXPANSION, ERR_INVALID,
+              "unknown integrator mode");
+        // With every bond at capacity the expansion integrator never expands and its step is
+        // the fixed-step one (same prepare pass, ladder and sweeps); expansion into bonds below
+        // capacity is not built.
+        check(p->mode == TTN_MODE_FIXED_STEP || t->topo.at_capacity(), ERR_INVALID,
+              "subspace expansion into bonds below their capacity is not available; the expansion "
+              "integrator is accepted on trees whose bonds are all at capacity");
         check(p->dt > 0.0, ERR_INVALID, "dt must be positive");
         auto in = std::make_unique<ttn_integ>();
         in->ctx = ctx;

@@ -43,6 +43,16 @@ struct Topology {
 
     int size() const { return static_cast<int>(nodes.size()); }
     int nmodes() const { return static_cast<int>(leaf_node.size()); }
+    // every bond at its capacity: the subspace expansion has no room to add functions
+    // (subspace_expansion_projector_splitting_engine.hpp:214 and :378 return at capacity)
+    bool at_capacity() const {
+        for (const Node& nd : nodes) {
+            if (nd.hrank < nd.cap_hrank) return false;
+            for (size_t c = 0; c < nd.dims.size(); ++c)
+                if (nd.dims[c] < nd.cap_dims[c]) return false;
+        }
+        return true;
+    }
 };
 
 // Build a topology from flat pre-order arrays (the C-ABI form):
