@@ -339,6 +339,8 @@ class sop_operator {
 // ------------------------------------------------------------------ integrator
 template <typename T, typename backend = blas_backend>
 class fixed_step_projector_splitting_integrator;
+template <typename T, typename backend>
+class subspace_expansion_projector_splitting_integrator;
 
 template <typename T, typename backend = blas_backend>
 class matrix_element {
@@ -419,7 +421,10 @@ class fixed_step_projector_splitting_integrator {
         p.krylov_dim = static_cast<int64_t>(m_kdim);
         p.krylov_tol = m_tol;
         p.dt = m_dt;
-        p.mode = TTN_MODE_FIXED_STEP;
+        p.mode = m_mode;
+        p.spawning_threshold = m_spawning_threshold;
+        p.unoccupied_threshold = m_unoccupied_threshold;
+        p.minimum_unoccupied = static_cast<int64_t>(m_minimum_unoccupied);
         detail::check(ttn_integrator_create(m_ctx, m_tree, m_sop, &p, &m_integ), m_ctx, "ttn_integrator_create");
     }
     void sync_to_device(httensor<T, backend>& A) {
@@ -455,6 +460,35 @@ class fixed_step_projector_splitting_integrator {
     bool m_synced = false;
     uint64_t m_version = 0;
     matrix_element<T, backend> m_mel;
+
+  protected:
+    int m_mode = TTN_MODE_FIXED_STEP;
+    real_type m_spawning_threshold = -1.0;
+    real_type m_unoccupied_threshold = -1.0;
+    size_type m_minimum_unoccupied = 0;
+};
+
+// subspace_expansion_projector_splitting_integrator (projector_splitting_integrator.hpp:229-326):
+// the same call pattern plus the expansion parameters the driver sets
+// (src/spin_boson.cpp:652-658).  The library accepts it on trees whose bonds are all at
+// capacity, where the reference's expansion never adds functions and the step is the
+// fixed-step one; on a tree with room to grow a bond the first step throws (the expansion
+// itself is not available).
+template <typename T, typename backend = blas_backend>
+class subspace_expansion_projector_splitting_integrator : public fixed_step_projector_splitting_integrator<T, backend> {
+    using base = fixed_step_projector_splitting_integrator<T, backend>;
+
+  public:
+    using real_type = typename base::real_type;
+    subspace_expansion_projector_splitting_integrator(httensor<T, backend>& A, sop_operator<T, backend>& H,
+                                                      size_type krylov_dim = 4, real_type krylov_tol = 1e-12,
+                                                      size_type nthreads = 1)
+        : base(A, H, krylov_dim, krylov_tol, nthreads) {
+        this->m_mode = TTN_MODE_SUBSPACE_EXPANSION;
+    }
+    real_type& spawning_threshold() { return this->m_spawning_threshold; }
+    real_type& unoccupied_threshold() { return this->m_unoccupied_threshold; }
+    size_type& minimum_unoccupied() { return this->m_minimum_unoccupied; }
 };
 
 }  // namespace ttns

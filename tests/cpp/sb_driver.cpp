@@ -152,8 +152,11 @@ int main(int argc, char** argv) {
         }
         A.set_is_orthogonalised();
 
-        fixed_step_projector_splitting_integrator<complex_type, backend_type> tdvp(A, H, krylov_dim, 1e-12, 1);
+        subspace_expansion_projector_splitting_integrator<complex_type, backend_type> tdvp(A, H, krylov_dim, 1e-12, 1);
         tdvp.dt() = dt;
+        tdvp.spawning_threshold() = 1e-7;
+        tdvp.unoccupied_threshold() = 1e-4;
+        tdvp.minimum_unoccupied() = 0;
         ops::dense_matrix_operator<complex_type, backend_type> mop(Sz);
         std::printf("%.17g %.17g\n", 0.0, std::real(tdvp.mel()(mop, 0, A)));
         for (int s = 0; s < nsteps; ++s) {
