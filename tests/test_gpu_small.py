@@ -105,3 +105,18 @@ def test_small_max_rejects_negative(hip):
         lib.set_small(100)
     finally:
         lib.close()
+
+
+def test_node_stats_account_for_every_node_application(hip):
+    # per-node H-application counts of the node propagations add up to the integrator's
+    # counter, on the small propagator and on the general path alike; every Krylov size is
+    # between 1 and the Krylov dimension
+    for m in (sb_model(8, 6, 4), sb_model(8, 8, 24)):
+        ig = make_integrator(hip, m)[2]
+        ig.step(3)
+        st = ig.node_stats()
+        assert st[:, 0].sum() == ig.counters()[0]
+        for napp, sumj in ((st[:, 0], st[:, 1]), (st[:, 2], st[:, 3])):
+            used = napp > 0
+            assert np.all(sumj[used] >= napp[used]) and np.all(sumj[used] <= 12 * napp[used])
+        assert st[:, 2].sum() > 0
