@@ -125,6 +125,7 @@ int orc_integrator_create(orc_ctx* ctx, const orc_tree* t, const orc_sop* s, con
               ERR_INVALID, "oracle: the expansion integrator is restated only on trees at capacity (where it is "
                            "the fixed-step one)");
         check(p->krylov_dim >= 1 && p->krylov_dim <= 64, ERR_INVALID, "krylov_dim must be in [1, 64]");
+        check(p->dt > 0.0, ERR_INVALID, "dt must be positive");
         orc::Params pp;
         pp.krylov_dim = static_cast<int>(p->krylov_dim);
         pp.krylov_tol = p->krylov_tol;
