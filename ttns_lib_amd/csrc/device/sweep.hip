@@ -621,8 +621,80 @@ __device__ void small_walk(const Ctx& c, int n, bool bond, F&& f) {
     }
 }
 
+// The descriptor records of the node whose small propagation is running (node, terms, their
+// sub-terms and operator chains, and the CSR index arrays of its leaf operators), copied into
+// LDS once per propagation so that the H-action reads them with LDS loads instead of dependent
+// global round trips in every Krylov iteration.
+constexpr int NC_TERMS = 8, NC_SUBS = 24, NC_KS = 32, NC_INTS = 256;
+struct NodeCache {
+    DNode nd;
+    DTerm terms[NC_TERMS];
+    DSub subs[NC_SUBS];
+    DK ks[NC_KS];
+    int64_t ints[NC_INTS];  // CSR rowptr / colind of the cached leaf operators (DK::idx remapped)
+    int sub_base, k_base;
+};
+__shared__ NodeCache g_nc;
+
+// ranges of node n's records: [s0, s1) sub-terms, [k0, k1) chain operators, CSR ints needed
+__device__ void node_record_ranges(const Ctx& c, int n, int& s0, int& s1, int& k0, int& k1, int& nints) {
+    const DNode& nd = c.node(n);
+    s0 = 1 << 30, s1 = 0, k0 = 1 << 30, k1 = 0, nints = 0;
+    for (int ti = 0; ti < nd.nterms; ++ti) {
+        const DTerm& tm = c.I.terms[nd.term0 + ti];
+        if (tm.nsub == 0) continue;
+        s0 = min(s0, tm.sub0);
+        s1 = max(s1, tm.sub0 + tm.nsub);
+    }
+    for (int s = s0; s < s1; ++s) {
+        const DSub& sb = c.I.subs[s];
+        if (sb.nk == 0) continue;
+        k0 = min(k0, sb.k0);
+        k1 = max(k1, sb.k0 + sb.nk);
+    }
+    for (int k = k0; k < k1; ++k) {
+        const DK& dk = c.I.ks[k];
+        if (nd.leaf >= 0 && dk.kind == 2) nints += dk.dim + 1 + dk.nnz;
+    }
+    if (s1 <= s0) s0 = s1 = 0;
+    if (k1 <= k0) k0 = k1 = 0;
+}
+
+__device__ bool node_cacheable(const Ctx& c, int n) {
+    int s0, s1, k0, k1, nints;
+    node_record_ranges(c, n, s0, s1, k0, k1, nints);
+    return c.node(n).nterms <= NC_TERMS && s1 - s0 <= NC_SUBS && k1 - k0 <= NC_KS && nints <= NC_INTS;
+}
+
+// fill g_nc for node n (thread 0 copies, everyone waits); the caller checked node_cacheable
+__device__ void cache_node(const Ctx& c, int n) {
+    if (threadIdx.x == 0) {
+        int s0, s1, k0, k1, nints;
+        node_record_ranges(c, n, s0, s1, k0, k1, nints);
+        const DNode& nd = c.node(n);
+        g_nc.nd = nd;
+        for (int ti = 0; ti < nd.nterms; ++ti) g_nc.terms[ti] = c.I.terms[nd.term0 + ti];
+        for (int s = s0; s < s1; ++s) g_nc.subs[s - s0] = c.I.subs[s];
+        int64_t at = 0;
+        for (int k = k0; k < k1; ++k) {
+            DK dk = c.I.ks[k];
+            if (nd.leaf >= 0 && dk.kind == 2) {
+                const int64_t cnt = dk.dim + 1 + dk.nnz;
+                for (int64_t e = 0; e < cnt; ++e) g_nc.ints[at + e] = c.I.ints[dk.idx + e];
+                dk.idx = at;
+                at += cnt;
+            }
+            g_nc.ks[k - k0] = dk;
+        }
+        g_nc.sub_base = s0;
+        g_nc.k_base = k0;
+    }
+    __syncthreads();
+}
+
 __device__ bool small_eligible(const Ctx& c, int n, bool bond, int64_t size) {
     if (c.small == nullptr || size > c.small_max || size > SMALL_N) return false;
+    if (!node_cacheable(c, n)) return false;
     int total = 0;
     small_walk(c, n, bond, [&](int64_t, int cnt) { total += cnt; });
     return total <= SMALL_OPS;
@@ -704,16 +776,16 @@ __device__ __forceinline__ void s_prim(const DK& k, const double2* op, const int
 }
 
 // heff_node on the LDS vector g_small.v into g_small.w
-__device__ void small_heff_node(const Ctx& c, int n) {
+__device__ void small_heff_node() {
     SmallLds& L = g_small;
     const double2* v = L.v;
     double2* res = L.w;
-    const DNode& nd = c.node(n);
+    const DNode& nd = g_nc.nd;
     const int r = nd.hrank, D = nd.dimen, size = nd.size, rr = r * r;
     int cur = 0;
     bool first = true;
     for (int ti = 0; ti < nd.nterms; ++ti) {
-        const DTerm& tm = c.I.terms[nd.term0 + ti];
+        const DTerm& tm = g_nc.terms[ti];
         if (tm.id_spf) {
             const double2 coeff = cmk(tm.ire, tm.iim);
             if (tm.id_mf) {
@@ -726,22 +798,22 @@ __device__ void small_heff_node(const Ctx& c, int n) {
             continue;
         }
         for (int i = 0; i < tm.nsub; ++i) {
-            const DSub& sb = c.I.subs[tm.sub0 + i];
+            const DSub& sb = g_nc.subs[tm.sub0 + i - g_nc.sub_base];
             const double2 ci = (i == 0) ? (tm.nsub > 1 ? cmk(sb.are, sb.aim) : cmk(tm.cre, tm.cim)) : cmk(sb.are, sb.aim);
             double2* dst = tm.id_mf ? res : L.t3;
             const bool acc = tm.id_mf ? !first : i > 0;
             if (sb.nk == 0) {
                 s_axpby(dst, ci, v, size, acc);
             } else if (nd.leaf >= 0) {
-                const DK& k = c.I.ks[sb.k0];
-                s_prim(k, L.ops + cur, c.I.ints, v, dst, r, ci, acc);
+                const DK& k = g_nc.ks[sb.k0 - g_nc.k_base];
+                s_prim(k, L.ops + cur, g_nc.ints, v, dst, r, ci, acc);
                 cur += prim_count(k);
             } else {
                 const double2* src = v;
                 for (int q = 0; q < sb.nk; ++q) {
                     const bool last = q == sb.nk - 1;
                     double2* out = last ? dst : (q % 2 == 0 ? L.t1 : L.t2);
-                    const DK& k = c.I.ks[sb.k0 + q];
+                    const DK& k = g_nc.ks[sb.k0 + q - g_nc.k_base];
                     int64_t LL, RR;
                     mode_view(nd, k.nu, LL, RR);
                     s_mode_contract(L.ops + cur, static_cast<int>(LL), nd.dims[k.nu], static_cast<int>(RR), src, out,
@@ -762,16 +834,16 @@ __device__ void small_heff_node(const Ctx& c, int n) {
 }
 
 // heff_bond on LDS data
-__device__ void small_heff_bond(const Ctx& c, int n) {
+__device__ void small_heff_bond() {
     SmallLds& L = g_small;
     const double2* v = L.v;
     double2* res = L.w;
-    const DNode& nd = c.node(n);
+    const DNode& nd = g_nc.nd;
     const int r = nd.hrank, sz = r * r;
     int cur = 0;
     s_zero(res, sz);
     for (int ti = 0; ti < nd.nterms; ++ti) {
-        const DTerm& tm = c.I.terms[nd.term0 + ti];
+        const DTerm& tm = g_nc.terms[ti];
         const double2* x = v;
         if (!tm.id_spf) {
             s_mode_contract(L.ops + cur, 1, r, r, v, L.t1, cmk(1.0, 0.0), false);
@@ -795,6 +867,7 @@ __device__ int64_t expv_small(Ctx& c, int n_op, bool bond, double2* x, int n, do
     SmallLds& L = g_small;
     const int kdim = c.I.kdim;
     const double tol = c.I.tol;
+    cache_node(c, n_op);
     // stage the operators (written by other team members: coherent loads)
     int cur = 0;
     small_walk(c, n_op, bond, [&](int64_t off, int cnt) {
@@ -855,9 +928,9 @@ __device__ int64_t expv_small(Ctx& c, int n_op, bool bond, double2* x, int n, do
                     }
             __syncthreads();
             if (bond) {
-                PROFILE(PROF_HBOND, small_heff_bond(c, n_op));
+                PROFILE(PROF_HBOND, small_heff_bond());
             } else {
-                PROFILE(PROF_HEFF, small_heff_node(c, n_op));
+                PROFILE(PROF_HEFF, small_heff_node());
             }
             ++napply;
             sumj += j + 1;
