@@ -267,7 +267,7 @@ __global__ __launch_bounds__(GT) void zdots_finish_kernel(const double* partial,
     red[threadIdx.x] = s;
     __syncthreads();
     for (int w = GT / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
     if (threadIdx.x == 0) h[i] = red[0];

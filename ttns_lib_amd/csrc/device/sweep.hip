@@ -510,7 +510,7 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
                 nn[0] += cabs2(w0) + cabs2(w1);
             }
             team_sum<1>(T, nn, nno);
-            if (threadIdx.x <= j) {
+            if (static_cast<int>(threadIdx.x) <= j) {
                 const int i = threadIdx.x;
                 g_sh.Hm[i * MAXK + j] =
                     cadd(cadd(g_sh.Hm[i * MAXK + j], cmk(h1[2 * i], h1[2 * i + 1])), cmk(h2[2 * i], h2[2 * i + 1]));
@@ -968,7 +968,7 @@ __device__ int64_t expv_small(Ctx& c, int n_op, bool bond, double2* x, int n, do
 #pragma unroll
             for (int e = 0; e < SMALL_E; ++e) nn[0] += cabs2(wv[e]);
             block_sum<1>(nn, nno);
-            if (threadIdx.x <= j) {
+            if (static_cast<int>(threadIdx.x) <= j) {
                 const int i = threadIdx.x;
                 g_sh.Hm[i * MAXK + j] = cadd(cadd(g_sh.Hm[i * MAXK + j], cmk(g_sh.hb[0][2 * i], g_sh.hb[0][2 * i + 1])),
                                            cmk(g_sh.hb[1][2 * i], g_sh.hb[1][2 * i + 1]));
