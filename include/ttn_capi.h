@@ -133,20 +133,41 @@ int ttn_trajectory(const ttn_integ* integ, int64_t* len, double* out);
 
 /* ---- spin-boson driver helpers (src/spin_boson.cpp:447-649) ---- */
 typedef struct {
-    int64_t nmodes;             /* bath modes, one per leaf */
+    int64_t nmodes;             /* bath modes */
     const double* wk;           /* frequencies [nmodes] */
     const double* gk;           /* couplings [nmodes] */
-    const int64_t* boson_dim;   /* Fock truncation [nmodes] */
+    const int64_t* boson_dim;   /* Fock truncation [nmodes], one mode per leaf (nblocks == 0) */
     int64_t nspf, nspf_lower, nspf_cap, nspf_lower_cap;
     int64_t nhilb;              /* system dimension */
     const double* hsys;         /* nhilb x nhilb complex */
     const double* scoup;        /* nhilb x nhilb complex */
     const double* psi0;         /* nhilb complex */
     uint64_t seed;              /* std::mt19937 seed of the initial state */
+    /* Leaves holding several modes (src/spin_boson.cpp:418-441): nblocks > 0 splits the modes
+     * into consecutive blocks of block_sizes[b] modes, one block per leaf, each mode keeping
+     * min(block_nmax[b], floor(ecut / |w_k|)) Fock states (boson_dim is then unused).
+     * nblocks == 0: one mode per leaf with boson_dim. */
+    int64_t nblocks;
+    const int64_t* block_sizes; /* [nblocks] */
+    const int64_t* block_nmax;  /* [nblocks] */
+    double ecut;
 } ttn_sb_params;
 
 int ttn_sb_problem(ttn_ctx* ctx, const ttn_sb_params* p, ttn_tree** tree, ttn_sop** sop);
 int ttn_sb_initial_state(ttn_integ* integ, const ttn_sb_params* p);
+/* partition_modes (src/mode_data.hpp:818-886, called at src/spin_boson.cpp:420 with
+ * wc = 2 * high frequency bound, nmax = target dimension, nmax4 = 250, nmax2 = maximum
+ * dimension, nmax3 = maximum dimension for high frequencies): sizes of consecutive mode
+ * blocks; wk sorted by magnitude.  partition: room for nmodes entries, or NULL to query. */
+int ttn_sb_partition_modes(ttn_ctx* ctx, int64_t nmodes, const double* wk, double ecut, double wc, int64_t nmax,
+                           int64_t nmax4, int64_t nmax2, int64_t nmax3, int64_t* nblocks, int64_t* partition);
+/* One leaf's mode block (harmonic_mode_combination::construct_basis_topology, H0, Hc;
+ * src/mode_data.hpp:136-361): *nstates and *nnz, then, for non-NULL outputs, the
+ * occupations [nstates x nmodes], the H0 diagonal [nstates], and Hc in CSR
+ * (rowptr [nstates + 1], colind [nnz], real values [nnz]). */
+int ttn_sb_mode_block(ttn_ctx* ctx, int64_t nmodes, const double* w, const double* g, double ecut, int64_t nmax,
+                      int64_t* nstates, int64_t* nnz, int64_t* occupations, double* h0, int64_t* hc_rowptr,
+                      int64_t* hc_colind, double* hc_vals);
 
 /* ---- batched persistent sweep: many instances advance in one launch ---- */
 int ttn_batch_create(ttn_ctx* ctx, int64_t n, ttn_integ* const* integs, ttn_batch** out);

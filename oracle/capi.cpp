@@ -216,6 +216,18 @@ int orc_sb_initial_state(orc_integ* in, const ttn_sb_params* p) {
     });
 }
 
+int orc_sb_partition_modes(orc_ctx* ctx, int64_t nmodes, const double* wk, double ecut, double wc, int64_t nmax,
+                           int64_t nmax4, int64_t nmax2, int64_t nmax3, int64_t* nblocks, int64_t* partition) {
+    return guard(ctx, [&] { sb_partition_c(nmodes, wk, ecut, wc, nmax, nmax4, nmax2, nmax3, nblocks, partition); });
+}
+int orc_sb_mode_block(orc_ctx* ctx, int64_t nmodes, const double* w, const double* g, double ecut, int64_t nmax,
+                      int64_t* nstates, int64_t* nnz, int64_t* occupations, double* h0, int64_t* hc_rowptr,
+                      int64_t* hc_colind, double* hc_vals) {
+    return guard(ctx, [&] {
+        sb_mode_block_c(nmodes, w, g, ecut, nmax, nstates, nnz, occupations, h0, hc_rowptr, hc_colind, hc_vals);
+    });
+}
+
 // ---- building blocks ----
 static std::vector<cplx> cvec(const double* p, i64 n) {
     std::vector<cplx> v(n);

@@ -45,6 +45,11 @@ int orc_expect(orc_integ* integ, int64_t mode, const double* op, double* out);
 int orc_norm2(orc_integ* integ, double* out);
 int orc_sb_problem(orc_ctx* ctx, const ttn_sb_params* p, orc_tree** tree, orc_sop** sop);
 int orc_sb_initial_state(orc_integ* integ, const ttn_sb_params* p);
+int orc_sb_partition_modes(orc_ctx* ctx, int64_t nmodes, const double* wk, double ecut, double wc, int64_t nmax,
+                           int64_t nmax4, int64_t nmax2, int64_t nmax3, int64_t* nblocks, int64_t* partition);
+int orc_sb_mode_block(orc_ctx* ctx, int64_t nmodes, const double* w, const double* g, double ecut, int64_t nmax,
+                      int64_t* nstates, int64_t* nnz, int64_t* occupations, double* h0, int64_t* hc_rowptr,
+                      int64_t* hc_colind, double* hc_vals);
 
 /* building blocks for unit tests: apply the node / bond effective Hamiltonian with the
  * current environments, refresh environments, and the dense kernels */

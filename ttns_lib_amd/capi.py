@@ -61,6 +61,10 @@ class SBParamsC(C.Structure):
         ("scoup", dblp),
         ("psi0", dblp),
         ("seed", C.c_uint64),
+        ("nblocks", C.c_int64),
+        ("block_sizes", i64p),
+        ("block_nmax", i64p),
+        ("ecut", C.c_double),
     ]
 
 
@@ -90,6 +94,9 @@ _SIGS = {
     "trajectory": [vp, i64p, dblp],
     "sb_problem": [vp, C.POINTER(SBParamsC), C.POINTER(vp), C.POINTER(vp)],
     "sb_initial_state": [vp, C.POINTER(SBParamsC)],
+    "sb_partition_modes": [vp, C.c_int64, dblp, C.c_double, C.c_double, C.c_int64, C.c_int64, C.c_int64, C.c_int64,
+                           i64p, i64p],
+    "sb_mode_block": [vp, C.c_int64, dblp, dblp, C.c_double, C.c_int64, i64p, i64p, i64p, dblp, i64p, i64p, dblp],
     "batch_create": [vp, i64, C.POINTER(vp), C.POINTER(vp)],
     "batch_destroy": [vp],
     "batch_step": [vp, C.c_double, C.c_double, i64],
@@ -183,11 +190,13 @@ class Library:
 
 @dataclass
 class SpinBoson:
-    """Spin-boson model with one bath mode per leaf (src/spin_boson.cpp:447-649)."""
+    """Spin-boson model (src/spin_boson.cpp:418-649): one bath mode per leaf with the Fock
+    truncations boson_dim, or, when block_sizes is given, consecutive blocks of modes on one
+    leaf each, mode k keeping min(block_nmax[b], floor(ecut / |w_k|)) states."""
 
     wk: Sequence[float]
     gk: Sequence[float]
-    boson_dim: Sequence[int]
+    boson_dim: Optional[Sequence[int]] = None
     nspf: int = 8
     nspf_lower: int = 8
     nspf_cap: int = 0
@@ -196,23 +205,75 @@ class SpinBoson:
     scoup: np.ndarray = field(default_factory=lambda: np.array([[1, 0], [0, -1]], dtype=complex))
     psi0: np.ndarray = field(default_factory=lambda: np.array([1, 0], dtype=complex))
     seed: int = 0
+    block_sizes: Optional[Sequence[int]] = None
+    block_nmax: Optional[Sequence[int]] = None
+    ecut: float = 3000.0
 
     def to_c(self) -> SBParamsC:
+        blocks = self.block_sizes is not None
+        if not blocks and self.boson_dim is None:
+            raise ValueError("SpinBoson needs boson_dim or block_sizes")
         self._keep = [
             np.ascontiguousarray(self.wk, dtype=np.float64),
             np.ascontiguousarray(self.gk, dtype=np.float64),
-            np.ascontiguousarray(self.boson_dim, dtype=np.int64),
+            np.ascontiguousarray(self.boson_dim if not blocks else [], dtype=np.int64),
             _cplx_in(self.hsys),
             _cplx_in(self.scoup),
             _cplx_in(self.psi0),
+            np.ascontiguousarray(self.block_sizes if blocks else [], dtype=np.int64),
+            np.ascontiguousarray(self.block_nmax if blocks else [], dtype=np.int64),
         ]
-        wk, gk, bd, hs, sc, ps = self._keep
+        wk, gk, bd, hs, sc, ps, bs, bn = self._keep
         nh = int(np.asarray(self.hsys).shape[0])
         return SBParamsC(
-            len(wk), _dp(wk), _dp(gk), _ip(bd), self.nspf, self.nspf_lower,
+            len(wk), _dp(wk), _dp(gk), _ip(bd) if not blocks else None, self.nspf, self.nspf_lower,
             max(self.nspf_cap, self.nspf), max(self.nspf_lower_cap, self.nspf_lower), nh,
-            _dp(hs), _dp(sc), _dp(ps), self.seed,
+            _dp(hs), _dp(sc), _dp(ps), self.seed, len(bs) if blocks else 0,
+            _ip(bs) if blocks else None, _ip(bn) if blocks else None, float(self.ecut),
         )
+
+
+def partition_modes(lib: Library, wk, ecut: float, wc: float, nmax: int, nmax4: int, nmax2: int,
+                    nmax3: int) -> list:
+    """Block sizes of the reference's partition_modes (src/mode_data.hpp:818-886)."""
+    w = np.ascontiguousarray(wk, dtype=np.float64)
+    nb = i64()
+    out = np.zeros(len(w), dtype=np.int64)
+    lib._call("sb_partition_modes", lib.ctx, len(w), _dp(w), float(ecut), float(wc), int(nmax), int(nmax4),
+              int(nmax2), int(nmax3), C.byref(nb), _ip(out))
+    return [int(x) for x in out[: nb.value]]
+
+
+def driver_blocks(lib: Library, wk, ecut: float = 3000.0, high_frequency_bound: float = 10.0,
+                  target_dimension: int = 1000, maximum_dimension: int = 100,
+                  maximum_dimension_high_frequency: Optional[int] = None):
+    """(block_sizes, block_nmax) as the reference driver derives them from its input keys
+    (src/spin_boson.cpp:194-209, 418-441): partition_modes with wc = 2 * high frequency bound,
+    and each block's cutoff cap chosen by its first mode's frequency."""
+    nhigh = maximum_dimension if maximum_dimension_high_frequency is None else maximum_dimension_high_frequency
+    wc2 = 2.0 * high_frequency_bound
+    sizes = partition_modes(lib, wk, ecut, wc2, target_dimension, 250, maximum_dimension, nhigh)
+    nmax, at = [], 0
+    for s in sizes:
+        nmax.append(nhigh if wk[at] > wc2 else maximum_dimension)
+        at += s
+    return sizes, nmax
+
+
+def mode_block(lib: Library, w, g, ecut: float, nmax: int) -> dict:
+    """Occupation table, H0 diagonal and Hc (CSR) of one leaf's mode block."""
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    ns, nz = i64(), i64()
+    args = (lib.ctx, len(w), _dp(w), _dp(g), float(ecut), int(nmax), C.byref(ns), C.byref(nz))
+    lib._call("sb_mode_block", *args, None, None, None, None, None)
+    occ = np.zeros((ns.value, len(w)), dtype=np.int64)
+    h0 = np.zeros(ns.value)
+    rp = np.zeros(ns.value + 1, dtype=np.int64)
+    ci = np.zeros(max(nz.value, 1), dtype=np.int64)
+    va = np.zeros(max(nz.value, 1))
+    lib._call("sb_mode_block", *args, _ip(occ), _dp(h0), _ip(rp), _ip(ci), _dp(va))
+    return dict(occupations=occ, h0=h0, rowptr=rp, colind=ci[: nz.value], vals=va[: nz.value])
 
 
 class Tree:

@@ -938,6 +938,19 @@ int ttn_sb_initial_state(ttn_integ* in, const ttn_sb_params* p) {
     });
 }
 
+int ttn_sb_partition_modes(ttn_ctx* ctx, int64_t nmodes, const double* wk, double ecut, double wc, int64_t nmax,
+                           int64_t nmax4, int64_t nmax2, int64_t nmax3, int64_t* nblocks, int64_t* partition) {
+    return guard(ctx, [&] { sb_partition_c(nmodes, wk, ecut, wc, nmax, nmax4, nmax2, nmax3, nblocks, partition); });
+}
+
+int ttn_sb_mode_block(ttn_ctx* ctx, int64_t nmodes, const double* w, const double* g, double ecut, int64_t nmax,
+                      int64_t* nstates, int64_t* nnz, int64_t* occupations, double* h0, int64_t* hc_rowptr,
+                      int64_t* hc_colind, double* hc_vals) {
+    return guard(ctx, [&] {
+        sb_mode_block_c(nmodes, w, g, ecut, nmax, nstates, nnz, occupations, h0, hc_rowptr, hc_colind, hc_vals);
+    });
+}
+
 int ttn_batch_create(ttn_ctx* ctx, int64_t n, ttn_integ* const* integs, ttn_batch** out) {
     return guard(ctx, [&] {
         check(n >= 1, ERR_INVALID, "empty batch");

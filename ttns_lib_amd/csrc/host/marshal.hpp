@@ -67,7 +67,7 @@ inline SBParams sb_from_c(const ttn_sb_params* p) {
     SBParams s;
     s.wk.assign(p->wk, p->wk + p->nmodes);
     s.gk.assign(p->gk, p->gk + p->nmodes);
-    s.boson_dim.assign(p->boson_dim, p->boson_dim + p->nmodes);
+    check(p->wk != nullptr && p->gk != nullptr, ERR_INVALID, "spin-boson: wk / gk is NULL");
     s.nspf = p->nspf;
     s.nspf_lower = p->nspf_lower;
     s.nspf_cap = p->nspf_cap;
@@ -79,7 +79,50 @@ inline SBParams sb_from_c(const ttn_sb_params* p) {
     }
     for (i64 i = 0; i < p->nhilb; ++i) s.psi0.push_back(cget(p->psi0, i));
     s.seed = p->seed;
+    if (p->nblocks != 0) {
+        check(p->nblocks > 0 && p->block_sizes != nullptr && p->block_nmax != nullptr, ERR_INVALID,
+              "spin-boson: nblocks needs block_sizes and block_nmax");
+        s.block_sizes.assign(p->block_sizes, p->block_sizes + p->nblocks);
+        s.block_nmax.assign(p->block_nmax, p->block_nmax + p->nblocks);
+        s.ecut = p->ecut;
+    } else {
+        check(p->boson_dim != nullptr, ERR_INVALID, "spin-boson: boson_dim is NULL");
+        s.boson_dim.assign(p->boson_dim, p->boson_dim + p->nmodes);
+    }
     return s;
+}
+
+inline void sb_partition_c(int64_t nmodes, const double* wk, double ecut, double wc, int64_t nmax, int64_t nmax4,
+                           int64_t nmax2, int64_t nmax3, int64_t* nblocks, int64_t* partition) {
+    check(nmodes >= 1 && wk != nullptr && nblocks != nullptr, ERR_INVALID, "partition_modes: bad arguments");
+    const auto part = partition_modes(std::vector<double>(wk, wk + nmodes), ecut, wc, nmax, nmax4, nmax2, nmax3);
+    *nblocks = static_cast<int64_t>(part.size());
+    if (partition)
+        for (size_t i = 0; i < part.size(); ++i) partition[i] = part[i];
+}
+
+inline void sb_mode_block_c(int64_t nmodes, const double* w, const double* g, double ecut, int64_t nmax,
+                            int64_t* nstates, int64_t* nnz, int64_t* occupations, double* h0, int64_t* hc_rowptr,
+                            int64_t* hc_colind, double* hc_vals) {
+    check(nmodes >= 1 && w != nullptr && g != nullptr && nstates != nullptr && nnz != nullptr, ERR_INVALID,
+          "mode block: bad arguments");
+    const ModeBlock b = mode_block(std::vector<double>(w, w + nmodes), std::vector<double>(g, g + nmodes), ecut, nmax);
+    const PrimOp hc = block_hc(b);
+    *nstates = b.nstates;
+    *nnz = hc.nnz();
+    if (occupations)
+        for (i64 s = 0; s < b.nstates; ++s)
+            for (int64_t k = 0; k < nmodes; ++k) occupations[s * nmodes + k] = b.occ(s, static_cast<size_t>(k));
+    if (h0) {
+        const PrimOp d = block_h0(b);
+        for (i64 s = 0; s < b.nstates; ++s) h0[s] = d.diag[s].real();
+    }
+    if (hc_rowptr)
+        for (size_t i = 0; i < hc.rowptr.size(); ++i) hc_rowptr[i] = hc.rowptr[i];
+    if (hc_colind)
+        for (size_t i = 0; i < hc.colind.size(); ++i) hc_colind[i] = hc.colind[i];
+    if (hc_vals)
+        for (size_t i = 0; i < hc.vals.size(); ++i) hc_vals[i] = hc.vals[i].real();
 }
 
 }  // namespace ttnh

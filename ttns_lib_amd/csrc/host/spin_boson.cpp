@@ -5,8 +5,31 @@
 
 namespace ttnh {
 
-Topology sb_topology(const SBParams& p) {
-    const size_t nblocks = p.boson_dim.size();
+std::vector<ModeBlock> sb_blocks(const SBParams& p) {
+    const size_t n = p.wk.size();
+    check(n >= 1 && p.gk.size() == n, ERR_INVALID, "spin-boson: wk and gk must have equal, nonzero length");
+    std::vector<ModeBlock> out;
+    if (p.block_sizes.empty()) {
+        check(p.boson_dim.size() == n, ERR_INVALID, "wk, gk and boson_dim must have equal length");
+        for (size_t i = 0; i < n; ++i) out.push_back(single_mode(p.wk[i], p.gk[i], p.boson_dim[i]));
+        return out;
+    }
+    check(p.block_nmax.size() == p.block_sizes.size(), ERR_INVALID, "spin-boson: one cutoff cap per block");
+    size_t at = 0;
+    for (size_t b = 0; b < p.block_sizes.size(); ++b) {
+        const i64 m = p.block_sizes[b];
+        check(m >= 1 && at + static_cast<size_t>(m) <= n, ERR_INVALID, "spin-boson: block sizes must cover the modes");
+        out.push_back(mode_block(std::vector<double>(p.wk.begin() + at, p.wk.begin() + at + m),
+                                 std::vector<double>(p.gk.begin() + at, p.gk.begin() + at + m), p.ecut,
+                                 p.block_nmax[b]));
+        at += static_cast<size_t>(m);
+    }
+    check(at == n, ERR_INVALID, "spin-boson: block sizes must add up to the number of modes");
+    return out;
+}
+
+Topology sb_topology(const SBParams& p, const std::vector<i64>& leaf_dims) {
+    const size_t nblocks = leaf_dims.size();
     check(nblocks >= 1, ERR_INVALID, "spin-boson model needs at least one bath mode");
     const size_t nlevels = static_cast<size_t>(std::log2(static_cast<double>(nblocks)));
     auto level_fn = [nlevels](i64 top, i64 bottom) {
@@ -23,42 +46,13 @@ Topology sb_topology(const SBParams& p) {
         NTree t(1);
         t.insert(p.nhilb).insert(p.nhilb);
         NTree& bath = t.insert(p.nhilb);
-        htucker_subtree(bath, p.boson_dim, 2, level_fn(top, bottom));
+        htucker_subtree(bath, leaf_dims, 2, level_fn(top, bottom));
         sanitise_tree(t);
         return t;
     };
     NTree topo = build(p.nspf, p.nspf_lower);
     NTree cap = build(std::max(p.nspf, p.nspf_cap), std::max(p.nspf_lower, p.nspf_lower_cap));
     return topology_from_ntree(topo, cap);
-}
-
-PrimOp harmonic_h0(double w, i64 d) {
-    PrimOp op;
-    op.kind = OP_DIAG;
-    op.dim = d;
-    op.diag.resize(d);
-    for (i64 n = 0; n < d; ++n) op.diag[n] = cplx(w * (static_cast<double>(n) + 0.5 - 0.5), 0.0);
-    return op;
-}
-
-PrimOp harmonic_hc(double g, i64 d) {
-    // row n: lowering entry (n-1) then raising entry (n+1)
-    PrimOp op;
-    op.kind = OP_CSR;
-    op.dim = d;
-    op.rowptr.push_back(0);
-    for (i64 n = 0; n < d; ++n) {
-        if (n > 0) {
-            op.colind.push_back(n - 1);
-            op.vals.push_back(cplx(std::sqrt(static_cast<double>(n)) * g, 0.0));
-        }
-        if (n + 1 < d) {
-            op.colind.push_back(n + 1);
-            op.vals.push_back(cplx(std::sqrt(static_cast<double>(n) + 1.0) * g, 0.0));
-        }
-        op.rowptr.push_back(static_cast<i64>(op.vals.size()));
-    }
-    return op;
 }
 
 std::vector<std::vector<cplx>> sb_initial_state(const Topology& t, const std::vector<cplx>& psi0,
@@ -118,17 +112,19 @@ std::vector<std::vector<cplx>> sb_initial_state(const Topology& t, const std::ve
 }
 
 SBProblem sb_build(const SBParams& p) {
-    const size_t nb = p.boson_dim.size();
-    check(p.wk.size() == nb && p.gk.size() == nb, ERR_INVALID, "wk, gk and boson_dim must have equal length");
+    const std::vector<ModeBlock> blocks = sb_blocks(p);
+    const size_t nb = blocks.size();
+    std::vector<i64> leaf_dims(nb);
+    for (size_t i = 0; i < nb; ++i) leaf_dims[i] = blocks[i].nstates;
     check(static_cast<i64>(p.hsys.size()) == p.nhilb * p.nhilb &&
               static_cast<i64>(p.scoup.size()) == p.nhilb * p.nhilb,
           ERR_INVALID, "Hsys / Scoup must be nhilb x nhilb");
     SBProblem prob;
-    prob.topo = sb_topology(p);
+    prob.topo = sb_topology(p, leaf_dims);
 
     std::vector<i64> mode_dims(nb + 1);
     mode_dims[0] = p.nhilb;
-    for (size_t i = 0; i < nb; ++i) mode_dims[i + 1] = p.boson_dim[i];
+    for (size_t i = 0; i < nb; ++i) mode_dims[i + 1] = leaf_dims[i];
     SOP H(static_cast<int>(1 + 2 * nb), mode_dims);
 
     PrimOp hs;
@@ -142,8 +138,8 @@ SBProblem sb_build(const SBParams& p) {
     for (size_t i = 0; i < nb; ++i) zr[i] = static_cast<int>(2 * i + 2);
     H.bind(sc, zr, 0);
     for (size_t i = 0; i < nb; ++i) {
-        H.bind(harmonic_h0(p.wk[i], p.boson_dim[i]), {static_cast<int>(1 + 2 * i)}, static_cast<int>(i + 1));
-        H.bind(harmonic_hc(p.gk[i], p.boson_dim[i]), {static_cast<int>(2 + 2 * i)}, static_cast<int>(i + 1));
+        H.bind(block_h0(blocks[i]), {static_cast<int>(1 + 2 * i)}, static_cast<int>(i + 1));
+        H.bind(block_hc(blocks[i]), {static_cast<int>(2 + 2 * i)}, static_cast<int>(i + 1));
     }
     prob.sop = std::move(H);
     prob.state = sb_initial_state(prob.topo, p.psi0, p.seed);
