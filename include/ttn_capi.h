@@ -158,7 +158,8 @@ int ttn_sb_initial_state(ttn_integ* integ, const ttn_sb_params* p);
 /* partition_modes (src/mode_data.hpp:818-886, called at src/spin_boson.cpp:420 with
  * wc = 2 * high frequency bound, nmax = target dimension, nmax4 = 250, nmax2 = maximum
  * dimension, nmax3 = maximum dimension for high frequencies): sizes of consecutive mode
- * blocks; wk sorted by magnitude.  partition: room for nmodes entries, or NULL to query. */
+ * blocks; wk sorted by magnitude.  partition: room for nmodes entries, or NULL to query.
+ * This and ttn_sb_mode_block are host-only and accept ctx == NULL (no error text is then kept). */
 int ttn_sb_partition_modes(ttn_ctx* ctx, int64_t nmodes, const double* wk, double ecut, double wc, int64_t nmax,
                            int64_t nmax4, int64_t nmax2, int64_t nmax3, int64_t* nblocks, int64_t* partition);
 /* One leaf's mode block (harmonic_mode_combination::construct_basis_topology, H0, Hc;

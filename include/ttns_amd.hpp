@@ -47,7 +47,7 @@ namespace detail {
 inline void check(int st, ttn_ctx* ctx, const char* what) {
     if (st != TTN_OK)
         throw std::runtime_error(std::string("Failed to apply projector splitting integrator.\n") + what + ": " +
-                                 ttn_last_error(ctx));
+                                 (ctx ? ttn_last_error(ctx) : "invalid arguments"));
 }
 }  // namespace detail
 
@@ -265,6 +265,80 @@ struct sparse_matrix_operator {
     csr_matrix<T> mat;
 };
 }  // namespace ops
+
+// ------------------------------------------------------------------ bath modes of the driver
+// The spin-boson driver's mode handling (src/mode_data.hpp): partition_modes (:818-886) and
+// harmonic_mode_combination (:21-361) with the calls the driver makes (set_wk / set_gk over
+// iterator ranges, construct_basis_topology, H0, Hc; src/spin_boson.cpp:418-441, 515-538).
+template <typename T>
+bool partition_modes(const std::vector<T>& wk, T ecut, T wc, std::vector<size_t>& partition, size_t nmax,
+                     size_t nmax4, size_t nmax2, size_t nmax3) {
+    std::vector<double> w(wk.begin(), wk.end());
+    std::vector<int64_t> part(w.size());
+    int64_t nb = 0;
+    detail::check(ttn_sb_partition_modes(nullptr, static_cast<int64_t>(w.size()), w.data(), ecut, wc,
+                                         static_cast<int64_t>(nmax), static_cast<int64_t>(nmax4),
+                                         static_cast<int64_t>(nmax2), static_cast<int64_t>(nmax3), &nb, part.data()),
+                  nullptr, "partition_modes");
+    partition.assign(part.begin(), part.begin() + nb);
+    return true;
+}
+
+template <typename T>
+class harmonic_mode_combination {
+  public:
+    template <class It>
+    void set_wk(It a, It b) { m_w.assign(a, b); }
+    template <class It>
+    void set_gk(It a, It b) { m_g.assign(a, b); }
+    // number of states of the block (the leaf dimension)
+    size_t construct_basis_topology(T ecut, size_t nmax) {
+        if (m_g.size() != m_w.size()) m_g.assign(m_w.size(), T(0));
+        m_ecut = ecut;
+        m_nmax = nmax;
+        int64_t ns = 0, nnz = 0;
+        detail::check(ttn_sb_mode_block(nullptr, static_cast<int64_t>(m_w.size()), m_w.data(), m_g.data(), ecut,
+                                        static_cast<int64_t>(nmax), &ns, &nnz, nullptr, nullptr, nullptr, nullptr,
+                                        nullptr),
+                      nullptr, "construct_basis_topology");
+        return static_cast<size_t>(ns);
+    }
+    template <typename U>
+    void H0(diagonal_matrix<U>& H) const {
+        std::vector<double> d;
+        tables<std::complex<double>>(&d, nullptr);
+        H.diag.assign(d.begin(), d.end());
+    }
+    template <typename U>
+    void Hc(csr_matrix<U>& H) const {
+        tables(nullptr, &H);
+    }
+
+  private:
+    template <typename U>
+    void tables(std::vector<double>* h0, csr_matrix<U>* hc) const {
+        int64_t ns = 0, nnz = 0;
+        const int64_t n = static_cast<int64_t>(m_w.size());
+        detail::check(ttn_sb_mode_block(nullptr, n, m_w.data(), m_g.data(), m_ecut, static_cast<int64_t>(m_nmax), &ns,
+                                        &nnz, nullptr, nullptr, nullptr, nullptr, nullptr),
+                      nullptr, "harmonic_mode_combination");
+        std::vector<double> d(ns), vals(nnz);
+        std::vector<int64_t> rp(ns + 1), ci(nnz);
+        detail::check(ttn_sb_mode_block(nullptr, n, m_w.data(), m_g.data(), m_ecut, static_cast<int64_t>(m_nmax), &ns,
+                                        &nnz, nullptr, d.data(), rp.data(), ci.data(), vals.data()),
+                      nullptr, "harmonic_mode_combination");
+        if (h0) *h0 = d;
+        if (hc) {
+            hc->n = static_cast<size_type>(ns);
+            hc->rowptr = rp;
+            hc->colind = ci;
+            hc->values.assign(vals.begin(), vals.end());
+        }
+    }
+    std::vector<double> m_w, m_g;
+    T m_ecut = T(0);
+    size_t m_nmax = 0;
+};
 
 template <typename T, typename backend = blas_backend>
 class sop_operator {

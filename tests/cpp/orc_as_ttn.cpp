@@ -15,6 +15,14 @@ int ttn_tree_create(ttn_ctx* c, int64_t n, const int64_t* parent, const int64_t*
                     const int64_t* leaf, ttn_tree** out) {
     return orc_tree_create(O(ctx)(c), n, parent, hrank, cap, leaf, reinterpret_cast<orc_tree**>(out));
 }
+int ttn_sb_partition_modes(ttn_ctx* c, int64_t n, const double* wk, double ecut, double wc, int64_t nmax, int64_t nmax4,
+                           int64_t nmax2, int64_t nmax3, int64_t* nb, int64_t* part) {
+    return orc_sb_partition_modes(O(ctx)(c), n, wk, ecut, wc, nmax, nmax4, nmax2, nmax3, nb, part);
+}
+int ttn_sb_mode_block(ttn_ctx* c, int64_t n, const double* w, const double* g, double ecut, int64_t nmax,
+                      int64_t* ns, int64_t* nnz, int64_t* occ, double* h0, int64_t* rp, int64_t* ci, double* v) {
+    return orc_sb_mode_block(O(ctx)(c), n, w, g, ecut, nmax, ns, nnz, occ, h0, rp, ci, v);
+}
 int ttn_tree_destroy(ttn_tree* t) { return orc_tree_destroy(O(tree)(t)); }
 int ttn_sop_create(ttn_ctx* c, int64_t nmodes, const int64_t* dims, int64_t nterms, const double* coeff, int64_t nops,
                    const int32_t* mode, const int32_t* kind, const int64_t* coff, const double* cdata,

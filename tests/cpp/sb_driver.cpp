@@ -4,7 +4,12 @@
 // the tests as the drop-in check: its <sigma_z(t)> must equal the oracle's on the same model.
 //
 // usage: sb_driver nmodes boson_dim nspf nsteps dt alpha wc seed krylov_dim
+//                  [ecut high_frequency_bound target_dimension maximum_dimension_high_frequency]
 // Bath: w_k = wc * 4 k / n (k = 1..n), g_k = sqrt(J(w_k) dw / pi), J(w) = pi/2 alpha w e^{-w/wc}.
+// The modes are split into leaves with partition_modes and each leaf's operators come from
+// harmonic_mode_combination, as in the reference driver (src/spin_boson.cpp:418-441,
+// 515-538); boson_dim is the maximum dimension.  Without the optional arguments every leaf
+// holds one mode (target dimension = boson_dim, no high-frequency modes).
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -21,27 +26,46 @@ using backend_type = blas_backend;
 
 int main(int argc, char** argv) {
     if (argc < 10) {
-        std::cerr << "usage: sb_driver nmodes boson_dim nspf nsteps dt alpha wc seed krylov_dim\n";
+        std::cerr << "usage: sb_driver nmodes boson_dim nspf nsteps dt alpha wc seed krylov_dim"
+                     " [ecut high_frequency_bound target_dimension maximum_dimension_high_frequency]\n";
         return 2;
     }
-    const size_type nblocks = std::strtoul(argv[1], nullptr, 10);
+    const size_type nblocks_arg = std::strtoul(argv[1], nullptr, 10);
     const size_type nmax = std::strtoul(argv[2], nullptr, 10);
     const size_type nspf = std::strtoul(argv[3], nullptr, 10), nspf_lower = nspf;
     const int nsteps = std::atoi(argv[4]);
     const double dt = std::atof(argv[5]), alpha = std::atof(argv[6]), wc = std::atof(argv[7]);
     const unsigned seed = static_cast<unsigned>(std::strtoul(argv[8], nullptr, 10));
     const size_type krylov_dim = std::strtoul(argv[9], nullptr, 10);
+    const bool blocks = argc >= 14;
+    const double ecut = blocks ? std::atof(argv[10]) : 3000.0;
+    const double hfb = blocks ? std::atof(argv[11]) : 1e300;
+    const size_type ntarget = blocks ? std::strtoul(argv[12], nullptr, 10) : nmax;
+    const size_type nmaxlargefreq = blocks ? std::strtoul(argv[13], nullptr, 10) : nmax;
     try {
         const size_type nhilb = 2;
-        std::vector<double> wk(nblocks), gk(nblocks);
-        for (size_type k = 0; k < nblocks; ++k) wk[k] = wc * 4.0 * static_cast<double>(k + 1) / nblocks;
-        const double dw = nblocks > 1 ? wk[1] - wk[0] : wk[0];
-        for (size_type k = 0; k < nblocks; ++k) {
+        const size_type N = nblocks_arg;
+        std::vector<double> wk(N), gk(N);
+        for (size_type k = 0; k < N; ++k) wk[k] = wc * 4.0 * static_cast<double>(k + 1) / N;
+        const double dw = N > 1 ? wk[1] - wk[0] : wk[0];
+        for (size_type k = 0; k < N; ++k) {
             const double J = 0.5 * std::acos(-1.0) * alpha * wk[k] * std::exp(-wk[k] / wc);
             gk[k] = std::sqrt(J * dw / std::acos(-1.0));
         }
-        std::vector<size_type> mode_dimensions(nblocks + 1, nmax), nmodes(nblocks, nmax);
+        std::vector<size_t> partitions;
+        partition_modes(wk, ecut, 2 * hfb, partitions, ntarget, 250, nmax, nmaxlargefreq);
+        const size_type nblocks = partitions.size();
+        std::vector<size_type> mode_dimensions(nblocks + 1), nmodes(nblocks), nskip(nblocks);
         mode_dimensions[0] = nhilb;
+        harmonic_mode_combination<double> mode_combination;
+        for (size_type i = 0, counter = 0; i < nblocks; ++i) {
+            nskip[i] = counter;
+            const size_type nmaxt = wk[counter] > 2 * hfb ? nmaxlargefreq : nmax;
+            mode_combination.set_wk(wk.begin() + counter, wk.begin() + counter + partitions[i]);
+            mode_combination.set_gk(gk.begin() + counter, gk.begin() + counter + partitions[i]);
+            nmodes[i] = mode_dimensions[i + 1] = mode_combination.construct_basis_topology(ecut, nmaxt);
+            counter += partitions[i];
+        }
 
         // topology: root -> [system node -> primitive], [bath node -> balanced binary subtree]
         auto build = [&](size_type top, size_type bottom) {
@@ -82,23 +106,15 @@ int main(int argc, char** argv) {
         for (size_type i = 0; i < nblocks; ++i) sigma_z_indices[i] = 1 + 2 * i + 1;
         H.bind(ops::dense_matrix_operator<complex_type, backend_type>{Scoup}, sigma_z_indices, 0);
         for (size_type i = 0; i < nblocks; ++i) {
+            const size_type nmaxt = wk[nskip[i]] > 2 * hfb ? nmaxlargefreq : nmax;
+            mode_combination.set_wk(wk.begin() + nskip[i], wk.begin() + nskip[i] + partitions[i]);
+            mode_combination.set_gk(gk.begin() + nskip[i], gk.begin() + nskip[i] + partitions[i]);
+            mode_combination.construct_basis_topology(ecut, nmaxt);
             diagonal_matrix<complex_type> H0;
-            for (size_type n = 0; n < nmax; ++n) H0.diag.push_back(wk[i] * static_cast<double>(n));
+            mode_combination.H0(H0);
             H.bind(ops::diagonal_matrix_operator<complex_type, backend_type>(H0), {1 + 2 * i}, i + 1);
             csr_matrix<complex_type> Hc;
-            Hc.n = nmax;
-            Hc.rowptr.push_back(0);
-            for (size_type n = 0; n < nmax; ++n) {
-                if (n > 0) {
-                    Hc.colind.push_back(static_cast<int64_t>(n - 1));
-                    Hc.values.push_back(std::sqrt(static_cast<double>(n)) * gk[i]);
-                }
-                if (n + 1 < nmax) {
-                    Hc.colind.push_back(static_cast<int64_t>(n + 1));
-                    Hc.values.push_back(std::sqrt(static_cast<double>(n) + 1.0) * gk[i]);
-                }
-                Hc.rowptr.push_back(static_cast<int64_t>(Hc.values.size()));
-            }
+            mode_combination.Hc(Hc);
             H.bind(ops::sparse_matrix_operator<complex_type, backend_type>(Hc), {1 + 2 * i + 1}, i + 1);
         }
 
