@@ -59,3 +59,41 @@ def test_block_helpers_exported_by_the_hip_library(hip, orc):
     a, b = mode_block(hip, [0.5, 0.9], [0.2, 0.3], 3000.0, 4), mode_block(orc, [0.5, 0.9], [0.2, 0.3], 3000.0, 4)
     for k in a:
         assert np.array_equal(a[k], b[k])
+
+
+def test_batch_with_block_leaves_equals_individual_runs(hip):
+    # a partitioned bath and a one-mode-per-leaf bath in one launch: each instance runs alone
+    # on its workgroup (team of one), so the batch gives the individual results bit for bit
+    from helpers import sb_model
+    from ttns_lib_amd.capi import Batch
+
+    models = [_partitioned_bath(hip), sb_model(8, 6, 4)]
+    singles = [make_integrator(hip, m, team_size=1)[2] for m in models]
+    batched = [make_integrator(hip, m, team_size=1)[2] for m in models]
+    for a in singles:
+        a.step(3)
+    B = Batch(hip, batched)
+    try:
+        B.step(3)
+    finally:
+        B.destroy()
+    for a, b in zip(singles, batched):
+        for nd in range(len(a.shapes)):
+            assert np.array_equal(a.get_node(nd), b.get_node(nd))
+
+
+def test_block_leaves_spread_over_a_team_match_oracle(hip, orc):
+    # leaves of 216 states x 10 functions (2160 entries) reach the team threshold: their
+    # H-actions, QRs and environment products are spread over a team of 4 workgroups
+    m_hip, m_orc = _partitioned_bath(hip), _partitioned_bath(orc)
+    for m in (m_hip, m_orc):
+        m.nspf, m.nspf_lower = 10, 10
+    _, _, a = make_integrator(hip, m_hip, team_size=4)
+    _, _, o = make_integrator(orc, m_orc)
+    assert (216, 10) in a.shapes
+    for _ in range(3):
+        a.step(1)
+        o.step(1)
+        for op in "xyz":
+            assert abs(a.expect(0, SIGMA[op]) - o.expect(0, SIGMA[op])) < 1e-10
+        assert abs(a.norm2() - o.norm2()) < 1e-12
