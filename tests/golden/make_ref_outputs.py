@@ -1,0 +1,60 @@
+"""Writes tests/golden/ref_outputs.json from data files the reference ships: the exact run
+inputs (temp/infiles_{1,4,5}), the discretised baths those runs printed (temp/errfiles_*,
+the w_k g_k lines between the "w_k \\t g_k" header and "bath discretised.") and the first rows
+of their trajectories (example_outputs/*.out: t <sx> <sy> <sz> <1> Nonesite Ntwosite nh/|A|
+[hranks of every non-root node in pre-order]).  Run once in a tree where the reference is
+present; the tests read only the JSON.
+
+usage: python tests/golden/make_ref_outputs.py [reference root]
+"""
+import json
+import os
+import sys
+
+REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_outputs.json")
+RUNS = {"superohmic_exponential": 1, "ohmic_debye": 4, "subohmic_exponential_t0": 5}
+KEYS = ["Hsys", "Scoup", "psi0", "bath", "dt", "nspf", "nspf lower", "nspf capacity", "nspf lower capacity", "nmodes",
+        "maximum dimension", "maximum dimension high frequency", "high frequency bound", "target dimension",
+        "krylov tolerance", "krylov dimension", "spawning parameter", "unoccupied threshold", "minimum unoccupied",
+        "seed", "print bond dimension"]
+NROWS = 11
+
+
+def bath(path):
+    with open(path, "rb") as f:
+        lines = f.read().decode().split("\n")
+    start = next(i for i, ln in enumerate(lines) if ln.rstrip().endswith("g_k")) + 1
+    end = next(i for i, ln in enumerate(lines) if ln.strip() == "bath discretised.")
+    wg = [tuple(float(x) for x in ln.split()) for ln in lines[start:end]]
+    return [w for w, _ in wg], [g for _, g in wg]
+
+
+def rows(path):
+    out = []
+    with open(path) as f:
+        for ln in f:
+            parts = ln.split()
+            if len(out) == NROWS or not parts or parts[0].startswith("nh_app"):
+                break
+            out.append(dict(values=[float(x) for x in parts[:5]], counters=[float(x) for x in parts[5:8]],
+                            hranks=[int(x) for x in parts[8:]]))
+    return out
+
+
+def main():
+    fixture = {}
+    for name, k in RUNS.items():
+        with open(os.path.join(REF, "temp", f"infiles_{k}")) as f:
+            cfg = json.load(f)
+        wk, gk = bath(os.path.join(REF, "temp", f"errfiles_{k}"))
+        assert len(wk) == cfg["nmodes"], (name, len(wk))
+        fixture[name] = dict(input={key: cfg[key] for key in KEYS if key in cfg}, output_file=cfg["output file"],
+                             wk=wk, gk=gk, rows=rows(os.path.join(REF, cfg["output file"])))
+    with open(OUT, "w") as f:
+        json.dump(fixture, f, indent=0)
+    print(f"wrote {OUT}")
+
+
+if __name__ == "__main__":
+    main()

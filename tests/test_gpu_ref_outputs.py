@@ -1,0 +1,36 @@
+"""GPU: the HIP library against the reference's own outputs on its exact 512-mode runs (the
+oracle passes the same checks in test_ref_outputs.py), and against the oracle."""
+import pytest
+
+from test_ref_outputs import TRAJECTORY_CASES, check_trajectory, fixed_step_integrator, ref_model, ref_runs
+from ttns_lib_amd.capi import SIGMA, sb_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def test_initial_bond_dimensions_match_reference_output(hip):
+    run = ref_runs()["ohmic_debye"]
+    tree, _ = sb_problem(hip, ref_model(hip, run))
+    n, _ = tree.size()
+    assert [tree.node(i)["hrank"] for i in range(1, n)] == run["rows"][0]["hranks"]
+
+
+@pytest.mark.parametrize("name,steps,bound", TRAJECTORY_CASES)
+def test_gpu_matches_reference_trajectory(hip, name, steps, bound):
+    print(f"[reference run] {name}, {steps} steps", flush=True)
+    run = ref_runs()[name]
+    _, _, ig = fixed_step_integrator(hip, run)
+    check_trajectory(ig, run, steps, bound)
+
+
+@pytest.mark.parametrize("name", [c[0] for c in TRAJECTORY_CASES])
+def test_gpu_matches_oracle_on_reference_run(hip, orc, name):
+    run = ref_runs()[name]
+    _, _, a = fixed_step_integrator(hip, run)
+    _, _, o = fixed_step_integrator(orc, run)
+    for _ in range(2):
+        a.step(1)
+        o.step(1)
+        for op in "xyz":
+            assert abs(a.expect(0, SIGMA[op]) - o.expect(0, SIGMA[op])) < 1e-10
+        assert abs(a.norm2() - o.norm2()) < 1e-12
