@@ -23,14 +23,19 @@ def test_gpu_matches_reference_trajectory(hip, name, steps, bound):
     check_trajectory(ig, run, steps, bound)
 
 
-@pytest.mark.parametrize("name", [c[0] for c in TRAJECTORY_CASES])
+# per-step bounds, about ten times the oracle's own spread under a 1e-15 relative perturbation
+# of the initial state (subohmic: 1.4e-12, 6e-11; superohmic: 5e-9 after one step)
+ORACLE_BOUNDS = {"subohmic_exponential_t0": (2e-11, 5e-10), "superohmic_exponential": (5e-8,)}
+
+
+@pytest.mark.parametrize("name", list(ORACLE_BOUNDS))
 def test_gpu_matches_oracle_on_reference_run(hip, orc, name):
     run = ref_runs()[name]
     _, _, a = fixed_step_integrator(hip, run)
     _, _, o = fixed_step_integrator(orc, run)
-    for _ in range(2):
+    for bound in ORACLE_BOUNDS[name]:
         a.step(1)
         o.step(1)
         for op in "xyz":
-            assert abs(a.expect(0, SIGMA[op]) - o.expect(0, SIGMA[op])) < 1e-10
+            assert abs(a.expect(0, SIGMA[op]) - o.expect(0, SIGMA[op])) < bound
         assert abs(a.norm2() - o.norm2()) < 1e-12
