@@ -5,15 +5,26 @@ of their trajectories (example_outputs/*.out: t <sx> <sy> <sz> <1> Nonesite Ntwo
 [hranks of every non-root node in pre-order]).  Run once in a tree where the reference is
 present; the tests read only the JSON.
 
+Two more outputs (beta_10/ohmic exponential and sudden) have no run input in temp/; their
+inputs are the matching files under inputs/, and their baths are the temp/errfiles_* whose
+tree gives the printed initial bond dimensions and whose first step reproduces the printed
+<sigma> (errfiles_2: 3.5e-12, errfiles_3: 2.2e-12; every other bath of the same tree shape
+differs by 5e-9 or more).
+
 usage: python tests/golden/make_ref_outputs.py [reference root]
 """
 import json
 import os
+import re
 import sys
 
 REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_outputs.json")
 RUNS = {"superohmic_exponential": 1, "ohmic_debye": 4, "subohmic_exponential_t0": 5}
+# name -> (input file under inputs/, output file, errfiles index)
+MATCHED = {"ohmic_exponential": ("beta_10/ohmic/spin_boson_exponential.in",
+                                 "example_outputs/beta_10/ohmic/spin_boson_exponential.out", 2),
+           "ohmic_sudden": ("beta_10/ohmic/spin_boson_sudden.in", "example_outputs/beta_10/ohmic/spin_boson_sudden.out", 3)}
 KEYS = ["Hsys", "Scoup", "psi0", "bath", "dt", "nspf", "nspf lower", "nspf capacity", "nspf lower capacity", "nmodes",
         "maximum dimension", "maximum dimension high frequency", "high frequency bound", "target dimension",
         "krylov tolerance", "krylov dimension", "spawning parameter", "unoccupied threshold", "minimum unoccupied",
@@ -42,8 +53,20 @@ def rows(path):
     return out
 
 
+def load_input(path):
+    # the files under inputs/ carry a trailing comma before the closing brace
+    with open(path) as f:
+        return json.loads(re.sub(r",\s*}\s*$", "}", f.read()))
+
+
 def main():
     fixture = {}
+    for name, (inp, out, k) in MATCHED.items():
+        cfg = load_input(os.path.join(REF, "inputs", inp))
+        cfg = {key.replace(" ", "").lower(): v for key, v in cfg.items()}
+        cfg = {key: cfg[key.replace(" ", "").lower()] for key in KEYS if key.replace(" ", "").lower() in cfg}
+        wk, gk = bath(os.path.join(REF, "temp", f"errfiles_{k}"))
+        fixture[name] = dict(input=cfg, output_file=out, wk=wk, gk=gk, rows=rows(os.path.join(REF, out)))
     for name, k in RUNS.items():
         with open(os.path.join(REF, "temp", f"infiles_{k}")) as f:
             cfg = json.load(f)

@@ -1,10 +1,15 @@
 """The oracle against the reference's own outputs (tests/golden/ref_outputs.json, written by
 make_ref_outputs.py from the reference's temp/infiles_*, temp/errfiles_* and
-example_outputs/*.out): three 512-mode runs with their exact discretised baths.
+example_outputs/*.out): five 512-mode runs with their exact discretised baths.
 
-* ohmic Debye, beta = 10: the first output row lists the initial bond dimension of every
-  non-root node in pre-order (686 numbers); the tree built here from the same bath
-  (partition_modes, mode blocks, htucker_subtree, sanitise_tree) must give them exactly.
+* ohmic Debye, exponential and sudden baths, beta = 10: the first output row lists the
+  initial bond dimension of every non-root node in pre-order (686, 1024 and 1024 numbers); the tree built
+  here from the same bath (partition_modes, mode blocks, htucker_subtree, sanitise_tree) must
+  give them exactly.
+* ohmic exponential and sudden baths, beta = 10 (1025-node trees): three steps agree
+  with the printed <sigma> within 1e-9 and 1e-10 (measured 3.5e-12, 5.1e-11, 2.6e-10 and
+  2.2e-12, 6.8e-12, 1.9e-11; the oracle's spread under a 1e-15 perturbation of its initial
+  state is 2.8e-13, 3.6e-11, 4.6e-10 and 1.5e-13, 2.5e-13, 1.0e-11).
 * subohmic exponential, T = 0 (256 leaves of two modes, 900 states each): the first two
   TDVP steps agree with the printed <sigma_x,y,z>(t) to 5e-10 (measured 3e-11 and 1.8e-10).
   The run amplifies rounding: perturbing the oracle's initial state by 1e-15 relative moves
@@ -62,7 +67,9 @@ def fixed_step_integrator(lib, run, **kw):
 
 
 # (run, steps compared, bound on <sigma>)
-TRAJECTORY_CASES = [("subohmic_exponential_t0", 2, 5e-10), ("superohmic_exponential", 1, 2e-8)]
+TRAJECTORY_CASES = [("subohmic_exponential_t0", 2, 5e-10), ("superohmic_exponential", 1, 2e-8),
+                    ("ohmic_exponential", 3, 1e-9), ("ohmic_sudden", 3, 1e-10)]
+BOND_DIMENSION_RUNS = ["ohmic_debye", "ohmic_exponential", "ohmic_sudden"]
 
 
 def check_trajectory(ig, run, steps, bound):
@@ -75,13 +82,13 @@ def check_trajectory(ig, run, steps, bound):
         assert abs(ig.norm2() - one) < 1e-12
 
 
-def test_initial_bond_dimensions_match_reference_output(orc):
-    run = ref_runs()["ohmic_debye"]
+@pytest.mark.parametrize("name", BOND_DIMENSION_RUNS)
+def test_initial_bond_dimensions_match_reference_output(orc, name):
+    run = ref_runs()[name]
     tree, _ = sb_problem(orc, ref_model(orc, run))
     n, _ = tree.size()
     hranks = [tree.node(i)["hrank"] for i in range(1, n)]
     assert hranks == run["rows"][0]["hranks"]
-    assert n == 687
 
 
 def test_initial_row_is_the_unperturbed_spin(orc):

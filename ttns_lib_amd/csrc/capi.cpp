@@ -109,7 +109,7 @@ void build_instance(ttn_integ* in) {
     const Topology& t = in->topo;
     const CompiledSOP& ops = in->ops;
     const int n = t.size();
-    check(n <= 1024, ERR_INVALID, "device path supports at most 1024 nodes");
+    check(n <= ttnd::MAX_NODES, ERR_INVALID, "device path supports at most 4096 nodes");
     const int kdim = static_cast<int>(in->p.krylov_dim);
     check(kdim >= 1 && kdim <= ttnd::MAXK, ERR_INVALID, "krylov_dim must be in [1, 16] on the device path");
 

@@ -15,6 +15,7 @@ namespace ttnd {
 constexpr int MAXC = 4;        // max children of a branch node on the device path
 constexpr int MAXK = 16;       // max Krylov dimension on the device path
 constexpr int NWORK = 6;       // generic work buffers, each max node size
+constexpr int MAX_NODES = 4096;  // nodes per tree (one byte of visit count each in LDS)
 
 // device wall-clock phase slots (100 MHz ticks, accumulated by thread 0)
 enum ProfSlot { PROF_HEFF = 0, PROF_HBOND, PROF_ORTH, PROF_EXPM, PROF_QR, PROF_SPF, PROF_MF, PROF_GAUGE,

@@ -1108,7 +1108,7 @@ __device__ void bond_update(Ctx& c, SweepState& st, int n, int ops) {
     }
 }
 
-__device__ void forward(Ctx& c, SweepState& st, bool evolve, int* visits) {
+__device__ void forward(Ctx& c, SweepState& st, bool evolve, unsigned char* visits) {
     for (int e = threadIdx.x; e < c.I.nnodes; e += NT) visits[e] = 0;
     __syncthreads();
     for (int s = 0; s < c.I.tour_len; ++s) {
@@ -1117,7 +1117,7 @@ __device__ void forward(Ctx& c, SweepState& st, bool evolve, int* visits) {
         const DNode& nd = c.node(id);
         const int tv = visits[id];
         __syncthreads();
-        if (threadIdx.x == 0) visits[id] = tv + 1;
+        if (threadIdx.x == 0) visits[id] = static_cast<unsigned char>(tv + 1);
         __syncthreads();
         if (tv != nd.nmodes) {
             if (tv == 0 && nd.parent >= 0) PROFILE(PROF_GAUGE, apply_from_parent(c, id));
@@ -1137,7 +1137,7 @@ __device__ void forward(Ctx& c, SweepState& st, bool evolve, int* visits) {
     }
 }
 
-__device__ void backward(Ctx& c, SweepState& st, bool evolve, int* visits) {
+__device__ void backward(Ctx& c, SweepState& st, bool evolve, unsigned char* visits) {
     for (int e = threadIdx.x; e < c.I.nnodes; e += NT) visits[e] = 0;
     __syncthreads();
     for (int s = c.I.tour_len - 1; s >= 0; --s) {
@@ -1146,7 +1146,7 @@ __device__ void backward(Ctx& c, SweepState& st, bool evolve, int* visits) {
         const DNode& nd = c.node(id);
         const int tv = visits[id];
         __syncthreads();
-        if (threadIdx.x == 0) visits[id] = tv + 1;
+        if (threadIdx.x == 0) visits[id] = static_cast<unsigned char>(tv + 1);
         __syncthreads();
         if (tv != nd.nmodes) {
             const int mode = nd.nmodes - (tv + 1);
@@ -1226,7 +1226,8 @@ __device__ __forceinline__ int xcd_id() {
 // the team that slot names; words[i] are the team words of instance i.
 __global__ __launch_bounds__(NT) void tdvp_run_kernel(const DInst* insts, RunArgs a, const TeamPlan* plan,
                                                       TeamWords* words, unsigned* xcd_fill) {
-    __shared__ int visits[1024];
+    // visit count of every node in the current sweep (at most 1 + MAXC)
+    __shared__ unsigned char visits[MAX_NODES];
     __shared__ int slot_info[3];
     if (plan == nullptr) {
         if (threadIdx.x == 0) {
