@@ -29,7 +29,7 @@ KEYS = ["Hsys", "Scoup", "psi0", "bath", "dt", "nspf", "nspf lower", "nspf capac
         "maximum dimension", "maximum dimension high frequency", "high frequency bound", "target dimension",
         "krylov tolerance", "krylov dimension", "spawning parameter", "unoccupied threshold", "minimum unoccupied",
         "seed", "print bond dimension"]
-NROWS = 11
+NROWS = 4
 
 
 def bath(path):
@@ -75,7 +75,7 @@ def main():
         fixture[name] = dict(input={key: cfg[key] for key in KEYS if key in cfg}, output_file=cfg["output file"],
                              wk=wk, gk=gk, rows=rows(os.path.join(REF, cfg["output file"])))
     with open(OUT, "w") as f:
-        json.dump(fixture, f, indent=0)
+        json.dump(fixture, f, separators=(",", ":"))
     print(f"wrote {OUT}")
 
 
