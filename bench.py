@@ -48,7 +48,7 @@ def make(lib, inst):
     return ig
 
 
-def workload(name):
+def workload(name, lib=None):
     from ttns_lib_amd import workloads as W
 
     if name == "c3":
@@ -67,6 +67,10 @@ def workload(name):
         return [W.c1_instance()], "Debye alpha 0.5 wc 2 beta 10, 128 modes, boson 16, bond 32 (fixed), dt 0.025"
     if name == "c4":
         return [W.c4_instance()], "Ohmic alpha 0.2 wc 5 T=0, 64 modes, boson 12, bond 24, dt 0.05"
+    if name in ("refruns", "refruns256"):
+        n = 64 if name == "refruns" else 256
+        return W.reference_runs(lib, n), (f"{n} instances of the reference's five 512-mode example runs (exact "
+                                          "baths, driver mode blocks, bonds 4-8, dt 0.005), seeds varied")
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -379,7 +383,7 @@ def main():
     from ttns_lib_amd.capi import Batch
 
     lib = ttns_lib_amd.load_library()
-    insts, desc = workload(args.workload)
+    insts, desc = workload(args.workload, lib)
     print(f"[bench] workload {args.workload}: {len(insts)} instances", file=sys.stderr, flush=True)
     igs = [make(lib, i) for i in insts]
     batch = Batch(lib, igs)

@@ -127,3 +127,33 @@ def c3_instances(path: str = C3_FILE):
                       nspf_lower=s["bond_dim"], seed=s["index"])
         insts.append(Instance(m, f"c3[{s['index']}]", 0.05, 12))
     return insts
+
+
+REF_FILE = os.path.join(GOLDEN, "ref_outputs.json")
+
+
+def reference_runs(lib, count: int = 64, path: str = REF_FILE):
+    """The reference's own 512-mode example runs (exact discretised baths and input keys of
+    example_outputs/{beta_10/{superohmic,ohmic},zero_temperature/subohmic}), built as its driver
+    builds them (partition_modes, mode blocks; src/spin_boson.cpp:418-538) at their initial bond
+    dimensions 4-8, dt 0.005: `count` instances cycling over the five runs, each copy with its own
+    seed for the random unoccupied functions."""
+    from .capi import driver_blocks
+
+    with open(path) as f:
+        runs = json.load(f)
+    names = sorted(runs)
+    out = []
+    for i in range(count):
+        name = names[i % len(names)]
+        run, cfg = runs[name], runs[name]["input"]
+        sizes, nmax = driver_blocks(lib, run["wk"], ecut=3000.0, high_frequency_bound=cfg["high frequency bound"],
+                                    target_dimension=cfg["target dimension"],
+                                    maximum_dimension=cfg["maximum dimension"],
+                                    maximum_dimension_high_frequency=cfg["maximum dimension high frequency"])
+        m = SpinBoson(wk=np.array(run["wk"]), gk=np.array(run["gk"]), block_sizes=sizes, block_nmax=nmax,
+                      nspf=cfg["nspf"], nspf_lower=cfg["nspf lower"], seed=cfg["seed"] + i // len(names),
+                      hsys=np.array(cfg["Hsys"], dtype=complex), scoup=np.array(cfg["Scoup"], dtype=complex),
+                      psi0=np.array(cfg["psi0"], dtype=complex))
+        out.append(Instance(m, f"{name}#{i // len(names)}", cfg["dt"], cfg["krylov dimension"]))
+    return out
