@@ -1,0 +1,92 @@
+"""The driver front-end (ttns_lib_amd/driver.py) on the reference's own example runs: it reads
+the reference's input keys and the bath log format, and its output rows must match the
+reference's .out files (the t = 0 row and bond dimensions exactly, the first steps' <O> within
+the bounds the oracle meets in test_ref_outputs.py).  On the CPU the driver runs over the
+oracle library (the fake backend of these tests); the GPU half runs the command line over the
+HIP library."""
+import io
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from test_ref_outputs import ref_runs
+from ttns_lib_amd import driver
+
+def write_case(tmp_path, name, steps):
+    run = ref_runs()[name]
+    cfg = dict(run["input"])
+    cfg["tmax"] = cfg["dt"] * steps
+    cfg["ops"] = [[[0, 1], [1, 0]], [[0, "-1.0j"], ["1.0j", 0]], [[1, 0], [0, -1]], [[1, 0], [0, 1]]]
+    cfg["output file"] = str(tmp_path / "out.out")
+    inp = tmp_path / "input.json"
+    # a trailing comma before the closing brace, as in the reference's inputs/
+    inp.write_text(json.dumps(cfg, indent=1)[:-1] + ",\n}")
+    # the reference's stderr log: a CR-separated debug line, the header, the pairs, the marker
+    lines = ["0 1.0\r1 2.0\rw_k \t g_k"] + [f"{w!r} {g!r}" for w, g in zip(run["wk"], run["gk"])]
+    lines += ["bath discretised.", "tree topology constructed", ""]
+    bath = tmp_path / "bath.log"
+    bath.write_text("\n".join(lines))
+    return run, str(inp), str(bath), cfg["output file"]
+
+
+def parse(text):
+    rows = []
+    for ln in text.splitlines():
+        if ln.startswith("nh_app"):
+            continue
+        p = ln.split()
+        rows.append(dict(values=[float(x) for x in p[:5]], hranks=[int(x) for x in p[8:]], text=ln))
+    return rows
+
+
+def check_rows(rows, run, steps, bound):
+    ref = run["rows"]
+    assert len(rows) == steps + 1
+    assert rows[0]["text"].startswith("0 0 0 1 1  0 0 0 ")
+    assert rows[0]["hranks"] == ref[0]["hranks"]
+    for r, e in zip(rows[1:], ref[1:1 + steps]):
+        assert abs(r["values"][0] - e["values"][0]) < 1e-15
+        assert np.abs(np.array(r["values"][1:4]) - np.array(e["values"][1:4])).max() < bound
+        assert abs(r["values"][4] - e["values"][4]) < 1e-12
+
+
+def test_bath_log_and_input_parsing(tmp_path):
+    run, inp, bath, _ = write_case(tmp_path, "ohmic_sudden", 1)
+    wk, gk = driver.load_bath(bath)
+    assert np.array_equal(wk, run["wk"]) and np.array_equal(gk, run["gk"])
+    cfg = driver.load_input(inp)
+    assert cfg["nspflower"] == run["input"]["nspf lower"] and cfg["krylovtolerance"] == 1e-12
+    plain = tmp_path / "bath.txt"
+    plain.write_text("# w g\n" + "\n".join(f"{w!r} {g!r}" for w, g in zip(run["wk"], run["gk"])))
+    w2, g2 = driver.load_bath(str(plain))
+    assert np.array_equal(w2, wk) and np.array_equal(g2, gk)
+    assert [o.shape for o in driver.observables(cfg)] == [(2, 2)] * 4
+    assert driver.observables(cfg)[1][0, 1] == -1j
+
+
+@pytest.mark.parametrize("name,steps,bound", [("ohmic_sudden", 2, 1e-10), ("subohmic_exponential_t0", 2, 5e-10)])
+def test_driver_over_oracle_matches_reference_output(orc, tmp_path, name, steps, bound):
+    run, inp, bath, _ = write_case(tmp_path, name, steps)
+    cfg = driver.load_input(inp)
+    wk, gk = driver.load_bath(bath)
+    out, log = io.StringIO(), io.StringIO()
+    driver.run(orc, cfg, wk, gk, out, log=log)
+    assert "not available" in log.getvalue()
+    check_rows(parse(out.getvalue()), run, steps, bound)
+    assert out.getvalue().rstrip().splitlines()[-1].startswith("nh_app: ")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,steps,bound", [("ohmic_sudden", 2, 1e-10), ("subohmic_exponential_t0", 2, 5e-10)])
+def test_driver_command_on_gpu_matches_reference_output(tmp_path, name, steps, bound):
+    run, inp, bath, out = write_case(tmp_path, name, steps)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    print(f"[driver] {name}, {steps} steps", flush=True)
+    subprocess.run([sys.executable, "-m", "ttns_lib_amd.driver", inp, "--bath", bath], check=True, cwd=root,
+                   timeout=300)
+    with open(out) as f:
+        check_rows(parse(f.read()), run, steps, bound)
