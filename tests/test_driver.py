@@ -16,6 +16,7 @@ import pytest
 from test_ref_outputs import ref_runs
 from ttns_lib_amd import driver
 
+
 def write_case(tmp_path, name, steps):
     run = ref_runs()[name]
     cfg = dict(run["input"])
