@@ -176,7 +176,8 @@ def report(insts, igs, teams, dflops, p0, steps, s0=None):
     for tot, k, d in rows:
         m = insts[k].model
         gf = dflops[k] / steps / 1e9
-        print(f"[report] inst {k:2d} modes {len(m.wk):3d} d {m.boson_dim[0]:2d} r {m.nspf:2d} team {teams[k]:2d} "
+        d0 = f"{m.boson_dim[0]:2d}" if m.boson_dim is not None else "bl"  # "bl": leaves hold mode blocks
+        print(f"[report] inst {k:2d} modes {len(m.wk):3d} d {d0} r {m.nspf:2d} team {teams[k]:2d} "
               f"{gf:8.1f} GFLOP {tot:6.2f} s {gf / max(tot, 1e-9):6.1f} GFLOP/s | "
               + " ".join(f"{n} {v:.2f}" for n, v in d.items() if n != "total"), file=sys.stderr, flush=True)
     if s0 is not None:

@@ -20,9 +20,11 @@ example_outputs/*.out): five 512-mode runs with their exact discretised baths.
   These <sigma> do not depend on the random
   unoccupied functions of the initial state (seed 1 gives the same differences), so this pins
   the bath, leaves, tree, Hamiltonian and step, not the initial-state random stream.
-* superohmic exponential, beta = 10: the first step agrees to 2e-8 (measured 9e-9).  This run
-  is sensitive to rounding: perturbing the oracle's initial state by 1e-15 relative moves
-  <sigma> by 5e-9 after one step and 2e-6 after two, so one step at 2e-8 is what it can pin.
+* superohmic exponential, beta = 10: the first step agrees to 1e-7 (measured 9e-9 on the oracle,
+  2.6e-8 on the GPU).  This run is sensitive to rounding: perturbing the oracle's initial state
+  by 1e-15 relative moves <sigma> by 5e-9 after one step and 2e-6 after two, and a different
+  summation order everywhere moves it further, so one step at 1e-7 is what it can pin (a wrong
+  bath of the same tree shape is off by 1e-5 or more).
   (The same perturbation moves the subohmic run by 1.4e-12 and 6e-11 after one and two steps.)
 """
 import json
@@ -67,7 +69,7 @@ def fixed_step_integrator(lib, run, **kw):
 
 
 # (run, steps compared, bound on <sigma>)
-TRAJECTORY_CASES = [("subohmic_exponential_t0", 2, 5e-10), ("superohmic_exponential", 1, 2e-8),
+TRAJECTORY_CASES = [("subohmic_exponential_t0", 2, 5e-10), ("superohmic_exponential", 1, 1e-7),
                     ("ohmic_exponential", 3, 1e-9), ("ohmic_sudden", 3, 1e-10)]
 BOND_DIMENSION_RUNS = ["ohmic_debye", "ohmic_exponential", "ohmic_sudden"]
 

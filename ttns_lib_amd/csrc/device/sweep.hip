@@ -624,15 +624,17 @@ __device__ void small_walk(const Ctx& c, int n, bool bond, F&& f) {
 // The descriptor records of the node whose small propagation is running (node, terms, their
 // sub-terms and operator chains, and the CSR index arrays of its leaf operators), copied into
 // LDS once per propagation so that the H-action reads them with LDS loads instead of dependent
-// global round trips in every Krylov iteration.
-constexpr int NC_TERMS = 8, NC_SUBS = 24, NC_KS = 32, NC_INTS = 256;
+// global round trips in every Krylov iteration.  NC_INTS holds the CSR arrays of a leaf of up
+// to ~340 states with two couplings per row (a 100-state mode needs 101 + 198).
+constexpr int NC_TERMS = 8, NC_SUBS = 24, NC_KS = 32, NC_INTS = 1024;
 struct NodeCache {
     DNode nd;
     DTerm terms[NC_TERMS];
     DSub subs[NC_SUBS];
     DK ks[NC_KS];
     int64_t ints[NC_INTS];  // CSR rowptr / colind of the cached leaf operators (DK::idx remapped)
-    int sub_base, k_base;
+    int64_t copy_src[NC_KS], copy_dst[NC_KS], copy_cnt[NC_KS];  // CSR arrays to copy into ints
+    int sub_base, k_base, ncopy;
 };
 __shared__ NodeCache g_nc;
 
@@ -666,7 +668,8 @@ __device__ bool node_cacheable(const Ctx& c, int n) {
     return c.node(n).nterms <= NC_TERMS && s1 - s0 <= NC_SUBS && k1 - k0 <= NC_KS && nints <= NC_INTS;
 }
 
-// fill g_nc for node n (thread 0 copies, everyone waits); the caller checked node_cacheable
+// fill g_nc for node n: thread 0 copies the records and lists the CSR arrays, then the
+// whole workgroup copies those; the caller checked node_cacheable
 __device__ void cache_node(const Ctx& c, int n) {
     if (threadIdx.x == 0) {
         int s0, s1, k0, k1, nints;
@@ -676,18 +679,30 @@ __device__ void cache_node(const Ctx& c, int n) {
         for (int ti = 0; ti < nd.nterms; ++ti) g_nc.terms[ti] = c.I.terms[nd.term0 + ti];
         for (int s = s0; s < s1; ++s) g_nc.subs[s - s0] = c.I.subs[s];
         int64_t at = 0;
+        int nc = 0;
         for (int k = k0; k < k1; ++k) {
             DK dk = c.I.ks[k];
             if (nd.leaf >= 0 && dk.kind == 2) {
                 const int64_t cnt = dk.dim + 1 + dk.nnz;
-                for (int64_t e = 0; e < cnt; ++e) g_nc.ints[at + e] = c.I.ints[dk.idx + e];
+                g_nc.copy_src[nc] = dk.idx;
+                g_nc.copy_dst[nc] = at;
+                g_nc.copy_cnt[nc] = cnt;
+                ++nc;
                 dk.idx = at;
                 at += cnt;
             }
             g_nc.ks[k - k0] = dk;
         }
+        g_nc.ncopy = nc;
         g_nc.sub_base = s0;
         g_nc.k_base = k0;
+    }
+    __syncthreads();
+    for (int q = 0; q < g_nc.ncopy; ++q) {
+        const int64_t* src = c.I.ints + g_nc.copy_src[q];
+        int64_t* dst = g_nc.ints + g_nc.copy_dst[q];
+        const int64_t cnt = g_nc.copy_cnt[q];
+        for (int64_t e = static_cast<int64_t>(threadIdx.x); e < cnt; e += NT) dst[e] = src[e];
     }
     __syncthreads();
 }
