@@ -94,7 +94,8 @@ def run(lib: Library, cfg: dict, wk, gk, out: TextIO, max_steps: Optional[int] =
         print("bond growth (subspace expansion) is not available: bonds stay at their initial dimensions",
               file=log)
     tree, sop = sb_problem(lib, model)
-    ig = Integrator(lib, tree, sop, dt=float(cfg["dt"]), krylov_dim=int(cfg.get("krylovdimension", 4)),
+    # defaults of the reference driver (src/spin_boson.cpp:251-255)
+    ig = Integrator(lib, tree, sop, dt=float(cfg["dt"]), krylov_dim=int(cfg.get("krylovdimension", 6)),
                     krylov_tol=float(cfg.get("krylovtolerance", 1e-12)))
     try:
         ig.set_sb_initial_state(model)
