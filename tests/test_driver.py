@@ -91,3 +91,23 @@ def test_driver_command_on_gpu_matches_reference_output(tmp_path, name, steps, b
                    timeout=300)
     with open(out) as f:
         check_rows(parse(f.read()), run, steps, bound)
+
+
+@pytest.mark.gpu
+def test_driver_batch_of_inputs_on_gpu_matches_reference_output(tmp_path):
+    # two inputs in one command: advanced together in one batched launch per step
+    cases = [("ohmic_sudden", 2, 1e-10), ("subohmic_exponential_t0", 2, 5e-10)]
+    made = []
+    for name, steps, bound in cases:
+        d = tmp_path / name
+        d.mkdir()
+        made.append((write_case(d, name, steps), steps, bound))
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    cmd = [sys.executable, "-m", "ttns_lib_amd.driver"] + [m[0][1] for m in made]
+    for m in made:
+        cmd += ["--bath", m[0][2]]
+    print("[driver] batch of two inputs", flush=True)
+    subprocess.run(cmd, check=True, cwd=root, timeout=300)
+    for (run, _, _, out), steps, bound in made:
+        with open(out) as f:
+            check_rows(parse(f.read()), run, steps, bound)

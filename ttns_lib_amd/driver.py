@@ -18,7 +18,12 @@ not available on the device, so an input whose capacities exceed the initial bon
 runs the one-site step (as the reference does while no bond grows) and says so on stderr.  The
 expansion counters (Nonesite, Ntwosite) are written as 0.
 
-usage: python -m ttns_lib_amd.driver input.json --bath bath.txt [--output out] [--steps N]
+Several inputs (each with its own --bath, in order) run as one batch: all their instances
+advance together in one persistent launch per step (ttn_batch_step), in place of the
+reference's process pool over inputs (run_examples.py); each writes its own output file.
+
+usage: python -m ttns_lib_amd.driver input.json [input2.json ...] --bath bath.txt [--bath bath2.txt ...]
+                                     [--output out] [--steps N]
 """
 from __future__ import annotations
 
@@ -30,7 +35,7 @@ from typing import Optional, TextIO
 
 import numpy as np
 
-from .capi import Integrator, Library, SpinBoson, driver_blocks, sb_problem
+from .capi import Batch, Integrator, Library, SpinBoson, driver_blocks, sb_problem
 
 
 def normalise_keys(doc: dict) -> dict:
@@ -86,56 +91,111 @@ def _g(x: float) -> str:
     return format(float(x), ".16g")
 
 
+class Job:
+    """One input: its integrator and the rows it writes."""
+
+    def __init__(self, lib: Library, cfg: dict, wk, gk, out: TextIO, log: TextIO = sys.stderr):
+        model = model_from_input(lib, cfg, wk, gk)
+        if int(cfg.get("nspfcapacity", model.nspf)) > model.nspf or \
+                int(cfg.get("nspflowercapacity", model.nspf_lower)) > model.nspf_lower:
+            print("bond growth (subspace expansion) is not available: bonds stay at their initial dimensions",
+                  file=log)
+        self.tree, self.sop = sb_problem(lib, model)
+        # defaults of the reference driver (src/spin_boson.cpp:251-255)
+        self.ig = Integrator(lib, self.tree, self.sop, dt=float(cfg["dt"]),
+                             krylov_dim=int(cfg.get("krylovdimension", 6)),
+                             krylov_tol=float(cfg.get("krylovtolerance", 1e-12)))
+        self.ig.set_sb_initial_state(model)
+        self.ops = observables(cfg)
+        self.hranks = " ".join(str(r) for _, r in self.ig.shapes[1:]) + " " if cfg.get("printbonddimension") else ""
+        self.nnodes = len(self.ig.shapes)
+        self.tmax, self.steps, self.out = float(cfg["tmax"]), 0, out
+        out.write("0 " + "".join(_g(self.ig.expect(0, o).real) + " " for o in self.ops) + " 0 0 0 " +
+                  self.hranks + "\n")
+
+    def running(self, max_steps: Optional[int]) -> bool:
+        return self.ig.time() < self.tmax and (max_steps is None or self.steps < max_steps)
+
+    def record(self):
+        """The row of the step just taken."""
+        self.steps += 1
+        vals = "".join(_g(self.ig.expect(0, o).real) + " " for o in self.ops)
+        nh = self.ig.counters()[0]
+        self.out.write(f"{_g(self.ig.time())} {vals}0 0 {_g(nh / self.nnodes)} {self.hranks}\n")
+        self.out.flush()
+
+    def finish(self):
+        self.out.write(f"nh_app: {_g(self.ig.counters()[0] / self.nnodes)}\n")
+        self.ig.destroy()
+
+
 def run(lib: Library, cfg: dict, wk, gk, out: TextIO, max_steps: Optional[int] = None, log: TextIO = sys.stderr):
     """Build the run described by `cfg` over `lib` and write its output rows to `out`."""
-    model = model_from_input(lib, cfg, wk, gk)
-    if int(cfg.get("nspfcapacity", model.nspf)) > model.nspf or \
-            int(cfg.get("nspflowercapacity", model.nspf_lower)) > model.nspf_lower:
-        print("bond growth (subspace expansion) is not available: bonds stay at their initial dimensions",
-              file=log)
-    tree, sop = sb_problem(lib, model)
-    # defaults of the reference driver (src/spin_boson.cpp:251-255)
-    ig = Integrator(lib, tree, sop, dt=float(cfg["dt"]), krylov_dim=int(cfg.get("krylovdimension", 6)),
-                    krylov_tol=float(cfg.get("krylovtolerance", 1e-12)))
+    job = Job(lib, cfg, wk, gk, out, log)
     try:
-        ig.set_sb_initial_state(model)
-        ops = observables(cfg)
-        hranks = " ".join(str(r) for _, r in ig.shapes[1:]) + " " if cfg.get("printbonddimension") else ""
-        nnodes = len(ig.shapes)
-        out.write("0 " + "".join(_g(ig.expect(0, o).real) + " " for o in ops) + " 0 0 0 " + hranks + "\n")
-        tmax, steps = float(cfg["tmax"]), 0
-        while ig.time() < tmax and (max_steps is None or steps < max_steps):
-            ig.step(1)
-            steps += 1
-            vals = "".join(_g(ig.expect(0, o).real) + " " for o in ops)
-            nh = ig.counters()[0]
-            out.write(f"{_g(ig.time())} {vals}0 0 {_g(nh / nnodes)} {hranks}\n")
-            out.flush()
-        out.write(f"nh_app: {_g(ig.counters()[0] / nnodes)}\n")
+        while job.running(max_steps):
+            job.ig.step(1)
+            job.record()
     finally:
-        ig.destroy()
+        job.finish()
+
+
+def run_batch(lib: Library, jobs_in, max_steps: Optional[int] = None, log: TextIO = sys.stderr):
+    """Several runs, [(cfg, wk, gk, out), ...], advanced together: one batched launch per step
+    over the runs that have not reached their tmax (the batch is rebuilt when one finishes)."""
+    jobs = [Job(lib, cfg, wk, gk, out, log) for cfg, wk, gk, out in jobs_in]
+    try:
+        active = [j for j in jobs if j.running(max_steps)]
+        while active:
+            batch = Batch(lib, [j.ig for j in active])
+            try:
+                while all(j.running(max_steps) for j in active):
+                    batch.step(1)
+                    for j in active:
+                        j.record()
+            finally:
+                batch.destroy()
+            active = [j for j in active if j.running(max_steps)]
+    finally:
+        for j in jobs:
+            j.finish()
 
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("input")
-    ap.add_argument("--bath", required=True, help="w_k g_k pairs, or the reference's stderr log of them")
-    ap.add_argument("--output", help="output file (default: the input's 'output file', else stdout)")
+    ap.add_argument("inputs", nargs="+")
+    ap.add_argument("--bath", action="append", required=True,
+                    help="w_k g_k pairs, or the reference's stderr log of them (one per input, in order)")
+    ap.add_argument("--output", help="output file of a single input (default: its 'output file', else stdout)")
     ap.add_argument("--steps", type=int, help="stop after this many steps")
     args = ap.parse_args(argv)
     from . import load_library
 
-    cfg = load_input(args.input)
-    wk, gk = load_bath(args.bath)
-    if "nmodes" in cfg and int(cfg["nmodes"]) != len(wk):
-        raise SystemExit(f"the input asks for {cfg['nmodes']} modes, the bath file has {len(wk)}")
+    if len(args.bath) != len(args.inputs):
+        raise SystemExit("give one --bath per input")
+    cfgs = [load_input(p) for p in args.inputs]
+    baths = [load_bath(p) for p in args.bath]
+    for p, cfg, (wk, _) in zip(args.inputs, cfgs, baths):
+        if "nmodes" in cfg and int(cfg["nmodes"]) != len(wk):
+            raise SystemExit(f"{p}: the input asks for {cfg['nmodes']} modes, the bath file has {len(wk)}")
     lib = load_library()
-    path = args.output or cfg.get("outputfile")
-    if path:
-        with open(path, "w") as f:
-            run(lib, cfg, wk, gk, f, args.steps)
-    else:
-        run(lib, cfg, wk, gk, sys.stdout, args.steps)
+    if len(cfgs) == 1:
+        path = args.output or cfgs[0].get("outputfile")
+        wk, gk = baths[0]
+        if path:
+            with open(path, "w") as f:
+                run(lib, cfgs[0], wk, gk, f, args.steps)
+        else:
+            run(lib, cfgs[0], wk, gk, sys.stdout, args.steps)
+        return 0
+    if args.output or any("outputfile" not in c for c in cfgs):
+        raise SystemExit("several inputs: each needs its own 'output file' key (and no --output)")
+    files = [open(c["outputfile"], "w") for c in cfgs]
+    try:
+        run_batch(lib, [(c, wk, gk, f) for c, (wk, gk), f in zip(cfgs, baths, files)], args.steps)
+    finally:
+        for f in files:
+            f.close()
     return 0
 
 
