@@ -47,6 +47,10 @@ def test_partition_edge_cases(orc):
     assert partition_modes(orc, [-1.0, 2.0, -600.0, -700.0], **REF_KEYS) == [1, 1, 2]
     with pytest.raises(TTNError):
         partition_modes(orc, [2.0, 1.0], **REF_KEYS)
+    # a bound near the int64 range: the state count saturates instead of overflowing, and
+    # 100^9 = 1e18 <= 4e18 < 100^10 gives blocks of nine modes
+    huge = dict(REF_KEYS, nmax=4 * 10 ** 18)
+    assert partition_modes(orc, [1.0] * 20, **huge) == [9, 9, 2]
 
 
 @pytest.mark.parametrize("seed", range(6))

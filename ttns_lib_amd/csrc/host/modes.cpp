@@ -107,7 +107,13 @@ std::vector<i64> partition_modes(const std::vector<double>& wk, double ecut, dou
             // states of a block of ntoadd modes, counted with the first mode's cutoff
             // (compute_nstates, mode_data.hpp:228-233, 785-792), saturated above nmaxp
             i64 nt = 1;
-            for (size_t i = 0; i < ntoadd && nt <= nmaxp; ++i) nt *= nper;
+            for (size_t i = 0; i < ntoadd && nt <= nmaxp; ++i) {
+                if (nper > 0 && nt > nmaxp / nper) {
+                    nt = nmaxp + 1;  // saturated: the product exceeds the bound
+                    break;
+                }
+                nt *= nper;
+            }
             if (nt > nmaxp) {
                 if (ntoadd > 1) --ntoadd;
                 break;
