@@ -34,7 +34,7 @@ PEAK_FP64_TFLOPS = 78.6
 # MI355X (FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0: the factors that make the counters
 # read the 1 GiB device copy of `bench.py --workload copy` as 1 GiB each way); the commands and
 # raw counts are in profiles/hbm_traffic_counters.md.  Not measured by this script.
-TRAFFIC_PER_BATCH_STEP = 5.98e12   # the 64-instance batch, one regular TDVP step
+TRAFFIC_PER_BATCH_STEP = 5.93e12   # the 64-instance batch, one regular TDVP step
 TRAFFIC_PER_NODE_HEFF = 8.45e8     # 64 x 64 x 64 node, 40 terms, one H-action (all GEMM stages)
 
 
