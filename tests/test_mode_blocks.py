@@ -51,6 +51,9 @@ def test_partition_edge_cases(orc):
     # 100^9 = 1e18 <= 4e18 < 100^10 gives blocks of nine modes
     huge = dict(REF_KEYS, nmax=4 * 10 ** 18)
     assert partition_modes(orc, [1.0] * 20, **huge) == [9, 9, 2]
+    # the largest int64 bound: 100^9 <= 2^63 - 1 < 100^10, still blocks of nine
+    top = dict(REF_KEYS, nmax=2 ** 63 - 1)
+    assert partition_modes(orc, [1.0] * 20, **top) == [9, 9, 2]
 
 
 @pytest.mark.parametrize("seed", range(6))

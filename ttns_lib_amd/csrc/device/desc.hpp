@@ -112,14 +112,14 @@ constexpr uint64_t TEAM_TIMEOUT = 200000000ull;   // 2 s of 100 MHz ticks at one
 
 // Team words of one team in global memory (zeroed before every launch): barrier arrival
 // count and completed generations on separate 128-byte lines, the team-wide failure
-// record, and an abort flag for a team that did not assemble.
+// record, and the roll-call outcome (0 undecided, 1 formed, 2 aborted).
 struct TeamWords {
     unsigned count;
     unsigned pad0[31];
     unsigned gen;
     unsigned pad1[31];
     int fail;
-    int aborted;
+    int outcome;
     unsigned pad2[30];
 };
 

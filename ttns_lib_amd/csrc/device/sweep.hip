@@ -1280,22 +1280,12 @@ __global__ __launch_bounds__(NT) void tdvp_run_kernel(const DInst* insts, RunArg
     T.pbuf = 0;
     T.part = reinterpret_cast<double*>(I.base + I.team_part);
     T.tiles = I.base + I.team_tiles;
-    if (size > 1) {
-        // roll call: nothing is touched unless the whole team has assembled
-        T.sync();
-        if (threadIdx.x == 0) {
-            if (g_sh.fail != 0)
-                __hip_atomic_store(&T.w->aborted, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            else if (__hip_atomic_load(&T.w->aborted, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
-                g_sh.fail = FAIL_TEAM_FORM;
-            if (g_sh.fail != 0) {
-                g_sh.fail = FAIL_TEAM_FORM;
-                __hip_atomic_store(I.status, static_cast<int>(FAIL_TEAM_FORM), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
-        __syncthreads();
-        if (g_sh.fail != 0) return;
+    // roll call: nothing is touched unless the whole team has assembled, and then by every
+    // member (a team that did not form leaves its instance untouched)
+    if (!T.roll_call()) {
+        if (threadIdx.x == 0)
+            __hip_atomic_store(I.status, static_cast<int>(FAIL_TEAM_FORM), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
     }
     Ctx c(I, &T, a.team_min);
     c.small = &g_small;

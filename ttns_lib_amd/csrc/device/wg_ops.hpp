@@ -190,6 +190,43 @@ struct Team {
         __syncthreads();
     }
 
+    // Roll call before anything is touched: decides once, in the outcome word, whether the
+    // whole team assembled.  The last arrival claims FORMED (0 -> 1) before it publishes the
+    // first generation; a member that gives up waiting claims ABORTED (0 -> 2).  Whichever
+    // claim lands first wins, and every member reads the same decided word afterwards, so
+    // either all members proceed or none does.  Returns true when the team formed.
+    __device__ bool roll_call() {
+        if (size == 1) return true;
+        if (threadIdx.x == 0) {
+            const unsigned old = __hip_atomic_fetch_add(&w->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (old + 1 == static_cast<unsigned>(size)) {
+                int expected = 0;
+                __hip_atomic_compare_exchange_strong(&w->outcome, &expected, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&w->gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                const uint64_t t0 = wall_clock64();
+                while (__hip_atomic_load(&w->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < 1u) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (wall_clock64() - t0 > TEAM_TIMEOUT) {
+                        int expected = 0;
+                        __hip_atomic_compare_exchange_strong(&w->outcome, &expected, 2, __ATOMIC_RELAXED,
+                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                }
+            }
+            // decided here: by the last arrival before it published the generation, or by
+            // this member's own claim (or the earlier one that beat it)
+            g_sh.fail = __hip_atomic_load(&w->outcome, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 1
+                            ? 0
+                            : static_cast<int>(FAIL_TEAM_FORM);
+        }
+        ++gen;
+        __syncthreads();
+        return g_sh.fail == 0;
+    }
+
     // record a failure on this member and in the team word (read by the others at their
     // next barrier)
     __device__ void fail(int code) {

@@ -107,14 +107,15 @@ std::vector<i64> partition_modes(const std::vector<double>& wk, double ecut, dou
             // states of a block of ntoadd modes, counted with the first mode's cutoff
             // (compute_nstates, mode_data.hpp:228-233, 785-792), saturated above nmaxp
             i64 nt = 1;
-            for (size_t i = 0; i < ntoadd && nt <= nmaxp; ++i) {
+            bool over = false;  // the product exceeds the bound (tracked apart: nmaxp may be INT64_MAX)
+            for (size_t i = 0; i < ntoadd; ++i) {
                 if (nper > 0 && nt > nmaxp / nper) {
-                    nt = nmaxp + 1;  // saturated: the product exceeds the bound
+                    over = true;
                     break;
                 }
                 nt *= nper;
             }
-            if (nt > nmaxp) {
+            if (over || nt > nmaxp) {
                 if (ntoadd > 1) --ntoadd;
                 break;
             }

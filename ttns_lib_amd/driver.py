@@ -142,20 +142,26 @@ def run(lib: Library, cfg: dict, wk, gk, out: TextIO, max_steps: Optional[int] =
 
 def run_batch(lib: Library, jobs_in, max_steps: Optional[int] = None, log: TextIO = sys.stderr):
     """Several runs, [(cfg, wk, gk, out), ...], advanced together: one batched launch per step
-    over the runs that have not reached their tmax (the batch is rebuilt when one finishes)."""
+    over the runs that have not reached their tmax (the batch is rebuilt when one finishes).
+    A batched launch advances instances that share dt, so runs are grouped by dt and the
+    groups run one after another."""
     jobs = [Job(lib, cfg, wk, gk, out, log) for cfg, wk, gk, out in jobs_in]
+    groups = {}
+    for j in jobs:
+        groups.setdefault(j.ig.params.dt, []).append(j)
     try:
-        active = [j for j in jobs if j.running(max_steps)]
-        while active:
-            batch = Batch(lib, [j.ig for j in active])
-            try:
-                while all(j.running(max_steps) for j in active):
-                    batch.step(1)
-                    for j in active:
-                        j.record()
-            finally:
-                batch.destroy()
-            active = [j for j in active if j.running(max_steps)]
+        for group in groups.values():
+            active = [j for j in group if j.running(max_steps)]
+            while active:
+                batch = Batch(lib, [j.ig for j in active])
+                try:
+                    while all(j.running(max_steps) for j in active):
+                        batch.step(1)
+                        for j in active:
+                            j.record()
+                finally:
+                    batch.destroy()
+                active = [j for j in active if j.running(max_steps)]
     finally:
         for j in jobs:
             j.finish()

@@ -106,8 +106,9 @@ try:
             m = dataclasses.replace(model, gk=g.detach().cpu().numpy().astype(float),
                                     wk=w.detach().cpu().numpy().astype(float))
             r = observable_and_gradient(lib, m, nsteps, dt, op=op, h=h, krylov_tol=krylov_tol)
-            ctx.save_for_backward(torch.as_tensor(r.dg, dtype=g.dtype), torch.as_tensor(r.dw, dtype=w.dtype))
-            return torch.tensor(r.value, dtype=g.dtype)
+            ctx.save_for_backward(torch.as_tensor(r.dg, dtype=g.dtype, device=g.device),
+                                  torch.as_tensor(r.dw, dtype=w.dtype, device=w.device))
+            return torch.tensor(r.value, dtype=g.dtype, device=g.device)
 
         @staticmethod
         def backward(ctx, grad_out):
