@@ -241,6 +241,21 @@ int ttn_node_expv(ttn_node* node, double coeff_re, double coeff_im, double tau, 
                   int64_t* napply);
 int ttn_node_last(const ttn_node* node, double* ms, double* flops);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * ttn_node_heff H-action, res = sum_t coeff_t (O0_t (x) O1_t) A MF_t^T, with every
+ * intermediate in a fixed device scratch array of 64 MiB that each GPU holds until the process
+ * ends.  Same arguments as ttn_node_create + ttn_node_heff, but every array is a DEVICE pointer
+ * the caller owns (coeff: nterms complex; op0: nterms d0 x d0; op1: nterms d1 x d1; mf: nterms
+ * r x r; A and res: (d0 d1) x r), and the work is enqueued on `stream` (a hipStream_t, NULL for
+ * the default stream) without synchronising.  Terms are processed in chunks that fit the
+ * scratch; a node whose mean-field factors and one term's intermediates exceed it returns
+ * TTN_ERR_CAPACITY.  Allocates nothing.  All calls on one GPU share its scratch: calls that
+ * overlap on one GPU (other streams, other threads) race and must be serialised by the caller. */
+int ttn_node_heff_noalloc(ttn_ctx* ctx, int64_t d0, int64_t d1, int64_t r, int64_t nterms, const double* coeff,
+                          const double* op0, const double* op1, const double* mf, const double* A, double* res,
+                          void* stream);
+
 /* ---- timing helpers (device-side event timing of the last ttn_step / ttn_batch_step) ---- */
 int ttn_last_step_ms(const ttn_integ* integ, double* ms);
 int ttn_batch_last_step_ms(const ttn_batch* batch, double* ms);

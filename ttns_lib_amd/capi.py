@@ -118,6 +118,7 @@ _SIGS = {
     "node_heff": [vp, i64],
     "node_mf": [vp, i64, i64, dblp],
     "node_expv": [vp, C.c_double, C.c_double, C.c_double, i64, C.c_double, i64p],
+    "node_heff_noalloc": [vp, i64, i64, i64, i64, vp, vp, vp, vp, vp, vp, vp],
     "node_last": [vp, dblp, dblp],
 }
 _RESTYPE = {"last_error": C.c_char_p}

@@ -689,6 +689,7 @@ int ttn_ctx_create(int device, ttn_ctx** out) {
         hip_check(hipEventCreate(&c->ev0), "hipEventCreate");
         hip_check(hipEventCreate(&c->ev1), "hipEventCreate");
         team_census(c.get());
+        ttnrt_noalloc_setup(c.get());
         // tuning knobs: TTN_TEAM_MIN (elements below which an operation runs on a team's
         // leader alone), TTN_TEAMS=0 switches teams off, TTN_SMALL_MAX (largest propagation
         // run in one workgroup's LDS, 0: none)

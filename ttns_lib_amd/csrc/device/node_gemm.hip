@@ -345,9 +345,29 @@ __global__ __launch_bounds__(GT) void znorm_kernel(const double2* x, int64_t n, 
     }
 }
 
+// out[t](p, j) = coeff[t] * mf[t](j, p): the mean-field factors of the term-blocked sum
+__global__ __launch_bounds__(GT) void zmft_kernel(const double2* coeff, const double2* mf, int64_t nt, int64_t r,
+                                                  double2* out) {
+    const int64_t rr = r * r, total = nt * rr;
+    for (int64_t e = blockIdx.x * static_cast<int64_t>(GT) + threadIdx.x; e < total;
+         e += static_cast<int64_t>(gridDim.x) * GT) {
+        const int64_t t = e / rr, pj = e - t * rr, p = pj / r, j = pj - p * r;
+        const double2 c = coeff[t], m = mf[t * rr + j * r + p];
+        out[e] = make_double2(c.x * m.x - c.y * m.y, c.x * m.y + c.y * m.x);
+    }
+}
+
 }  // namespace ttnd
 
 // ---------------------------------------------------------------- launchers
+extern "C" hipError_t ttnd_zmft(const double2* coeff, const double2* mf, int64_t nt, int64_t r, double2* out,
+                                hipStream_t s) {
+    using namespace ttnd;
+    const int64_t total = nt * r * r;
+    const int blocks = static_cast<int>(total / GT + 1 < 1024 ? total / GT + 1 : 1024);
+    hipLaunchKernelGGL(zmft_kernel, dim3(blocks), dim3(GT), 0, s, coeff, mf, nt, r, out);
+    return hipGetLastError();
+}
 extern "C" hipError_t ttnd_zgemm(const ttnd::GemmDesc* g, int nsplit, hipStream_t s) {
     using namespace ttnd;
     const int tiles = ((g->M + BM - 1) / BM) * ((g->N + BN - 1) / BN);

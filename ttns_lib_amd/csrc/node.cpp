@@ -13,6 +13,8 @@
 #include <cmath>
 #include <complex>
 #include <memory>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "device/node_gemm.hpp"
@@ -29,6 +31,17 @@ extern "C" hipError_t ttnd_zscale(double2* y, const double2* x, double ar, doubl
 extern "C" hipError_t ttnd_zcombine(double2* y, const double2* V, int64_t ld, int m, const double* c, double beta,
                                     int64_t n, int nblk, hipStream_t s);
 extern "C" hipError_t ttnd_znorm(const double2* x, int64_t n, double* partial, int nblk, hipStream_t s);
+extern "C" hipError_t ttnd_zmft(const double2* coeff, const double2* mf, int64_t nt, int64_t r, double2* out,
+                                hipStream_t s);
+
+// ---------------------------------------------------------------- allocation-free scratch
+// Device memory of ttn_node_heff_noalloc: the mean-field factors coeff_t MF_t^T, the per-term
+// intermediates of one chunk of terms and the K-split workspace.  Every GPU the library runs
+// on holds its own copy until the process ends.  Host code never passes `scratch` itself to
+// a kernel: its device address is taken once per device (ttnrt_noalloc_setup, called when a
+// context is created) and only that address is used.
+constexpr int64_t NOALLOC_SCRATCH_BYTES = int64_t(64) << 20;
+static __device__ __align__(256) double2 scratch[NOALLOC_SCRATCH_BYTES / sizeof(double2)];
 
 using namespace ttnh;
 using namespace ttnrt;
@@ -70,91 +83,114 @@ ttnd::GemmDesc desc0() {
     return g;
 }
 
-void run(ttn_node* nd, ttnd::GemmDesc g) {
+// The buffers one H-action works on: a node's own allocations, or slices of the
+// allocation-free scratch (ttn_node_heff_noalloc).
+struct NodeView {
+    int64_t d0, d1, r, nt, D, size;
+    const double2 *op0, *op1, *mft;  // operators: nt d0 x d0, nt d1 x d1, nt r x r (coeff_t MF_t^T)
+    double2 *Y, *Z;                  // per-term intermediates, nt x size each
+    double2* ws;                     // K-split workspace
+    int64_t ws_n;                    // its entries
+    bool ws_cap;                     // run unsplit when a split would not fit (else an error)
+    hipStream_t s;
+};
+
+NodeView view(ttn_node* nd) {
+    return NodeView{nd->d0, nd->d1, nd->r, nd->nt, nd->D, nd->size, nd->op0.p, nd->op1.p, nd->mft.p,
+                    nd->Y.p, nd->Z.p, nd->ws.p, static_cast<int64_t>(nd->ws.n), false, nd->ctx->stream};
+}
+
+void run(const NodeView& v, ttnd::GemmDesc g) {
     if (g.kb <= 0) g.kb = g.K;
-    const int ns = split_for(g.M, g.N, static_cast<int64_t>(g.nb0) * g.nb1, g.K);
+    int ns = split_for(g.M, g.N, static_cast<int64_t>(g.nb0) * g.nb1, g.K);
     if (ns > 1) {
         const int64_t need = static_cast<int64_t>(ns) * g.M * g.N * g.nb0 * g.nb1;
-        check(need <= static_cast<int64_t>(nd->ws.n), ERR_INVALID, "node workspace too small for the K split");
-        g.W = nd->ws.p;
+        if (need > v.ws_n && v.ws_cap) ns = 1;
+        check(need <= v.ws_n || ns == 1, ERR_INVALID, "node workspace too small for the K split");
+    }
+    if (ns > 1) {
+        g.W = v.ws;
         g.w_batch = static_cast<int64_t>(g.M) * g.N;
         g.w_split = g.w_batch * g.nb0 * g.nb1;
     }
-    hip_check(ttnd_zgemm(&g, ns, nd->ctx->stream), "zgemm");
+    hip_check(ttnd_zgemm(&g, ns, v.s), "zgemm");
 }
+void run(ttn_node* nd, const ttnd::GemmDesc& g) { run(view(nd), g); }
 
 // Y_t = O1_t x_1 src for every term
-void stage_mode1(ttn_node* nd, const double2* src) {
+void stage_mode1(const NodeView& v, const double2* src) {
     ttnd::GemmDesc g = desc0();
-    g.M = static_cast<int>(nd->d1);
-    g.N = static_cast<int>(nd->r);
-    g.K = static_cast<int>(nd->d1);
-    g.nb0 = static_cast<int>(nd->d0);
-    g.nb1 = static_cast<int>(nd->nt);
-    g.A = nd->op1.p;
-    g.a_b1 = nd->d1 * nd->d1;
-    g.a_r = nd->d1;
+    g.M = static_cast<int>(v.d1);
+    g.N = static_cast<int>(v.r);
+    g.K = static_cast<int>(v.d1);
+    g.nb0 = static_cast<int>(v.d0);
+    g.nb1 = static_cast<int>(v.nt);
+    g.A = v.op1;
+    g.a_b1 = v.d1 * v.d1;
+    g.a_r = v.d1;
     g.a_k = 1;
     g.B = src;
-    g.b_b0 = nd->d1 * nd->r;
-    g.b_k = nd->r;
+    g.b_b0 = v.d1 * v.r;
+    g.b_k = v.r;
     g.b_c = 1;
-    g.C = nd->Y.p;
-    g.c_b0 = nd->d1 * nd->r;
-    g.c_b1 = nd->size;
-    g.c_r = nd->r;
+    g.C = v.Y;
+    g.c_b0 = v.d1 * v.r;
+    g.c_b1 = v.size;
+    g.c_r = v.r;
     g.c_c = 1;
-    run(nd, g);
+    run(v, g);
 }
 
 // dst_t = O0_t x_0 src_t (src batch stride src_b: size for Y, 0 for a single tensor)
-void stage_mode0(ttn_node* nd, const double2* src, int64_t src_b, double2* dst) {
+void stage_mode0(const NodeView& v, const double2* src, int64_t src_b, double2* dst) {
     ttnd::GemmDesc g = desc0();
-    g.M = static_cast<int>(nd->d0);
-    g.N = static_cast<int>(nd->d1 * nd->r);
-    g.K = static_cast<int>(nd->d0);
-    g.nb0 = static_cast<int>(nd->nt);
-    g.A = nd->op0.p;
-    g.a_b0 = nd->d0 * nd->d0;
-    g.a_r = nd->d0;
+    g.M = static_cast<int>(v.d0);
+    g.N = static_cast<int>(v.d1 * v.r);
+    g.K = static_cast<int>(v.d0);
+    g.nb0 = static_cast<int>(v.nt);
+    g.A = v.op0;
+    g.a_b0 = v.d0 * v.d0;
+    g.a_r = v.d0;
     g.a_k = 1;
     g.B = src;
     g.b_b0 = src_b;
-    g.b_k = nd->d1 * nd->r;
+    g.b_k = v.d1 * v.r;
     g.b_c = 1;
     g.C = dst;
-    g.c_b0 = nd->size;
-    g.c_r = nd->d1 * nd->r;
+    g.c_b0 = v.size;
+    g.c_r = v.d1 * v.r;
     g.c_c = 1;
-    run(nd, g);
+    run(v, g);
 }
 
-// res = sum_t Z_t (coeff_t MF_t^T): K = nt r, blocked by term
-void stage_mf_sum(ttn_node* nd, double2* res) {
+// res (=|+=) sum_t Z_t (coeff_t MF_t^T): K = nt r, blocked by term
+void stage_mf_sum(const NodeView& v, double2* res, bool accumulate) {
     ttnd::GemmDesc g = desc0();
-    g.M = static_cast<int>(nd->D);
-    g.N = static_cast<int>(nd->r);
-    g.K = static_cast<int>(nd->nt * nd->r);
-    g.kb = static_cast<int>(nd->r);
-    g.A = nd->Z.p;
-    g.a_r = nd->r;
+    g.M = static_cast<int>(v.D);
+    g.N = static_cast<int>(v.r);
+    g.K = static_cast<int>(v.nt * v.r);
+    g.kb = static_cast<int>(v.r);
+    g.accumulate = accumulate ? 1 : 0;
+    g.A = v.Z;
+    g.a_r = v.r;
     g.a_k = 1;
-    g.a_kblk = nd->size;
-    g.B = nd->mft.p;
-    g.b_k = nd->r;
-    g.b_kblk = nd->r * nd->r;
+    g.a_kblk = v.size;
+    g.B = v.mft;
+    g.b_k = v.r;
+    g.b_kblk = v.r * v.r;
     g.b_c = 1;
     g.C = res;
-    g.c_r = nd->r;
+    g.c_r = v.r;
     g.c_c = 1;
-    run(nd, g);
+    run(v, g);
 }
 
-void heff(ttn_node* nd, const double2* v, double2* res) {
-    stage_mode1(nd, v);
-    stage_mode0(nd, nd->Y.p, nd->size, nd->Z.p);
-    stage_mf_sum(nd, res);
+void heff(const NodeView& v, const double2* x, double2* res) {
+    stage_mode1(v, x);
+    stage_mode0(v, v.Y, v.size, v.Z);
+    stage_mf_sum(v, res, false);
 }
+void heff(ttn_node* nd, const double2* x, double2* res) { heff(view(nd), x, res); }
 
 double heff_flops(const ttn_node* nd) {
     return 8.0 * static_cast<double>(nd->nt) * static_cast<double>(nd->size) *
@@ -288,7 +324,23 @@ struct Timer {
     }
 };
 
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};  // device address of `scratch`, per device
+
 }  // namespace
+
+void ttnrt_noalloc_setup(ttn_ctx* c) {
+    check(c->device >= 0 && c->device < MAX_DEVICES, ERR_DEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[c->device] == nullptr) {
+        void* p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        check(p != nullptr, ERR_DEVICE, "hipGetSymbolAddress(scratch) returned a null address");
+        g_scratch_ptr[c->device] = p;
+    }
+    c->noalloc_scratch = g_scratch_ptr[c->device];
+}
 
 extern "C" {
 
@@ -396,9 +448,9 @@ int ttn_node_mf(ttn_node* nd, int64_t child, int64_t reps, double* out) {
         for (int64_t k = 0; k < reps; ++k) {
             // the other child's operator on A, then MF_t^T: Z_t
             if (child == 0)
-                stage_mode1(nd, nd->A.p);
+                stage_mode1(view(nd), nd->A.p);
             else
-                stage_mode0(nd, nd->A.p, 0, nd->Y.p);
+                stage_mode0(view(nd), nd->A.p, 0, nd->Y.p);
             ttnd::GemmDesc g = desc0();
             g.M = static_cast<int>(nd->D);
             g.N = static_cast<int>(nd->r);
@@ -464,6 +516,54 @@ int ttn_node_expv(ttn_node* nd, double cr, double ci, double tau, int64_t kdim, 
         tm.stop();
         if (napply) *napply = na;
         nd->last_flops = static_cast<double>(na) * heff_flops(nd);
+    });
+}
+
+int ttn_node_heff_noalloc(ttn_ctx* ctx, int64_t d0, int64_t d1, int64_t r, int64_t nterms, const double* coeff,
+                          const double* op0, const double* op1, const double* mf, const double* A, double* res,
+                          void* stream) {
+    return guard(ctx, [&] {
+        check(ctx != nullptr, ERR_INVALID, "null context");
+        check(d0 >= 1 && d1 >= 1 && r >= 1 && nterms >= 1, ERR_INVALID, "node dimensions must be positive");
+        check(coeff && op0 && op1 && mf && A && res, ERR_INVALID, "null device pointer");
+        check(ctx->noalloc_scratch != nullptr, ERR_DEVICE, "the context has no scratch address");
+        check(d0 <= (int64_t(1) << 20) && d1 <= (int64_t(1) << 20) && r <= (int64_t(1) << 20) &&
+                  nterms <= (int64_t(1) << 20),
+              ERR_CAPACITY, "ttn_node_heff_noalloc: dimension out of range");
+        const int64_t cap = NOALLOC_SCRATCH_BYTES / static_cast<int64_t>(sizeof(double2));
+        const int64_t size = d0 * d1 * r, rr = r * r;
+        // scratch layout (entries, each slice on a 256-byte boundary): mean-field factors
+        // (nterms r^2), Y and Z for a chunk of tc terms, then the rest as K-split workspace
+        const int64_t off_y = align16(nterms * rr);
+        auto end_of = [&](int64_t t) { return align16(off_y + t * size) + t * size; };
+        int64_t tc = std::min<int64_t>(nterms, std::max<int64_t>(0, (cap - off_y) / (2 * size)));
+        while (tc > 0 && end_of(tc) > cap) --tc;
+        check(tc >= 1, ERR_CAPACITY,
+              "ttn_node_heff_noalloc: the node needs " + std::to_string(end_of(1) * 16) +
+                  " bytes of scratch (mean-field factors and one term's intermediates); the maximum is " +
+                  std::to_string(NOALLOC_SCRATCH_BYTES));
+        double2* base = static_cast<double2*>(ctx->noalloc_scratch);
+        double2* mft = base;
+        double2* Y = base + off_y;
+        const int64_t off_z = align16(off_y + tc * size);
+        double2* Z = base + off_z;
+        const int64_t off_ws = std::min(cap, align16(off_z + tc * size));
+        double2* ws = base + off_ws;
+        const int64_t ws_n = cap - off_ws;
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        hip_check(ttnd_zmft(reinterpret_cast<const double2*>(coeff), reinterpret_cast<const double2*>(mf), nterms, r,
+                            mft, s),
+                  "mean-field factors");
+        for (int64_t t0 = 0; t0 < nterms; t0 += tc) {
+            const int64_t cnt = std::min(tc, nterms - t0);
+            NodeView v{d0, d1, r, cnt, d0 * d1, size,
+                       reinterpret_cast<const double2*>(op0) + t0 * d0 * d0,
+                       reinterpret_cast<const double2*>(op1) + t0 * d1 * d1, mft + t0 * rr,
+                       Y, Z, ws, ws_n, true, s};
+            stage_mode1(v, reinterpret_cast<const double2*>(A));
+            stage_mode0(v, v.Y, v.size, v.Z);
+            stage_mf_sum(v, reinterpret_cast<double2*>(res), t0 > 0);
+        }
     });
 }
 

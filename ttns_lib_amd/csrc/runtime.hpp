@@ -58,7 +58,11 @@ struct ttn_ctx {
     ttnrt::DevBuf<unsigned> team_words;  // XCD slot counters (MAX_XCD lines) + TeamWords per instance
     int64_t team_words_n = 0;     // instances the buffer holds
     ttnrt::DevBuf<ttnd::TeamPlan> plan;
+    void* noalloc_scratch = nullptr;  // this device's copy of the allocation-free scratch (node.cpp)
 };
+
+// Takes the device address of the allocation-free scratch on the context's (current) device.
+void ttnrt_noalloc_setup(ttn_ctx* c);
 
 namespace ttnrt {
 
