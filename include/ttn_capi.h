@@ -17,7 +17,9 @@
  *   ttn_state_set/get    httensor node matrix access (A[i]().as_matrix())  include/ttns_lib/ttn_nodes/httensor_node.hpp:20-285
  *   ttn_step             integrator operator()(A, H, coeff)           projector_splitting_integrator.hpp:136-169,281-326
  *   ttn_time             integrator t()                               projector_splitting_integrator.hpp:188,359
- *   ttn_counters         nh_applications(), engine().Nonesite/Ntwosite  projector_splitting_integrator.hpp:190,361-365
+ *   ttn_counters         nh_applications() and the step count         projector_splitting_integrator.hpp:218,361,67
+ *   ttn_expansion_counters  engine().Nonesite()/Ntwosite()          subspace_expansion_projector_splitting_engine.hpp:513-514
+ *   ttn_set_dt, ttn_get_dt, ttn_reset_nsteps  set_dt()/dt(), reset_nsteps()  projector_splitting_integrator.hpp:33,184-187,352-353
  *   ttn_expect           mel()(op, mode, A)                           include/ttns_lib/observables/matrix_element.hpp:264-314
  *   ttn_traversal        tdvp_tree_traversal order                   include/ttns_lib/tdvp_core/tdvp_tree_traversal.hpp:37-87
  *   ttn_sb_problem, ttn_sb_initial_state
@@ -47,11 +49,11 @@ extern "C" {
 #define TTN_OP_CSR 2
 #define TTN_OP_IDENTITY 3
 
+/* fixed_step_projector_splitting_integrator (projector_splitting_integrator.hpp:77-223) */
 #define TTN_MODE_FIXED_STEP 0
-/* subspace_expansion_projector_splitting_integrator (projector_splitting_integrator.hpp:281-326):
- * accepted on trees whose bonds are all at capacity, where the reference's expansion returns
- * without adding functions (subspace_expansion_projector_splitting_engine.hpp:214, :378) and the
- * step equals the fixed-step one; TTN_ERR_INVALID when a bond is below its capacity. */
+/* subspace_expansion_projector_splitting_integrator (projector_splitting_integrator.hpp:225-397):
+ * bonds below their capacity grow during the sweeps (subspace_expansion_projector_splitting_engine.hpp:
+ * 207-510); node storage is laid out at capacity, so a bond grows inside a launch */
 #define TTN_MODE_SUBSPACE_EXPANSION 1
 
 typedef struct ttn_ctx ttn_ctx;
@@ -108,6 +110,12 @@ typedef struct {
     int64_t minimum_unoccupied;
     int64_t neigenvalues;
     int64_t team_size;           /* workgroups cooperating on this instance (0 = automatic) */
+    double eigenvalue_tolerance; /* eigs tolerance of the downward expansion (0 = the engine's 1e-6) */
+    int32_t only_apply_when_no_unoccupied;  /* expand only while fewer than minimum_unoccupied functions
+                                               are unoccupied (projector_splitting_integrator.hpp:338-339) */
+    uint32_t rng_seed;           /* std::mt19937 seed of the expansion's trial vectors; 0 = the engine's
+                                    default-constructed generator (seed 5489,
+                                    two_site_energy_variations.hpp:34) */
 } ttn_integ_params;
 
 int ttn_integrator_create(ttn_ctx* ctx, const ttn_tree* tree, const ttn_sop* sop, const ttn_integ_params* p,
@@ -119,8 +127,25 @@ int ttn_state_get(ttn_integ* integ, int64_t node, double* data, int64_t rows, in
 /* nsteps integrator calls A <- TDVP(A, H, coeff) (coeff = -i for real time) */
 int ttn_step(ttn_integ* integ, double coeff_re, double coeff_im, int64_t nsteps);
 int ttn_time(const ttn_integ* integ, double* t);
-/* cumulative node-propagation H applications and completed steps */
+/* cumulative node-propagation H applications (nh_applications(), projector_splitting_integrator.hpp:
+ * 218, 361) and completed steps (the base class's m_nsteps, :67) */
 int ttn_counters(const ttn_integ* integ, int64_t* nh_applications, int64_t* nsteps);
+/* engine().Nonesite() / Ntwosite() (subspace_expansion_projector_splitting_engine.hpp:513-514, read by
+ * src/spin_boson.cpp:91): random one-site and two-site expansion attempts so far (0 for the
+ * fixed-step integrator) */
+int ttn_expansion_counters(const ttn_integ* integ, int64_t* n_onesite, int64_t* n_twosite);
+/* current bond dimension (hrank) of every node, pre-order (the root's is 1), as the driver prints
+ * them (src/spin_boson.cpp:95-103); they grow under the expansion integrator */
+int ttn_hranks(const ttn_integ* integ, int64_t* hranks);
+/* current shape of a node matrix (httensor_node_data::shape, httensor_node.hpp:20-285) */
+int ttn_state_shape(const ttn_integ* integ, int64_t node, int64_t* rows, int64_t* cols);
+/* dt() / set_dt() (projector_splitting_integrator.hpp:184-187, 352-353): read by every following
+ * ttn_step; t() stays dt * nsteps as in the reference */
+int ttn_set_dt(ttn_integ* integ, double dt);
+int ttn_get_dt(const ttn_integ* integ, double* dt);
+/* reset_nsteps() (projector_splitting_integrator.hpp:33): the next ttn_step prepares the
+ * evolution and runs the first-step ladder again */
+int ttn_reset_nsteps(ttn_integ* integ);
 /* <psi|O|psi> for a dense operator O (d x d complex) on primitive mode `mode` */
 int ttn_expect(ttn_integ* integ, int64_t mode, const double* op, double* out);
 /* <psi|psi> */

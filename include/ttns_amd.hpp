@@ -499,6 +499,7 @@ class fixed_step_projector_splitting_integrator {
         p.spawning_threshold = m_spawning_threshold;
         p.unoccupied_threshold = m_unoccupied_threshold;
         p.minimum_unoccupied = static_cast<int64_t>(m_minimum_unoccupied);
+        p.neigenvalues = static_cast<int64_t>(m_neigenvalues);
         detail::check(ttn_integrator_create(m_ctx, m_tree, m_sop, &p, &m_integ), m_ctx, "ttn_integrator_create");
     }
     void sync_to_device(httensor<T, backend>& A) {
@@ -540,6 +541,7 @@ class fixed_step_projector_splitting_integrator {
     real_type m_spawning_threshold = -1.0;
     real_type m_unoccupied_threshold = -1.0;
     size_type m_minimum_unoccupied = 0;
+    size_type m_neigenvalues = 2;
 };
 
 // subspace_expansion_projector_splitting_integrator (projector_splitting_integrator.hpp:229-326):

@@ -42,6 +42,18 @@ int guard(orc_ctx* ctx, F&& f) {
 }
 }  // namespace
 
+static std::vector<cplx> cvec(const double* p, i64 n) {
+    std::vector<cplx> v(n);
+    for (i64 i = 0; i < n; ++i) v[i] = cget(p, i);
+    return v;
+}
+static void cput(const std::vector<cplx>& v, double* p) {
+    for (size_t i = 0; i < v.size(); ++i) {
+        p[2 * i] = v[i].real();
+        p[2 * i + 1] = v[i].imag();
+    }
+}
+
 extern "C" {
 
 int orc_ctx_create(int, orc_ctx** out) {
@@ -121,15 +133,24 @@ int orc_sop_compiled_tables(const orc_tree* t, const orc_sop* s, int64_t* len, i
 int orc_integrator_create(orc_ctx* ctx, const orc_tree* t, const orc_sop* s, const ttn_integ_params* p,
                           orc_integ** out) {
     return guard(ctx, [&] {
-        check(p->mode == TTN_MODE_FIXED_STEP || (p->mode == TTN_MODE_SUBSPACE_EXPANSION && t->topo.at_capacity()),
-              ERR_INVALID, "oracle: the expansion integrator is restated only on trees at capacity (where it is "
-                           "the fixed-step one)");
+        check(p->mode == TTN_MODE_FIXED_STEP || p->mode == TTN_MODE_SUBSPACE_EXPANSION, ERR_INVALID,
+              "unknown integrator mode");
         check(p->krylov_dim >= 1 && p->krylov_dim <= 64, ERR_INVALID, "krylov_dim must be in [1, 64]");
         check(p->dt > 0.0, ERR_INVALID, "dt must be positive");
         orc::Params pp;
         pp.krylov_dim = static_cast<int>(p->krylov_dim);
         pp.krylov_tol = p->krylov_tol;
         pp.dt = p->dt;
+        pp.expansion = p->mode == TTN_MODE_SUBSPACE_EXPANSION;
+        pp.spawning_threshold = p->spawning_threshold;
+        pp.unoccupied_threshold = p->unoccupied_threshold;
+        pp.minimum_unoccupied = p->minimum_unoccupied;
+        pp.neigenvalues = p->neigenvalues;
+        if (p->eigenvalue_tolerance > 0.0) pp.eigenvalue_tolerance = p->eigenvalue_tolerance;
+        pp.only_apply_when_no_unoccupied = p->only_apply_when_no_unoccupied != 0;
+        pp.rng_seed = p->rng_seed != 0 ? p->rng_seed : 5489u;
+        check(!pp.expansion || pp.neigenvalues >= 1, ERR_INVALID, "neigenvalues must be positive");
+        check(pp.minimum_unoccupied >= 0, ERR_INVALID, "minimum_unoccupied must not be negative");
         auto in = std::make_unique<orc_integ>();
         in->ctx = ctx;
         in->eng = std::make_unique<orc::Engine>(t->topo, s->sop, pp);
@@ -178,6 +199,58 @@ int orc_counters(const orc_integ* in, int64_t* nh, int64_t* ns) {
     if (ns) *ns = in->eng->nsteps();
     return TTN_OK;
 }
+int orc_expansion_counters(const orc_integ* in, int64_t* n1, int64_t* n2) {
+    if (n1) *n1 = in->eng->n_onesite();
+    if (n2) *n2 = in->eng->n_twosite();
+    return TTN_OK;
+}
+int orc_hranks(const orc_integ* in, int64_t* hr) {
+    const auto& topo = in->eng->topo();
+    for (int n = 0; n < topo.size(); ++n) hr[n] = topo.nodes[n].hrank;
+    return TTN_OK;
+}
+int orc_state_shape(const orc_integ* in, int64_t node, int64_t* rows, int64_t* cols) {
+    return guard(in->ctx, [&] {
+        const auto& topo = in->eng->topo();
+        check(node >= 0 && node < topo.size(), ERR_INVALID, "node out of range");
+        if (rows) *rows = topo.nodes[node].dimen();
+        if (cols) *cols = topo.nodes[node].hrank;
+    });
+}
+int orc_set_dt(orc_integ* in, double dt) {
+    return guard(in->ctx, [&] {
+        check(dt > 0.0, ERR_INVALID, "dt must be positive");
+        in->eng->set_dt(dt);
+    });
+}
+int orc_get_dt(const orc_integ* in, double* dt) {
+    *dt = in->eng->dt();
+    return TTN_OK;
+}
+int orc_reset_nsteps(orc_integ* in) {
+    in->eng->reset_nsteps();
+    return TTN_OK;
+}
+int orc_normals_used(const orc_integ* in, int64_t* n) {
+    *n = in->eng->normals_used();
+    return TTN_OK;
+}
+int orc_singular_values(int64_t n, const double* M, double* sv) {
+    return guard(nullptr, [&] {
+        auto m = cvec(M, n * n);
+        auto s = orc::singular_values(m.data(), n);
+        for (i64 i = 0; i < n; ++i) sv[i] = s[i];
+    });
+}
+int orc_tridiag_eigen(int64_t n, const double* a, const double* b, double* w, double* Z) {
+    return guard(nullptr, [&] {
+        std::vector<double> av(a, a + n), bv(b, b + (n > 0 ? n - 1 : 0)), wv, Zv;
+        orc::tridiag_eigen(av, bv, wv, Zv);
+        std::copy(wv.begin(), wv.end(), w);
+        std::copy(Zv.begin(), Zv.end(), Z);
+    });
+}
+
 int orc_expect(orc_integ* in, int64_t mode, const double* op, double* out) {
     return guard(in->ctx, [&] {
         const auto& topo = in->eng->topo();
@@ -229,18 +302,6 @@ int orc_sb_mode_block(orc_ctx* ctx, int64_t nmodes, const double* w, const doubl
 }
 
 // ---- building blocks ----
-static std::vector<cplx> cvec(const double* p, i64 n) {
-    std::vector<cplx> v(n);
-    for (i64 i = 0; i < n; ++i) v[i] = cget(p, i);
-    return v;
-}
-static void cput(const std::vector<cplx>& v, double* p) {
-    for (size_t i = 0; i < v.size(); ++i) {
-        p[2 * i] = v[i].real();
-        p[2 * i + 1] = v[i].imag();
-    }
-}
-
 int orc_heff_node(orc_integ* in, int64_t node, const double* v, double* out) {
     return guard(in->ctx, [&] {
         const i64 sz = in->eng->topo().nodes.at(node).size();

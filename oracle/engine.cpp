@@ -34,6 +34,7 @@ Engine::Engine(const ttnh::Topology& topo, const ttnh::SOP& sop, const Params& p
         }
     }
     m_dt = p.dt;
+    m_rng.seed(p.rng_seed);
 }
 
 // kronecker_product_operator::apply (kronecker_product_operator_helper.hpp:261-315):
@@ -138,7 +139,7 @@ void Engine::update_spf(int n) {
         const Term& tm = terms[ind];
         if (tm.id_spf) continue;
         auto& out = m_spf[n][ind];
-        for (auto& x : out) x = cplx(0.0, 0.0);
+        out.assign(r * r, cplx(0.0, 0.0));
         for (int i = 0; i < tm.nspf_terms(); ++i) {
             if (nd.is_leaf()) {
                 const auto& idx = tm.spf_index[i][0];
@@ -182,7 +183,7 @@ void Engine::update_mf(int n) {
         const Term& tm = terms[ind];
         if (tm.id_mf) continue;
         auto& out = m_mf[n][ind];
-        for (auto& x : out) x = cplx(0.0, 0.0);
+        out.assign(d * d, cplx(0.0, 0.0));
         for (int it = 0; it < tm.nmf_terms(); ++it) {
             const auto& mi = tm.mf_index[it];
             kron_siblings(p, mi.siblings, Ap, tmp.data(), HA.data());
@@ -276,7 +277,11 @@ void Engine::bond_update(int n, int ops) {
 }
 
 // ---- sweeps (tdvp_algorithm_engine.hpp:330-501) ----
-void Engine::forward(bool evolve) {
+// With `expand`, every root-to-leaf factorisation is followed by the downward expansion of
+// that bond and every leaf-to-root factorisation by the upward one (:373, :479).  Returns
+// whether any bond grew.
+bool Engine::forward(bool evolve, bool expand) {
+    bool grew = false;
     std::vector<int> visits(m_topo.size(), 0);
     for (int id : m_tour) {
         const Node& nd = m_topo.nodes[id];
@@ -286,7 +291,8 @@ void Engine::forward(bool evolve) {
             if (tv == 0 && !nd.is_root()) apply_from_parent(id);
             if (!nd.is_leaf()) {
                 r2l(id, mode);
-                update_mf(nd.children[mode]);
+                if (expand && expand_down(id, mode)) grew = true;
+                update_mf(m_topo.nodes[id].children[mode]);
             }
         } else {
             if (evolve) node_update(id);
@@ -298,9 +304,11 @@ void Engine::forward(bool evolve) {
             }
         }
     }
+    return grew;
 }
 
-void Engine::backward(bool evolve) {
+bool Engine::backward(bool evolve, bool expand) {
+    bool grew = false;
     std::vector<int> visits(m_topo.size(), 0);
     for (auto it = m_tour.rbegin(); it != m_tour.rend(); ++it) {
         const int id = *it;
@@ -321,24 +329,30 @@ void Engine::backward(bool evolve) {
             }
         } else if (!nd.is_root()) {
             l2r(id);
+            if (expand && expand_up(id)) grew = true;
             apply_to_parent(id);
             update_spf(id);
         }
     }
+    return grew;
 }
 
-void Engine::sweep(bool evolve) {
-    forward(evolve);
+bool Engine::sweep(bool evolve, bool expand) {
+    const bool f = forward(evolve, expand);
     m_t += m_sub_dt / 2.0;
-    backward(evolve);
+    const bool b = backward(evolve, expand);
     m_t += m_sub_dt / 2.0;
+    return f || b;
 }
 
-// tdvp_algorithm_base::prepare_evolution (:257-321) without subspace expansion
+// tdvp_algorithm_base::prepare_evolution (:257-321): single-particle operators bottom-up, then
+// sweeps without propagation until a sweep pair adds no function (one pair for the fixed-step
+// integrator, whose expansion never adds)
 void Engine::prepare_evolution() {
     for (int n = m_topo.size() - 1; n >= 0; --n)
         if (!m_topo.nodes[n].is_root()) update_spf(n);
-    sweep(false);
+    while (sweep(false, m_p.expansion)) {
+    }
 }
 
 void Engine::step(cplx coeff) {
@@ -347,18 +361,21 @@ void Engine::step(cplx coeff) {
         double tevolve = 0.0;
         double dt = m_dt / 1e5;
         m_sub_dt = (dt + tevolve <= m_dt ? dt : m_dt - tevolve);
+        m_exp_dt = m_dt;  // prepare_evolution runs with the engine's dt at the full step
         prepare_evolution();
         while (tevolve < m_dt) {
             m_sub_dt = (dt + tevolve <= m_dt ? dt : m_dt - tevolve);
+            m_exp_dt = m_sub_dt;
             m_t = tevolve;
-            sweep(true);
+            sweep(true, m_p.expansion);
             tevolve += dt;
             dt *= 10;
         }
     } else {
         m_sub_dt = m_dt;
+        m_exp_dt = m_dt;
         m_t = static_cast<double>(m_nsteps) * m_dt;
-        sweep(true);
+        sweep(true, m_p.expansion);
     }
     ++m_nsteps;
 }

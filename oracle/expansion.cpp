@@ -1,0 +1,575 @@
+// ORACLE — test infrastructure only (see oracle.hpp).  Restatement of the subspace
+// expansion of the reference's expansion integrator:
+//   subspace_expansion_down / _up   tdvp_core/subspace_expansion_projector_splitting_engine.hpp:207-510
+//   get_nunoccupied                 :577-585 (populations = the gauge factorisation's singular values)
+//   two-site projected energy       tdvp_core/two_site_energy_variations.hpp:49-338
+//   trial vectors, expand_*         two_site_energy_variations.hpp:341-562
+// The dominant eigenpairs come from utils::krylov_integrator::eigs, which is not vendored in
+// the reference (llindoy/utils_cpp, no pinned version).  They are restated here as restarted
+// Lanczos (Lanczos 1950) with full re-orthogonalisation (classical Gram-Schmidt applied twice),
+// the Ritz pairs of the tridiagonal matrix from cyclic Jacobi rotations (Jacobi 1846), and an
+// explicit restart from the sum of the leading neigenvalues Ritz vectors until each of their
+// residuals |beta_m s_{m,k}| is at most tol times the largest Ritz value.  Every Ritz pair of
+// the last Krylov space is returned (descending), and the spawning test counts all of them
+// (:280-290): with only the leading neigenvalues pairs counted, bonds of the reference's own
+// runs grow 2-3 times too slowly (a bond of its Debye run goes 8 -> 9 -> 14 -> 21 over three
+// steps, more than two functions per expansion), while counting the whole Ritz spectrum gives
+// its total growth to within a few functions (tests/test_expansion.py).  eigs returns the
+// eigenvectors of the map the two-site functor applies; the downward expansion conjugates them
+// (:278) and the upward one uses them as they are (:451-457), as the reference does.
+#include "oracle.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+namespace orc {
+
+using ttnh::check;
+using ttnh::ERR_INVALID;
+using ttnh::ERR_NUMERIC;
+using ttnh::Node;
+using ttnh::Term;
+
+namespace {
+
+// (L, d, R) of the rank-3 view of `mode` of a node; mode == nmodes is the bond index
+void rank3(const Node& nd, int mode, i64& L, i64& d, i64& R) {
+    if (mode == nd.nmodes()) {
+        L = nd.dimen();
+        d = nd.hrank;
+        R = 1;
+        return;
+    }
+    L = 1;
+    R = nd.hrank;
+    for (int c = 0; c < mode; ++c) L *= nd.dims[c];
+    for (int c = mode + 1; c < nd.nmodes(); ++c) R *= nd.dims[c];
+    d = nd.dims[mode];
+}
+
+double norm2(const std::vector<cplx>& x) {
+    double s = 0.0;
+    for (const cplx& v : x) s += std::norm(v);
+    return s;
+}
+
+// the (L, d, R) tensor with nadd slices appended to its middle index: slice d + k is
+// rows[k] (an L x R array) or zero
+std::vector<cplx> pad_middle(const std::vector<cplx>& a, i64 L, i64 d, i64 R, i64 nadd,
+                             const std::vector<std::vector<cplx>>* rows) {
+    const i64 dn = d + nadd;
+    std::vector<cplx> out(L * dn * R, cplx(0.0, 0.0));
+    for (i64 l = 0; l < L; ++l) {
+        for (i64 j = 0; j < d; ++j)
+            for (i64 k = 0; k < R; ++k) out[(l * dn + j) * R + k] = a[(l * d + j) * R + k];
+        if (rows)
+            for (i64 j = 0; j < nadd; ++j)
+                for (i64 k = 0; k < R; ++k) out[(l * dn + d + j) * R + k] = (*rows)[j][l * R + k];
+    }
+    return out;
+}
+
+// r (n x n) zero-padded to (n + nadd) x (n + nadd) (expand_matrix, :430-444)
+std::vector<cplx> pad_square(const std::vector<cplx>& r, i64 n, i64 nadd) {
+    const i64 m = n + nadd;
+    std::vector<cplx> out(m * m, cplx(0.0, 0.0));
+    for (i64 i = 0; i < n; ++i)
+        for (i64 j = 0; j < n; ++j) out[i * m + j] = r[i * n + j];
+    return out;
+}
+
+}  // namespace
+
+// ---- singular values: one-sided Jacobi on the columns of M ----
+std::vector<double> singular_values(const cplx* M, i64 n) {
+    std::vector<cplx> A(M, M + n * n);  // column j: A[i * n + j]
+    for (int sweep = 0; sweep < 80; ++sweep) {
+        bool rotated = false;
+        for (i64 p = 0; p < n; ++p)
+            for (i64 q = p + 1; q < n; ++q) {
+                double alpha = 0.0, beta = 0.0;
+                cplx gamma(0.0, 0.0);
+                for (i64 i = 0; i < n; ++i) {
+                    alpha += std::norm(A[i * n + p]);
+                    beta += std::norm(A[i * n + q]);
+                    gamma += std::conj(A[i * n + p]) * A[i * n + q];
+                }
+                const double g = std::abs(gamma);
+                if (g == 0.0 || g <= 1e-15 * std::sqrt(alpha * beta)) continue;
+                rotated = true;
+                const cplx e = gamma / g;  // phase of a_p^H a_q
+                const double zeta = (beta - alpha) / (2.0 * g);
+                const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+                const double c = 1.0 / std::sqrt(1.0 + t * t), s = c * t;
+                for (i64 i = 0; i < n; ++i) {
+                    const cplx ap = A[i * n + p], aq = std::conj(e) * A[i * n + q];
+                    A[i * n + p] = c * ap - s * aq;
+                    A[i * n + q] = (s * ap + c * aq) * e;
+                }
+            }
+        if (!rotated) break;
+    }
+    std::vector<double> sv(n);
+    for (i64 j = 0; j < n; ++j) {
+        double s = 0.0;
+        for (i64 i = 0; i < n; ++i) s += std::norm(A[i * n + j]);
+        sv[j] = std::sqrt(s);
+    }
+    std::sort(sv.begin(), sv.end(), std::greater<double>());
+    return sv;
+}
+
+// ---- eigenpairs of a real symmetric tridiagonal matrix: cyclic Jacobi ----
+void tridiag_eigen(const std::vector<double>& a, const std::vector<double>& b, std::vector<double>& w,
+                   std::vector<double>& Z) {
+    const int n = static_cast<int>(a.size());
+    std::vector<double> T(n * n, 0.0);
+    Z.assign(n * n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        T[i * n + i] = a[i];
+        Z[i * n + i] = 1.0;
+        if (i + 1 < n) T[i * n + i + 1] = T[(i + 1) * n + i] = b[i];
+    }
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                tot += T[i * n + j] * T[i * n + j];
+                if (i != j) off += T[i * n + j] * T[i * n + j];
+            }
+        if (off <= 1e-30 * tot || off == 0.0) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = T[p * n + q];
+                if (apq == 0.0) continue;
+                const double theta = (T[q * n + q] - T[p * n + p]) / (2.0 * apq);
+                const double t = (theta >= 0.0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < n; ++k) {
+                    if (k == p || k == q) continue;
+                    const double kp = T[k * n + p], kq = T[k * n + q];
+                    T[k * n + p] = T[p * n + k] = c * kp - s * kq;
+                    T[k * n + q] = T[q * n + k] = s * kp + c * kq;
+                }
+                T[p * n + p] -= t * apq;
+                T[q * n + q] += t * apq;
+                T[p * n + q] = T[q * n + p] = 0.0;
+                for (int k = 0; k < n; ++k) {
+                    const double zp = Z[k * n + p], zq = Z[k * n + q];
+                    Z[k * n + p] = c * zp - s * zq;
+                    Z[k * n + q] = s * zp + c * zq;
+                }
+            }
+    }
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return T[x * n + x] > T[y * n + y]; });
+    w.resize(n);
+    std::vector<double> Zs(n * n);
+    for (int k = 0; k < n; ++k) {
+        w[k] = T[order[k] * n + order[k]];
+        for (int i = 0; i < n; ++i) Zs[i * n + k] = Z[i * n + order[k]];
+    }
+    Z.swap(Zs);
+}
+
+// ---- restarted Lanczos for the neigenvalues largest eigenpairs of a Hermitian map ----
+void Engine::eigs(const std::function<void(const cplx*, cplx*)>& F, std::vector<cplx> x, int kd, double tol,
+                  std::vector<double>& S, std::vector<std::vector<cplx>>& V) const {
+    const i64 m = static_cast<i64>(x.size());
+    const int neig = static_cast<int>(m_p.neigenvalues);
+    check(kd >= 1 && neig >= 1, ERR_INVALID, "eigs: empty Krylov space");
+    for (int cycle = 0;; ++cycle) {
+        const double xn = std::sqrt(norm2(x));
+        check(std::isfinite(xn) && xn > 0.0, ERR_NUMERIC, "eigs: start vector has no norm");
+        std::vector<std::vector<cplx>> Q(1, x);
+        for (cplx& v : Q[0]) v /= xn;
+        std::vector<double> al, be;
+        double beta_last = 0.0, anorm = 0.0;
+        bool invariant = false;
+        std::vector<cplx> w(m);
+        for (int j = 0; j < kd; ++j) {
+            F(Q[j].data(), w.data());
+            std::vector<cplx> h(j + 1, cplx(0.0, 0.0));
+            for (int pass = 0; pass < 2; ++pass) {
+                std::vector<cplx> hp(j + 1, cplx(0.0, 0.0));
+                for (int i = 0; i <= j; ++i)
+                    for (i64 q = 0; q < m; ++q) hp[i] += std::conj(Q[i][q]) * w[q];
+                for (int i = 0; i <= j; ++i) {
+                    for (i64 q = 0; q < m; ++q) w[q] -= hp[i] * Q[i][q];
+                    h[i] += hp[i];
+                }
+            }
+            const double bj = std::sqrt(norm2(w));
+            check(std::isfinite(bj), ERR_NUMERIC, "eigs: non-finite Lanczos vector");
+            al.push_back(h[j].real());
+            anorm = std::max({anorm, std::fabs(h[j].real()), bj});
+            if (j + 1 == kd) {
+                beta_last = bj;
+                break;
+            }
+            if (bj <= 1e-12 * anorm) {
+                invariant = true;
+                break;
+            }
+            be.push_back(bj);
+            Q.emplace_back(m);
+            for (i64 q = 0; q < m; ++q) Q.back()[q] = w[q] / bj;
+        }
+        const int mm = static_cast<int>(al.size());
+        std::vector<double> th, Z;
+        tridiag_eigen(al, be, th, Z);
+        const int k = std::min(neig, mm);
+        bool converged = invariant;
+        if (!converged) {
+            converged = true;
+            const double scale = std::max(std::fabs(th[0]), 1e-300);
+            for (int i = 0; i < k; ++i)
+                if (beta_last * std::fabs(Z[(mm - 1) * mm + i]) > tol * scale) converged = false;
+        }
+        std::vector<std::vector<cplx>> Y(mm, std::vector<cplx>(m, cplx(0.0, 0.0)));
+        for (int i = 0; i < mm; ++i)
+            for (int j = 0; j < mm; ++j)
+                for (i64 q = 0; q < m; ++q) Y[i][q] += Z[j * mm + i] * Q[j][q];
+        if (converged || cycle == 99) {
+            S = th;
+            V.swap(Y);
+            return;
+        }
+        x.assign(m, cplx(0.0, 0.0));
+        for (int i = 0; i < k; ++i)
+            for (i64 q = 0; q < m; ++q) x[q] += Y[i][q];
+    }
+}
+
+// get_nunoccupied (:577-585): populations relative to the largest one
+int Engine::n_unoccupied(const std::vector<cplx>& R, i64 n) const {
+    const std::vector<double> sv = singular_values(R.data(), n);
+    int cnt = 0;
+    for (double s : sv)
+        if (s / sv[0] < m_p.unoccupied_threshold) ++cnt;
+    return cnt;
+}
+
+// generate_orthogonal_trial_vector (two_site_energy_variations.hpp:341-372): a random vector of
+// the complement of `mode` of node n, drawn from the normal stream (each entry takes two draws,
+// the first for the imaginary part: GCC evaluates the arguments of complex(draw, draw) right to
+// left), then freed, one slice at a time, of the unconjugated projection on the slices and
+// renormalised.
+void Engine::trial_vector(int n, int mode, std::vector<cplx>& x) {
+    const Node& nd = m_topo.nodes[n];
+    i64 L, d, R;
+    rank3(nd, mode, L, d, R);
+    const cplx* a = m_A[n].data();
+    x.assign(L * R, cplx(0.0, 0.0));
+    if (L * R == d) return;
+    for (int attempt = 0;; ++attempt) {
+        check(attempt < 1000, ERR_NUMERIC, "trial vector: no orthogonal component found");
+        for (cplx& v : x) {
+            const double im = m_ndist(m_rng);
+            const double re = m_ndist(m_rng);
+            v = cplx(re, im);
+        }
+        m_normals += 2 * static_cast<i64>(x.size());
+        double nrm = std::sqrt(norm2(x));
+        for (cplx& v : x) v /= nrm;
+        bool ok = true;
+        for (i64 k = 0; k < d && ok; ++k) {
+            cplx uix(0.0, 0.0);
+            for (i64 l = 0; l < L; ++l)
+                for (i64 j = 0; j < R; ++j) uix += a[(l * d + k) * R + j] * x[l * R + j];
+            for (i64 l = 0; l < L; ++l)
+                for (i64 j = 0; j < R; ++j) x[l * R + j] -= uix * a[(l * d + k) * R + j];
+            nrm = std::sqrt(norm2(x));
+            if (nrm > 1e-14)
+                for (cplx& v : x) v /= nrm;
+            else
+                ok = false;
+        }
+        if (ok) return;
+    }
+}
+
+// terms of node c's operator with neither an identity mean field nor identity spf (:49-73)
+std::vector<int> Engine::two_site_terms(int c) const {
+    std::vector<int> out;
+    const auto& terms = m_ops.nodes[c].terms;
+    for (int i = 0; i < static_cast<int>(terms.size()); ++i)
+        if (!terms[i].id_mf && !terms[i].id_spf) out.push_back(i);
+    return out;
+}
+
+// construct_two_site_energy_terms_lower (:78-142): (sum_i accum_i K_i A) r, projected onto
+// the complement of A's columns
+void Engine::two_site_lower(int c, int ind, const std::vector<cplx>& r, std::vector<cplx>& out) const {
+    const Node& nd = m_topo.nodes[c];
+    const i64 D = nd.dimen(), rk = nd.hrank, sz = D * rk;
+    const Term& tm = m_ops.nodes[c].terms[ind];
+    const cplx* a = m_A[c].data();
+    std::vector<cplx> res(sz, cplx(0.0, 0.0)), HA(sz), tmp(sz);
+    for (int i = 0; i < tm.nspf_terms(); ++i) {
+        if (nd.is_leaf()) {
+            const auto& idx = tm.spf_index[i][0];
+            m_sop.ops[idx.first][idx.second].op.apply(a, HA.data(), rk);
+        } else {
+            kpo(c, tm.spf_index[i], a, HA.data(), tmp.data());
+        }
+        for (i64 q = 0; q < sz; ++q) res[q] += tm.accum[i] * HA[q];
+    }
+    for (i64 x = 0; x < D; ++x)
+        for (i64 j = 0; j < rk; ++j) {
+            cplx s(0.0, 0.0);
+            for (i64 k = 0; k < rk; ++k) s += res[x * rk + k] * r[k * rk + j];
+            HA[x * rk + j] = s;
+        }
+    std::vector<cplx> T(rk * rk);
+    adjoint_mul(a, HA.data(), D, rk, T.data());
+    out.assign(sz, cplx(0.0, 0.0));
+    for (i64 x = 0; x < D; ++x)
+        for (i64 j = 0; j < rk; ++j) {
+            cplx s(0.0, 0.0);
+            for (i64 k = 0; k < rk; ++k) s += a[x * rk + k] * T[k * rk + j];
+            out[x * rk + j] = HA[x * rk + j] - s;
+        }
+}
+
+// construct_two_site_energy_terms_upper (:144-222): sum_it accum_it (sibling spf) A_p mf_p^T on
+// the parent, with the projection on the parent's slices of c's mode removed
+void Engine::two_site_upper(int c, int ind, std::vector<cplx>& out) const {
+    const Node& nd = m_topo.nodes[c];
+    const int p = nd.parent;
+    const Node& pn = m_topo.nodes[p];
+    const i64 psz = pn.size(), pD = pn.dimen(), pr = pn.hrank;
+    i64 L, d, R;
+    rank3(pn, nd.child_id, L, d, R);
+    const Term& tm = m_ops.nodes[c].terms[ind];
+    const auto& pterms = m_ops.nodes[p].terms;
+    const cplx* Ap = m_A[p].data();
+    std::vector<cplx> res(psz, cplx(0.0, 0.0)), tmp(psz), HA(psz);
+    for (int it = 0; it < tm.nmf_terms(); ++it) {
+        const auto& mi = tm.mf_index[it];
+        kron_siblings(p, mi.siblings, Ap, tmp.data(), HA.data());
+        const cplx* t = tmp.data();
+        if (!pterms[mi.parent_index].id_mf) {
+            mul_transposed(tmp.data(), pD, pr, m_mf[p][mi.parent_index].data(), HA.data());
+            t = HA.data();
+        }
+        for (i64 q = 0; q < psz; ++q) res[q] += tm.accum[it] * t[q];
+    }
+    // HA(j, j') = sum_{l,k} res(l,j,k) conj(A(l,j',k));  res(l,j,k) -= sum_j' HA(j,j') A(l,j',k)
+    std::vector<cplx> M(d * d, cplx(0.0, 0.0));
+    for (i64 l = 0; l < L; ++l)
+        for (i64 j = 0; j < d; ++j)
+            for (i64 jp = 0; jp < d; ++jp) {
+                cplx s(0.0, 0.0);
+                for (i64 k = 0; k < R; ++k) s += res[(l * d + j) * R + k] * std::conj(Ap[(l * d + jp) * R + k]);
+                M[j * d + jp] += s;
+            }
+    out.assign(psz, cplx(0.0, 0.0));
+    mode_contract(M.data(), L, d, R, Ap, out.data());
+    for (i64 q = 0; q < psz; ++q) out[q] = res[q] - out[q];
+}
+
+void Engine::grow(int c, i64 nadd) {
+    Node& cn = m_topo.nodes[c];
+    cn.hrank += nadd;
+    if (!cn.is_root()) m_topo.nodes[cn.parent].dims[cn.child_id] += nadd;
+}
+
+// subspace_expansion_down (:207-358): child c = children[mode] of p; r = p's gauge factor
+// for that mode (d x d); new slices of p's mode from the dominant right singular vectors of
+// the projected two-site H psi, zero columns in c.
+bool Engine::expand_down(int p, int mode) {
+    const int c = m_topo.nodes[p].children[mode];
+    const Node& pn = m_topo.nodes[p];
+    const Node& cn = m_topo.nodes[c];
+    if (pn.dims[mode] >= pn.cap_dims[mode] || cn.hrank >= cn.cap_hrank) return false;
+    i64 max_add = pn.cap_dims[mode] - pn.dims[mode];
+    i64 L, d, R;
+    rank3(pn, mode, L, d, R);
+    const i64 maxdim = L * R;
+    if (d >= maxdim) return false;
+    const bool twosite = cn.hrank < cn.dimen() && cn.hrank != 1;
+    int n_unocc = 0;
+    if (m_p.only_apply_when_no_unoccupied || !twosite) n_unocc = n_unoccupied(m_R[p], d);
+    if (m_p.only_apply_when_no_unoccupied && n_unocc >= m_p.minimum_unoccupied) return false;
+    max_add = std::min(max_add, maxdim - d);
+    i64 required = n_unocc < m_p.minimum_unoccupied ? m_p.minimum_unoccupied - n_unocc : 0;
+    required = std::min(required, max_add);
+
+    const i64 D1 = cn.dimen(), r1 = cn.hrank;
+    i64 nadd = 0;
+    std::vector<std::vector<cplx>> rows;
+    if (twosite) {
+        const std::vector<int> inds = two_site_terms(c);
+        std::vector<std::vector<cplx>> op1(inds.size()), op2(inds.size());
+        for (size_t t = 0; t < inds.size(); ++t) {
+            two_site_lower(c, inds[t], m_R[p], op1[t]);
+            two_site_upper(c, inds[t], op2[t]);
+        }
+        std::vector<cplx> x;
+        trial_vector(p, mode, x);
+        // M^H M v with M = sum_t op1_t (op2_t as d x LR) (the functor with MconjM, :265-301)
+        auto F = [&](const cplx* v, cplx* out) {
+            std::fill(out, out + maxdim, cplx(0.0, 0.0));
+            if (inds.empty()) return;
+            std::vector<cplx> t1(D1, cplx(0.0, 0.0)), t2(d);
+            for (size_t t = 0; t < inds.size(); ++t) {
+                for (i64 j = 0; j < d; ++j) {
+                    cplx s(0.0, 0.0);
+                    for (i64 l = 0; l < L; ++l)
+                        for (i64 k = 0; k < R; ++k) s += op2[t][(l * d + j) * R + k] * v[l * R + k];
+                    t2[j] = s;
+                }
+                for (i64 i = 0; i < D1; ++i)
+                    for (i64 j = 0; j < d; ++j) t1[i] += op1[t][i * r1 + j] * t2[j];
+            }
+            for (size_t t = 0; t < inds.size(); ++t) {
+                for (i64 j = 0; j < d; ++j) {
+                    cplx s(0.0, 0.0);
+                    for (i64 i = 0; i < D1; ++i) s += std::conj(op1[t][i * r1 + j]) * t1[i];
+                    t2[j] = s;
+                }
+                for (i64 l = 0; l < L; ++l)
+                    for (i64 k = 0; k < R; ++k) {
+                        cplx s(0.0, 0.0);
+                        for (i64 j = 0; j < d; ++j) s += std::conj(op2[t][(l * d + j) * R + k]) * t2[j];
+                        out[l * R + k] += s;
+                    }
+            }
+        };
+        std::vector<double> S;
+        const int kd = static_cast<int>(std::min<i64>(m_p.krylov_dim, maxdim));
+        eigs(F, x, kd, m_p.eigenvalue_tolerance, S, rows);
+        for (auto& row : rows)
+            for (cplx& v : row) v = std::conj(v);
+        for (double s : S) {
+            const double sv = s > 0.0 ? std::sqrt(s) * (m_exp_dt / 2.0) : 0.0;
+            if (!std::isnan(sv) && sv > m_p.spawning_threshold) ++nadd;
+        }
+        nadd = std::min(nadd, max_add);
+        ++m_twosite;
+    } else {
+        // random one-site directions, one at a time (:319-350)
+        while (nadd < required) {
+            std::vector<cplx> x;
+            trial_vector(p, mode, x);
+            const std::vector<std::vector<cplx>> one(1, x);
+            i64 Lp, dp, Rp;
+            rank3(m_topo.nodes[p], mode, Lp, dp, Rp);
+            m_A[c] = pad_middle(m_A[c], m_topo.nodes[c].dimen(), m_topo.nodes[c].hrank, 1, 1, nullptr);
+            m_R[p] = pad_square(m_R[p], dp, 1);
+            m_A[p] = pad_middle(m_A[p], Lp, dp, Rp, 1, &one);
+            grow(c, 1);
+            ++nadd;
+        }
+        ++m_onesite;
+        return nadd > 0;
+    }
+    if (nadd == 0) return false;
+    rows.resize(nadd);
+    m_A[c] = pad_middle(m_A[c], D1, r1, 1, nadd, nullptr);
+    m_R[p] = pad_square(m_R[p], d, nadd);
+    m_A[p] = pad_middle(m_A[p], L, d, R, nadd, &rows);
+    grow(c, nadd);
+    return true;
+}
+
+// subspace_expansion_up (:361-510): c and its parent p after c's leaf-to-root factorisation
+// (r = c's R, r1 x r1); new columns of c from the dominant left singular vectors of the
+// projected two-site H psi, zero slices in p.
+bool Engine::expand_up(int c) {
+    const Node& cn = m_topo.nodes[c];
+    const int p = cn.parent;
+    if (cn.hrank >= cn.cap_hrank) return false;
+    i64 max_add = cn.cap_hrank - cn.hrank;
+    const i64 D1 = cn.dimen(), r1 = cn.hrank;
+    if (r1 >= D1) return false;
+    const Node& pn = m_topo.nodes[p];
+    i64 L, d, R;
+    rank3(pn, cn.child_id, L, d, R);
+    const bool twosite = d < L * R && d != 1;
+    int n_unocc = 0;
+    if (m_p.only_apply_when_no_unoccupied || !twosite) n_unocc = n_unoccupied(m_R[c], r1);
+    if (m_p.only_apply_when_no_unoccupied && n_unocc >= m_p.minimum_unoccupied) return false;
+    max_add = std::min(max_add, D1 - r1);
+    i64 required = n_unocc < m_p.minimum_unoccupied ? m_p.minimum_unoccupied - n_unocc : 0;
+    required = std::min(required, max_add);
+
+    i64 nadd = 0;
+    std::vector<std::vector<cplx>> cols;
+    if (twosite) {
+        const std::vector<int> inds = two_site_terms(c);
+        std::vector<std::vector<cplx>> op1(inds.size()), op2(inds.size());
+        for (size_t t = 0; t < inds.size(); ++t) {
+            two_site_upper(c, inds[t], op2[t]);
+            two_site_lower(c, inds[t], m_R[c], op1[t]);
+        }
+        std::vector<cplx> x;
+        trial_vector(c, cn.nmodes(), x);
+        // conj(M) M^T v (the functor without MconjM, :302-330)
+        auto F = [&](const cplx* v, cplx* out) {
+            std::fill(out, out + D1, cplx(0.0, 0.0));
+            if (inds.empty()) return;
+            std::vector<cplx> t1(L * R, cplx(0.0, 0.0)), t2(d);
+            for (size_t t = 0; t < inds.size(); ++t) {
+                for (i64 j = 0; j < d; ++j) {
+                    cplx s(0.0, 0.0);
+                    for (i64 i = 0; i < D1; ++i) s += op1[t][i * r1 + j] * v[i];
+                    t2[j] = s;
+                }
+                for (i64 l = 0; l < L; ++l)
+                    for (i64 k = 0; k < R; ++k) {
+                        cplx s(0.0, 0.0);
+                        for (i64 j = 0; j < d; ++j) s += op2[t][(l * d + j) * R + k] * t2[j];
+                        t1[l * R + k] += s;
+                    }
+            }
+            for (size_t t = 0; t < inds.size(); ++t) {
+                for (i64 j = 0; j < d; ++j) {
+                    cplx s(0.0, 0.0);
+                    for (i64 l = 0; l < L; ++l)
+                        for (i64 k = 0; k < R; ++k) s += std::conj(op2[t][(l * d + j) * R + k]) * t1[l * R + k];
+                    t2[j] = s;
+                }
+                for (i64 i = 0; i < D1; ++i)
+                    for (i64 j = 0; j < d; ++j) out[i] += std::conj(op1[t][i * r1 + j]) * t2[j];
+            }
+        };
+        std::vector<double> S;
+        const int kd = static_cast<int>(std::min<i64>(m_p.krylov_dim, D1));
+        eigs(F, x, kd, 1e-4, S, cols);
+        for (double s : S) {
+            const double sv = s > 0.0 ? std::sqrt(s) * m_exp_dt / 2.0 : 0.0;
+            if (!std::isnan(sv) && sv > m_p.spawning_threshold) ++nadd;
+        }
+        nadd = std::min(nadd, max_add);
+        if (nadd == 0) return false;
+        ++m_twosite;
+    } else {
+        while (nadd < required) {
+            std::vector<cplx> x;
+            trial_vector(c, m_topo.nodes[c].nmodes(), x);
+            const std::vector<std::vector<cplx>> one(1, x);
+            i64 Lp, dp, Rp;
+            rank3(m_topo.nodes[p], m_topo.nodes[c].child_id, Lp, dp, Rp);
+            const i64 rc = m_topo.nodes[c].hrank;
+            m_A[p] = pad_middle(m_A[p], Lp, dp, Rp, 1, nullptr);
+            m_R[c] = pad_square(m_R[c], rc, 1);
+            m_A[c] = pad_middle(m_A[c], D1, rc, 1, 1, &one);
+            grow(c, 1);
+            ++nadd;
+        }
+        ++m_onesite;
+        return nadd > 0;
+    }
+    cols.resize(nadd);
+    m_A[p] = pad_middle(m_A[p], L, d, R, nadd, nullptr);
+    m_R[c] = pad_square(m_R[c], r1, nadd);
+    m_A[c] = pad_middle(m_A[c], D1, r1, 1, nadd, &cols);
+    grow(c, nadd);
+    return true;
+}
+
+}  // namespace orc

@@ -38,6 +38,7 @@
 #include "../ttns_lib_amd/csrc/host/topology.hpp"
 
 #include <functional>
+#include <random>
 
 namespace orc {
 
@@ -68,11 +69,28 @@ struct KrylovStats {
 i64 expv(const std::function<void(const cplx*, cplx*)>& H, cplx* x, i64 n, cplx c, double tau, int kdim,
          double tol);
 
+// ---- subspace expansion helpers (expansion.cpp) ----
+// singular values of a dense n x n matrix, descending (one-sided Jacobi, Hestenes 1958)
+std::vector<double> singular_values(const cplx* M, i64 n);
+// eigenvalues (descending) and eigenvectors (columns of Z, n x n row-major) of a real
+// symmetric tridiagonal matrix (diagonal a, off-diagonal b), cyclic Jacobi rotations
+void tridiag_eigen(const std::vector<double>& a, const std::vector<double>& b, std::vector<double>& w,
+                   std::vector<double>& Z);
+
 // ---- the TDVP engine ----
 struct Params {
     int krylov_dim = 12;
     double krylov_tol = 1e-12;
     double dt = 0.05;
+    // subspace_expansion_projector_splitting_integrator (projector_splitting_integrator.hpp:225-397)
+    bool expansion = false;
+    double spawning_threshold = -1.0;       // engine defaults (subspace_expansion_..._engine.hpp:45)
+    double unoccupied_threshold = -1.0;
+    i64 minimum_unoccupied = 0;
+    i64 neigenvalues = 2;
+    double eigenvalue_tolerance = 1e-6;
+    bool only_apply_when_no_unoccupied = false;
+    unsigned rng_seed = 5489u;              // std::mt19937 default seed (two_site_energy_variations.hpp:34)
 };
 
 class Engine {
@@ -83,11 +101,20 @@ public:
     const ttnh::Topology& topo() const { return m_topo; }
     const ttnh::CompiledSOP& compiled() const { return m_ops; }
 
-    // one integrator call (projector_splitting_integrator.hpp:136-169)
+    // one integrator call (projector_splitting_integrator.hpp:136-169, 281-326)
     void step(cplx coeff);
     double time() const { return m_dt * static_cast<double>(m_nsteps); }
     i64 nsteps() const { return m_nsteps; }
     i64 nh_applications() const { return m_nh; }
+    // engine().Nonesite() / Ntwosite() (subspace_expansion_projector_splitting_engine.hpp:513-514)
+    i64 n_onesite() const { return m_onesite; }
+    i64 n_twosite() const { return m_twosite; }
+    void set_dt(double dt) { m_dt = dt; }
+    double dt() const { return m_dt; }
+    void reset_nsteps() { m_nsteps = 0; }
+    // normal draws consumed by the expansion so far (the position in the std::mt19937 +
+    // std::normal_distribution stream)
+    i64 normals_used() const { return m_normals; }
 
     // <psi|O_mode|psi> with O acting on one primitive mode (full contraction, no
     // orthonormality assumption) and <psi|psi>
@@ -108,15 +135,28 @@ private:
     void kron_siblings(int n, const std::vector<std::pair<int, int>>& sib, const cplx* Ap, cplx* out,
                        cplx* tmp) const;
     void prepare_evolution();
-    void sweep(bool evolve);
-    void forward(bool evolve);
-    void backward(bool evolve);
+    bool sweep(bool evolve, bool expand);
+    bool forward(bool evolve, bool expand);
+    bool backward(bool evolve, bool expand);
     void node_update(int n);
     void bond_update(int n, int with_ops_of);
     void r2l(int n, int mode);
     void l2r(int n);
     void apply_from_parent(int n);
     void apply_to_parent(int n);
+
+    // subspace expansion (subspace_expansion_projector_splitting_engine.hpp:194-510,
+    // two_site_energy_variations.hpp:49-562)
+    bool expand_down(int p, int mode);
+    bool expand_up(int c);
+    int n_unoccupied(const std::vector<cplx>& R, i64 n) const;
+    void trial_vector(int n, int mode, std::vector<cplx>& x);
+    void two_site_lower(int c, int ind, const std::vector<cplx>& r, std::vector<cplx>& out) const;
+    void two_site_upper(int c, int ind, std::vector<cplx>& out) const;
+    std::vector<int> two_site_terms(int c) const;
+    void eigs(const std::function<void(const cplx*, cplx*)>& F, std::vector<cplx> x, int kd, double tol,
+              std::vector<double>& S, std::vector<std::vector<cplx>>& V) const;
+    void grow(int c, i64 nadd);   // child c's bond dimension += nadd (shapes only)
 
     ttnh::Topology m_topo;
     ttnh::SOP m_sop;
@@ -126,8 +166,12 @@ private:
     std::vector<std::vector<cplx>> m_A, m_R;
     std::vector<std::vector<std::vector<cplx>>> m_spf, m_mf;
     double m_dt = 0.0, m_sub_dt = 0.0, m_t = 0.0;
+    double m_exp_dt = 0.0;    // the engine's dt() seen by the spawning criterion
     cplx m_coeff = cplx(0.0, -1.0);
     i64 m_nsteps = 0, m_nh = 0;
+    i64 m_onesite = 0, m_twosite = 0, m_normals = 0;
+    std::mt19937 m_rng;
+    std::normal_distribution<double> m_ndist{0.0, 1.0};
 };
 
 }  // namespace orc

@@ -41,6 +41,12 @@ int orc_state_get(orc_integ* integ, int64_t node, double* data, int64_t rows, in
 int orc_step(orc_integ* integ, double coeff_re, double coeff_im, int64_t nsteps);
 int orc_time(const orc_integ* integ, double* t);
 int orc_counters(const orc_integ* integ, int64_t* nh_applications, int64_t* nsteps);
+int orc_expansion_counters(const orc_integ* integ, int64_t* n_onesite, int64_t* n_twosite);
+int orc_hranks(const orc_integ* integ, int64_t* hranks);
+int orc_state_shape(const orc_integ* integ, int64_t node, int64_t* rows, int64_t* cols);
+int orc_set_dt(orc_integ* integ, double dt);
+int orc_get_dt(const orc_integ* integ, double* dt);
+int orc_reset_nsteps(orc_integ* integ);
 int orc_expect(orc_integ* integ, int64_t mode, const double* op, double* out);
 int orc_norm2(orc_integ* integ, double* out);
 int orc_sb_problem(orc_ctx* ctx, const ttn_sb_params* p, orc_tree** tree, orc_sop** sop);
@@ -65,6 +71,11 @@ int orc_expv(int64_t n, const double* Hdense, double* x, double c_re, double c_i
              double tol, int64_t* napply);
 int orc_expm(int64_t n, const double* M, double* E);
 int orc_qr(int64_t m, int64_t n, const double* A, double* Q, double* R);
+/* expansion helpers: normal draws consumed so far; singular values (descending) of a dense n x n
+ * matrix; eigenpairs of a real symmetric tridiagonal matrix (eigenvectors as columns of Z) */
+int orc_normals_used(const orc_integ* integ, int64_t* n);
+int orc_singular_values(int64_t n, const double* M, double* sv);
+int orc_tridiag_eigen(int64_t n, const double* a, const double* b, double* w, double* Z);
 
 #ifdef __cplusplus
 }

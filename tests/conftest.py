@@ -51,4 +51,7 @@ ORACLE_EXTRA = {
     "expm": [_C.c_int64, _d, _d],
     "qr": [_C.c_int64, _C.c_int64, _d, _d, _d],
     "node_expv": [_C.c_void_p, _C.c_int64, _C.c_double, _C.c_double, _C.c_double, _C.c_int64, _C.c_double, _i],
+    "normals_used": [_C.c_void_p, _i],
+    "singular_values": [_C.c_int64, _d, _d],
+    "tridiag_eigen": [_C.c_int64, _d, _d, _d, _d],
 }

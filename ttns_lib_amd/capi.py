@@ -43,6 +43,9 @@ class IntegParams(C.Structure):
         ("minimum_unoccupied", C.c_int64),
         ("neigenvalues", C.c_int64),
         ("team_size", C.c_int64),
+        ("eigenvalue_tolerance", C.c_double),
+        ("only_apply_when_no_unoccupied", C.c_int32),
+        ("rng_seed", C.c_uint32),
     ]
 
 
@@ -88,6 +91,12 @@ _SIGS = {
     "step": [vp, C.c_double, C.c_double, i64],
     "time": [vp, dblp],
     "counters": [vp, i64p, i64p],
+    "expansion_counters": [vp, i64p, i64p],
+    "hranks": [vp, i64p],
+    "state_shape": [vp, i64, i64p, i64p],
+    "set_dt": [vp, C.c_double],
+    "get_dt": [vp, dblp],
+    "reset_nsteps": [vp],
     "expect": [vp, i64, dblp, dblp],
     "norm2": [vp, dblp],
     "set_observable": [vp, i64, dblp],
@@ -373,16 +382,52 @@ def sb_problem(lib: Library, model: SpinBoson):
 
 
 class Integrator:
+    """An integrator handle (ttn_integrator_create).  mode=MODE_SUBSPACE_EXPANSION is the
+    reference's subspace_expansion_projector_splitting_integrator: bonds below their capacity
+    grow, so node shapes are read back from the library (`shape`, `hranks`)."""
+
     def __init__(self, lib: Library, tree: Tree, sop: SOPHandle, *, dt: float, krylov_dim: int = 12,
                  krylov_tol: float = 1e-12, mode: int = MODE_FIXED_STEP, team_size: int = 0,
                  spawning_threshold: float = 1e-12, unoccupied_threshold: float = 1e-12,
-                 minimum_unoccupied: int = 0, neigenvalues: int = 2):
+                 minimum_unoccupied: int = 0, neigenvalues: int = 2, eigenvalue_tolerance: float = 0.0,
+                 only_apply_when_no_unoccupied: bool = False, rng_seed: int = 0):
         self.lib, self.tree = lib, tree
         self.params = IntegParams(krylov_dim, krylov_tol, dt, mode, spawning_threshold, unoccupied_threshold,
-                                  minimum_unoccupied, neigenvalues, team_size)
+                                  minimum_unoccupied, neigenvalues, team_size, eigenvalue_tolerance,
+                                  int(bool(only_apply_when_no_unoccupied)), rng_seed)
         self.h = vp()
         lib._call("integrator_create", lib.ctx, tree.h, sop.h, C.byref(self.params), C.byref(self.h))
         self.shapes = tree.shapes()
+
+    def shape(self, i):
+        """current (rows, cols) of node i"""
+        if not self.lib.has("state_shape"):
+            return self.shapes[i]
+        r, c = i64(), i64()
+        self.lib._call("state_shape", self.h, i, C.byref(r), C.byref(c))
+        return r.value, c.value
+
+    def hranks(self) -> np.ndarray:
+        out = np.zeros(len(self.shapes), dtype=np.int64)
+        self.lib._call("hranks", self.h, _ip(out))
+        return out
+
+    def expansion_counters(self):
+        """(Nonesite, Ntwosite) of the expansion engine"""
+        a, b = i64(), i64()
+        self.lib._call("expansion_counters", self.h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def set_dt(self, dt: float):
+        self.lib._call("set_dt", self.h, float(dt))
+
+    def get_dt(self) -> float:
+        out = C.c_double()
+        self.lib._call("get_dt", self.h, C.byref(out))
+        return out.value
+
+    def reset_nsteps(self):
+        self.lib._call("reset_nsteps", self.h)
 
     def set_sb_initial_state(self, model: SpinBoson):
         p = model.to_c()
@@ -390,13 +435,13 @@ class Integrator:
 
     def set_node(self, i, a):
         a = np.asarray(a, dtype=complex)
-        rows, cols = self.shapes[i]
+        rows, cols = self.shape(i)
         assert a.shape == (rows, cols), (a.shape, rows, cols)
         buf = _cplx_in(a)
         self.lib._call("state_set", self.h, i, _dp(buf), rows, cols)
 
     def get_node(self, i):
-        rows, cols = self.shapes[i]
+        rows, cols = self.shape(i)
         buf = np.zeros(2 * rows * cols, dtype=np.float64)
         self.lib._call("state_get", self.h, i, _dp(buf), rows, cols)
         return buf.view(np.complex128).reshape(rows, cols)

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -67,6 +68,7 @@ struct ttn_integ {
     int obs_leaf = -1;
     int64_t obs_dim = 0;
     int64_t nsteps = 0;
+    int64_t n_onesite = 0, n_twosite = 0;  // expansion attempts (engine().Nonesite/Ntwosite)
     double last_ms = 0.0;
     std::vector<double> trajectory;  // (re <O>, <1>) per completed step
 };
@@ -881,6 +883,37 @@ int ttn_counters(const ttn_integ* in, int64_t* nh, int64_t* ns) {
         if (nh) *nh = c[0];
         if (ns) *ns = in->nsteps;
     });
+}
+
+int ttn_expansion_counters(const ttn_integ* in, int64_t* n1, int64_t* n2) {
+    if (n1) *n1 = in->n_onesite;
+    if (n2) *n2 = in->n_twosite;
+    return TTN_OK;
+}
+int ttn_hranks(const ttn_integ* in, int64_t* hr) {
+    for (int n = 0; n < in->topo.size(); ++n) hr[n] = in->topo.nodes[n].hrank;
+    return TTN_OK;
+}
+int ttn_state_shape(const ttn_integ* in, int64_t node, int64_t* rows, int64_t* cols) {
+    return guard(in->ctx, [&] {
+        check(node >= 0 && node < in->topo.size(), ERR_INVALID, "node out of range");
+        if (rows) *rows = in->topo.nodes[node].dimen();
+        if (cols) *cols = in->topo.nodes[node].hrank;
+    });
+}
+int ttn_set_dt(ttn_integ* in, double dt) {
+    return guard(in->ctx, [&] {
+        check(dt > 0.0 && std::isfinite(dt), ERR_INVALID, "dt must be positive");
+        in->p.dt = dt;
+    });
+}
+int ttn_get_dt(const ttn_integ* in, double* dt) {
+    *dt = in->p.dt;
+    return TTN_OK;
+}
+int ttn_reset_nsteps(ttn_integ* in) {
+    in->nsteps = 0;
+    return TTN_OK;
 }
 
 static void run_expect(ttn_integ* in, int leaf, int64_t op_off, double* out3) {
