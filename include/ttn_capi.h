@@ -146,6 +146,16 @@ int ttn_get_dt(const ttn_integ* integ, double* dt);
 /* reset_nsteps() (projector_splitting_integrator.hpp:33): the next ttn_step prepares the
  * evolution and runs the first-step ladder again */
 int ttn_reset_nsteps(ttn_integ* integ);
+/* normal draws the expansion's trial vectors have taken from its std::mt19937 +
+ * std::normal_distribution stream so far.  Instrumentation (no reference counterpart). */
+int ttn_expansion_draws(const ttn_integ* integ, int64_t* n);
+/* Expansion log (instrumentation, no reference counterpart): with a capacity > 0 every two-site
+ * expansion attempt appends a record of 20 doubles: the child node, the direction (0 down, 1 up),
+ * the number of Ritz values, the functions added and, for the 16 largest Ritz values S, the
+ * spawning values sqrt(S) dt / 2 that the spawning threshold is applied to (descending).
+ * ttn_expansion_log copies up to *len records (out may be NULL) and sets *len to the count. */
+int ttn_set_expansion_log(ttn_integ* integ, int64_t capacity);
+int ttn_expansion_log(const ttn_integ* integ, int64_t* len, double* out);
 /* <psi|O|psi> for a dense operator O (d x d complex) on primitive mode `mode` */
 int ttn_expect(ttn_integ* integ, int64_t mode, const double* op, double* out);
 /* <psi|psi> */

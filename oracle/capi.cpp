@@ -231,6 +231,21 @@ int orc_reset_nsteps(orc_integ* in) {
     in->eng->reset_nsteps();
     return TTN_OK;
 }
+int orc_set_expansion_log(orc_integ* in, int64_t cap) {
+    in->eng->set_log(cap);
+    return TTN_OK;
+}
+int orc_expansion_log(const orc_integ* in, int64_t* len, double* out) {
+    const auto& lg = in->eng->log();
+    const int64_t n = static_cast<int64_t>(lg.size() / 20);
+    if (out) std::copy(lg.begin(), lg.begin() + 20 * std::min(n, *len), out);
+    *len = n;
+    return TTN_OK;
+}
+int orc_expansion_draws(const orc_integ* in, int64_t* n) {
+    *n = in->eng->normals_used();
+    return TTN_OK;
+}
 int orc_normals_used(const orc_integ* in, int64_t* n) {
     *n = in->eng->normals_used();
     return TTN_OK;

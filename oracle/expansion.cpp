@@ -371,6 +371,18 @@ void Engine::two_site_upper(int c, int ind, std::vector<cplx>& out) const {
     for (i64 q = 0; q < psz; ++q) out[q] = res[q] - out[q];
 }
 
+// one record per two-site attempt: node, direction, Ritz values, functions added and the
+// spawning values sqrt(S) dt / 2 of the 16 largest Ritz values S (what the threshold is applied to)
+void Engine::log_attempt(int c, int dir, const std::vector<double>& S, i64 nadd) {
+    if (static_cast<i64>(m_log.size()) >= 20 * m_log_cap) return;
+    m_log.push_back(c);
+    m_log.push_back(dir);
+    m_log.push_back(static_cast<double>(S.size()));
+    m_log.push_back(static_cast<double>(nadd));
+    for (size_t i = 0; i < 16; ++i)
+        m_log.push_back(i < S.size() && S[i] > 0.0 ? std::sqrt(S[i]) * m_exp_dt / 2.0 : 0.0);
+}
+
 void Engine::grow(int c, i64 nadd) {
     Node& cn = m_topo.nodes[c];
     cn.hrank += nadd;
@@ -449,6 +461,7 @@ bool Engine::expand_down(int p, int mode) {
             if (!std::isnan(sv) && sv > m_p.spawning_threshold) ++nadd;
         }
         nadd = std::min(nadd, max_add);
+        log_attempt(c, 0, S, nadd);
         ++m_twosite;
     } else {
         // random one-site directions, one at a time (:319-350)
@@ -545,6 +558,7 @@ bool Engine::expand_up(int c) {
             if (!std::isnan(sv) && sv > m_p.spawning_threshold) ++nadd;
         }
         nadd = std::min(nadd, max_add);
+        log_attempt(c, 1, S, nadd);
         if (nadd == 0) return false;
         ++m_twosite;
     } else {

@@ -115,6 +115,13 @@ public:
     // normal draws consumed by the expansion so far (the position in the std::mt19937 +
     // std::normal_distribution stream)
     i64 normals_used() const { return m_normals; }
+    // log of the two-site expansion attempts: records of 20 doubles (child node, direction 0
+    // down / 1 up, Ritz values kept, functions added, 16 Ritz values), at most cap records
+    void set_log(i64 cap) {
+        m_log_cap = cap;
+        m_log.clear();
+    }
+    const std::vector<double>& log() const { return m_log; }
 
     // <psi|O_mode|psi> with O acting on one primitive mode (full contraction, no
     // orthonormality assumption) and <psi|psi>
@@ -157,6 +164,7 @@ private:
     void eigs(const std::function<void(const cplx*, cplx*)>& F, std::vector<cplx> x, int kd, double tol,
               std::vector<double>& S, std::vector<std::vector<cplx>>& V) const;
     void grow(int c, i64 nadd);   // child c's bond dimension += nadd (shapes only)
+    void log_attempt(int c, int dir, const std::vector<double>& S, i64 nadd);
 
     ttnh::Topology m_topo;
     ttnh::SOP m_sop;
@@ -170,6 +178,8 @@ private:
     cplx m_coeff = cplx(0.0, -1.0);
     i64 m_nsteps = 0, m_nh = 0;
     i64 m_onesite = 0, m_twosite = 0, m_normals = 0;
+    std::vector<double> m_log;
+    i64 m_log_cap = 0;
     std::mt19937 m_rng;
     std::normal_distribution<double> m_ndist{0.0, 1.0};
 };

@@ -47,6 +47,9 @@ int orc_state_shape(const orc_integ* integ, int64_t node, int64_t* rows, int64_t
 int orc_set_dt(orc_integ* integ, double dt);
 int orc_get_dt(const orc_integ* integ, double* dt);
 int orc_reset_nsteps(orc_integ* integ);
+int orc_expansion_draws(const orc_integ* integ, int64_t* n);
+int orc_set_expansion_log(orc_integ* integ, int64_t capacity);
+int orc_expansion_log(const orc_integ* integ, int64_t* len, double* out);
 int orc_expect(orc_integ* integ, int64_t mode, const double* op, double* out);
 int orc_norm2(orc_integ* integ, double* out);
 int orc_sb_problem(orc_ctx* ctx, const ttn_sb_params* p, orc_tree** tree, orc_sop** sop);

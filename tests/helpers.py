@@ -105,3 +105,85 @@ def exact_sigma(model, psi0, times, op="z"):
 def random_hermitian(rng, n, scale=1.0):
     X = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
     return scale * 0.5 * (X + X.conj().T)
+
+
+def perturb_nodes(integ, seed, eps=1e-15):
+    """Multiplies every node tensor entry by (1 + eps N(0,1)): a run that differs from the
+    unperturbed one only by rounding-sized changes."""
+    rng = np.random.default_rng(seed)
+    for i in range(len(integ.shapes)):
+        x = integ.get_node(i)
+        integ.set_node(i, x * (1 + eps * rng.standard_normal(x.shape)))
+    return integ
+
+
+LOG_CAPACITY = 1 << 16
+
+
+def decision_margin(log, spawn):
+    """How far the closest spawning decision of an expansion log (Integrator.expansion_log) was
+    from the threshold: the smallest |ln(v / spawn)| over its positive spawning values v."""
+    n = log[:, 2].astype(int)
+    v = log[:, 4:][(np.arange(16)[None, :] < n[:, None]) & (log[:, 4:] > 0)]
+    return float(np.abs(np.log(v / spawn)).min()) if v.size else np.inf
+
+
+def fragile_decisions(log, perturbed_logs, spawn):
+    """Spawning values of a run's expansion log whose decision rounding could flip: those whose
+    distance from the threshold, |v - spawn|, is at most twice how far they moved, |v_p - v|, in
+    a run that differs from this one only by rounding (the same attempt and Ritz index in its
+    log; a Ritz value one run lacks counts as 0).  Returns the count, or -1 once the runs make
+    different attempts."""
+    v = np.where(np.arange(16)[None, :] < log[:, 2:3].astype(int), np.maximum(log[:, 4:], 0.0), 0.0)
+    fragile = np.zeros(v.shape, dtype=bool)
+    for lp in perturbed_logs:
+        if lp.shape != log.shape or not np.array_equal(lp[:, :2], log[:, :2]):
+            return -1
+        vp = np.where(np.arange(16)[None, :] < lp[:, 2:3].astype(int), np.maximum(lp[:, 4:], 0.0), 0.0)
+        fragile |= 2 * np.abs(vp - v) >= np.abs(v - spawn)
+    return int(fragile.sum())
+
+
+class RoundingYardstick:
+    """Checks a run against a reference run step by step, measured by how far runs of the
+    reference that differ from it only by rounding (perturb_nodes) drift from it.
+
+    Each step passes (bond dimensions, [Nonesite, Ntwosite], <sigma_x,y,z>) for the run under
+    test, the reference and every perturbed reference, and whether the step's spawning decisions
+    were robust to rounding (no fragile_decisions).  While every
+    step so far was robust and all perturbed runs still have the reference's bond dimensions and
+    counters, the run under test must have them too.  Afterwards
+    its number of differing bonds, the difference of the summed bond dimensions and of Ntwosite
+    stay within twice the largest seen so far in a perturbed run plus a slack of four bonds / eight
+    attempts.  <sigma> agrees within max(1e-10, 10 x the largest perturbed difference so far)."""
+
+    def __init__(self):
+        self.exact = True
+        self.exact_steps = 0
+        self.env = np.zeros(4)  # differing bonds, |sum diff|, |Nonesite diff|, |Ntwosite diff|
+        self.sigma_env = 0.0
+        self.worst = np.zeros(5)
+
+    @staticmethod
+    def _dist(a, b):
+        ha, ca, _ = a
+        hb, cb, _ = b
+        ha, hb = np.asarray(ha, dtype=np.int64), np.asarray(hb, dtype=np.int64)
+        return np.array([(ha != hb).sum(), abs(ha.sum() - hb.sum()), abs(ca[0] - cb[0]), abs(ca[1] - cb[1])])
+
+    def check(self, k, run, ref, perturbed, robust=True):
+        dp = np.max([self._dist(p, ref) for p in perturbed], axis=0)
+        self.env = np.maximum(self.env, dp)
+        self.sigma_env = max(self.sigma_env,
+                             max(np.abs(np.asarray(p[2]) - np.asarray(ref[2])).max() for p in perturbed))
+        self.exact = self.exact and bool(robust) and not dp.any()
+        d = self._dist(run, ref)
+        ds = np.abs(np.asarray(run[2]) - np.asarray(ref[2])).max()
+        self.worst = np.maximum(self.worst, np.append(d, ds))
+        if self.exact:
+            self.exact_steps = k + 1
+            assert np.array_equal(np.asarray(run[0]), np.asarray(ref[0])), (k, d)
+            assert list(run[1]) == list(ref[1]), (k, run[1], ref[1])
+        else:
+            assert np.all(d <= 2 * self.env + np.array([4, 4, 4, 8])), (k, d, self.env)
+        assert ds <= max(1e-10, 10 * self.sigma_env), (k, ds, self.sigma_env)

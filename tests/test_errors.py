@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from helpers import make_integrator, sb_model
-from ttns_lib_amd.capi import OP_DENSE, Batch, Integrator, SOPHandle, Tree, TTNError, sb_problem
+from ttns_lib_amd.capi import MODE_SUBSPACE_EXPANSION, OP_DENSE, Batch, Integrator, SOPHandle, Tree, TTNError, sb_problem
 
 INVALID = 3
 
@@ -90,4 +90,13 @@ def test_gpu_device_limits(hip):
     with pytest.raises(TTNError) as e:
         batch.step(1)   # checked when the batch steps (instances may change between calls)
     assert e.value.code == INVALID and "dt" in str(e.value)
+    batch.destroy()
+    # an expansion instance whose bonds can still grow is stepped on its own
+    g = sb_model(4, 4, 2, nspf_cap=4, nspf_lower_cap=4)
+    a = make_integrator(hip, g, mode=MODE_SUBSPACE_EXPANSION)[2]
+    b = make_integrator(hip, g)[2]
+    batch = Batch(hip, [a, b])
+    with pytest.raises(TTNError) as e:
+        batch.step(1)
+    assert e.value.code == INVALID and "capacity" in str(e.value)
     batch.destroy()

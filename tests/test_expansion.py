@@ -22,7 +22,8 @@ integrator), restated in the oracle, pinned independently of its own code:
 import numpy as np
 import pytest
 
-from helpers import exact_sigma, make_integrator, sb_model, tree_children, ttns_vector
+from helpers import LOG_CAPACITY, RoundingYardstick, exact_sigma, fragile_decisions, make_integrator, perturb_nodes, \
+    sb_model, tree_children, ttns_vector
 from test_ref_outputs import ref_model, ref_runs
 from ttns_lib_amd.capi import MODE_FIXED_STEP, MODE_SUBSPACE_EXPANSION, SIGMA, Integrator, SpinBoson, sb_problem
 
@@ -143,9 +144,13 @@ def test_oracle_no_spawn_equals_fixed_step(orc):
     check_no_spawn_equals_fixed_step(orc)
 
 
-def _ref_expansion_integrator(lib, run):
+def _ref_expansion_integrator(lib, run, cap=None):
+    """The reference's run with expansion on; `cap` lowers the bond capacity (192 in these runs)
+    to what the device path holds, which changes nothing while every bond stays below it."""
     cfg = run["input"]
     m = ref_model(lib, run)
+    if cap is not None:
+        m.nspf_cap, m.nspf_lower_cap = min(m.nspf_cap, cap), min(m.nspf_lower_cap, cap)
     tree, sop = sb_problem(lib, m)
     ig = Integrator(lib, tree, sop, dt=cfg["dt"], krylov_dim=cfg["krylov dimension"],
                     krylov_tol=cfg["krylov tolerance"], mode=MODE_SUBSPACE_EXPANSION,
@@ -159,9 +164,9 @@ def _ref_expansion_integrator(lib, run):
 EXACT_GROWTH = [("ohmic_exponential", 1, [1e-10]), ("ohmic_sudden", 2, [1e-10, 1e-10])]
 
 
-def check_reference_growth(lib, name, steps, bounds):
+def check_reference_growth(lib, name, steps, bounds, cap=None):
     run = ref_runs()[name]
-    ig = _ref_expansion_integrator(lib, run)
+    ig = _ref_expansion_integrator(lib, run, cap)
     for r, bound in zip(run["rows"][1:1 + steps], bounds):
         ig.step(1)
         n1, n2 = ig.expansion_counters()
@@ -172,14 +177,15 @@ def check_reference_growth(lib, name, steps, bounds):
     return ig
 
 
+@pytest.mark.parametrize("cap", [None, 64])
 @pytest.mark.parametrize("name,steps,bounds", EXACT_GROWTH)
-def test_oracle_reproduces_reference_bond_growth(orc, name, steps, bounds):
-    check_reference_growth(orc, name, steps, bounds)
+def test_oracle_reproduces_reference_bond_growth(orc, name, steps, bounds, cap):
+    check_reference_growth(orc, name, steps, bounds, cap)
 
 
-def test_oracle_debye_growth_tracks_reference(orc):
+def check_debye_growth(lib, cap=None):
     run = ref_runs()["ohmic_debye"]
-    ig = _ref_expansion_integrator(orc, run)
+    ig = _ref_expansion_integrator(lib, run, cap)
     for r in run["rows"][1:4]:
         ig.step(1)
         hr, ref = np.array(ig.hranks()[1:]), np.array(r["hranks"])
@@ -189,3 +195,83 @@ def test_oracle_debye_growth_tracks_reference(orc):
         assert np.abs(hr - ref).sum() <= 2 * max(20, (hr != ref).sum())
         got = [ig.expect(0, SIGMA[q]).real for q in "xyz"]
         assert np.abs(np.array(got) - np.array(r["values"][1:4])).max() < 5e-8
+
+
+def test_oracle_debye_growth_tracks_reference(orc):
+    check_debye_growth(orc)
+
+
+def _state(ig):
+    return np.array(ig.hranks()), list(ig.expansion_counters()), np.array([ig.expect(0, SIGMA[q]).real for q in "xyz"])
+
+
+def side_by_side(lib, orc, model, steps, team_size=0, seeds=(7, 8, 9), **override):
+    """lib's expansion run against the oracle's, step by step, through helpers.RoundingYardstick
+    (three perturbed oracle runs; a step is robust when its spawning decisions have no fragile
+    values).  `override` changes lib's parameters only."""
+    args = dict(dt=0.05, mode=MODE_SUBSPACE_EXPANSION, spawning_threshold=1e-8, unoccupied_threshold=1e-4,
+                minimum_unoccupied=2)
+    g = make_integrator(lib, model, team_size=team_size, **{**args, **override})[2]
+    o = make_integrator(orc, model, **args)[2]
+    ps = [perturb_nodes(make_integrator(orc, model, **args)[2], s) for s in seeds]
+    ys = RoundingYardstick()
+    for k in range(steps):
+        for ig in [o] + ps:
+            ig.set_expansion_log(LOG_CAPACITY)
+        for ig in [g, o] + ps:
+            ig.step(1)
+        spawn = args["spawning_threshold"]
+        lo = o.expansion_log()
+        robust = fragile_decisions(lo, [p.expansion_log() for p in ps], spawn) == 0
+        ys.check(k, _state(g), _state(o), [_state(p) for p in ps], robust)
+        assert abs(g.norm2() - 1.0) < 1e-12
+    print(f"exact for {ys.exact_steps} of {steps} steps; largest differences (bonds, sum, Nonesite, Ntwosite, "
+          f"sigma) {ys.worst}, perturbed oracle {ys.env} {ys.sigma_env:.1e}")
+    return g, ys
+
+
+def _c1x_against_fixture(lib, nsteps, **override):
+    """Runs the single larger run with expansion on (128 modes, bond 8 -> 32) through
+    helpers.RoundingYardstick against tests/golden/c1x_expansion.npz."""
+    import os
+
+    from ttns_lib_amd.workloads import C1_EXPANSION, GOLDEN, c1_expansion_instance
+
+    fx = np.load(os.path.join(GOLDEN, "c1x_expansion.npz"))
+    inst = c1_expansion_instance()
+    tree, sop = sb_problem(lib, inst.model)
+    ig = Integrator(lib, tree, sop, dt=inst.dt, krylov_dim=inst.krylov_dim, krylov_tol=float(fx["krylov_tol"]),
+                    mode=MODE_SUBSPACE_EXPANSION, **{**C1_EXPANSION, **override})
+    ig.set_sb_initial_state(inst.model)
+    ys = RoundingYardstick()
+    for k in range(nsteps):
+        ig.step(1)
+        state = (np.array(ig.hranks()), list(ig.expansion_counters()),
+                 np.array([ig.expect(0, SIGMA[q]).real for q in "xyz"]))
+        ys.check(k, state, (fx["hranks"][k], fx["counters"][k], fx["sigma"][k]),
+                 [(fx["hranks_p"][p, k], fx["counters_p"][p, k], fx["sigma_p"][p, k]) for p in range(3)],
+                 fx["fragile"][k] == 0)
+    return ys, fx
+
+
+def test_oracle_reproduces_its_c1_expansion_fixture(orc):
+    # same library, same inputs: bit for bit over the first 12 steps (growth starts at step 10)
+    ys, fx = _c1x_against_fixture(orc, 12)
+    assert not ys.worst.any()
+    assert fx["hranks"][9].sum() > fx["hranks"][0].sum()
+    # the decisions of the first 10 steps (the first growth included) are robust to rounding,
+    # from step 11 on some are not
+    robust = fx["fragile"] == 0
+    assert robust[:10].all() and not robust[10:].any()
+
+
+def test_rounding_yardstick_flags_a_halved_spawning_value(orc):
+    # sqrt(S) dt / 4 in place of sqrt(S) dt / 2 in the spawning test adds fewer functions
+    with pytest.raises(AssertionError):
+        _c1x_against_fixture(orc, 12, spawning_threshold=2e-8)
+
+
+def test_rounding_yardstick_flags_a_doubled_spawning_threshold_on_a_small_tree(orc):
+    side_by_side(orc, orc, sb_model(8, 6, 2, nspf_cap=6, nspf_lower_cap=6), 8)   # itself: passes
+    with pytest.raises(AssertionError):
+        side_by_side(orc, orc, sb_model(8, 6, 2, nspf_cap=6, nspf_lower_cap=6), 8, spawning_threshold=2e-8)

@@ -97,6 +97,9 @@ _SIGS = {
     "set_dt": [vp, C.c_double],
     "get_dt": [vp, dblp],
     "reset_nsteps": [vp],
+    "expansion_draws": [vp, i64p],
+    "set_expansion_log": [vp, i64],
+    "expansion_log": [vp, i64p, dblp],
     "expect": [vp, i64, dblp, dblp],
     "norm2": [vp, dblp],
     "set_observable": [vp, i64, dblp],
@@ -428,6 +431,24 @@ class Integrator:
 
     def reset_nsteps(self):
         self.lib._call("reset_nsteps", self.h)
+
+    def set_expansion_log(self, capacity: int):
+        self.lib._call("set_expansion_log", self.h, int(capacity))
+
+    def expansion_log(self) -> np.ndarray:
+        """records (node, direction, Ritz values, functions added, spawning values sqrt(S) dt / 2
+        of the 16 largest Ritz values S)"""
+        n = i64(0)
+        self.lib._call("expansion_log", self.h, C.byref(n), None)
+        out = np.zeros(20 * n.value)
+        self.lib._call("expansion_log", self.h, C.byref(n), _dp(out))
+        return out.reshape(-1, 20)
+
+    def expansion_draws(self) -> int:
+        """normal draws the expansion's trial vectors have taken so far"""
+        n = i64()
+        self.lib._call("expansion_draws", self.h, C.byref(n))
+        return n.value
 
     def set_sb_initial_state(self, model: SpinBoson):
         p = model.to_c()

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <random>
 #include <string>
 
 #include "../../include/ttn_capi.h"
@@ -58,6 +59,18 @@ struct ttn_integ {
     DevBuf<double> scratch;
     DevBuf<int64_t> node_stats;
     DevBuf<int64_t> prof;
+    DevBuf<int> hr;                 // live bond dimension of every node
+    // subspace expansion: the normal stream of the trial vectors (std::mt19937 +
+    // std::normal_distribution, the generator the reference's two_site_variations holds),
+    // generated here and uploaded ahead of each launch; pending holds the draws not yet consumed
+    bool expand = false;
+    std::mt19937 rng;
+    std::normal_distribution<double> ndist{0.0, 1.0};
+    std::vector<double> pending;
+    int64_t normals_used = 0;
+    DevBuf<double> d_rng;
+    DevBuf<int64_t> d_exp;          // ExpWord words
+    DevBuf<double> d_exp_log;       // expansion log (ttn_set_expansion_log)
     FlopModel fm;
     int64_t sweep_pairs = 0;   // forward+backward sweeps run (with or without propagation)
     int64_t prepares = 0;
@@ -126,36 +139,35 @@ void build_instance(ttn_integ* in) {
     std::vector<ttnd::DNode> dn(n);
     int64_t maxsize = 1;
     in->node_A.assign(n, 0);
+    // storage at capacity: a bond grows inside a launch without moving anything
     for (int i = 0; i < n; ++i) {
         const Node& nd = t.nodes[i];
         check(nd.nmodes() <= ttnd::MAXC, ERR_INVALID, "device path supports at most 4 children per node");
-        check(nd.hrank <= 64, ERR_INVALID, "device path supports bond dimensions up to 64");
-        for (i64 d : nd.dims) check(d <= 64 || nd.is_leaf(), ERR_INVALID, "device path supports mode dims up to 64");
+        check(nd.cap_hrank <= 64, ERR_INVALID, "device path supports bond dimensions (and capacities) up to 64");
+        for (size_t c = 0; c < nd.dims.size(); ++c)
+            check(nd.cap_dims[c] <= 64 || nd.is_leaf(), ERR_INVALID, "device path supports mode dims up to 64");
+        check(nd.capacity() < (int64_t(1) << 31), ERR_INVALID, "node tensor too large for the device path");
         ttnd::DNode& d = dn[i];
         d.parent = nd.parent;
         d.child_id = nd.child_id;
         d.nchild = static_cast<int>(nd.children.size());
         d.leaf = nd.leaf_index;
         d.nmodes = nd.nmodes();
-        d.hrank = static_cast<int>(nd.hrank);
-        d.dimen = static_cast<int>(nd.dimen());
-        d.size = static_cast<int>(nd.size());
-        for (int c = 0; c < ttnd::MAXC; ++c) {
-            d.dims[c] = c < nd.nmodes() ? static_cast<int>(nd.dims[c]) : 1;
-            d.children[c] = c < static_cast<int>(nd.children.size()) ? nd.children[c] : -1;
-        }
-        d.A = alloc(nd.size());
+        d.leafdim = nd.is_leaf() ? static_cast<int>(nd.dims[0]) : 0;
+        d.cap_hrank = static_cast<int>(nd.cap_hrank);
+        for (int c = 0; c < ttnd::MAXC; ++c) d.children[c] = c < static_cast<int>(nd.children.size()) ? nd.children[c] : -1;
+        d.A = alloc(nd.capacity());
         in->node_A[i] = d.A;
-        int64_t rmax = nd.hrank;
+        int64_t rmax = nd.cap_hrank;
         if (!nd.is_leaf())
-            for (i64 x : nd.dims) rmax = std::max<int64_t>(rmax, x);
+            for (i64 x : nd.cap_dims) rmax = std::max<int64_t>(rmax, x);
         d.R = alloc(rmax * rmax);
-        maxsize = std::max<int64_t>(maxsize, nd.size());
+        maxsize = std::max<int64_t>(maxsize, nd.capacity());
     }
-    // environment matrices
+    // environment matrices (capacity hrank squared)
     std::vector<std::vector<int64_t>> spf_off(n), mf_off(n);
     for (int i = 0; i < n; ++i) {
-        const i64 r = t.nodes[i].hrank;
+        const i64 r = t.nodes[i].cap_hrank;
         const auto& terms = ops.nodes[i].terms;
         spf_off[i].assign(terms.size(), -1);
         mf_off[i].assign(terms.size(), -1);
@@ -220,6 +232,8 @@ void build_instance(ttn_integ* in) {
             }
             d.ire = ic.real();
             d.iim = ic.imag();
+            d.a0re = tm.accum.empty() ? 0.0 : tm.accum[0].real();
+            d.a0im = tm.accum.empty() ? 0.0 : tm.accum[0].imag();
             d.spf = spf_off[i][k];
             d.mf = mf_off[i][k];
             d.sub0 = static_cast<int>(subs.size());
@@ -239,7 +253,7 @@ void build_instance(ttn_integ* in) {
                         ttnd::DK dk{};
                         dk.nu = pr.first;
                         dk.kind = 0;
-                        dk.dim = static_cast<int>(nd.dims[pr.first]);
+                        dk.dim = static_cast<int>(nd.cap_dims[pr.first]);
                         dk.mat = spf_off[ch][pr.second];
                         check(dk.mat >= 0, ERR_INVALID, "spf chain references an identity term");
                         ks.push_back(dk);
@@ -264,7 +278,7 @@ void build_instance(ttn_integ* in) {
                     ttnd::DK dk{};
                     dk.nu = sb.first;
                     dk.kind = 0;
-                    dk.dim = static_cast<int>(t.nodes[p].dims[sb.first]);
+                    dk.dim = static_cast<int>(t.nodes[p].cap_dims[sb.first]);
                     dk.mat = spf_off[sib][sb.second];
                     check(dk.mat >= 0, ERR_INVALID, "mean-field chain references an identity term");
                     ks.push_back(dk);
@@ -273,6 +287,14 @@ void build_instance(ttn_integ* in) {
             }
             terms.push_back(d);
         }
+    }
+    // expansion: two-site factors and vectors
+    int nt2 = 0;
+    for (int i = 0; i < n; ++i) {
+        int k = 0;
+        for (const Term& tm : ops.nodes[i].terms)
+            if (!tm.id_spf && !tm.id_mf) ++k;
+        nt2 = std::max(nt2, k);
     }
     // observable slot and work buffers
     int64_t maxdim = 1;
@@ -286,6 +308,14 @@ void build_instance(ttn_integ* in) {
     // team scratch: partial sums (doubles, two buffers) and partial products per member
     I.team_part = alloc(ttnd::TEAM_MAX * ttnd::PART_K);
     I.team_tiles = alloc(static_cast<int64_t>(ttnd::TEAM_MAX) * ttnd::TILE_MAX);
+    in->expand = in->p.mode == TTN_MODE_SUBSPACE_EXPANSION;
+    if (in->expand) {
+        check(nt2 <= 8, ERR_INVALID, "device expansion supports at most 8 two-site terms per node");
+        I.nt2 = std::max(nt2, 1);
+        I.e_op1 = alloc(I.nt2 * maxsize);
+        I.e_op2 = alloc(I.nt2 * maxsize);
+        I.e_vec = alloc(2 * maxsize);
+    }
     in->arena_size = align16(off);
 
     // device allocations
@@ -317,6 +347,11 @@ void build_instance(ttn_integ* in) {
               "upload tables");
     in->counters.alloc(2);
     hip_check(hipMemsetAsync(in->counters.p, 0, 2 * sizeof(int64_t), in->ctx->stream), "memset counters");
+    // live bond dimensions
+    std::vector<int> hr(n);
+    for (int i = 0; i < n; ++i) hr[i] = static_cast<int>(t.nodes[i].hrank);
+    in->hr.alloc(n);
+    hip_check(hipMemcpyAsync(in->hr.p, hr.data(), n * sizeof(int), hipMemcpyHostToDevice, in->ctx->stream), "hranks");
     in->status.alloc(2);
     hip_check(hipMemsetAsync(in->status.p, 0, 2 * sizeof(int), in->ctx->stream), "memset status");
     in->scratch.alloc(4);
@@ -346,6 +381,20 @@ void build_instance(ttn_integ* in) {
     I.node_stats = in->node_stats.p;
     I.prof = in->prof.p;
     I.status = in->status.p;
+    I.hr = in->hr.p;
+    in->d_exp.alloc(ttnd::EXP_NWORDS);
+    hip_check(hipMemsetAsync(in->d_exp.p, 0, ttnd::EXP_NWORDS * sizeof(int64_t), in->ctx->stream), "memset exp");
+    I.exp = in->d_exp.p;
+    I.expand = in->expand ? 1 : 0;
+    I.neig = static_cast<int>(in->p.neigenvalues);
+    I.min_unocc = static_cast<int>(in->p.minimum_unoccupied);
+    I.only_unocc = in->p.only_apply_when_no_unoccupied != 0 ? 1 : 0;
+    I.spawn = in->p.spawning_threshold;
+    I.unocc_thr = in->p.unoccupied_threshold;
+    I.eig_tol = in->p.eigenvalue_tolerance > 0.0 ? in->p.eigenvalue_tolerance : 1e-6;
+    in->rng.seed(in->p.rng_seed != 0 ? in->p.rng_seed : 5489u);
+    I.rng = nullptr;
+    I.rng_n = 0;
     in->d_inst.alloc(1);
     hip_check(hipMemcpyAsync(in->d_inst.p, &I, sizeof(I), hipMemcpyHostToDevice, in->ctx->stream), "upload inst");
     hip_check(hipStreamSynchronize(in->ctx->stream), "sync");
@@ -372,6 +421,7 @@ void make_args(const ttn_integ* in, double cr, double ci, int64_t nsteps, ttnd::
     a.small_max = in->ctx->small_max;
     if (in->nsteps == 0) {
         a.prepare = 1;
+        a.prepare_spf = 1;
         a.nladder = ladder(in->p.dt, a.ladder_dt);
         a.nregular = static_cast<int>(nsteps - 1);
     } else {
@@ -405,13 +455,18 @@ std::string failure_message(const int st[2]) {
                                     "a Krylov basis vector has a non-finite norm",
                                     "no Krylov sub-step met the tolerance",
                                     "a team barrier timed out",
-                                    "the team's workgroups did not assemble"};
+                                    "the team's workgroups did not assemble",
+                                    "the expansion needed more random draws than were uploaded",
+                                    "the expansion found no orthogonal trial vector or a non-finite value"};
     const int reason = st[0] & 15, bond = (st[0] >> 4) & 1, node = st[0] >> 5;
     if (reason == ttnd::FAIL_TEAM_TIMEOUT || reason == ttnd::FAIL_TEAM_FORM)
         return std::string("TDVP sweep failed at step ") + std::to_string(st[1]) + ": " + reasons[reason];
+    if (reason == ttnd::FAIL_RNG_EXHAUSTED || reason == ttnd::FAIL_EXPANSION)
+        return std::string("TDVP sweep failed at step ") + std::to_string(st[1]) + ", bond above node " +
+               std::to_string(node) + ": " + reasons[reason];
     std::string m = "TDVP sweep failed at step " + std::to_string(st[1]) + ", " +
                     (bond ? "bond propagation above node " : "node propagation of node ") + std::to_string(node) +
-                    ": " + (reason < 6 ? reasons[reason] : "unknown reason");
+                    ": " + (reason < 8 ? reasons[reason] : "unknown reason");
     return m;
 }
 
@@ -425,6 +480,68 @@ void finish(ttn_integ* in, int64_t nsteps) {
     }
     if (st[0] != 0) fail(ERR_NUMERIC, failure_message(st));
     in->nsteps += nsteps;
+}
+
+// ---------------------------------------------------------------- expansion launches
+// Upper bound on the normal draws one sweep pair of the expansion integrator can take: a bond
+// below its capacity makes at most one downward and one upward attempt per sweep pair (at
+// capacity an attempt returns before drawing), each trial vector takes two draws per entry of
+// the complement (sized here at capacity), a two-site attempt draws one vector and a random
+// one-site attempt up to minimum_unoccupied, and redrawn vectors are covered by a factor 2.
+int64_t rng_bound(const ttn_integ* in) {
+    const Topology& t = in->topo;
+    int64_t tot = 0;
+    const int64_t vecs = 1 + std::max<int64_t>(in->p.minimum_unoccupied, 0);
+    for (int n = 1; n < t.size(); ++n) {
+        const Node& nd = t.nodes[n];
+        if (nd.hrank >= nd.cap_hrank) continue;
+        const Node& pn = t.nodes[nd.parent];
+        const int64_t m_down = pn.capacity() / pn.cap_dims[nd.child_id];
+        const int64_t m_up = nd.cap_dimen();
+        tot += 2 * 2 * (m_down + m_up) * vecs;
+    }
+    return tot;
+}
+
+// Makes at least `need` normal draws available to the next launch: the draws the last launch
+// consumed are dropped, the stream is continued on the host and the pending draws uploaded.
+void top_up_rng(ttn_integ* in, int64_t need) {
+    int64_t pos = 0;
+    hip_check(hipMemcpy(&pos, in->d_exp.p + ttnd::EXP_RNG_POS, sizeof(pos), hipMemcpyDeviceToHost), "rng cursor");
+    check(pos >= 0 && pos <= static_cast<int64_t>(in->pending.size()), ERR_DEVICE, "random stream cursor out of range");
+    in->pending.erase(in->pending.begin(), in->pending.begin() + pos);
+    in->normals_used += pos;
+    while (static_cast<int64_t>(in->pending.size()) < need) in->pending.push_back(in->ndist(in->rng));
+    const int64_t n = static_cast<int64_t>(in->pending.size());
+    if (static_cast<int64_t>(in->d_rng.n) < n) in->d_rng.alloc(static_cast<size_t>(std::max<int64_t>(n, 1)));
+    if (n > 0)
+        hip_check(hipMemcpy(in->d_rng.p, in->pending.data(), n * sizeof(double), hipMemcpyHostToDevice), "rng");
+    const int64_t zero = 0;
+    hip_check(hipMemcpy(in->d_exp.p + ttnd::EXP_RNG_POS, &zero, sizeof(zero), hipMemcpyHostToDevice), "rng cursor");
+    in->inst.rng = in->d_rng.p;
+    in->inst.rng_n = n;
+}
+
+// Live bond dimensions and expansion counters of the last launch into the host copy of the tree
+// (the node shapes ttn_state_get/set and the flop model use).  Returns the device's grew flag.
+bool sync_shapes(ttn_integ* in) {
+    const int n = in->topo.size();
+    std::vector<int> hr(n);
+    hip_check(hipMemcpy(hr.data(), in->hr.p, n * sizeof(int), hipMemcpyDeviceToHost), "hranks");
+    bool changed = false;
+    for (int i = 0; i < n; ++i) {
+        Node& nd = in->topo.nodes[i];
+        if (nd.hrank == hr[i]) continue;
+        nd.hrank = hr[i];
+        if (nd.parent >= 0) in->topo.nodes[nd.parent].dims[nd.child_id] = hr[i];
+        changed = true;
+    }
+    if (changed) in->fm = flop_model(in->topo, in->ops, in->sop);
+    int64_t w[ttnd::EXP_NWORDS];
+    hip_check(hipMemcpy(w, in->d_exp.p, sizeof(w), hipMemcpyDeviceToHost), "expansion words");
+    in->n_onesite = w[ttnd::EXP_ONESITE];
+    in->n_twosite = w[ttnd::EXP_TWOSITE];
+    return w[ttnd::EXP_GREW] != 0;
 }
 
 // ---------------------------------------------------------------- teams
@@ -791,13 +908,9 @@ int ttn_integrator_create(ttn_ctx* ctx, const ttn_tree* t, const ttn_sop* s, con
         check(p != nullptr, ERR_INVALID, "null parameters");
         check(p->mode == TTN_MODE_FIXED_STEP || p->mode == TTN_MODE_SUBSPACE_EXPANSION, ERR_INVALID,
               "unknown integrator mode");
-        // With every bond at capacity the expansion integrator never expands and its step is
-        // the fixed-step one (same prepare pass, ladder and sweeps); expansion into bonds below
-        // capacity is not built.
-        check(p->mode == TTN_MODE_FIXED_STEP || t->topo.at_capacity(), ERR_INVALID,
-              "subspace expansion into bonds below their capacity is not available; the expansion "
-              "integrator is accepted on trees whose bonds are all at capacity");
         check(p->dt > 0.0, ERR_INVALID, "dt must be positive");
+        check(p->mode == TTN_MODE_FIXED_STEP || p->neigenvalues >= 1, ERR_INVALID, "neigenvalues must be positive");
+        check(p->minimum_unoccupied >= 0, ERR_INVALID, "minimum_unoccupied must not be negative");
         auto in = std::make_unique<ttn_integ>();
         in->ctx = ctx;
         in->topo = t->topo;
@@ -855,9 +968,83 @@ int ttn_trajectory(const ttn_integ* in, int64_t* len, double* out) {
     return TTN_OK;
 }
 
+// The expansion integrator's steps: prepare_evolution one sweep pair per launch until a pair
+// adds nothing, the first-step ladder, then the regular steps in launches short enough for the
+// normal draws uploaded ahead of them (one launch once every bond is at capacity).
+void step_expanding(ttn_integ* in, double cr, double ci, int64_t nsteps) {
+    constexpr int64_t RNG_BUDGET = int64_t(1) << 23;  // draws per launch (64 MiB)
+    prepare_traj(in, nsteps);
+    hipStream_t s = in->ctx->stream;
+    int64_t done = 0;
+    double ms_total = 0.0;
+    int st[2] = {0, 0};
+    auto run = [&](ttnd::RunArgs& a, int64_t pairs) {
+        top_up_rng(in, pairs * rng_bound(in));
+        hip_check(hipMemcpyAsync(in->d_inst.p, &in->inst, sizeof(in->inst), hipMemcpyHostToDevice, s), "inst");
+        launch(in->ctx, in->d_inst.p, {in->inst}, a, {auto_team(in)});
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, in->ctx->ev0, in->ctx->ev1), "elapsed");
+        ms_total += ms;
+        account(in, a);
+        const bool grew = sync_shapes(in);
+        hip_check(hipMemcpy(st, in->status.p, sizeof(st), hipMemcpyDeviceToHost), "status");
+        return grew;
+    };
+    ttnd::RunArgs a;
+    make_args(in, cr, ci, 1, a);
+    if (in->nsteps == 0) {
+        for (int it = 0; st[0] == 0; ++it) {
+            check(it < 1000000, ERR_NUMERIC, "prepare_evolution: the expansion did not settle");
+            ttnd::RunArgs b = a;
+            b.prepare = 1;
+            b.prepare_spf = it == 0 ? 1 : 0;
+            b.nladder = 0;
+            b.nregular = 0;
+            if (!run(b, 1)) break;
+        }
+        if (st[0] == 0) {
+            ttnd::RunArgs b = a;
+            b.prepare = 0;
+            b.prepare_spf = 0;
+            b.nregular = 0;
+            run(b, b.nladder);
+            if (st[0] == 0) done = 1;
+        }
+    }
+    while (st[0] == 0 && done < nsteps) {
+        const int64_t bound = rng_bound(in);
+        const int64_t chunk =
+            bound == 0 ? nsteps - done : std::max<int64_t>(1, std::min<int64_t>(RNG_BUDGET / bound, nsteps - done));
+        ttnd::RunArgs b = a;
+        b.prepare = 0;
+        b.prepare_spf = 0;
+        b.nladder = 0;
+        b.nregular = static_cast<int>(chunk);
+        b.step0 = done;
+        run(b, chunk);
+        if (st[0] == 0) done += chunk;
+    }
+    in->last_ms = ms_total;
+    if (st[0] != 0) {
+        // the steps completed before the failure are kept; the failing one is not
+        if (in->obs_leaf >= 0 && done > 0) {
+            std::vector<double> buf(2 * done);
+            hip_check(hipMemcpy(buf.data(), in->traj.p, buf.size() * sizeof(double), hipMemcpyDeviceToHost), "traj");
+            in->trajectory.insert(in->trajectory.end(), buf.begin(), buf.end());
+        }
+        in->nsteps += done;
+        fail(ERR_NUMERIC, failure_message(st));
+    }
+    finish(in, nsteps);
+}
+
 int ttn_step(ttn_integ* in, double cr, double ci, int64_t nsteps) {
     return guard(in->ctx, [&] {
         if (nsteps <= 0) return;
+        if (in->expand) {
+            step_expanding(in, cr, ci, nsteps);
+            return;
+        }
         ttnd::RunArgs a;
         make_args(in, cr, ci, nsteps, a);
         prepare_traj(in, nsteps);
@@ -910,6 +1097,40 @@ int ttn_set_dt(ttn_integ* in, double dt) {
 int ttn_get_dt(const ttn_integ* in, double* dt) {
     *dt = in->p.dt;
     return TTN_OK;
+}
+int ttn_set_expansion_log(ttn_integ* in, int64_t capacity) {
+    return guard(in->ctx, [&] {
+        check(capacity >= 0, ERR_INVALID, "log capacity must not be negative");
+        if (capacity == 0) {
+            in->d_exp_log.release();
+            in->inst.exp_log = nullptr;
+        } else {
+            in->d_exp_log.alloc(static_cast<size_t>(capacity * ttnd::EXP_LOG_REC));
+            in->inst.exp_log = in->d_exp_log.p;
+        }
+        in->inst.exp_log_cap = capacity;
+        const int64_t zero = 0;
+        hip_check(hipMemcpy(in->d_exp.p + ttnd::EXP_LOG_N, &zero, sizeof(zero), hipMemcpyHostToDevice), "log");
+    });
+}
+int ttn_expansion_log(const ttn_integ* in, int64_t* len, double* out) {
+    return guard(in->ctx, [&] {
+        int64_t n = 0;
+        hip_check(hipMemcpy(&n, in->d_exp.p + ttnd::EXP_LOG_N, sizeof(n), hipMemcpyDeviceToHost), "log");
+        n = std::min(n, in->inst.exp_log_cap);
+        if (out && n > 0)
+            hip_check(hipMemcpy(out, in->d_exp_log.p, std::min(n, *len) * ttnd::EXP_LOG_REC * sizeof(double),
+                                hipMemcpyDeviceToHost),
+                      "log");
+        *len = n;
+    });
+}
+int ttn_expansion_draws(const ttn_integ* in, int64_t* n) {
+    return guard(in->ctx, [&] {
+        int64_t pos = 0;
+        hip_check(hipMemcpy(&pos, in->d_exp.p + ttnd::EXP_RNG_POS, sizeof(pos), hipMemcpyDeviceToHost), "rng cursor");
+        *n = in->normals_used + pos;
+    });
 }
 int ttn_reset_nsteps(ttn_integ* in) {
     in->nsteps = 0;
@@ -1007,6 +1228,11 @@ int ttn_batch_step(ttn_batch* b, double cr, double ci, int64_t nsteps) {
         for (ttn_integ* in : b->integs) {
             check(in->nsteps == first->nsteps, ERR_INVALID, "batched instances must be at the same step");
             check(in->p.dt == first->p.dt, ERR_INVALID, "batched instances must share dt");
+            // growing bonds need the per-instance prepare loop and normal-stream top-ups of
+            // ttn_step; at capacity the expansion adds nothing and the step is the fixed one
+            check(!in->expand || rng_bound(in) == 0, ERR_INVALID,
+                  "batched stepping takes expansion-mode instances only once every bond is at capacity; "
+                  "step growing instances with ttn_step");
         }
         ttnd::RunArgs a;
         make_args(first, cr, ci, nsteps, a);

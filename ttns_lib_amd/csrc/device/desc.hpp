@@ -24,16 +24,28 @@ enum ProfSlot { PROF_HEFF = 0, PROF_HBOND, PROF_ORTH, PROF_EXPM, PROF_QR, PROF_S
                 PROF_NSMALL,   // number of those propagations (a count, not ticks)
                 PROF_NSLOTS = 16 };
 
+// Static part of a node.  Its shape is not stored here: bond dimensions grow under the
+// expansion integrator, so the sweep reads them from the instance's live bond-dimension
+// table (DInst::hr, mirrored in LDS) and derives the mode dimensions from the children's.
+// Storage (A, R and the environment matrices) is laid out at capacity.
 struct DNode {
     int parent, child_id, nchild, leaf;  // leaf = primitive mode index, -1 for branch nodes
-    int nmodes, hrank, dimen, size;      // size = dimen * hrank
-    int dims[MAXC];
+    int nmodes, leafdim, cap_hrank, pad;  // leafdim: primitive dimension of a leaf node
     int children[MAXC];
-    int64_t A, R;                         // arena offsets: node tensor, gauge matrix (rmax^2)
+    int64_t A, R;                         // arena offsets: node tensor (capacity), gauge matrix (rmax^2)
     int term0, nterms;                    // slice of DTerm
 };
 
-// operator_term (reference ttn_nodes/operator_node.hpp:78-229)
+// Live shape of a node: bond dimension, mode dimensions (the children's bond dimensions, or
+// the primitive dimension of a leaf), their product and the matrix size dimen x hrank.
+struct Shape {
+    int hrank, nmodes;
+    int dims[MAXC];
+    int dimen, size;
+};
+
+// operator_term (reference ttn_nodes/operator_node.hpp:78-229); spf and mf matrices are
+// hrank x hrank with the node's live hrank as their row stride, in slots sized for capacity
 struct DTerm {
     int id_spf, id_mf;
     double cre, cim;       // coeff()
@@ -41,6 +53,8 @@ struct DTerm {
     int sub0, nsub;        // spf sub-terms (DSub)
     int mf0, nmf;          // mean-field contributions (DMf)
     int64_t spf, mf;       // arena offsets of the hrank x hrank matrices, -1 for identity
+    double a0re, a0im;     // accum_coeff(0): the accumulation coefficient a single sub-term or
+                           // mean-field contribution carries in the two-site energy
 };
 
 // one spf sub-term: accumulation coefficient and the chain of mode operators
@@ -49,10 +63,10 @@ struct DSub {
     int k0, nk;            // slice of DK
 };
 
-// one operator of a chain.  Branch nodes: nu = child slot, mat = child's spf matrix
-// (dim x dim, kind = DENSE).  Leaf nodes: nu = primitive mode, kind/mat/idx/nnz describe
-// the primitive operator (dense d x d, diagonal d, or CSR values + (rowptr, colind) in the
-// integer table).
+// one operator of a chain.  Branch nodes: nu = child slot, mat = child's spf matrix (the
+// child's live hrank square, kind = DENSE; dim is not used).  Leaf nodes: nu = primitive mode,
+// kind/dim/mat/idx/nnz describe the primitive operator (dense d x d, diagonal d, or CSR values
+// + (rowptr, colind) in the integer table).
 struct DK {
     int nu, kind, dim, nnz;
     int64_t mat, idx;
@@ -74,7 +88,15 @@ enum FailReason {
     FAIL_KRYLOV_STEP = 3,       // no sub-step size met the Krylov tolerance
     FAIL_TEAM_TIMEOUT = 4,      // a team barrier did not complete in time
     FAIL_TEAM_FORM = 5,         // the team's workgroups did not assemble (state untouched)
+    FAIL_RNG_EXHAUSTED = 6,     // the expansion needed more normal draws than were uploaded
+    FAIL_EXPANSION = 7,         // an expansion step found no orthogonal trial vector or a non-finite value
 };
+
+// Expansion state of an instance in global memory (int64 words)
+enum ExpWord { EXP_RNG_POS = 0, EXP_ONESITE, EXP_TWOSITE, EXP_GREW, EXP_LOG_N, EXP_NWORDS = 8 };
+// one record of the expansion log: node, direction (0 down, 1 up), Ritz values kept, functions
+// added, then the Ritz values (descending, MAXK slots)
+constexpr int EXP_LOG_REC = 4 + MAXK;
 
 struct DInst {
     double2* base;                  // arena
@@ -102,6 +124,19 @@ struct DInst {
     int64_t* prof;                 // per-phase device wall-clock ticks (PROF_* slots), or null
     int64_t team_part;             // team partial sums (2 x TEAM_MAX x PART_K doubles)
     int64_t team_tiles;            // team partial products (TEAM_MAX x TILE_MAX)
+    int* hr;                       // live bond dimension of every node (written by expansions)
+    // subspace expansion (subspace_expansion_projector_splitting_engine.hpp:207-510); expand = 0
+    // for the fixed-step integrator
+    int expand, neig, min_unocc, only_unocc;
+    double spawn, unocc_thr, eig_tol;
+    const double* rng;             // normal draws of std::mt19937 + std::normal_distribution, in order
+    int64_t rng_n;                 // draws uploaded
+    int64_t* exp;                  // ExpWord words
+    int64_t e_op1, e_op2;          // arena: two-site factors, nt2 slots of maxsize each
+    int64_t e_vec;                 // arena: trial vector, restart vector and F temporaries (4 x maxsize)
+    int nt2, pad2;                 // largest number of two-site terms of a node
+    double* exp_log;               // expansion log (EXP_LOG_REC doubles per two-site attempt) or null
+    int64_t exp_log_cap;           // records it holds
 };
 
 // ---------------------------------------------------------------- teams
@@ -139,7 +174,8 @@ struct TeamPlan {
 
 struct RunArgs {
     double2 coeff;                 // -i for real time
-    int prepare;                   // run prepare_evolution first
+    int prepare;                   // prepare_evolution sweep pairs to run first (0: none)
+    int prepare_spf;               // run its bottom-up single-particle pass first
     int nladder;                   // ladder sub-steps (first step)
     double ladder_dt[8];
     int nregular;                  // regular steps after the ladder

@@ -28,6 +28,10 @@ struct SmallLds {
 __shared__ SmallLds g_small;
 constexpr int LDS_STAGE = (5 * SMALL_N + SMALL_OPS) / 2;  // entries per staged operand
 
+// Live bond dimension of every node of the workgroup's instance, loaded from DInst::hr when the
+// kernel starts and updated by every member after an expansion grows a bond.
+__shared__ unsigned char g_hr[MAX_NODES];
+
 struct Ctx {
     const DInst& I;
     Team* T;
@@ -41,6 +45,27 @@ struct Ctx {
         : I(inst), T(t), base(inst.base), team_min(tmin) {}
     __device__ double2* W(int k) const { return base + I.work[k]; }
     __device__ const DNode& node(int n) const { return I.nodes[n]; }
+    __device__ int hr(int n) const { return g_hr[n]; }
+    // live shape of node n (bond dimensions from the LDS table)
+    __device__ Shape shape(int n) const {
+        const DNode& nd = I.nodes[n];
+        Shape s;
+        s.hrank = g_hr[n];
+        s.nmodes = nd.nmodes;
+        if (nd.leaf >= 0) {
+            s.dims[0] = nd.leafdim;
+            for (int q = 1; q < MAXC; ++q) s.dims[q] = 1;
+            s.dimen = nd.leafdim;
+        } else {
+            s.dimen = 1;
+            for (int q = 0; q < MAXC; ++q) {
+                s.dims[q] = q < nd.nmodes ? static_cast<int>(g_hr[nd.children[q]]) : 1;
+                s.dimen *= s.dims[q];
+            }
+        }
+        s.size = s.dimen * s.hrank;
+        return s;
+    }
     // failures are detected on team-uniform values, so every thread of every member takes
     // the same branch; the LDS flag is read only after a barrier
     __device__ bool failed() const { return g_sh.fail != 0; }
@@ -83,18 +108,18 @@ __device__ __forceinline__ void prof_add(const Ctx& c, int slot, uint64_t t0) {
     }
 
 // (L, R) of the rank-3 view of mode `mode` of a node
-__device__ __forceinline__ void mode_view(const DNode& nd, int mode, int64_t& L, int64_t& R) {
+__device__ __forceinline__ void mode_view(const Shape& sh, int mode, int64_t& L, int64_t& R) {
     L = 1;
-    R = nd.hrank;
-    for (int c = 0; c < mode; ++c) L *= nd.dims[c];
-    for (int c = mode + 1; c < nd.nmodes; ++c) R *= nd.dims[c];
+    R = sh.hrank;
+    for (int c = 0; c < mode; ++c) L *= sh.dims[c];
+    for (int c = mode + 1; c < sh.nmodes; ++c) R *= sh.dims[c];
 }
 
 // kronecker_product_operator::apply (kronecker_product_operator_helper.hpp:261-315):
 // chain of mode contractions with spf matrices; the last contraction lands in out.
-__device__ void kpo(Ctx& c, const DNode& nd, int k0, int nk, const double2* v, double2* out, double2* tmp) {
+__device__ void kpo(Ctx& c, const Shape& sh, int k0, int nk, const double2* v, double2* out, double2* tmp) {
     if (nk == 0) {
-        t_copy(*c.T, out, v, nd.size);
+        t_copy(*c.T, out, v, sh.size);
         return;
     }
     const double2* src = v;
@@ -102,8 +127,8 @@ __device__ void kpo(Ctx& c, const DNode& nd, int k0, int nk, const double2* v, d
         double2* dst = ((nk - 1 - q) % 2 == 0) ? out : tmp;
         const DK& k = c.I.ks[k0 + q];
         int64_t L, R;
-        mode_view(nd, k.nu, L, R);
-        t_mode_contract(*c.T, c.base + k.mat, L, nd.dims[k.nu], R, src, dst, cmk(1.0, 0.0), false);
+        mode_view(sh, k.nu, L, R);
+        t_mode_contract(*c.T, c.base + k.mat, L, sh.dims[k.nu], R, src, dst, cmk(1.0, 0.0), false);
         src = dst;
     }
 }
@@ -111,17 +136,17 @@ __device__ void kpo(Ctx& c, const DNode& nd, int k0, int nk, const double2* v, d
 // coefficient_evolution_{leaf,branch} (projector_splitting_evolution_functions.hpp:65-258)
 // out (=|+=) alpha * K v for one spf sub-term: a leaf's primitive operator or a chain of
 // mode contractions (intermediates in t1/t2, the last one lands in out with the epilogue)
-__device__ void subterm_into(Ctx& c, const DNode& nd, const DSub& s, const double2* v, double2* out, double2 alpha,
-                             bool accumulate, double2* t1, double2* t2) {
+__device__ void subterm_into(Ctx& c, const DNode& nd, const Shape& sh, const DSub& s, const double2* v, double2* out,
+                             double2 alpha, bool accumulate, double2* t1, double2* t2) {
     if (s.nk == 0) {
         if (accumulate)
-            t_axpy(*c.T, out, alpha, v, nd.size);
+            t_axpy(*c.T, out, alpha, v, sh.size);
         else
-            t_scale_to(*c.T, out, alpha, v, nd.size);
+            t_scale_to(*c.T, out, alpha, v, sh.size);
         return;
     }
     if (nd.leaf >= 0) {
-        t_prim_apply(*c.T, c.I.ks[s.k0], c.base, c.I.ints, v, out, nd.hrank, alpha, accumulate);
+        t_prim_apply(*c.T, c.I.ks[s.k0], c.base, c.I.ints, v, out, sh.hrank, alpha, accumulate);
         return;
     }
     const double2* src = v;
@@ -130,8 +155,8 @@ __device__ void subterm_into(Ctx& c, const DNode& nd, const DSub& s, const doubl
         double2* dst = last ? out : (q % 2 == 0 ? t1 : t2);
         const DK& k = c.I.ks[s.k0 + q];
         int64_t L, R;
-        mode_view(nd, k.nu, L, R);
-        t_mode_contract(*c.T, c.base + k.mat, L, nd.dims[k.nu], R, src, dst, last ? alpha : cmk(1.0, 0.0),
+        mode_view(sh, k.nu, L, R);
+        t_mode_contract(*c.T, c.base + k.mat, L, sh.dims[k.nu], R, src, dst, last ? alpha : cmk(1.0, 0.0),
                         last && accumulate);
         src = dst;
     }
@@ -141,8 +166,9 @@ __device__ void subterm_into(Ctx& c, const DNode& nd, const DSub& s, const doubl
 // accumulation and coefficients are folded into the GEMM epilogues.  Uses W1..W3.
 __device__ void heff_node(Ctx& c, int n, const double2* v, double2* res) {
     const DNode& nd = c.node(n);
-    const int r = nd.hrank;
-    const int64_t D = nd.dimen;
+    const Shape sh = c.shape(n);
+    const int r = sh.hrank;
+    const int64_t D = sh.dimen;
     double2 *t1 = c.W(1), *t2 = c.W(2), *t3 = c.W(3);
     bool first = true;  // res not written yet
     for (int ti = 0; ti < nd.nterms; ++ti) {
@@ -151,9 +177,9 @@ __device__ void heff_node(Ctx& c, int n, const double2* v, double2* res) {
             const double2 coeff = cmk(tm.ire, tm.iim);
             if (tm.id_mf) {
                 if (first)
-                    t_scale_to(*c.T, res, coeff, v, nd.size);
+                    t_scale_to(*c.T, res, coeff, v, sh.size);
                 else
-                    t_axpy(*c.T, res, coeff, v, nd.size);
+                    t_axpy(*c.T, res, coeff, v, sh.size);
             } else {
                 t_mul_transposed(*c.T, v, D, r, c.base + tm.mf, res, coeff, !first);
             }
@@ -164,10 +190,10 @@ __device__ void heff_node(Ctx& c, int n, const double2* v, double2* res) {
             const DSub& s = c.I.subs[tm.sub0 + i];
             const double2 ci = (i == 0) ? (tm.nsub > 1 ? cmk(s.are, s.aim) : cmk(tm.cre, tm.cim)) : cmk(s.are, s.aim);
             if (tm.id_mf) {
-                subterm_into(c, nd, s, v, res, ci, !first, t1, t2);
+                subterm_into(c, nd, sh, s, v, res, ci, !first, t1, t2);
                 first = false;
             } else {
-                subterm_into(c, nd, s, v, t3, ci, i > 0, t1, t2);
+                subterm_into(c, nd, sh, s, v, t3, ci, i > 0, t1, t2);
             }
         }
         if (!tm.id_mf) {
@@ -175,7 +201,7 @@ __device__ void heff_node(Ctx& c, int n, const double2* v, double2* res) {
             first = false;
         }
     }
-    if (first) t_zero(*c.T, res, nd.size);
+    if (first) t_zero(*c.T, res, sh.size);
 }
 
 // dst (=|+=) alpha * P Q on one workgroup, P and Q r x r matrices copied into LDS first, with
@@ -213,7 +239,7 @@ __device__ void wg_square_product_lds(const double2* P, const double2* Q, int r,
 // full_hamiltonian_action (projector_splitting_evolution_functions.hpp:7-62); uses W1, W2
 __device__ void heff_bond(Ctx& c, int n, const double2* v, double2* res) {
     const DNode& nd = c.node(n);
-    const int r = nd.hrank;
+    const int r = c.hr(n);
     const int64_t sz = static_cast<int64_t>(r) * r;
     double2* t1 = c.W(1);
     t_zero(*c.T, res, sz);
@@ -252,8 +278,9 @@ __device__ void heff_bond(Ctx& c, int n, const double2* v, double2* res) {
 // single_particle_operator_engine::evaluate (single_particle_operator.hpp:26-103); uses W1, W2
 __device__ void update_spf_body(Ctx& c, int n) {
     const DNode& nd = c.node(n);
-    const int r = nd.hrank;
-    const int64_t D = nd.dimen;
+    const Shape sh = c.shape(n);
+    const int r = sh.hrank;
+    const int64_t D = sh.dimen;
     const double2* a = c.base + nd.A;
     double2 *HA = c.W(1), *tmp = c.W(2);
     for (int ti = 0; ti < nd.nterms; ++ti) {
@@ -265,7 +292,7 @@ __device__ void update_spf_body(Ctx& c, int n) {
             if (nd.leaf >= 0 && s.nk > 0)
                 t_prim_apply(*c.T, c.I.ks[s.k0], c.base, c.I.ints, a, HA, r);
             else
-                kpo(c, nd, s.k0, s.nk, a, HA, tmp);
+                kpo(c, sh, s.k0, s.nk, a, HA, tmp);
             if (tm.nsub == 1)
                 t_adjoint_mul(*c.T, a, HA, D, r, out, cmk(1.0, 0.0), false);
             else
@@ -274,7 +301,7 @@ __device__ void update_spf_body(Ctx& c, int n) {
     }
 }
 __device__ void update_spf(Ctx& c, int n) {
-    on_team(c, c.node(n).size, [&](Ctx& cc) { update_spf_body(cc, n); });
+    on_team(c, c.shape(n).size, [&](Ctx& cc) { update_spf_body(cc, n); });
 }
 
 // mean_field_operator_engine::evaluate (mean_field_operator.hpp:37-135) for child n;
@@ -282,12 +309,13 @@ __device__ void update_spf(Ctx& c, int n) {
 __device__ void update_mf_body(Ctx& c, int n) {
     const DNode& nd = c.node(n);
     const DNode& pn = c.node(nd.parent);
+    const Shape ps = c.shape(nd.parent);
     const int mode = nd.child_id;
     int64_t L, R;
-    mode_view(pn, mode, L, R);
-    const int d = pn.dims[mode];
-    const int64_t pD = pn.dimen;
-    const int pr = pn.hrank;
+    mode_view(ps, mode, L, R);
+    const int d = ps.dims[mode];
+    const int64_t pD = ps.dimen;
+    const int pr = ps.hrank;
     const double2* Ap = c.base + pn.A;
     double2 *t1 = c.W(1), *t2 = c.W(2), *t3 = c.W(3);
     for (int ti = 0; ti < nd.nterms; ++ti) {
@@ -298,7 +326,7 @@ __device__ void update_mf_body(Ctx& c, int n) {
             const DMf& m = c.I.mfs[tm.mf0 + it];
             const double2* HA = Ap;
             if (m.nk > 0) {
-                kpo(c, pn, m.k0, m.nk, Ap, t1, t2);
+                kpo(c, ps, m.k0, m.nk, Ap, t1, t2);
                 HA = t1;
             }
             if (m.pmf >= 0) {
@@ -315,7 +343,7 @@ __device__ void update_mf_body(Ctx& c, int n) {
 __device__ void update_mf(Ctx& c, int n) {
     const DNode& nd = c.node(n);
     if (nd.parent < 0) return;
-    on_team(c, c.node(nd.parent).size, [&](Ctx& cc) { update_mf_body(cc, n); });
+    on_team(c, c.shape(nd.parent).size, [&](Ctx& cc) { update_mf_body(cc, n); });
 }
 
 // ---------------------------------------------------------------- gauge moves
@@ -323,11 +351,12 @@ __device__ void update_mf(Ctx& c, int n) {
 __device__ void r2l_body(Ctx& c, int n, int mode) {
     Team& T = *c.T;
     const DNode& nd = c.node(n);
-    const int64_t sz = nd.size;
+    const Shape sh = c.shape(n);
+    const int64_t sz = sh.size;
     int64_t id1 = 1;
-    for (int q = 0; q <= mode; ++q) id1 *= nd.dims[q];
+    for (int q = 0; q <= mode; ++q) id1 *= sh.dims[q];
     const int64_t id2 = sz / id1;
-    const int d2 = nd.dims[mode];
+    const int d2 = sh.dims[mode];
     const int d1 = static_cast<int>(sz / d2);
     double2* A = c.base + nd.A;
     double2 *U = c.W(5), *Q = c.W(1), *Rm = c.W(2);
@@ -349,42 +378,45 @@ __device__ void r2l_body(Ctx& c, int n, int mode) {
     T.sync();
 }
 __device__ void r2l(Ctx& c, int n, int mode) {
-    on_team(c, c.node(n).size, [&](Ctx& cc) { r2l_body(cc, n, mode); });
+    on_team(c, c.shape(n).size, [&](Ctx& cc) { r2l_body(cc, n, mode); });
 }
 
 // leaf_to_root_decomposition_engine::evaluate + apply_to_node (leaf_to_root_decomposition.hpp:56-101)
 __device__ void l2r(Ctx& c, int n) {
-    on_team(c, c.node(n).size, [&](Ctx& cc) {
+    on_team(c, c.shape(n).size, [&](Ctx& cc) {
         const DNode& nd = cc.node(n);
+        const Shape sh = cc.shape(n);
         double2* A = cc.base + nd.A;
         double2* U = cc.W(5);
-        t_copy(*cc.T, U, A, nd.size);
-        t_householder_qr(*cc.T, U, nd.dimen, nd.hrank, A, cc.base + nd.R);
+        t_copy(*cc.T, U, A, sh.size);
+        t_householder_qr(*cc.T, U, sh.dimen, sh.hrank, A, cc.base + nd.R);
     });
 }
 
 // root_to_leaf_decomposition_engine::apply_from_parent (:208-233): A <- A * R_parent
 __device__ void apply_from_parent(Ctx& c, int n) {
-    on_team(c, c.node(n).size, [&](Ctx& cc) {
+    on_team(c, c.shape(n).size, [&](Ctx& cc) {
         const DNode& nd = cc.node(n);
+        const Shape sh = cc.shape(n);
         double2* A = cc.base + nd.A;
         double2* Tm = cc.W(5);
-        t_mul(*cc.T, A, nd.dimen, nd.hrank, cc.base + cc.node(nd.parent).R, Tm);
-        t_copy(*cc.T, A, Tm, nd.size);
+        t_mul(*cc.T, A, sh.dimen, sh.hrank, cc.base + cc.node(nd.parent).R, Tm);
+        t_copy(*cc.T, A, Tm, sh.size);
     });
 }
 
 // leaf_to_root_decomposition_engine::apply_to_parent (leaf_to_root_decomposition.hpp:105-138)
 __device__ void apply_to_parent(Ctx& c, int n) {
-    on_team(c, c.node(c.node(n).parent).size, [&](Ctx& cc) {
+    on_team(c, c.shape(c.node(n).parent).size, [&](Ctx& cc) {
         const DNode& nd = cc.node(n);
         const DNode& pn = cc.node(nd.parent);
+        const Shape ps = cc.shape(nd.parent);
         int64_t L, R;
-        mode_view(pn, nd.child_id, L, R);
+        mode_view(ps, nd.child_id, L, R);
         double2* Tm = cc.W(5);
         double2* Ap = cc.base + pn.A;
-        t_mode_contract(*cc.T, cc.base + nd.R, L, pn.dims[nd.child_id], R, Ap, Tm, cmk(1.0, 0.0), false);
-        t_copy(*cc.T, Ap, Tm, pn.size);
+        t_mode_contract(*cc.T, cc.base + nd.R, L, ps.dims[nd.child_id], R, Ap, Tm, cmk(1.0, 0.0), false);
+        t_copy(*cc.T, Ap, Tm, ps.size);
     });
 }
 
@@ -592,7 +624,7 @@ __device__ __forceinline__ int prim_count(const DK& k) { return k.kind == 0 ? k.
 template <class F>
 __device__ void small_walk(const Ctx& c, int n, bool bond, F&& f) {
     const DNode& nd = c.node(n);
-    const int rr = nd.hrank * nd.hrank;
+    const int rr = c.hr(n) * c.hr(n);
     for (int ti = 0; ti < nd.nterms; ++ti) {
         const DTerm& tm = c.I.terms[nd.term0 + ti];
         if (bond) {
@@ -613,7 +645,8 @@ __device__ void small_walk(const Ctx& c, int n, bool bond, F&& f) {
             } else {
                 for (int q = 0; q < sb.nk; ++q) {
                     const DK& k = c.I.ks[sb.k0 + q];
-                    f(k.mat, k.dim * k.dim);
+                    const int dk = c.hr(nd.children[k.nu]);
+                    f(k.mat, dk * dk);
                 }
             }
         }
@@ -629,6 +662,7 @@ __device__ void small_walk(const Ctx& c, int n, bool bond, F&& f) {
 constexpr int NC_TERMS = 8, NC_SUBS = 24, NC_KS = 32, NC_INTS = 1024;
 struct NodeCache {
     DNode nd;
+    Shape sh;  // its live shape
     DTerm terms[NC_TERMS];
     DSub subs[NC_SUBS];
     DK ks[NC_KS];
@@ -676,6 +710,7 @@ __device__ void cache_node(const Ctx& c, int n) {
         node_record_ranges(c, n, s0, s1, k0, k1, nints);
         const DNode& nd = c.node(n);
         g_nc.nd = nd;
+        g_nc.sh = c.shape(n);
         for (int ti = 0; ti < nd.nterms; ++ti) g_nc.terms[ti] = c.I.terms[nd.term0 + ti];
         for (int s = s0; s < s1; ++s) g_nc.subs[s - s0] = c.I.subs[s];
         int64_t at = 0;
@@ -796,7 +831,8 @@ __device__ void small_heff_node() {
     const double2* v = L.v;
     double2* res = L.w;
     const DNode& nd = g_nc.nd;
-    const int r = nd.hrank, D = nd.dimen, size = nd.size, rr = r * r;
+    const Shape& sh = g_nc.sh;
+    const int r = sh.hrank, D = sh.dimen, size = sh.size, rr = r * r;
     int cur = 0;
     bool first = true;
     for (int ti = 0; ti < nd.nterms; ++ti) {
@@ -830,10 +866,11 @@ __device__ void small_heff_node() {
                     double2* out = last ? dst : (q % 2 == 0 ? L.t1 : L.t2);
                     const DK& k = g_nc.ks[sb.k0 + q - g_nc.k_base];
                     int64_t LL, RR;
-                    mode_view(nd, k.nu, LL, RR);
-                    s_mode_contract(L.ops + cur, static_cast<int>(LL), nd.dims[k.nu], static_cast<int>(RR), src, out,
+                    mode_view(sh, k.nu, LL, RR);
+                    const int dk = sh.dims[k.nu];
+                    s_mode_contract(L.ops + cur, static_cast<int>(LL), dk, static_cast<int>(RR), src, out,
                                     last ? ci : cmk(1.0, 0.0), last && acc);
-                    cur += k.dim * k.dim;
+                    cur += dk * dk;
                     src = out;
                 }
             }
@@ -854,7 +891,7 @@ __device__ void small_heff_bond() {
     const double2* v = L.v;
     double2* res = L.w;
     const DNode& nd = g_nc.nd;
-    const int r = nd.hrank, sz = r * r;
+    const int r = g_nc.sh.hrank, sz = r * r;
     int cur = 0;
     s_zero(res, sz);
     for (int ti = 0; ti < nd.nterms; ++ti) {
@@ -1066,28 +1103,32 @@ __device__ int64_t run_small(Ctx& c, int n_op, bool bond, double2* x, int n, dou
     return expv_small<MAXK>(c, n_op, bond, x, n, coeff, tau, sumj);
 }
 
+#include "expand.hpp"
+
 // ---------------------------------------------------------------- sweeps
 struct SweepState {
     double2 coeff;
     double sub_dt;
+    double exp_dt;   // the engine's dt() seen by the spawning test of the expansion
     int64_t nh;
 };
 
 __device__ void node_update(Ctx& c, SweepState& st, int n) {
     const DNode& nd = c.node(n);
+    const int size = c.shape(n).size;
     c.where = n;
     c.bond = 0;
     int64_t na = 0, sumj = 0;
     const uint64_t pt0 = prof_now(c);
-    if (small_eligible(c, n, false, nd.size)) {
+    if (small_eligible(c, n, false, size)) {
         // one workgroup (the leader, other members wait at the barrier that follows)
-        on_team(c, 0, [&](Ctx& cc) { na = run_small(cc, n, false, cc.base + nd.A, nd.size, st.coeff, st.sub_dt / 2.0, sumj); });
+        on_team(c, 0, [&](Ctx& cc) { na = run_small(cc, n, false, cc.base + nd.A, size, st.coeff, st.sub_dt / 2.0, sumj); });
         count_small(c, pt0);
     } else {
-        on_team(c, nd.size, [&](Ctx& cc) {
+        on_team(c, size, [&](Ctx& cc) {
             na = expv(
                 cc, [&](const double2* v, double2* res) { PROFILE(PROF_HEFF, heff_node(cc, n, v, res)); },
-                cc.base + nd.A, nd.size, st.coeff, st.sub_dt / 2.0, sumj);
+                cc.base + nd.A, size, st.coeff, st.sub_dt / 2.0, sumj);
         });
     }
     prof_add(c, PROF_EXPV_NODE, pt0);
@@ -1099,7 +1140,7 @@ __device__ void node_update(Ctx& c, SweepState& st, int n) {
 }
 
 __device__ void bond_update(Ctx& c, SweepState& st, int n, int ops) {
-    const int r = c.node(ops).hrank;
+    const int r = c.hr(ops);
     c.where = ops;
     c.bond = 1;
     int64_t na = 0, sumj = 0;
@@ -1123,11 +1164,15 @@ __device__ void bond_update(Ctx& c, SweepState& st, int n, int ops) {
     }
 }
 
-__device__ void forward(Ctx& c, SweepState& st, bool evolve, unsigned char* visits) {
+// With expand, every root-to-leaf factorisation is followed by the downward expansion of that
+// bond and every leaf-to-root factorisation by the upward one (tdvp_algorithm_engine.hpp:373,
+// 479).  Returns whether a bond grew (the same on every member).
+__device__ bool forward(Ctx& c, SweepState& st, bool evolve, bool expand, unsigned char* visits) {
+    bool grew = false;
     for (int e = threadIdx.x; e < c.I.nnodes; e += NT) visits[e] = 0;
     __syncthreads();
     for (int s = 0; s < c.I.tour_len; ++s) {
-        if (c.failed()) return;
+        if (c.failed()) return grew;
         const int id = c.I.tour[s];
         const DNode& nd = c.node(id);
         const int tv = visits[id];
@@ -1138,6 +1183,9 @@ __device__ void forward(Ctx& c, SweepState& st, bool evolve, unsigned char* visi
             if (tv == 0 && nd.parent >= 0) PROFILE(PROF_GAUGE, apply_from_parent(c, id));
             if (nd.leaf < 0) {
                 PROFILE(PROF_QR, r2l(c, id, tv));
+                if (expand && expand_on_leader(c, nd.children[tv],
+                                               [&](Ctx& cc) { return expand_down_body(cc, id, tv, st.exp_dt); }))
+                    grew = true;
                 PROFILE(PROF_MF, update_mf(c, nd.children[tv]));
             }
         } else {
@@ -1150,13 +1198,15 @@ __device__ void forward(Ctx& c, SweepState& st, bool evolve, unsigned char* visi
             }
         }
     }
+    return grew;
 }
 
-__device__ void backward(Ctx& c, SweepState& st, bool evolve, unsigned char* visits) {
+__device__ bool backward(Ctx& c, SweepState& st, bool evolve, bool expand, unsigned char* visits) {
+    bool grew = false;
     for (int e = threadIdx.x; e < c.I.nnodes; e += NT) visits[e] = 0;
     __syncthreads();
     for (int s = c.I.tour_len - 1; s >= 0; --s) {
-        if (c.failed()) return;
+        if (c.failed()) return grew;
         const int id = c.I.tour[s];
         const DNode& nd = c.node(id);
         const int tv = visits[id];
@@ -1178,10 +1228,13 @@ __device__ void backward(Ctx& c, SweepState& st, bool evolve, unsigned char* vis
             }
         } else if (nd.parent >= 0) {
             PROFILE(PROF_QR, l2r(c, id));
+            if (expand && expand_on_leader(c, id, [&](Ctx& cc) { return expand_up_body(cc, id, st.exp_dt); }))
+                grew = true;
             PROFILE(PROF_GAUGE, apply_to_parent(c, id));
             PROFILE(PROF_SPF, update_spf(c, id));
         }
     }
+    return grew;
 }
 
 // <psi|O|psi> for O on the leaf node `leaf` (path contraction to the root, the other
@@ -1189,24 +1242,26 @@ __device__ void backward(Ctx& c, SweepState& st, bool evolve, unsigned char* vis
 __device__ void path_expect(Ctx& c, int leaf, int64_t op_off, double2& val, double& nrm) {
     Team& T = *c.T;
     const DNode& ln = c.node(leaf);
+    const Shape ls = c.shape(leaf);
     double2 *HA = c.W(1), *M = c.W(2), *Tm = c.W(3), *M2 = c.W(4);
     DK k;
     k.nu = ln.leaf;
     k.kind = 0;
-    k.dim = ln.dimen;
+    k.dim = ls.dimen;
     k.nnz = 0;
     k.mat = op_off;
     k.idx = 0;
-    t_prim_apply(T, k, c.base, c.I.ints, c.base + ln.A, HA, ln.hrank);
-    t_adjoint_mul(T, c.base + ln.A, HA, ln.dimen, ln.hrank, M, cmk(1.0, 0.0), false);
+    t_prim_apply(T, k, c.base, c.I.ints, c.base + ln.A, HA, ls.hrank);
+    t_adjoint_mul(T, c.base + ln.A, HA, ls.dimen, ls.hrank, M, cmk(1.0, 0.0), false);
     int n = leaf;
     while (c.node(n).parent >= 0) {
         const DNode& nd = c.node(n);
         const DNode& pn = c.node(nd.parent);
+        const Shape ps = c.shape(nd.parent);
         int64_t L, R;
-        mode_view(pn, nd.child_id, L, R);
-        t_mode_contract(T, M, L, pn.dims[nd.child_id], R, c.base + pn.A, Tm, cmk(1.0, 0.0), false);
-        t_adjoint_mul(T, c.base + pn.A, Tm, pn.dimen, pn.hrank, M2, cmk(1.0, 0.0), false);
+        mode_view(ps, nd.child_id, L, R);
+        t_mode_contract(T, M, L, ps.dims[nd.child_id], R, c.base + pn.A, Tm, cmk(1.0, 0.0), false);
+        t_adjoint_mul(T, c.base + pn.A, Tm, ps.dimen, ps.hrank, M2, cmk(1.0, 0.0), false);
         double2* tmp = M;
         M = M2;
         M2 = tmp;
@@ -1214,7 +1269,7 @@ __device__ void path_expect(Ctx& c, int leaf, int64_t op_off, double2& val, doub
     }
     val = ldc(M);
     const DNode& root = c.node(0);
-    nrm = team_norm2(T, c.base + root.A, root.size);
+    nrm = team_norm2(T, c.base + root.A, c.shape(0).size);
 }
 
 __device__ void step_done(Ctx& c, int64_t step_index) {
@@ -1290,29 +1345,41 @@ __global__ __launch_bounds__(NT) void tdvp_run_kernel(const DInst* insts, RunArg
     Ctx c(I, &T, a.team_min);
     c.small = &g_small;
     c.small_max = a.small_max;
+    for (int e = threadIdx.x; e < I.nnodes; e += NT) g_hr[e] = static_cast<unsigned char>(I.hr[e]);
+    __syncthreads();
     const uint64_t prof_t0 = prof_now(c);
     SweepState st;
     st.coeff = a.coeff;
     st.nh = 0;
     int64_t step_index = a.step0;
-    if (a.prepare) {
-        // tdvp_algorithm_base::prepare_evolution (:257-321) without subspace expansion
+    // tdvp_algorithm_base::prepare_evolution (:257-321): the bottom-up single-particle pass,
+    // then sweep pairs without propagation
+    if (a.prepare_spf)
         for (int n = I.nnodes - 1; n >= 0; --n)
             if (I.nodes[n].parent >= 0) update_spf(c, n);
+    const bool expand = I.expand != 0;
+    bool grew = false;
+    for (int pi = 0; pi < a.prepare && !c.failed(); ++pi) {
+        // the engine's dt is the full step during prepare_evolution
         st.sub_dt = a.nladder > 0 ? a.ladder_dt[0] : a.dt;
-        forward(c, st, false, visits);
-        backward(c, st, false, visits);
+        st.exp_dt = a.dt;
+        grew = forward(c, st, false, expand, visits);
+        grew = backward(c, st, false, expand, visits) || grew;
+        if (expand && !grew) break;
     }
+    if (a.prepare > 0 && expand && c.writer()) I.exp[EXP_GREW] = grew ? 1 : 0;
     for (int l = 0; l < a.nladder && !c.failed(); ++l) {
         st.sub_dt = a.ladder_dt[l];
-        forward(c, st, true, visits);
-        backward(c, st, true, visits);
+        st.exp_dt = a.ladder_dt[l];
+        forward(c, st, true, expand, visits);
+        backward(c, st, true, expand, visits);
     }
     if (a.nladder > 0 && !c.failed()) step_done(c, step_index++);
     for (int s = 0; s < a.nregular && !c.failed(); ++s) {
         st.sub_dt = a.dt;
-        forward(c, st, true, visits);
-        backward(c, st, true, visits);
+        st.exp_dt = a.dt;
+        forward(c, st, true, expand, visits);
+        backward(c, st, true, expand, visits);
         step_done(c, step_index++);
     }
     __syncthreads();
@@ -1342,6 +1409,8 @@ __global__ __launch_bounds__(NT) void expect_kernel(const DInst* inst, int leaf,
     T.part = nullptr;
     T.tiles = nullptr;
     Ctx c(*inst, &T, 0);
+    for (int e = threadIdx.x; e < inst->nnodes; e += NT) g_hr[e] = static_cast<unsigned char>(inst->hr[e]);
+    __syncthreads();
     double2 v;
     double nrm;
     path_expect(c, leaf, op_off, v, nrm);

@@ -91,6 +91,21 @@ def c1_instance() -> Instance:
     return Instance(SpinBoson(wk=w, gk=g, boson_dim=[16] * 128, nspf=32, nspf_lower=32), "c1", 0.025, 12)
 
 
+# the expansion knobs of the single larger run: spawning 1e-8, unoccupied threshold 1e-4, at least
+# two unoccupied functions (the reference driver's keys "spawning parameter", "unoccupied
+# threshold", "minimum unoccupied", src/spin_boson.cpp:258-278)
+C1_EXPANSION = dict(spawning_threshold=1e-8, unoccupied_threshold=1e-4, minimum_unoccupied=2)
+
+
+def c1_expansion_instance() -> Instance:
+    """The single larger run as configured, subspace expansion on: the tree of c1_instance with
+    initial bond dimension 8 and capacity 32 (leaves capped at their 16 states), the bonds grown by
+    the expansion integrator with the C1_EXPANSION knobs."""
+    w, g = debye_bath(128, 0.5, 2.0, beta=10.0)
+    return Instance(SpinBoson(wk=w, gk=g, boson_dim=[16] * 128, nspf=8, nspf_lower=8, nspf_cap=32, nspf_lower_cap=32),
+                    "c1x", 0.025, 12)
+
+
 def c4_instance() -> Instance:
     """Adjoint config: Ohmic alpha 0.2, wc 5, T = 0, 64 modes, boson dim 12, bond dim 24."""
     w, g = ohmic_bath(64, 0.2, 5.0)
