@@ -146,6 +146,15 @@ int ttn_get_dt(const ttn_integ* integ, double* dt);
 /* reset_nsteps() (projector_splitting_integrator.hpp:33): the next ttn_step prepares the
  * evolution and runs the first-step ladder again */
 int ttn_reset_nsteps(ttn_integ* integ);
+/* new values of the SOP's coefficients (nterms complex, interleaved re/im; sop_operator::coeff(r),
+ * sop_operator.hpp:471-473), which the reference's operator()(A, op, coeff) reads from `op` on
+ * every call: the following ttn_step uses them */
+int ttn_set_coefficients(ttn_integ* integ, int64_t nterms, const double* coeff);
+/* the expansion knobs of a TTN_MODE_SUBSPACE_EXPANSION integrator, which the reference's engine
+ * reads on every call (spawning_threshold(), unoccupied_threshold(), minimum_unoccupied(),
+ * neigenvalues(), only_apply_when_no_unoccupied(), projector_splitting_integrator.hpp:338-357):
+ * those fields of *p (and eigenvalue_tolerance) replace the current ones; the others are ignored */
+int ttn_set_expansion_params(ttn_integ* integ, const ttn_integ_params* p);
 /* normal draws the expansion's trial vectors have taken from its std::mt19937 +
  * std::normal_distribution stream so far.  Instrumentation (no reference counterpart). */
 int ttn_expansion_draws(const ttn_integ* integ, int64_t* n);

@@ -11,9 +11,11 @@
 //                                     include/ttns_lib/operators/matrix_operators.hpp:14-474
 //   sop_operator<T, backend>(nterms, mode_dims).bind(op, r_indices, mode)
 //                                     include/ttns_lib/operators/sop_operator.hpp:234-392
-//   fixed_step_projector_splitting_integrator<T, backend>(A, H, krylov_dim, krylov_tol, nthreads):
-//       dt(), t(), operator()(A, H, coeff), nh_applications(), mel()(op, mode, A)
-//                                     include/ttns_lib/integrators/projector_splitting_integrator.hpp:76-218,
+//   fixed_step_ / subspace_expansion_projector_splitting_integrator<T, backend>(A, H, krylov_dim,
+//       krylov_tol, nthreads): initialise(), clear(), reset_nsteps(), dt(), set_dt(), t(),
+//       operator()(A, H, coeff), nh_applications(), mel()(op, mode, A), the expansion knobs and
+//       engine().Nonesite() / Ntwosite()
+//                                     include/ttns_lib/integrators/projector_splitting_integrator.hpp:18-397,
 //                                     include/ttns_lib/observables/matrix_element.hpp:264-314
 //
 // Semantics kept: node matrices are row-major (prod child dims) x hrank with child 0
@@ -21,14 +23,16 @@
 // call runs the prepare pass and the six-sub-step ladder) and returns dt; failures throw a
 // std::runtime_error whose text starts with the reference's "Failed to apply projector
 // splitting integrator.".  All arithmetic runs on the GPU; A is copied to the device when
-// the integrator is first used or after the caller changed it, and copied back after every
-// step.  Only complex<double> and static dense / diagonal / CSR leaf operators exist.
+// the integrator is first used or after the caller changed it, and copied back only when the
+// caller next reads a node matrix (bond dimensions are updated after every step).  Only
+// complex<double> and dense / diagonal / CSR leaf operators exist.
 #pragma once
 
 #include <cmath>
 #include <complex>
 #include <cstdint>
 #include <functional>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -175,6 +179,10 @@ struct ntree_builder {
 };
 
 // ------------------------------------------------------------------ TTNS storage
+// After an integrator step the device holds the newer state: the tensor records that (and the
+// bond dimensions, which the expansion integrator grows) without copying, and the first access
+// of a node matrix copies every node back.  Writing through a node (non-const access) marks the
+// host copy newer, and the integrator copies it to the device before its next step.
 template <typename T, typename backend = blas_backend>
 class httensor {
   public:
@@ -186,22 +194,26 @@ class httensor {
         int64_t hrank() const { return m_hrank; }
         int64_t parent() const { return m_parent; }
         matrix<T>& operator()() {
-            ++*m_version;
+            m_owner->pull();
+            ++m_owner->m_version;
             return m_A;
         }
-        const matrix<T>& operator()() const { return m_A; }
+        const matrix<T>& operator()() const {
+            m_owner->pull();
+            return m_A;
+        }
 
       private:
         friend class httensor;
         int64_t m_parent = -1, m_hrank = 1, m_leaf = -1;
         matrix<T> m_A;
-        uint64_t* m_version = nullptr;
+        httensor* m_owner = nullptr;
     };
 
     httensor(const ntree<size_type>& topology, const ntree<size_type>& capacity) {
         int64_t leaves = 0;
         flatten(topology(), capacity(), -1, leaves);
-        for (node& n : m_nodes) n.m_version = &m_version;
+        for (node& n : m_nodes) n.m_owner = this;
     }
     httensor(const httensor&) = delete;
     httensor& operator=(const httensor&) = delete;
@@ -211,14 +223,44 @@ class httensor {
     const node& operator[](size_type i) const { return m_nodes[i]; }
     typename std::vector<node>::iterator begin() { return m_nodes.begin(); }
     typename std::vector<node>::iterator end() { return m_nodes.end(); }
+    typename std::vector<node>::const_iterator begin() const { return m_nodes.begin(); }
+    typename std::vector<node>::const_iterator end() const { return m_nodes.end(); }
     void set_is_orthogonalised() {}
 
-    // flat pre-order arrays of the C ABI (ttn_tree_create)
+    // flat pre-order arrays of the C ABI (ttn_tree_create); hranks() are the current ones
     const std::vector<int64_t>& parent() const { return m_par; }
     const std::vector<int64_t>& hranks() const { return m_hr; }
     const std::vector<int64_t>& cap_hranks() const { return m_cap; }
     const std::vector<int64_t>& leaf_dims() const { return m_leafdim; }
     uint64_t version() const { return m_version; }
+
+    // ---- used by the integrators
+    // the device now holds the newer state, produced by `source`; `pull(A)` copies it into the
+    // node matrices (through raw()) when a node matrix is next accessed
+    void mark_device_newer(const void* source, std::function<void(httensor&)> pull,
+                           const std::vector<int64_t>& hranks) {
+        m_source = source;
+        m_pull = std::move(pull);
+        for (size_type i = 0; i < m_nodes.size(); ++i) m_nodes[i].m_hrank = m_hr[i] = hranks[i];
+    }
+    // the state the device holds came from `source` and the host copy has not changed since
+    bool device_current(const void* source, uint64_t version) const {
+        return m_source == source && version == m_version;
+    }
+    // forget `source` (its device state is about to go away): copy its newer state first
+    void detach(const void* source) {
+        if (m_source != source) return;
+        pull();
+        m_source = nullptr;
+    }
+    matrix<T>& raw(size_type i) { return m_nodes[i].m_A; }
+    void pull() const {
+        if (!m_pull) return;
+        auto f = std::move(m_pull);
+        m_pull = nullptr;
+        f(const_cast<httensor&>(*this));
+    }
+    std::weak_ptr<int> alive() const { return m_alive; }
 
   private:
     void flatten(const ntree_node<size_type>& t, const ntree_node<size_type>& c, int64_t parent, int64_t& leaves) {
@@ -244,6 +286,9 @@ class httensor {
     std::vector<node> m_nodes;
     std::vector<int64_t> m_par, m_hr, m_cap, m_leafdim;
     uint64_t m_version = 0;
+    const void* m_source = nullptr;
+    mutable std::function<void(httensor&)> m_pull;
+    std::shared_ptr<int> m_alive = std::make_shared<int>(0);
 };
 
 // ------------------------------------------------------------------ operators
@@ -360,18 +405,25 @@ class sop_operator {
         idx.insert(idx.end(), op.mat.colind.begin(), op.mat.colind.end());
         add(mode, TTN_OP_CSR, op.mat.values.data(), op.mat.values.size(), idx, r);
     }
+    // coeff() / coeff(r) (sop_operator.hpp:471-473): read by the integrator on every step
     std::vector<T>& coeff() { return m_coeff_store(); }
+    T& coeff(size_type r) { return m_coeff_store().at(r); }
+    T coeff(size_type r) const { return r < m_coeff.size() ? m_coeff[r] : T(1.0); }
     size_type nterms() const { return m_nterms; }
+    // interleaved re/im of every coefficient (1 where none was set)
+    std::vector<double> coefficients() const {
+        std::vector<double> c(2 * m_nterms);
+        for (size_type t = 0; t < m_nterms; ++t) {
+            const T v = coeff(t);
+            c[2 * t] = v.real();
+            c[2 * t + 1] = v.imag();
+        }
+        return c;
+    }
 
     // C-ABI handle on a context (rebuilt lazily by the integrator)
     ttn_sop* create(ttn_ctx* ctx) const {
-        std::vector<double> coeff(2 * m_nterms);
-        const std::vector<T>& c = m_coeff;
-        for (size_type t = 0; t < m_nterms; ++t) {
-            const T v = t < c.size() ? c[t] : T(1.0);
-            coeff[2 * t] = v.real();
-            coeff[2 * t + 1] = v.imag();
-        }
+        const std::vector<double> coeff = coefficients();
         std::vector<int32_t> mode(m_mode.begin(), m_mode.end()), kind(m_kind.begin(), m_kind.end());
         ttn_sop* s = nullptr;
         detail::check(ttn_sop_create(ctx, static_cast<int64_t>(m_dims.size()), m_dims.data(),
@@ -419,7 +471,7 @@ class subspace_expansion_projector_splitting_integrator;
 template <typename T, typename backend = blas_backend>
 class matrix_element {
   public:
-    // Re/Im <psi|O|psi> for a dense operator on primitive mode `mode`
+    // <psi|O|psi> for a dense operator on primitive mode `mode` (matrix_element.hpp:264-314)
     T operator()(const ops::dense_matrix_operator<T, backend>& op, size_type mode, httensor<T, backend>& A) {
         return m_integ->expect(op, mode, A);
     }
@@ -429,58 +481,102 @@ class matrix_element {
     fixed_step_projector_splitting_integrator<T, backend>* m_integ = nullptr;
 };
 
+// The integrators (projector_splitting_integrator.hpp:18-397).  The reference's state lives in
+// the integrator object: dt(), krylov_dim(), krylov_tolerance(), nthreads() and the expansion
+// knobs are plain members the caller assigns, and every call reads them.  Here the device
+// integrator is created on first use (a step or mel()) with the values set by then; dt, the
+// operator's coefficients and the expansion knobs are forwarded again whenever they changed
+// since the last call.  krylov_dim / krylov_tolerance take effect at initialise().
 template <typename T, typename backend>
 class fixed_step_projector_splitting_integrator {
   public:
     using real_type = double;
+    fixed_step_projector_splitting_integrator() { m_mel.m_integ = this; }
     fixed_step_projector_splitting_integrator(httensor<T, backend>& A, sop_operator<T, backend>& H,
                                               size_type krylov_dim = 4, real_type krylov_tol = 1e-12,
                                               size_type nthreads = 1)
         : m_kdim(krylov_dim), m_tol(krylov_tol), m_nthreads(nthreads) {
-        detail::check(ttn_ctx_create(0, &m_ctx), m_ctx, "ttn_ctx_create");
-        const auto& p = A.parent();
-        detail::check(ttn_tree_create(m_ctx, static_cast<int64_t>(p.size()), p.data(), A.hranks().data(),
-                                      A.cap_hranks().data(), A.leaf_dims().data(), &m_tree),
-                      m_ctx, "ttn_tree_create");
-        m_sop = H.create(m_ctx);
         m_mel.m_integ = this;
+        initialise(A, H);
     }
-    ~fixed_step_projector_splitting_integrator() {
-        if (m_integ) ttn_integrator_destroy(m_integ);
-        if (m_sop) ttn_sop_destroy(m_sop);
-        if (m_tree) ttn_tree_destroy(m_tree);
+    virtual ~fixed_step_projector_splitting_integrator() {
+        release();
         if (m_ctx) ttn_ctx_destroy(m_ctx);
     }
     fixed_step_projector_splitting_integrator(const fixed_step_projector_splitting_integrator&) = delete;
     fixed_step_projector_splitting_integrator& operator=(const fixed_step_projector_splitting_integrator&) = delete;
 
+    // initialise(A, op) (:100-116): binds the tree (its current bond dimensions and capacities)
+    // and the operator; the previous device state, if any, is copied into its tensor first
+    void initialise(httensor<T, backend>& A, sop_operator<T, backend>& H) {
+        release();
+        if (!m_ctx) detail::check(ttn_ctx_create(0, &m_ctx), m_ctx, "ttn_ctx_create");
+        const auto& p = A.parent();
+        detail::check(ttn_tree_create(m_ctx, static_cast<int64_t>(p.size()), p.data(), A.hranks().data(),
+                                      A.cap_hranks().data(), A.leaf_dims().data(), &m_tree),
+                      m_ctx, "ttn_tree_create");
+        m_sop = H.create(m_ctx);
+        m_coeff_sent = H.coefficients();
+    }
+    // clear() (:118-137): unbinds and returns to the default parameters
+    void clear() {
+        release();
+        m_kdim = 4;
+        m_tol = 1e-12;
+        m_nthreads = 1;
+        m_dt = 0.0;
+    }
+    // reset_nsteps() (:33): t() restarts at 0 and the next call is a first step again
+    // (prepare_evolution and the sub-step ladder)
+    void reset_nsteps() {
+        if (m_integ) detail::check(ttn_reset_nsteps(m_integ), m_ctx, "ttn_reset_nsteps");
+    }
+
     real_type& dt() { return m_dt; }
     const real_type& dt() const { return m_dt; }
-    real_type t() const {
-        double v = 0.0;
-        if (m_integ) ttn_time(m_integ, &v);
-        return v;
-    }
-    size_type krylov_dim() const { return m_kdim; }
-    real_type krylov_tolerance() const { return m_tol; }
-    size_type nthreads() const { return m_nthreads; }
+    void set_dt(real_type dt) { m_dt = dt; }
+    // t() = dt() * nsteps, with the current dt() (:188, 362)
+    real_type t() const { return m_dt * static_cast<real_type>(nsteps()); }
+    size_type& krylov_dim() { return m_kdim; }
+    const size_type& krylov_dim() const { return m_kdim; }
+    real_type& krylov_tolerance() { return m_tol; }
+    const real_type& krylov_tolerance() const { return m_tol; }
+    size_type& nthreads() { return m_nthreads; }
+    const size_type& nthreads() const { return m_nthreads; }
     size_type nh_applications() const {
         int64_t nh = 0, ns = 0;
-        if (m_integ) ttn_counters(m_integ, &nh, &ns);
+        if (m_integ) detail::check(ttn_counters(m_integ, &nh, &ns), m_ctx, "ttn_counters");
         return static_cast<size_type>(nh);
     }
     matrix_element<T, backend>& mel() { return m_mel; }
 
-    // one step A <- TDVP(A, H, coeff) of length dt (coeff = -i for real time); returns dt
-    real_type operator()(httensor<T, backend>& A, sop_operator<T, backend>&, T coeff) {
-        sync_to_device(A);
+    // one call A <- TDVP(A, H, coeff) of length dt() (coeff = -i for real time); returns dt().
+    // The first call after construction or reset_nsteps() runs prepare_evolution and the
+    // sub-step ladder (:145-168, 284-312).
+    real_type operator()(httensor<T, backend>& A, sop_operator<T, backend>& H, T coeff) {
+        if (!(m_dt > 0.0)) throw std::runtime_error("Failed to apply projector splitting integrator.\ndt must be positive");
+        bind(A);
+        if (m_dt != m_dt_sent) {
+            detail::check(ttn_set_dt(m_integ, m_dt), m_ctx, "ttn_set_dt");
+            m_dt_sent = m_dt;
+        }
+        const std::vector<double> c = H.coefficients();
+        if (c != m_coeff_sent) {
+            detail::check(ttn_set_coefficients(m_integ, static_cast<int64_t>(H.nterms()), c.data()), m_ctx,
+                          "ttn_set_coefficients");
+            m_coeff_sent = c;
+        }
+        forward_parameters();
         detail::check(ttn_step(m_integ, coeff.real(), coeff.imag(), 1), m_ctx, "ttn_step");
-        sync_from_device(A);
+        std::vector<int64_t> hr(A.size());
+        detail::check(ttn_hranks(m_integ, hr.data()), m_ctx, "ttn_hranks");
+        A.mark_device_newer(this, [this](httensor<T, backend>& B) { copy_from_device(B); }, hr);
+        m_version = A.version();
         return m_dt;
     }
 
     T expect(const ops::dense_matrix_operator<T, backend>& op, size_type mode, httensor<T, backend>& A) {
-        sync_to_device(A);
+        bind(A);
         double out[2] = {0.0, 0.0};
         detail::check(ttn_expect(m_integ, static_cast<int64_t>(mode),
                                  reinterpret_cast<const double*>(op.mat.data()), out),
@@ -488,68 +584,124 @@ class fixed_step_projector_splitting_integrator {
         return T(out[0], out[1]);
     }
 
-  private:
-    void ensure() {
-        if (m_integ) return;
+  protected:
+    // the expansion integrator forwards its knobs here
+    virtual void forward_parameters() {}
+    ttn_integ_params params() const {
         ttn_integ_params p{};
         p.krylov_dim = static_cast<int64_t>(m_kdim);
         p.krylov_tol = m_tol;
-        p.dt = m_dt;
+        p.dt = m_dt > 0.0 ? m_dt : 1.0;  // forwarded again before every step
         p.mode = m_mode;
         p.spawning_threshold = m_spawning_threshold;
         p.unoccupied_threshold = m_unoccupied_threshold;
         p.minimum_unoccupied = static_cast<int64_t>(m_minimum_unoccupied);
         p.neigenvalues = static_cast<int64_t>(m_neigenvalues);
-        detail::check(ttn_integrator_create(m_ctx, m_tree, m_sop, &p, &m_integ), m_ctx, "ttn_integrator_create");
+        p.only_apply_when_no_unoccupied = m_only_unoccupied ? 1 : 0;
+        return p;
     }
-    void sync_to_device(httensor<T, backend>& A) {
+    ttn_integ* handle() const { return m_integ; }
+    ttn_ctx* context() const { return m_ctx; }
+
+    int m_mode = TTN_MODE_FIXED_STEP;
+    real_type m_spawning_threshold = -1.0;
+    real_type m_unoccupied_threshold = -1.0;
+    size_type m_minimum_unoccupied = 0;
+    size_type m_neigenvalues = 2;
+    bool m_only_unoccupied = false;
+
+  private:
+    int64_t nsteps() const {
+        int64_t nh = 0, ns = 0;
+        if (m_integ) detail::check(ttn_counters(m_integ, &nh, &ns), m_ctx, "ttn_counters");
+        return ns;
+    }
+    void ensure() {
+        if (m_integ) return;
+        if (!m_tree) throw std::runtime_error("Failed to apply projector splitting integrator.\nnot initialised");
+        const ttn_integ_params p = params();
+        detail::check(ttn_integrator_create(m_ctx, m_tree, m_sop, &p, &m_integ), m_ctx, "ttn_integrator_create");
+        m_dt_sent = p.dt;
+    }
+    // make the device hold A: copy A up unless the device state is A's current one
+    void bind(httensor<T, backend>& A) {
         ensure();
-        if (m_synced && A.version() == m_version) return;
+        if (A.device_current(this, m_version)) return;
+        // the tensor bound so far keeps this device's newer state; A's own newer state (from
+        // another integrator) is copied into A before A goes up
+        if (m_bound && m_bound != &A && !m_alive.expired()) m_bound->detach(this);
+        A.pull();
         for (size_type i = 0; i < A.size(); ++i) {
-            const matrix<T>& m = static_cast<const httensor<T, backend>&>(A)[i]();
+            const matrix<T>& m = A.raw(i);
             detail::check(ttn_state_set(m_integ, static_cast<int64_t>(i), reinterpret_cast<const double*>(m.data()),
                                         static_cast<int64_t>(m.size(0)), static_cast<int64_t>(m.size(1))),
                           m_ctx, "ttn_state_set");
         }
-        m_synced = true;
+        m_bound = &A;
+        m_alive = A.alive();
+        A.mark_device_newer(this, nullptr, A.hranks());
         m_version = A.version();
     }
-    void sync_from_device(httensor<T, backend>& A) {
+    void copy_from_device(httensor<T, backend>& A) {
         for (size_type i = 0; i < A.size(); ++i) {
-            matrix<T>& m = A[i]();
-            detail::check(ttn_state_get(m_integ, static_cast<int64_t>(i), reinterpret_cast<double*>(m.data()),
-                                        static_cast<int64_t>(m.size(0)), static_cast<int64_t>(m.size(1))),
+            int64_t rows = 0, cols = 0;
+            detail::check(ttn_state_shape(m_integ, static_cast<int64_t>(i), &rows, &cols), m_ctx, "ttn_state_shape");
+            matrix<T>& m = A.raw(i);
+            if (m.size(0) != static_cast<size_type>(rows) || m.size(1) != static_cast<size_type>(cols))
+                m.resize(static_cast<size_type>(rows), static_cast<size_type>(cols));
+            detail::check(ttn_state_get(m_integ, static_cast<int64_t>(i), reinterpret_cast<double*>(m.data()), rows, cols),
                           m_ctx, "ttn_state_get");
         }
-        m_version = A.version();  // the copies above are the device state, not caller edits
+    }
+    // the device state is about to go: the bound tensor gets it first
+    void release() {
+        if (m_bound && !m_alive.expired()) m_bound->detach(this);
+        m_bound = nullptr;
+        m_alive.reset();
+        if (m_integ) ttn_integrator_destroy(m_integ);
+        if (m_sop) ttn_sop_destroy(m_sop);
+        if (m_tree) ttn_tree_destroy(m_tree);
+        m_integ = nullptr;
+        m_sop = nullptr;
+        m_tree = nullptr;
     }
 
     ttn_ctx* m_ctx = nullptr;
     ttn_tree* m_tree = nullptr;
     ttn_sop* m_sop = nullptr;
     ttn_integ* m_integ = nullptr;
-    size_type m_kdim;
-    real_type m_tol;
-    size_type m_nthreads;
-    real_type m_dt = 0.0;
-    bool m_synced = false;
+    size_type m_kdim = 4;
+    real_type m_tol = 1e-12;
+    size_type m_nthreads = 1;
+    real_type m_dt = 0.0, m_dt_sent = 0.0;
+    std::vector<double> m_coeff_sent;
+    httensor<T, backend>* m_bound = nullptr;
+    std::weak_ptr<int> m_alive;
     uint64_t m_version = 0;
     matrix_element<T, backend> m_mel;
-
-  protected:
-    int m_mode = TTN_MODE_FIXED_STEP;
-    real_type m_spawning_threshold = -1.0;
-    real_type m_unoccupied_threshold = -1.0;
-    size_type m_minimum_unoccupied = 0;
-    size_type m_neigenvalues = 2;
 };
 
-// subspace_expansion_projector_splitting_integrator (projector_splitting_integrator.hpp:229-326):
-// the same call pattern plus the expansion parameters the driver sets
-// (src/spin_boson.cpp:652-658).  The library accepts it on trees whose bonds are all at
-// capacity, where the reference's expansion never adds functions and the step is the
-// fixed-step one; on a tree with room to grow a bond the first step throws (the expansion
-// itself is not available).
+// engine() of the expansion integrator: the counters the reference's driver prints
+// (subspace_expansion_projector_splitting_engine.hpp:513-514, src/spin_boson.cpp:91)
+class expansion_engine_view {
+  public:
+    size_type Nonesite() const { return counters().first; }
+    size_type Ntwosite() const { return counters().second; }
+
+  private:
+    template <typename, typename>
+    friend class subspace_expansion_projector_splitting_integrator;
+    std::pair<size_type, size_type> counters() const {
+        int64_t n1 = 0, n2 = 0;
+        if (m_integ && *m_integ) detail::check(ttn_expansion_counters(*m_integ, &n1, &n2), nullptr, "ttn_expansion_counters");
+        return {static_cast<size_type>(n1), static_cast<size_type>(n2)};
+    }
+    ttn_integ* const* m_integ = nullptr;
+};
+
+// subspace_expansion_projector_splitting_integrator (projector_splitting_integrator.hpp:229-397):
+// the bonds grow up to the capacities of the tree's second ntree; the knobs are those the
+// driver sets (src/spin_boson.cpp:652-658)
 template <typename T, typename backend = blas_backend>
 class subspace_expansion_projector_splitting_integrator : public fixed_step_projector_splitting_integrator<T, backend> {
     using base = fixed_step_projector_splitting_integrator<T, backend>;
@@ -563,8 +715,37 @@ class subspace_expansion_projector_splitting_integrator : public fixed_step_proj
         this->m_mode = TTN_MODE_SUBSPACE_EXPANSION;
     }
     real_type& spawning_threshold() { return this->m_spawning_threshold; }
+    const real_type& spawning_threshold() const { return this->m_spawning_threshold; }
     real_type& unoccupied_threshold() { return this->m_unoccupied_threshold; }
+    const real_type& unoccupied_threshold() const { return this->m_unoccupied_threshold; }
     size_type& minimum_unoccupied() { return this->m_minimum_unoccupied; }
+    const size_type& minimum_unoccupied() const { return this->m_minimum_unoccupied; }
+    size_type& neigenvalues() { return this->m_neigenvalues; }
+    const size_type& neigenvalues() const { return this->m_neigenvalues; }
+    bool& only_apply_when_no_unoccupied() { return this->m_only_unoccupied; }
+    const bool& only_apply_when_no_unoccupied() const { return this->m_only_unoccupied; }
+
+    const expansion_engine_view& engine() const {
+        m_view.m_integ = &m_handle;
+        m_handle = this->handle();
+        return m_view;
+    }
+
+  protected:
+    void forward_parameters() override {
+        const ttn_integ_params p = this->params();
+        const std::vector<double> now = {p.spawning_threshold, p.unoccupied_threshold,
+                                         static_cast<double>(p.minimum_unoccupied), static_cast<double>(p.neigenvalues),
+                                         static_cast<double>(p.only_apply_when_no_unoccupied)};
+        if (now == m_sent) return;
+        detail::check(ttn_set_expansion_params(this->handle(), &p), this->context(), "ttn_set_expansion_params");
+        m_sent = now;
+    }
+
+  private:
+    mutable expansion_engine_view m_view;
+    mutable ttn_integ* m_handle = nullptr;
+    std::vector<double> m_sent;
 };
 
 }  // namespace ttns

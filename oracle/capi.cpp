@@ -5,6 +5,7 @@
 #include "../ttns_lib_amd/csrc/host/tables.hpp"
 #include "oracle.hpp"
 
+#include <cmath>
 #include <memory>
 #include <string>
 
@@ -230,6 +231,35 @@ int orc_get_dt(const orc_integ* in, double* dt) {
 int orc_reset_nsteps(orc_integ* in) {
     in->eng->reset_nsteps();
     return TTN_OK;
+}
+int orc_set_expansion_params(orc_integ* in, const ttn_integ_params* p) {
+    return guard(in->ctx, [&] {
+        check(p != nullptr, ERR_INVALID, "null parameters");
+        check(in->eng->params().expansion, ERR_INVALID, "expansion parameters need a subspace-expansion integrator");
+        check(p->neigenvalues >= 1, ERR_INVALID, "neigenvalues must be positive");
+        check(p->minimum_unoccupied >= 0, ERR_INVALID, "minimum_unoccupied must not be negative");
+        orc::Params q = in->eng->params();
+        q.spawning_threshold = p->spawning_threshold;
+        q.unoccupied_threshold = p->unoccupied_threshold;
+        q.minimum_unoccupied = p->minimum_unoccupied;
+        q.neigenvalues = p->neigenvalues;
+        q.eigenvalue_tolerance = p->eigenvalue_tolerance > 0.0 ? p->eigenvalue_tolerance : 1e-6;
+        q.only_apply_when_no_unoccupied = p->only_apply_when_no_unoccupied != 0;
+        in->eng->set_params(q);
+    });
+}
+int orc_set_coefficients(orc_integ* in, int64_t nterms, const double* coeff) {
+    return guard(in->ctx, [&] {
+        check(coeff != nullptr && nterms == in->eng->nterms(), ERR_INVALID,
+              "set_coefficients: term count differs from the operator's");
+        std::vector<cplx> c(static_cast<size_t>(nterms));
+        for (int64_t r = 0; r < nterms; ++r) {
+            check(std::isfinite(coeff[2 * r]) && std::isfinite(coeff[2 * r + 1]), ERR_INVALID,
+                  "set_coefficients: non-finite coefficient");
+            c[r] = cplx(coeff[2 * r], coeff[2 * r + 1]);
+        }
+        in->eng->set_coefficients(c);
+    });
 }
 int orc_set_expansion_log(orc_integ* in, int64_t cap) {
     in->eng->set_log(cap);

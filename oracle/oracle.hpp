@@ -112,6 +112,14 @@ public:
     void set_dt(double dt) { m_dt = dt; }
     double dt() const { return m_dt; }
     void reset_nsteps() { m_nsteps = 0; }
+    const Params& params() const { return m_p; }
+    void set_params(const Params& p) { m_p = p; }
+    // new SOP coefficients: only the coefficient fields of the compiled tables change
+    i64 nterms() const { return m_sop.nterms; }
+    void set_coefficients(const std::vector<cplx>& c) {
+        m_sop.coeff = c;
+        m_ops = ttnh::compile_sop(m_topo, m_sop);
+    }
     // normal draws consumed by the expansion so far (the position in the std::mt19937 +
     // std::normal_distribution stream)
     i64 normals_used() const { return m_normals; }

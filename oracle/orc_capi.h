@@ -48,6 +48,8 @@ int orc_set_dt(orc_integ* integ, double dt);
 int orc_get_dt(const orc_integ* integ, double* dt);
 int orc_reset_nsteps(orc_integ* integ);
 int orc_expansion_draws(const orc_integ* integ, int64_t* n);
+int orc_set_coefficients(orc_integ* integ, int64_t nterms, const double* coeff);
+int orc_set_expansion_params(orc_integ* integ, const ttn_integ_params* p);
 int orc_set_expansion_log(orc_integ* integ, int64_t capacity);
 int orc_expansion_log(const orc_integ* integ, int64_t* len, double* out);
 int orc_expect(orc_integ* integ, int64_t mode, const double* op, double* out);

@@ -48,4 +48,17 @@ int ttn_counters(const ttn_integ* i, int64_t* nh, int64_t* ns) { return orc_coun
 int ttn_expect(ttn_integ* i, int64_t mode, const double* op, double* out) {
     return orc_expect(O(integ)(i), mode, op, out);
 }
+int ttn_expansion_counters(const ttn_integ* i, int64_t* n1, int64_t* n2) {
+    return orc_expansion_counters(OC(integ)(i), n1, n2);
+}
+int ttn_hranks(const ttn_integ* i, int64_t* hr) { return orc_hranks(OC(integ)(i), hr); }
+int ttn_state_shape(const ttn_integ* i, int64_t node, int64_t* r, int64_t* c) {
+    return orc_state_shape(OC(integ)(i), node, r, c);
+}
+int ttn_set_dt(ttn_integ* i, double dt) { return orc_set_dt(O(integ)(i), dt); }
+int ttn_reset_nsteps(ttn_integ* i) { return orc_reset_nsteps(O(integ)(i)); }
+int ttn_set_coefficients(ttn_integ* i, int64_t n, const double* c) { return orc_set_coefficients(O(integ)(i), n, c); }
+int ttn_set_expansion_params(ttn_integ* i, const ttn_integ_params* p) {
+    return orc_set_expansion_params(O(integ)(i), p);
+}
 }

@@ -1,10 +1,20 @@
 // Spin-boson driver written against the reference's C++ API (include/ttns_amd.hpp), the
 // way the reference's own driver sets up and runs a trajectory (src/spin_boson.cpp:447-649
-// for the tree, operator binding and initial state, :41-107 for the step loop).  Used by
-// the tests as the drop-in check: its <sigma_z(t)> must equal the oracle's on the same model.
+// for the tree, operator binding and initial state, :41-107 for the step loop, whose calls
+// run() below makes: tdvp(A, H, -i), tdvp.t(), tdvp.mel()(op, 0, A), tdvp.engine().Nonesite()
+// / Ntwosite(), tdvp.nh_applications() and every non-root node's hrank()).  Used by the tests
+// as the drop-in check: its rows must equal the oracle's on the same model.
 //
 // usage: sb_driver nmodes boson_dim nspf nsteps dt alpha wc seed krylov_dim
 //                  [ecut high_frequency_bound target_dimension maximum_dimension_high_frequency]
+//                  [--capacity C] [--spawn S] [--unoccupied U] [--min-unoccupied M]
+//                  [--dt2 DT --dt2-at K] [--coeff0 F --coeff0-at K] [--reset-at K] [--clear-at K]
+// Rows: t <sigma_z> Nonesite Ntwosite nh_applications/size hranks...  The first row is t = 0
+// (no counters).  --capacity lets the bonds grow to C (the expansion integrator's tree
+// capacity); --dt2 sets tdvp.dt() = DT after K steps; --coeff0 multiplies the coefficient of
+// term 0 (the system Hamiltonian) by F after K steps; --reset-at calls reset_nsteps() after K
+// steps; --clear-at calls clear(), restores the parameters and initialise()s again after K
+// steps (each of these with the state the run has reached).
 // Bath: w_k = wc * 4 k / n (k = 1..n), g_k = sqrt(J(w_k) dw / pi), J(w) = pi/2 alpha w e^{-w/wc}.
 // The modes are split into leaves with partition_modes and each leaf's operators come from
 // harmonic_mode_combination, as in the reference driver (src/spin_boson.cpp:418-441,
@@ -14,8 +24,12 @@
 #include <complex>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
+#include <functional>
 #include <iostream>
+#include <map>
 #include <random>
+#include <string>
 #include <vector>
 
 #include "ttns_amd.hpp"
@@ -24,7 +38,41 @@ using namespace ttns;
 using complex_type = std::complex<double>;
 using backend_type = blas_backend;
 
+template <class integ_type, class op_type>
+void run(httensor<complex_type, backend_type>& A, integ_type& tdvp, sop_operator<complex_type, backend_type>& H,
+         const op_type& mop, int nsteps, const std::function<void(int)>& between) {
+    auto row_hranks = [&]() {
+        for (auto& c : A)
+            if (!c.is_root()) std::printf(" %lld", static_cast<long long>(c.hrank()));
+        std::printf("\n");
+    };
+    std::printf("%.17g %.17g", 0.0, std::real(tdvp.mel()(mop, 0, A)));
+    row_hranks();
+    for (int s = 0; s < nsteps; ++s) {
+        tdvp(A, H, complex_type(0, -1));
+        std::printf("%.17g %.17g %zu %zu %.17g", tdvp.t(), std::real(tdvp.mel()(mop, 0, A)),
+                    static_cast<size_t>(tdvp.engine().Nonesite()), static_cast<size_t>(tdvp.engine().Ntwosite()),
+                    static_cast<double>(tdvp.nh_applications()) / static_cast<double>(A.size()));
+        row_hranks();
+        between(s + 1);
+    }
+}
+
 int main(int argc, char** argv) {
+    std::vector<char*> pos;
+    std::map<std::string, double> opt;
+    for (int i = 1; i < argc; ++i) {
+        if (std::strncmp(argv[i], "--", 2) == 0 && i + 1 < argc) {
+            opt[argv[i] + 2] = std::atof(argv[i + 1]);
+            ++i;
+        } else {
+            pos.push_back(argv[i]);
+        }
+    }
+    auto get = [&](const char* k, double d) { return opt.count(k) ? opt[k] : d; };
+    argc = static_cast<int>(pos.size()) + 1;
+    pos.insert(pos.begin(), argv[0]);
+    argv = pos.data();
     if (argc < 10) {
         std::cerr << "usage: sb_driver nmodes boson_dim nspf nsteps dt alpha wc seed krylov_dim"
                      " [ecut high_frequency_bound target_dimension maximum_dimension_high_frequency]\n";
@@ -90,7 +138,8 @@ int main(int argc, char** argv) {
             ntree_builder<size_type>::sanitise_tree(t);
             return t;
         };
-        ntree<size_type> topology = build(nspf, nspf_lower), capacity = build(nspf, nspf_lower);
+        const size_type cap = static_cast<size_type>(get("capacity", static_cast<double>(nspf)));
+        ntree<size_type> topology = build(nspf, nspf_lower), capacity = build(cap, cap);
         httensor<complex_type, backend_type> A(topology, capacity);
 
         // Hamiltonian: Hsys on mode 0, Scoup on mode 0 for terms 2+2i, H0_i and Hc_i on mode i+1
@@ -169,17 +218,29 @@ int main(int argc, char** argv) {
         A.set_is_orthogonalised();
 
         subspace_expansion_projector_splitting_integrator<complex_type, backend_type> tdvp(A, H, krylov_dim, 1e-12, 1);
-        tdvp.dt() = dt;
-        tdvp.spawning_threshold() = 1e-7;
-        tdvp.unoccupied_threshold() = 1e-4;
-        tdvp.minimum_unoccupied() = 0;
+        auto knobs = [&]() {
+            tdvp.dt() = dt;
+            tdvp.spawning_threshold() = get("spawn", 1e-7);
+            tdvp.unoccupied_threshold() = get("unoccupied", 1e-4);
+            tdvp.minimum_unoccupied() = static_cast<size_type>(get("min-unoccupied", 0.0));
+        };
+        knobs();
         ops::dense_matrix_operator<complex_type, backend_type> mop(Sz);
-        std::printf("%.17g %.17g\n", 0.0, std::real(tdvp.mel()(mop, 0, A)));
-        for (int s = 0; s < nsteps; ++s) {
-            tdvp(A, H, complex_type(0, -1));
-            std::printf("%.17g %.17g %zu\n", tdvp.t(), std::real(tdvp.mel()(mop, 0, A)),
-                        static_cast<size_t>(tdvp.nh_applications()));
-        }
+        const int dt2_at = static_cast<int>(get("dt2-at", -1.0)), c0_at = static_cast<int>(get("coeff0-at", -1.0));
+        const int reset_at = static_cast<int>(get("reset-at", -1.0)), clear_at = static_cast<int>(get("clear-at", -1.0));
+        run(A, tdvp, H, mop, nsteps, [&](int done) {
+            if (done == dt2_at) tdvp.set_dt(get("dt2", dt));
+            if (done == c0_at) H.coeff(0) *= get("coeff0", 1.0);
+            if (done == reset_at) tdvp.reset_nsteps();
+            if (done == clear_at) {
+                const double dtnow = tdvp.dt();
+                tdvp.clear();
+                tdvp.krylov_dim() = krylov_dim;
+                knobs();
+                tdvp.dt() = dtnow;
+                tdvp.initialise(A, H);
+            }
+        });
     } catch (const std::exception& ex) {
         std::cerr << ex.what() << std::endl;
         return 1;

@@ -38,16 +38,22 @@ def test_node_mean_fields_match_oracle(hip, orc, d0, d1, r, nt, child):
     g.destroy()
 
 
-@pytest.mark.parametrize("d0,d1,r,nt,tau", [(16, 16, 16, 5, 0.05), (24, 20, 16, 4, 0.3)])
-def test_node_expv_matches_oracle(hip, orc, d0, d1, r, nt, tau):
+# the last case is the contraction microbenchmark's node (64 x 64 x 64, 40 terms) with tau chosen
+# so that the propagation fills the whole 16-vector Lanczos space in one sub-step (16 H-actions)
+@pytest.mark.parametrize("d0,d1,r,nt,tau,scale,na_want", [(16, 16, 16, 5, 0.05, None, None),
+                                                          (24, 20, 16, 4, 0.3, None, None),
+                                                          (64, 64, 64, 40, 0.12, 0.5, 16)])
+def test_node_expv_matches_oracle(hip, orc, d0, d1, r, nt, tau, scale, na_want):
     print(f"[node expv] {d0}x{d1}x{r} terms {nt} tau {tau}", flush=True)
-    P = NodeProblem(orc, d0, d1, r, nt, seed=11, scale=1.0 / d0)
+    P = NodeProblem(orc, d0, d1, r, nt, seed=11, scale=1.0 / d0 if scale is None else scale)
+    v = P.v / np.linalg.norm(P.v)
     g = Node(hip, P.op0, P.op1, P.mf)
-    g.set(P.v)
+    g.set(v)
     na = g.expv(tau, krylov_dim=16, tol=1e-12)
-    want, na_orc = P.expv(P.v, tau, 16, 1e-12)
+    want, na_orc = P.expv(v, tau, 16, 1e-12)
     assert na == na_orc
+    assert na_want is None or na == na_want
     assert _rel(g.get(), want) < 1e-11
     # unitary propagation
-    assert abs(np.linalg.norm(g.get()) - np.linalg.norm(P.v)) < 1e-11 * np.linalg.norm(P.v)
+    assert abs(np.linalg.norm(g.get()) - 1.0) < 1e-11
     g.destroy()

@@ -125,6 +125,32 @@ def test_nan_state_reports_numeric_error(hip):
     assert e.value.code == 1
 
 
+def test_batch_isolates_a_nan_instance(hip):
+    # one poisoned instance ends in TTN_ERR_NUMERIC naming it; every other instance of the
+    # launch completes its steps bit for bit as when stepped alone
+    print("[batch nan isolation] 5 instances, instance 2 poisoned", flush=True)
+    models = [sb_model(8, 6, 4, seed=s, alpha=0.05 * (s + 1)) for s in range(5)]
+    singles = [make_integrator(hip, m)[2] for m in models]
+    batched = [make_integrator(hip, m)[2] for m in models]
+    bad = batched[2].get_node(4).copy()
+    bad[0, 0] = np.nan
+    batched[2].set_node(4, bad)
+    for k, a in enumerate(singles):
+        if k != 2:
+            a.step(3)
+    B = Batch(hip, batched)
+    with pytest.raises(TTNError) as e:
+        B.step(3)
+    assert e.value.code == 1 and "batch instance 2" in str(e.value)
+    for k, (a, b) in enumerate(zip(singles, batched)):
+        if k == 2:
+            assert b.counters()[1] == 0  # the failing instance completed no step
+            continue
+        assert b.counters()[1] == 3 and b.time() == a.time()
+        for nd in range(len(a.shapes)):
+            assert np.array_equal(a.get_node(nd), b.get_node(nd))
+
+
 def _op(a):
     import ctypes as C
 

@@ -97,6 +97,8 @@ _SIGS = {
     "set_dt": [vp, C.c_double],
     "get_dt": [vp, dblp],
     "reset_nsteps": [vp],
+    "set_coefficients": [vp, i64, dblp],
+    "set_expansion_params": [vp, C.c_void_p],
     "expansion_draws": [vp, i64p],
     "set_expansion_log": [vp, i64],
     "expansion_log": [vp, i64p, dblp],
@@ -431,6 +433,21 @@ class Integrator:
 
     def reset_nsteps(self):
         self.lib._call("reset_nsteps", self.h)
+
+    def set_expansion_params(self, **kw):
+        """changes spawning_threshold, unoccupied_threshold, minimum_unoccupied, neigenvalues,
+        eigenvalue_tolerance or only_apply_when_no_unoccupied of an expansion integrator"""
+        for k, v in kw.items():
+            if k not in ("spawning_threshold", "unoccupied_threshold", "minimum_unoccupied", "neigenvalues",
+                         "eigenvalue_tolerance", "only_apply_when_no_unoccupied"):
+                raise TypeError(f"not an expansion parameter: {k}")
+            setattr(self.params, k, v)
+        self.lib._call("set_expansion_params", self.h, C.cast(C.pointer(self.params), C.c_void_p))
+
+    def set_coefficients(self, coeff):
+        """new SOP coefficients (one complex per term), used from the next step on"""
+        c = np.ascontiguousarray(np.asarray(coeff, dtype=np.complex128))
+        self.lib._call("set_coefficients", self.h, int(c.size), _dp(c.view(np.float64)))
 
     def set_expansion_log(self, capacity: int):
         self.lib._call("set_expansion_log", self.h, int(capacity))

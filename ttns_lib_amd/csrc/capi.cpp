@@ -71,6 +71,12 @@ struct ttn_integ {
     DevBuf<double> d_rng;
     DevBuf<int64_t> d_exp;          // ExpWord words
     DevBuf<double> d_exp_log;       // expansion log (ttn_set_expansion_log)
+    // host copies of the coefficient-bearing tables and their byte offsets in `tables`
+    // (ttn_set_coefficients patches and re-uploads them)
+    std::vector<ttnd::DTerm> h_terms;
+    std::vector<ttnd::DSub> h_subs;
+    std::vector<ttnd::DMf> h_mfs;
+    size_t o_terms = 0, o_subs = 0, o_mfs = 0;
     FlopModel fm;
     int64_t sweep_pairs = 0;   // forward+backward sweeps run (with or without propagation)
     int64_t prepares = 0;
@@ -118,6 +124,33 @@ struct ttn_batch {
 };
 
 namespace {
+
+// The coefficients of the compiled tables (a term's own factor, the sum of its accumulation
+// coefficients, accum_coeff(0), and the per-sub-term and per-mean-field accumulation
+// coefficients): the only fields of the tables that depend on the SOP's coefficients.
+void term_coeffs(const Term& tm, ttnd::DTerm& d) {
+    d.cre = tm.coeff.real();
+    d.cim = tm.coeff.imag();
+    cplx ic = tm.coeff;
+    if (tm.nspf_terms() > 1) {
+        ic = cplx(0.0, 0.0);
+        for (const cplx& a : tm.accum) ic += a;
+    }
+    d.ire = ic.real();
+    d.iim = ic.imag();
+    d.a0re = tm.accum.empty() ? 0.0 : tm.accum[0].real();
+    d.a0im = tm.accum.empty() ? 0.0 : tm.accum[0].imag();
+}
+void sub_coeff(const Term& tm, int s, ttnd::DSub& ds) {
+    const cplx a = (tm.nspf_terms() > 1) ? tm.accum.at(s) : cplx(0.0, 0.0);
+    ds.are = a.real();
+    ds.aim = a.imag();
+}
+void mf_coeff(const Term& tm, int m, ttnd::DMf& dm) {
+    const cplx a = (tm.nmf_terms() > 1) ? tm.accum.at(m) : cplx(0.0, 0.0);
+    dm.are = a.real();
+    dm.aim = a.imag();
+}
 
 // Flatten topology + compiled tables into device descriptors and lay out the arena.
 void build_instance(ttn_integ* in) {
@@ -223,26 +256,14 @@ void build_instance(ttn_integ* in) {
             ttnd::DTerm d{};
             d.id_spf = tm.id_spf;
             d.id_mf = tm.id_mf;
-            d.cre = tm.coeff.real();
-            d.cim = tm.coeff.imag();
-            cplx ic = tm.coeff;
-            if (tm.nspf_terms() > 1) {
-                ic = cplx(0.0, 0.0);
-                for (const cplx& a : tm.accum) ic += a;
-            }
-            d.ire = ic.real();
-            d.iim = ic.imag();
-            d.a0re = tm.accum.empty() ? 0.0 : tm.accum[0].real();
-            d.a0im = tm.accum.empty() ? 0.0 : tm.accum[0].imag();
+            term_coeffs(tm, d);
             d.spf = spf_off[i][k];
             d.mf = mf_off[i][k];
             d.sub0 = static_cast<int>(subs.size());
             d.nsub = tm.nspf_terms();
             for (int s = 0; s < tm.nspf_terms(); ++s) {
                 ttnd::DSub ds{};
-                const cplx a = (tm.nspf_terms() > 1) ? tm.accum.at(s) : cplx(0.0, 0.0);
-                ds.are = a.real();
-                ds.aim = a.imag();
+                sub_coeff(tm, s, ds);
                 ds.k0 = static_cast<int>(ks.size());
                 ds.nk = static_cast<int>(tm.spf_index[s].size());
                 for (const auto& pr : tm.spf_index[s]) {
@@ -266,9 +287,7 @@ void build_instance(ttn_integ* in) {
             for (int m = 0; m < tm.nmf_terms(); ++m) {
                 const MfIndex& mi = tm.mf_index[m];
                 ttnd::DMf dm{};
-                const cplx a = (tm.nmf_terms() > 1) ? tm.accum.at(m) : cplx(0.0, 0.0);
-                dm.are = a.real();
-                dm.aim = a.imag();
+                mf_coeff(tm, m, dm);
                 const int p = nd.parent;
                 dm.pmf = ops.nodes[p].terms[mi.parent_index].id_mf ? -1 : mf_off[p][mi.parent_index];
                 dm.k0 = static_cast<int>(ks.size());
@@ -342,6 +361,12 @@ void build_instance(ttn_integ* in) {
     const size_t o_mfs = pack(mfs.data(), mfs.size() * sizeof(ttnd::DMf));
     const size_t o_tour = pack(tour_i.data(), tour_i.size() * sizeof(int));
     const size_t o_ints = pack(ints.data(), ints.size() * sizeof(int64_t));
+    in->h_terms = terms;
+    in->h_subs = subs;
+    in->h_mfs = mfs;
+    in->o_terms = o_terms;
+    in->o_subs = o_subs;
+    in->o_mfs = o_mfs;
     in->tables.alloc(blob.size());
     hip_check(hipMemcpyAsync(in->tables.p, blob.data(), blob.size(), hipMemcpyHostToDevice, in->ctx->stream),
               "upload tables");
@@ -1135,6 +1160,61 @@ int ttn_expansion_draws(const ttn_integ* in, int64_t* n) {
 int ttn_reset_nsteps(ttn_integ* in) {
     in->nsteps = 0;
     return TTN_OK;
+}
+
+int ttn_set_expansion_params(ttn_integ* in, const ttn_integ_params* p) {
+    return guard(in->ctx, [&] {
+        check(p != nullptr, ERR_INVALID, "null parameters");
+        check(in->expand, ERR_INVALID, "expansion parameters need a subspace-expansion integrator");
+        check(p->neigenvalues >= 1, ERR_INVALID, "neigenvalues must be positive");
+        check(p->minimum_unoccupied >= 0, ERR_INVALID, "minimum_unoccupied must not be negative");
+        in->p.spawning_threshold = p->spawning_threshold;
+        in->p.unoccupied_threshold = p->unoccupied_threshold;
+        in->p.minimum_unoccupied = p->minimum_unoccupied;
+        in->p.neigenvalues = p->neigenvalues;
+        in->p.eigenvalue_tolerance = p->eigenvalue_tolerance;
+        in->p.only_apply_when_no_unoccupied = p->only_apply_when_no_unoccupied;
+        ttnd::DInst& I = in->inst;  // uploaded with every launch
+        I.neig = static_cast<int>(p->neigenvalues);
+        I.min_unocc = static_cast<int>(p->minimum_unoccupied);
+        I.only_unocc = p->only_apply_when_no_unoccupied != 0 ? 1 : 0;
+        I.spawn = p->spawning_threshold;
+        I.unocc_thr = p->unoccupied_threshold;
+        I.eig_tol = p->eigenvalue_tolerance > 0.0 ? p->eigenvalue_tolerance : 1e-6;
+    });
+}
+
+int ttn_set_coefficients(ttn_integ* in, int64_t nterms, const double* coeff) {
+    return guard(in->ctx, [&] {
+        check(nterms == in->sop.nterms, ERR_INVALID, "set_coefficients: term count differs from the operator's");
+        check(coeff != nullptr, ERR_INVALID, "set_coefficients: null coefficients");
+        for (int64_t r = 0; r < nterms; ++r) {
+            check(std::isfinite(coeff[2 * r]) && std::isfinite(coeff[2 * r + 1]), ERR_INVALID,
+                  "set_coefficients: non-finite coefficient");
+            in->sop.coeff[r] = cplx(coeff[2 * r], coeff[2 * r + 1]);
+        }
+        // coefficients are copied into the compiled tables, never used to shape them: the
+        // recompiled tables differ from the uploaded ones in the coefficient fields only
+        in->ops = compile_sop(in->topo, in->sop);
+        size_t ti = 0, si = 0, mi = 0;
+        for (int i = 0; i < in->topo.size(); ++i)
+            for (const Term& tm : in->ops.nodes[i].terms) {
+                check(ti < in->h_terms.size(), ERR_INVALID, "set_coefficients: compiled tables changed shape");
+                term_coeffs(tm, in->h_terms[ti++]);
+                for (int q = 0; q < tm.nspf_terms(); ++q) sub_coeff(tm, q, in->h_subs.at(si++));
+                for (int q = 0; q < tm.nmf_terms(); ++q) mf_coeff(tm, q, in->h_mfs.at(mi++));
+            }
+        check(ti == in->h_terms.size() && si == in->h_subs.size() && mi == in->h_mfs.size(), ERR_INVALID,
+              "set_coefficients: compiled tables changed shape");
+        hipStream_t s = in->ctx->stream;
+        hip_check(hipStreamSynchronize(s), "sync");
+        auto up = [&](size_t off, const void* src, size_t bytes) {
+            if (bytes) hip_check(hipMemcpy(in->tables.p + off, src, bytes, hipMemcpyHostToDevice), "coefficients");
+        };
+        up(in->o_terms, in->h_terms.data(), in->h_terms.size() * sizeof(ttnd::DTerm));
+        up(in->o_subs, in->h_subs.data(), in->h_subs.size() * sizeof(ttnd::DSub));
+        up(in->o_mfs, in->h_mfs.data(), in->h_mfs.size() * sizeof(ttnd::DMf));
+    });
 }
 
 static void run_expect(ttn_integ* in, int leaf, int64_t op_off, double* out3) {
