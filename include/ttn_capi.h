@@ -79,6 +79,9 @@ int ttn_tree_destroy(ttn_tree* tree);
 int ttn_tree_node(const ttn_tree* tree, int64_t node, int64_t* nchild, int64_t* rows, int64_t* hrank,
                   int64_t* leaf_index);
 int ttn_tree_size(const ttn_tree* tree, int64_t* n_nodes, int64_t* n_modes);
+/* bond capacity of every node, pre-order: the values of the capacity ntree the reference's
+ * httensor(topology, capacity) takes (httensor.hpp:128-160) */
+int ttn_tree_capacities(const ttn_tree* tree, int64_t* cap_hranks);
 /* sweep order; order may be NULL to query the length */
 int ttn_traversal(const ttn_tree* tree, int64_t* len, int64_t* order);
 

@@ -90,6 +90,12 @@ int orc_tree_node(const orc_tree* t, int64_t node, int64_t* nchild, int64_t* row
         if (leaf) *leaf = nd.leaf_index;
     });
 }
+int orc_tree_capacities(const orc_tree* t, int64_t* caps) {
+    return guard(t->ctx, [&] {
+        check(caps != nullptr, ERR_INVALID, "null output");
+        for (int i = 0; i < t->topo.size(); ++i) caps[i] = t->topo.nodes[i].cap_hrank;
+    });
+}
 int orc_tree_size(const orc_tree* t, int64_t* n, int64_t* m) {
     if (n) *n = t->topo.size();
     if (m) *m = t->topo.nmodes();

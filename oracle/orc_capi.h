@@ -25,6 +25,7 @@ int orc_tree_create(orc_ctx* ctx, int64_t n_nodes, const int64_t* parent, const 
 int orc_tree_destroy(orc_tree* tree);
 int orc_tree_node(const orc_tree* tree, int64_t node, int64_t* nchild, int64_t* rows, int64_t* hrank,
                   int64_t* leaf_index);
+int orc_tree_capacities(const orc_tree* tree, int64_t* cap_hranks);
 int orc_tree_size(const orc_tree* tree, int64_t* n_nodes, int64_t* n_modes);
 int orc_traversal(const orc_tree* tree, int64_t* len, int64_t* order);
 int orc_sop_create(orc_ctx* ctx, int64_t nmodes, const int64_t* mode_dims, int64_t nterms, const double* coeff,

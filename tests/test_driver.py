@@ -1,7 +1,10 @@
 """The driver front-end (ttns_lib_amd/driver.py) on the reference's own example runs: it reads
 the reference's input keys and the bath log format, and its output rows must match the
-reference's .out files (the t = 0 row and bond dimensions exactly, the first steps' <O> within
-the bounds the oracle meets in test_ref_outputs.py).  On the CPU the driver runs over the
+reference's .out files: the t = 0 row, the Nonesite / Ntwosite counters and every bond
+dimension exactly over the steps in which the oracle reproduces the reference's bond growth
+(test_expansion.py), <O> within the bounds the oracle meets in test_ref_outputs.py.  The
+nh/|A| column follows the reference's un-vendored Krylov routine, whose H-application count
+runs about 6 % below this library's; it is checked within 10 %.  On the CPU the driver runs over the
 oracle library (the fake backend of these tests); the GPU half runs the command line over the
 HIP library."""
 import io
@@ -40,7 +43,8 @@ def parse(text):
         if ln.startswith("nh_app"):
             continue
         p = ln.split()
-        rows.append(dict(values=[float(x) for x in p[:5]], hranks=[int(x) for x in p[8:]], text=ln))
+        rows.append(dict(values=[float(x) for x in p[:5]], counters=[float(x) for x in p[5:8]],
+                         hranks=[int(x) for x in p[8:]], text=ln))
     return rows
 
 
@@ -48,11 +52,16 @@ def check_rows(rows, run, steps, bound):
     ref = run["rows"]
     assert len(rows) == steps + 1
     assert rows[0]["text"].startswith("0 0 0 1 1  0 0 0 ")
-    assert rows[0]["hranks"] == ref[0]["hranks"]
+    if ref[0].get("hranks"):
+        assert rows[0]["hranks"] == ref[0]["hranks"]
     for r, e in zip(rows[1:], ref[1:1 + steps]):
         assert abs(r["values"][0] - e["values"][0]) < 1e-15
         assert np.abs(np.array(r["values"][1:4]) - np.array(e["values"][1:4])).max() < bound
         assert abs(r["values"][4] - e["values"][4]) < 1e-12
+        assert r["counters"][:2] == e["counters"][:2]
+        assert abs(r["counters"][2] / e["counters"][2] - 1.0) < 0.1
+        if e.get("hranks"):
+            assert r["hranks"] == e["hranks"]
 
 
 def test_bath_log_and_input_parsing(tmp_path):
@@ -76,7 +85,6 @@ def test_driver_over_oracle_matches_reference_output(orc, tmp_path, name, steps,
     wk, gk = driver.load_bath(bath)
     out, log = io.StringIO(), io.StringIO()
     driver.run(orc, cfg, wk, gk, out, log=log)
-    assert "not available" in log.getvalue()
     check_rows(parse(out.getvalue()), run, steps, bound)
     assert out.getvalue().rstrip().splitlines()[-1].startswith("nh_app: ")
 
@@ -87,15 +95,17 @@ def test_driver_command_on_gpu_matches_reference_output(tmp_path, name, steps, b
     run, inp, bath, out = write_case(tmp_path, name, steps)
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     print(f"[driver] {name}, {steps} steps", flush=True)
-    subprocess.run([sys.executable, "-m", "ttns_lib_amd.driver", inp, "--bath", bath], check=True, cwd=root,
-                   timeout=300)
+    # capacity 64 in place of 192: no bond grows past 12 in these steps (the oracle's growth is
+    # the same with either, test_expansion.py)
+    subprocess.run([sys.executable, "-m", "ttns_lib_amd.driver", inp, "--bath", bath, "--max-capacity", "64"],
+                   check=True, cwd=root, timeout=300)
     with open(out) as f:
         check_rows(parse(f.read()), run, steps, bound)
 
 
 @pytest.mark.gpu
 def test_driver_batch_of_inputs_on_gpu_matches_reference_output(tmp_path):
-    # two inputs in one command: advanced together in one batched launch per step
+    # two inputs in one command, advanced together (one at a time while their bonds can grow)
     cases = [("ohmic_sudden", 2, 1e-10), ("subohmic_exponential_t0", 2, 5e-10)]
     made = []
     for name, steps, bound in cases:
@@ -103,7 +113,7 @@ def test_driver_batch_of_inputs_on_gpu_matches_reference_output(tmp_path):
         d.mkdir()
         made.append((write_case(d, name, steps), steps, bound))
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    cmd = [sys.executable, "-m", "ttns_lib_amd.driver"] + [m[0][1] for m in made]
+    cmd = [sys.executable, "-m", "ttns_lib_amd.driver", "--max-capacity", "64"] + [m[0][1] for m in made]
     for m in made:
         cmd += ["--bath", m[0][2]]
     print("[driver] batch of two inputs", flush=True)

@@ -80,6 +80,7 @@ _SIGS = {
     "tree_destroy": [vp],
     "tree_node": [vp, i64, i64p, i64p, i64p, i64p],
     "tree_size": [vp, i64p, i64p],
+    "tree_capacities": [vp, i64p],
     "traversal": [vp, i64p, i64p],
     "sop_create": [vp, i64, i64p, i64, dblp, i64, i32p, i32p, i64p, dblp, i64p, i64p, i64p, i64p, C.POINTER(vp)],
     "sop_destroy": [vp],
@@ -311,6 +312,13 @@ class Tree:
         a, b, c, d = i64(), i64(), i64(), i64()
         self.lib._call("tree_node", self.h, i, C.byref(a), C.byref(b), C.byref(c), C.byref(d))
         return dict(nchild=a.value, rows=b.value, hrank=c.value, leaf_index=d.value)
+
+    def capacities(self) -> np.ndarray:
+        """bond capacity of every node (pre-order)"""
+        n, _ = self.size()
+        out = np.zeros(n, dtype=np.int64)
+        self.lib._call("tree_capacities", self.h, out.ctypes.data_as(i64p))
+        return out
 
     def shapes(self):
         n, _ = self.size()

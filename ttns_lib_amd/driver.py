@@ -13,17 +13,23 @@
   `t <O_1> ... <O_n> Nonesite Ntwosite nh/|A| [hranks of the non-root nodes]`, 16 significant
   digits, with the t = 0 row first.
 
-Bond dimensions stay at their initial values: the bond growth of the expansion integrator is
-not available on the device, so an input whose capacities exceed the initial bond dimensions
-runs the one-site step (as the reference does while no bond grows) and says so on stderr.  The
-expansion counters (Nonesite, Ntwosite) are written as 0.
+* runs the reference driver's integrator, the subspace-expansion one (:650-657), with its
+  defaults: spawning parameter and unoccupied threshold 1e-12, minimum unoccupied = the nspf
+  capacity, capacities at least the initial bond dimensions (:257-278).  The bonds grow up to
+  the capacities; Nonesite / Ntwosite and the current bond dimensions go into the rows.
 
-Several inputs (each with its own --bath, in order) run as one batch: all their instances
-advance together in one persistent launch per step (ttn_batch_step), in place of the
-reference's process pool over inputs (run_examples.py); each writes its own output file.
+The device holds bond dimensions up to 64.  An input whose capacities exceed that (the
+reference's examples ask for 192) is refused unless --max-capacity lowers them; with it, a
+note on stderr marks the first step at which a bond reaches a lowered capacity, from where the
+run may differ from the reference's.
+
+Several inputs (each with its own --bath, in order) advance together, in place of the
+reference's process pool over inputs (run_examples.py); each writes its own output file.  Runs
+that share dt and whose bonds are all at capacity step in one persistent launch
+(ttn_batch_step); runs whose bonds can still grow step one after another.
 
 usage: python -m ttns_lib_amd.driver input.json [input2.json ...] --bath bath.txt [--bath bath2.txt ...]
-                                     [--output out] [--steps N]
+                                     [--output out] [--steps N] [--max-capacity C]
 """
 from __future__ import annotations
 
@@ -35,7 +41,7 @@ from typing import Optional, TextIO
 
 import numpy as np
 
-from .capi import Batch, Integrator, Library, SpinBoson, driver_blocks, sb_problem
+from .capi import MODE_SUBSPACE_EXPANSION, Batch, Integrator, Library, SpinBoson, driver_blocks, sb_problem
 
 
 def normalise_keys(doc: dict) -> dict:
@@ -75,13 +81,23 @@ def observables(cfg: dict):
     return [np.array([[_cplx(v) for v in row] for row in m], dtype=complex) for m in mats]
 
 
-def model_from_input(lib: Library, cfg: dict, wk, gk) -> SpinBoson:
+def capacities(cfg: dict):
+    """(nspf capacity, nspf lower capacity): at least the initial bond dimensions (:269-275)"""
+    nspf, lower = int(cfg["nspf"]), int(cfg["nspflower"])
+    return max(int(cfg.get("nspfcapacity", nspf)), nspf), max(int(cfg.get("nspflowercapacity", lower)), lower)
+
+
+def model_from_input(lib: Library, cfg: dict, wk, gk, max_capacity: Optional[int] = None) -> SpinBoson:
     nmax = int(cfg["maximumdimension"])
+    cap, lower_cap = capacities(cfg)
+    if max_capacity is not None:
+        cap, lower_cap = min(cap, max(max_capacity, int(cfg["nspf"]))), min(lower_cap, max(max_capacity,
+                                                                                          int(cfg["nspflower"])))
     sizes, caps = driver_blocks(lib, wk, ecut=3000.0, high_frequency_bound=float(cfg.get("highfrequencybound", 10.0)),
                                 target_dimension=int(cfg["targetdimension"]), maximum_dimension=nmax,
                                 maximum_dimension_high_frequency=int(cfg.get("maximumdimensionhighfrequency", nmax)))
     return SpinBoson(wk=wk, gk=gk, block_sizes=sizes, block_nmax=caps, nspf=int(cfg["nspf"]),
-                     nspf_lower=int(cfg["nspflower"]), seed=int(cfg.get("seed", 0)),
+                     nspf_lower=int(cfg["nspflower"]), nspf_cap=cap, nspf_lower_cap=lower_cap, seed=int(cfg.get("seed", 0)),
                      hsys=np.array([[_cplx(v) for v in row] for row in cfg["hsys"]], dtype=complex),
                      scoup=np.array([[_cplx(v) for v in row] for row in cfg["scoup"]], dtype=complex),
                      psi0=np.array([_cplx(v) for v in cfg["psi0"]], dtype=complex))
@@ -94,24 +110,37 @@ def _g(x: float) -> str:
 class Job:
     """One input: its integrator and the rows it writes."""
 
-    def __init__(self, lib: Library, cfg: dict, wk, gk, out: TextIO, log: TextIO = sys.stderr):
-        model = model_from_input(lib, cfg, wk, gk)
-        if int(cfg.get("nspfcapacity", model.nspf)) > model.nspf or \
-                int(cfg.get("nspflowercapacity", model.nspf_lower)) > model.nspf_lower:
-            print("bond growth (subspace expansion) is not available: bonds stay at their initial dimensions",
-                  file=log)
+    def __init__(self, lib: Library, cfg: dict, wk, gk, out: TextIO, log: TextIO = sys.stderr,
+                 max_capacity: Optional[int] = None):
+        model = model_from_input(lib, cfg, wk, gk, max_capacity)
+        self.lowered = (model.nspf_cap, model.nspf_lower_cap) != capacities(cfg)
+        if self.lowered:
+            print(f"capacities lowered from {capacities(cfg)} to {(model.nspf_cap, model.nspf_lower_cap)}", file=log)
+        self.log = log
         self.tree, self.sop = sb_problem(lib, model)
-        # defaults of the reference driver (src/spin_boson.cpp:251-255)
+        cap = model.nspf_cap
+        # defaults of the reference driver (src/spin_boson.cpp:251-278)
         self.ig = Integrator(lib, self.tree, self.sop, dt=float(cfg["dt"]),
                              krylov_dim=int(cfg.get("krylovdimension", 6)),
-                             krylov_tol=float(cfg.get("krylovtolerance", 1e-12)))
+                             krylov_tol=float(cfg.get("krylovtolerance", 1e-12)), mode=MODE_SUBSPACE_EXPANSION,
+                             spawning_threshold=float(cfg.get("spawningparameter", 1e-12)),
+                             unoccupied_threshold=float(cfg.get("unoccupiedthreshold", 1e-12)),
+                             minimum_unoccupied=int(cfg.get("minimumunoccupied", capacities(cfg)[0])))
         self.ig.set_sb_initial_state(model)
+        self.caps = self.tree.capacities()
         self.ops = observables(cfg)
-        self.hranks = " ".join(str(r) for _, r in self.ig.shapes[1:]) + " " if cfg.get("printbonddimension") else ""
+        self.print_hranks = bool(cfg.get("printbonddimension"))
         self.nnodes = len(self.ig.shapes)
         self.tmax, self.steps, self.out = float(cfg["tmax"]), 0, out
+        self.noted = False
         out.write("0 " + "".join(_g(self.ig.expect(0, o).real) + " " for o in self.ops) + " 0 0 0 " +
-                  self.hranks + "\n")
+                  self.hranks() + "\n")
+
+    def hranks(self) -> str:
+        return " ".join(str(r) for r in self.ig.hranks()[1:]) + " " if self.print_hranks else ""
+
+    def can_grow(self) -> bool:
+        return bool(np.any(np.asarray(self.ig.hranks()) < self.caps))
 
     def running(self, max_steps: Optional[int]) -> bool:
         return self.ig.time() < self.tmax and (max_steps is None or self.steps < max_steps)
@@ -121,17 +150,23 @@ class Job:
         self.steps += 1
         vals = "".join(_g(self.ig.expect(0, o).real) + " " for o in self.ops)
         nh = self.ig.counters()[0]
-        self.out.write(f"{_g(self.ig.time())} {vals}0 0 {_g(nh / self.nnodes)} {self.hranks}\n")
+        n1, n2 = self.ig.expansion_counters()
+        self.out.write(f"{_g(self.ig.time())} {vals}{n1} {n2} {_g(nh / self.nnodes)} {self.hranks()}\n")
         self.out.flush()
+        if self.lowered and not self.noted and np.any(np.asarray(self.ig.hranks()) >= self.caps):
+            self.noted = True
+            print(f"step {self.steps}: a bond reached its lowered capacity; from here the run may differ from "
+                  "one with the input's capacities", file=self.log)
 
     def finish(self):
         self.out.write(f"nh_app: {_g(self.ig.counters()[0] / self.nnodes)}\n")
         self.ig.destroy()
 
 
-def run(lib: Library, cfg: dict, wk, gk, out: TextIO, max_steps: Optional[int] = None, log: TextIO = sys.stderr):
+def run(lib: Library, cfg: dict, wk, gk, out: TextIO, max_steps: Optional[int] = None, log: TextIO = sys.stderr,
+        max_capacity: Optional[int] = None):
     """Build the run described by `cfg` over `lib` and write its output rows to `out`."""
-    job = Job(lib, cfg, wk, gk, out, log)
+    job = Job(lib, cfg, wk, gk, out, log, max_capacity)
     try:
         while job.running(max_steps):
             job.ig.step(1)
@@ -140,12 +175,13 @@ def run(lib: Library, cfg: dict, wk, gk, out: TextIO, max_steps: Optional[int] =
         job.finish()
 
 
-def run_batch(lib: Library, jobs_in, max_steps: Optional[int] = None, log: TextIO = sys.stderr):
-    """Several runs, [(cfg, wk, gk, out), ...], advanced together: one batched launch per step
-    over the runs that have not reached their tmax (the batch is rebuilt when one finishes).
-    A batched launch advances instances that share dt, so runs are grouped by dt and the
-    groups run one after another."""
-    jobs = [Job(lib, cfg, wk, gk, out, log) for cfg, wk, gk, out in jobs_in]
+def run_batch(lib: Library, jobs_in, max_steps: Optional[int] = None, log: TextIO = sys.stderr,
+              max_capacity: Optional[int] = None):
+    """Several runs, [(cfg, wk, gk, out), ...], advanced together step by step.  A batched
+    launch advances instances that share dt and cannot grow, so runs are grouped by dt; within
+    a group the runs whose bonds can still grow step one at a time, the others in one batched
+    launch (rebuilt whenever the set changes)."""
+    jobs = [Job(lib, cfg, wk, gk, out, log, max_capacity) for cfg, wk, gk, out in jobs_in]
     groups = {}
     for j in jobs:
         groups.setdefault(j.ig.params.dt, []).append(j)
@@ -153,14 +189,19 @@ def run_batch(lib: Library, jobs_in, max_steps: Optional[int] = None, log: TextI
         for group in groups.values():
             active = [j for j in group if j.running(max_steps)]
             while active:
-                batch = Batch(lib, [j.ig for j in active])
-                try:
-                    while all(j.running(max_steps) for j in active):
+                fixed = [j for j in active if not j.can_grow()]
+                for j in active:
+                    if j not in fixed:
+                        j.ig.step(1)
+                        j.record()
+                if fixed:
+                    batch = Batch(lib, [j.ig for j in fixed])
+                    try:
                         batch.step(1)
-                        for j in active:
+                        for j in fixed:
                             j.record()
-                finally:
-                    batch.destroy()
+                    finally:
+                        batch.destroy()
                 active = [j for j in active if j.running(max_steps)]
     finally:
         for j in jobs:
@@ -174,6 +215,8 @@ def main(argv=None) -> int:
                     help="w_k g_k pairs, or the reference's stderr log of them (one per input, in order)")
     ap.add_argument("--output", help="output file of a single input (default: its 'output file', else stdout)")
     ap.add_argument("--steps", type=int, help="stop after this many steps")
+    ap.add_argument("--max-capacity", type=int,
+                    help="lower the bond capacities to at most this (the device holds up to 64)")
     args = ap.parse_args(argv)
     from . import load_library
 
@@ -190,15 +233,16 @@ def main(argv=None) -> int:
         wk, gk = baths[0]
         if path:
             with open(path, "w") as f:
-                run(lib, cfgs[0], wk, gk, f, args.steps)
+                run(lib, cfgs[0], wk, gk, f, args.steps, max_capacity=args.max_capacity)
         else:
-            run(lib, cfgs[0], wk, gk, sys.stdout, args.steps)
+            run(lib, cfgs[0], wk, gk, sys.stdout, args.steps, max_capacity=args.max_capacity)
         return 0
     if args.output or any("outputfile" not in c for c in cfgs):
         raise SystemExit("several inputs: each needs its own 'output file' key (and no --output)")
     files = [open(c["outputfile"], "w") for c in cfgs]
     try:
-        run_batch(lib, [(c, wk, gk, f) for c, (wk, gk), f in zip(cfgs, baths, files)], args.steps)
+        run_batch(lib, [(c, wk, gk, f) for c, (wk, gk), f in zip(cfgs, baths, files)], args.steps,
+                  max_capacity=args.max_capacity)
     finally:
         for f in files:
             f.close()
