@@ -158,6 +158,17 @@ int ttn_set_coefficients(ttn_integ* integ, int64_t nterms, const double* coeff);
  * neigenvalues(), only_apply_when_no_unoccupied(), projector_splitting_integrator.hpp:338-357):
  * those fields of *p (and eigenvalue_tolerance) replace the current ones; the others are ignored */
 int ttn_set_expansion_params(ttn_integ* integ, const ttn_integ_params* p);
+/* Checkpoint / restart (the reference's cereal save/load of the integrator and the httensor,
+ * projector_splitting_integrator.hpp:50-68, 198-216, 373-391, httensor.hpp:101-118): everything
+ * a following step reads -- node tensors at their current shapes, bond dimensions, the gauge,
+ * single-particle and mean-field matrices, step and H-application counters, Nonesite /
+ * Ntwosite, dt and the expansion's random stream -- so that an integrator built on the same
+ * tree, operator and parameters continues bit for bit after ttn_checkpoint_load.  Not kept: the
+ * recorded trajectory and the profiling counters.  ttn_checkpoint_save with buf == NULL sets
+ * *bytes to the size needed; otherwise *bytes is the buffer's size on entry and the size
+ * written on return.  Loading clears a failure left by an earlier step. */
+int ttn_checkpoint_save(ttn_integ* integ, int64_t* bytes, void* buf);
+int ttn_checkpoint_load(ttn_integ* integ, const void* buf, int64_t bytes);
 /* normal draws the expansion's trial vectors have taken from its std::mt19937 +
  * std::normal_distribution stream so far.  Instrumentation (no reference counterpart). */
 int ttn_expansion_draws(const ttn_integ* integ, int64_t* n);

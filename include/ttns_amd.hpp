@@ -575,6 +575,35 @@ class fixed_step_projector_splitting_integrator {
         return m_dt;
     }
 
+    // checkpoint / restart in this library's own format (ttn_checkpoint_save / _load; the
+    // reference's cereal archives, projector_splitting_integrator.hpp:50-68, 198-216): the
+    // integrator's state after the last call, and A taken back to that state
+    std::vector<char> checkpoint(httensor<T, backend>& A) {
+        bind(A);
+        int64_t n = 0;
+        detail::check(ttn_checkpoint_save(m_integ, &n, nullptr), m_ctx, "ttn_checkpoint_save");
+        std::vector<char> buf(static_cast<size_t>(n));
+        detail::check(ttn_checkpoint_save(m_integ, &n, buf.data()), m_ctx, "ttn_checkpoint_save");
+        buf.resize(static_cast<size_t>(n));
+        return buf;
+    }
+    void restore(const std::vector<char>& buf, httensor<T, backend>& A) {
+        ensure();
+        if (m_bound && !m_alive.expired()) m_bound->detach(this);
+        detail::check(ttn_checkpoint_load(m_integ, buf.data(), static_cast<int64_t>(buf.size())), m_ctx,
+                      "ttn_checkpoint_load");
+        double dt = 0.0;
+        detail::check(ttn_get_dt(m_integ, &dt), m_ctx, "ttn_get_dt");
+        m_dt = m_dt_sent = dt;
+        std::vector<int64_t> hr(A.size());
+        detail::check(ttn_hranks(m_integ, hr.data()), m_ctx, "ttn_hranks");
+        A.pull();
+        A.mark_device_newer(this, [this](httensor<T, backend>& B) { copy_from_device(B); }, hr);
+        m_bound = &A;
+        m_alive = A.alive();
+        m_version = A.version();
+    }
+
     T expect(const ops::dense_matrix_operator<T, backend>& op, size_type mode, httensor<T, backend>& A) {
         bind(A);
         double out[2] = {0.0, 0.0};

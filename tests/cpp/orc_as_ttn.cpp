@@ -61,4 +61,8 @@ int ttn_set_coefficients(ttn_integ* i, int64_t n, const double* c) { return orc_
 int ttn_set_expansion_params(ttn_integ* i, const ttn_integ_params* p) {
     return orc_set_expansion_params(O(integ)(i), p);
 }
+// checkpoints belong to the device library only
+int ttn_checkpoint_save(ttn_integ*, int64_t*, void*) { return TTN_ERR_INVALID; }
+int ttn_checkpoint_load(ttn_integ*, const void*, int64_t) { return TTN_ERR_INVALID; }
+int ttn_get_dt(const ttn_integ* i, double* dt) { return orc_get_dt(OC(integ)(i), dt); }
 }

@@ -9,12 +9,15 @@
 //                  [ecut high_frequency_bound target_dimension maximum_dimension_high_frequency]
 //                  [--capacity C] [--spawn S] [--unoccupied U] [--min-unoccupied M]
 //                  [--dt2 DT --dt2-at K] [--coeff0 F --coeff0-at K] [--reset-at K] [--clear-at K]
+//                  [--checkpoint-at K --restore-at J]
 // Rows: t <sigma_z> Nonesite Ntwosite nh_applications/size hranks...  The first row is t = 0
 // (no counters).  --capacity lets the bonds grow to C (the expansion integrator's tree
 // capacity); --dt2 sets tdvp.dt() = DT after K steps; --coeff0 multiplies the coefficient of
 // term 0 (the system Hamiltonian) by F after K steps; --reset-at calls reset_nsteps() after K
 // steps; --clear-at calls clear(), restores the parameters and initialise()s again after K
-// steps (each of these with the state the run has reached).
+// steps (each of these with the state the run has reached); --checkpoint-at takes a checkpoint
+// after K steps and --restore-at goes back to it after J > K steps (the rows after J then
+// repeat those after K).
 // Bath: w_k = wc * 4 k / n (k = 1..n), g_k = sqrt(J(w_k) dw / pi), J(w) = pi/2 alpha w e^{-w/wc}.
 // The modes are split into leaves with partition_modes and each leaf's operators come from
 // harmonic_mode_combination, as in the reference driver (src/spin_boson.cpp:418-441,
@@ -228,10 +231,14 @@ int main(int argc, char** argv) {
         ops::dense_matrix_operator<complex_type, backend_type> mop(Sz);
         const int dt2_at = static_cast<int>(get("dt2-at", -1.0)), c0_at = static_cast<int>(get("coeff0-at", -1.0));
         const int reset_at = static_cast<int>(get("reset-at", -1.0)), clear_at = static_cast<int>(get("clear-at", -1.0));
+        const int ck_at = static_cast<int>(get("checkpoint-at", -1.0)), rs_at = static_cast<int>(get("restore-at", -1.0));
+        std::vector<char> ck;
         run(A, tdvp, H, mop, nsteps, [&](int done) {
             if (done == dt2_at) tdvp.set_dt(get("dt2", dt));
             if (done == c0_at) H.coeff(0) *= get("coeff0", 1.0);
             if (done == reset_at) tdvp.reset_nsteps();
+            if (done == ck_at) ck = tdvp.checkpoint(A);
+            if (done == rs_at) tdvp.restore(ck, A);
             if (done == clear_at) {
                 const double dtnow = tdvp.dt();
                 tdvp.clear();

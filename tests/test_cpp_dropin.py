@@ -31,6 +31,9 @@ CASES = [(8, 6, 4, 12, 0.05, 0.1, 5.0, 0, 12), (16, 10, 8, 8, 0.05, 0.3, 5.0, 3,
          (8, 6, 4, 7, 0.05, 0.1, 5.0, 1, 12, "--coeff0", 2.0, "--coeff0-at", 2, "--reset-at", 4),
          # clear() + initialise() after 3 steps (the state carries over, the step count restarts)
          (8, 6, 4, 6, 0.05, 0.1, 5.0, 2, 12, "--clear-at", 3)]
+# checkpoint after 2 steps of a growing tree, back to it after 5: rows 6.. repeat rows 3..
+CHECKPOINT = (8, 6, 2, 8, 0.05, 0.1, 5.0, 0, 12, "--capacity", 6, "--spawn", 1e-8, "--min-unoccupied", 2,
+              "--checkpoint-at", 2, "--restore-at", 5)
 
 
 def _split(case):
@@ -109,6 +112,15 @@ def test_cpp_driver_over_oracle_reproduces_builtin_problem(orc, case):
     assert np.array_equal(head, ho)
     assert np.array_equal(counts, co)
     assert hr == hro
+
+
+@pytest.mark.gpu
+def test_cpp_driver_checkpoint_restore_on_gpu():
+    print(f"[cpp driver] checkpoint case {CHECKPOINT}", flush=True)
+    head, counts, hr = _run(_driver("sb_driver"), CHECKPOINT)
+    # rows after the restore (6..8) repeat the rows after the checkpoint (3..5) bit for bit
+    assert np.array_equal(head[6:9], head[3:6]) and np.array_equal(counts[5:8], counts[2:5])
+    assert hr[6:9] == hr[3:6] and hr[3] != hr[0]
 
 
 @pytest.mark.gpu

@@ -48,20 +48,32 @@ def parse(text):
     return rows
 
 
-def check_rows(rows, run, steps, bound):
+def check_rows(rows, run, steps, bounds, exact):
+    """`exact` steps with the reference's counters and bond dimensions (the steps whose
+    expansion decisions rounding cannot flip), then Ntwosite within 2 attempts"""
     ref = run["rows"]
     assert len(rows) == steps + 1
     assert rows[0]["text"].startswith("0 0 0 1 1  0 0 0 ")
     if ref[0].get("hranks"):
         assert rows[0]["hranks"] == ref[0]["hranks"]
-    for r, e in zip(rows[1:], ref[1:1 + steps]):
+    for k, (r, e, bound) in enumerate(zip(rows[1:], ref[1:1 + steps], bounds)):
         assert abs(r["values"][0] - e["values"][0]) < 1e-15
         assert np.abs(np.array(r["values"][1:4]) - np.array(e["values"][1:4])).max() < bound
         assert abs(r["values"][4] - e["values"][4]) < 1e-12
-        assert r["counters"][:2] == e["counters"][:2]
         assert abs(r["counters"][2] / e["counters"][2] - 1.0) < 0.1
-        if e.get("hranks"):
-            assert r["hranks"] == e["hranks"]
+        if k < exact:
+            assert r["counters"][:2] == e["counters"][:2]
+            if e.get("hranks"):
+                assert r["hranks"] == e["hranks"]
+        else:
+            assert r["counters"][0] == e["counters"][0] and abs(r["counters"][1] - e["counters"][1]) <= 2
+
+
+# (run, steps, <O> bound per step, steps compared exactly).  The subohmic run's second step
+# holds a two-site attempt that rounding decides: three oracle runs perturbed by 1e-15 end it
+# with Ntwosite 4085 or 4086 and <sigma> 1.7e-10 .. 2.3e-10 apart (the oracle is 1.8e-10 from
+# the reference there)
+DRIVER_CASES = [("ohmic_sudden", 2, [1e-10, 1e-10], 2), ("subohmic_exponential_t0", 2, [5e-10, 1e-9], 1)]
 
 
 def test_bath_log_and_input_parsing(tmp_path):
@@ -78,20 +90,20 @@ def test_bath_log_and_input_parsing(tmp_path):
     assert driver.observables(cfg)[1][0, 1] == -1j
 
 
-@pytest.mark.parametrize("name,steps,bound", [("ohmic_sudden", 2, 1e-10), ("subohmic_exponential_t0", 2, 5e-10)])
-def test_driver_over_oracle_matches_reference_output(orc, tmp_path, name, steps, bound):
+@pytest.mark.parametrize("name,steps,bounds,exact", DRIVER_CASES)
+def test_driver_over_oracle_matches_reference_output(orc, tmp_path, name, steps, bounds, exact):
     run, inp, bath, _ = write_case(tmp_path, name, steps)
     cfg = driver.load_input(inp)
     wk, gk = driver.load_bath(bath)
     out, log = io.StringIO(), io.StringIO()
     driver.run(orc, cfg, wk, gk, out, log=log)
-    check_rows(parse(out.getvalue()), run, steps, bound)
+    check_rows(parse(out.getvalue()), run, steps, bounds, exact)
     assert out.getvalue().rstrip().splitlines()[-1].startswith("nh_app: ")
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name,steps,bound", [("ohmic_sudden", 2, 1e-10), ("subohmic_exponential_t0", 2, 5e-10)])
-def test_driver_command_on_gpu_matches_reference_output(tmp_path, name, steps, bound):
+@pytest.mark.parametrize("name,steps,bounds,exact", DRIVER_CASES)
+def test_driver_command_on_gpu_matches_reference_output(tmp_path, name, steps, bounds, exact):
     run, inp, bath, out = write_case(tmp_path, name, steps)
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     print(f"[driver] {name}, {steps} steps", flush=True)
@@ -100,24 +112,23 @@ def test_driver_command_on_gpu_matches_reference_output(tmp_path, name, steps, b
     subprocess.run([sys.executable, "-m", "ttns_lib_amd.driver", inp, "--bath", bath, "--max-capacity", "64"],
                    check=True, cwd=root, timeout=300)
     with open(out) as f:
-        check_rows(parse(f.read()), run, steps, bound)
+        check_rows(parse(f.read()), run, steps, bounds, exact)
 
 
 @pytest.mark.gpu
 def test_driver_batch_of_inputs_on_gpu_matches_reference_output(tmp_path):
     # two inputs in one command, advanced together (one at a time while their bonds can grow)
-    cases = [("ohmic_sudden", 2, 1e-10), ("subohmic_exponential_t0", 2, 5e-10)]
     made = []
-    for name, steps, bound in cases:
+    for name, steps, bounds, exact in DRIVER_CASES:
         d = tmp_path / name
         d.mkdir()
-        made.append((write_case(d, name, steps), steps, bound))
+        made.append((write_case(d, name, steps), steps, bounds, exact))
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     cmd = [sys.executable, "-m", "ttns_lib_amd.driver", "--max-capacity", "64"] + [m[0][1] for m in made]
     for m in made:
         cmd += ["--bath", m[0][2]]
     print("[driver] batch of two inputs", flush=True)
     subprocess.run(cmd, check=True, cwd=root, timeout=300)
-    for (run, _, _, out), steps, bound in made:
+    for (run, _, _, out), steps, bounds, exact in made:
         with open(out) as f:
-            check_rows(parse(f.read()), run, steps, bound)
+            check_rows(parse(f.read()), run, steps, bounds, exact)

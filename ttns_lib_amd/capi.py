@@ -99,6 +99,8 @@ _SIGS = {
     "get_dt": [vp, dblp],
     "reset_nsteps": [vp],
     "set_coefficients": [vp, i64, dblp],
+    "checkpoint_save": [vp, i64p, C.c_void_p],
+    "checkpoint_load": [vp, C.c_void_p, i64],
     "set_expansion_params": [vp, C.c_void_p],
     "expansion_draws": [vp, i64p],
     "set_expansion_log": [vp, i64],
@@ -451,6 +453,19 @@ class Integrator:
                 raise TypeError(f"not an expansion parameter: {k}")
             setattr(self.params, k, v)
         self.lib._call("set_expansion_params", self.h, C.cast(C.pointer(self.params), C.c_void_p))
+
+    def checkpoint(self) -> bytes:
+        """the integrator's state (ttn_checkpoint_save)"""
+        n = i64(0)
+        self.lib._call("checkpoint_save", self.h, C.byref(n), None)
+        buf = C.create_string_buffer(n.value)
+        self.lib._call("checkpoint_save", self.h, C.byref(n), buf)
+        return buf.raw[:n.value]
+
+    def restore(self, data: bytes):
+        """continue from a checkpoint of an integrator on the same tree, operator and parameters"""
+        buf = C.create_string_buffer(bytes(data), len(data))
+        self.lib._call("checkpoint_load", self.h, buf, len(data))
 
     def set_coefficients(self, coeff):
         """new SOP coefficients (one complex per term), used from the next step on"""

@@ -13,6 +13,7 @@
 #include <map>
 #include <memory>
 #include <random>
+#include <sstream>
 #include <string>
 
 #include "../../include/ttn_capi.h"
@@ -77,6 +78,9 @@ struct ttn_integ {
     std::vector<ttnd::DSub> h_subs;
     std::vector<ttnd::DMf> h_mfs;
     size_t o_terms = 0, o_subs = 0, o_mfs = 0;
+    // arena regions that carry state from one step to the next besides the node tensors
+    // (gauge factors, single-particle and mean-field matrices): what a checkpoint stores
+    std::vector<std::pair<int64_t, int64_t>> carried;
     FlopModel fm;
     int64_t sweep_pairs = 0;   // forward+backward sweeps run (with or without propagation)
     int64_t prepares = 0;
@@ -195,6 +199,7 @@ void build_instance(ttn_integ* in) {
         if (!nd.is_leaf())
             for (i64 x : nd.cap_dims) rmax = std::max<int64_t>(rmax, x);
         d.R = alloc(rmax * rmax);
+        in->carried.emplace_back(d.R, rmax * rmax);
         maxsize = std::max<int64_t>(maxsize, nd.capacity());
     }
     // environment matrices (capacity hrank squared)
@@ -205,8 +210,14 @@ void build_instance(ttn_integ* in) {
         spf_off[i].assign(terms.size(), -1);
         mf_off[i].assign(terms.size(), -1);
         for (size_t k = 0; k < terms.size(); ++k) {
-            if (!terms[k].id_spf) spf_off[i][k] = alloc(r * r);
-            if (!terms[k].id_mf) mf_off[i][k] = alloc(r * r);
+            if (!terms[k].id_spf) {
+                spf_off[i][k] = alloc(r * r);
+                in->carried.emplace_back(spf_off[i][k], r * r);
+            }
+            if (!terms[k].id_mf) {
+                mf_off[i][k] = alloc(r * r);
+                in->carried.emplace_back(mf_off[i][k], r * r);
+            }
         }
     }
     // primitive operators (uploaded once per (mode, index))
@@ -1166,6 +1177,191 @@ int ttn_expansion_draws(const ttn_integ* in, int64_t* n) {
 int ttn_reset_nsteps(ttn_integ* in) {
     in->nsteps = 0;
     return TTN_OK;
+}
+
+// ---------------------------------------------------------------- checkpoints
+// Layout (native byte order, for resuming with the same library build): the 8-byte tag
+// "TTNCKPT1", a fixed block of int64 fields (nodes, terms, tree/operator fingerprint, steps,
+// H applications, sweeps completed, Nonesite, Ntwosite, normal draws consumed, pending draws,
+// length of the generator text, sweep pairs, prepare passes), dt, the bond dimension of every
+// node, the std::mt19937 / std::normal_distribution state as text, the pending normal draws,
+// every node tensor at its current shape and the carried matrices (gauge factors,
+// single-particle and mean-field matrices) at capacity.
+namespace {
+constexpr char CKPT_TAG[8] = {'T', 'T', 'N', 'C', 'K', 'P', 'T', '1'};
+constexpr int CKPT_NFIELDS = 13;
+enum CkptField { CK_NODES, CK_TERMS, CK_FPRINT, CK_NSTEPS, CK_NH, CK_SWEEPS, CK_ONESITE, CK_TWOSITE,
+                 CK_NORMALS, CK_PENDING, CK_RNGLEN, CK_PAIRS, CK_PREPARES };
+
+uint64_t fingerprint(const ttn_integ* in) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&h](int64_t v) {
+        for (int b = 0; b < 8; ++b) {
+            h ^= static_cast<uint64_t>((v >> (8 * b)) & 0xff);
+            h *= 1099511628211ull;
+        }
+    };
+    for (const Node& nd : in->topo.nodes) {
+        mix(nd.parent);
+        mix(nd.cap_hrank);
+        mix(nd.leaf_index);
+        mix(nd.is_leaf() ? nd.dims[0] : -1);
+    }
+    mix(in->sop.nterms);
+    mix(in->p.krylov_dim);
+    mix(in->p.mode);
+    return h;
+}
+
+std::string rng_text(const ttn_integ* in) {
+    std::ostringstream o;
+    o << in->rng << ' ' << in->ndist;
+    return o.str();
+}
+
+int64_t node_live(const Node& nd) { return nd.dimen() * nd.hrank; }
+
+// bytes of the checkpoint of `in` with `pending` pending draws and generator text `rt`
+int64_t ckpt_bytes(const ttn_integ* in, int64_t pending, int64_t rt) {
+    int64_t b = 8 + 8 * CKPT_NFIELDS + 8 + 8 * in->topo.size() + rt + 8 * pending;
+    for (const Node& nd : in->topo.nodes) b += 16 * node_live(nd);
+    for (const auto& r : in->carried) b += 16 * r.second;
+    return b;
+}
+}  // namespace
+
+int ttn_checkpoint_save(ttn_integ* in, int64_t* bytes, void* buf) {
+    return guard(in->ctx, [&] {
+        check(bytes != nullptr, ERR_INVALID, "checkpoint: null size");
+        int st[2] = {0, 0};
+        hip_check(hipMemcpy(st, in->status.p, sizeof(st), hipMemcpyDeviceToHost), "status");
+        check(st[0] == 0, ERR_INVALID, "checkpoint: the integrator's last step failed");
+        int64_t pos = 0;
+        hip_check(hipMemcpy(&pos, in->d_exp.p + ttnd::EXP_RNG_POS, sizeof(pos), hipMemcpyDeviceToHost), "rng cursor");
+        check(pos >= 0 && pos <= static_cast<int64_t>(in->pending.size()), ERR_DEVICE, "random stream cursor out of range");
+        const std::string rt = rng_text(in);
+        const int64_t npend = static_cast<int64_t>(in->pending.size()) - pos;
+        const int64_t need = ckpt_bytes(in, npend, static_cast<int64_t>(rt.size()));
+        if (buf == nullptr) {
+            *bytes = need;
+            return;
+        }
+        check(*bytes >= need, ERR_INVALID, "checkpoint: buffer too small");
+        int64_t c[2];
+        hip_check(hipMemcpy(c, in->counters.p, sizeof(c), hipMemcpyDeviceToHost), "counters");
+        char* w = static_cast<char*>(buf);
+        auto put = [&w](const void* src, size_t n) {
+            std::memcpy(w, src, n);
+            w += n;
+        };
+        put(CKPT_TAG, 8);
+        int64_t f[CKPT_NFIELDS];
+        f[CK_NODES] = in->topo.size();
+        f[CK_TERMS] = in->sop.nterms;
+        f[CK_FPRINT] = static_cast<int64_t>(fingerprint(in));
+        f[CK_NSTEPS] = in->nsteps;
+        f[CK_NH] = c[0];
+        f[CK_SWEEPS] = c[1];
+        f[CK_ONESITE] = in->n_onesite;
+        f[CK_TWOSITE] = in->n_twosite;
+        f[CK_NORMALS] = in->normals_used + pos;
+        f[CK_PENDING] = npend;
+        f[CK_RNGLEN] = static_cast<int64_t>(rt.size());
+        f[CK_PAIRS] = in->sweep_pairs;
+        f[CK_PREPARES] = in->prepares;
+        put(f, sizeof(f));
+        put(&in->p.dt, 8);
+        for (const Node& nd : in->topo.nodes) put(&nd.hrank, 8);
+        put(rt.data(), rt.size());
+        if (npend > 0) put(in->pending.data() + pos, 8 * npend);
+        for (int i = 0; i < in->topo.size(); ++i) {
+            const int64_t n = node_live(in->topo.nodes[i]);
+            hip_check(hipMemcpy(w, in->arena.p + in->node_A[i], 16 * n, hipMemcpyDeviceToHost), "checkpoint");
+            w += 16 * n;
+        }
+        for (const auto& r : in->carried) {
+            hip_check(hipMemcpy(w, in->arena.p + r.first, 16 * r.second, hipMemcpyDeviceToHost), "checkpoint");
+            w += 16 * r.second;
+        }
+        *bytes = need;
+    });
+}
+
+int ttn_checkpoint_load(ttn_integ* in, const void* buf, int64_t bytes) {
+    return guard(in->ctx, [&] {
+        check(buf != nullptr && bytes >= 8 + 8 * CKPT_NFIELDS + 8, ERR_INVALID, "checkpoint: truncated");
+        const char* r = static_cast<const char*>(buf);
+        const char* end = r + bytes;
+        auto get = [&r, end](void* dst, size_t n) {
+            check(r + n <= end, ERR_INVALID, "checkpoint: truncated");
+            std::memcpy(dst, r, n);
+            r += n;
+        };
+        char tag[8];
+        get(tag, 8);
+        check(std::memcmp(tag, CKPT_TAG, 8) == 0, ERR_INVALID, "checkpoint: not a checkpoint of this library");
+        int64_t f[CKPT_NFIELDS];
+        get(f, sizeof(f));
+        check(f[CK_NODES] == in->topo.size() && f[CK_TERMS] == in->sop.nterms &&
+                  f[CK_FPRINT] == static_cast<int64_t>(fingerprint(in)),
+              ERR_INVALID, "checkpoint: written for another tree, operator or integrator mode");
+        check(f[CK_PENDING] >= 0 && f[CK_RNGLEN] >= 0 && f[CK_NSTEPS] >= 0, ERR_INVALID, "checkpoint: corrupt header");
+        double dt = 0.0;
+        get(&dt, 8);
+        check(dt > 0.0 && std::isfinite(dt), ERR_INVALID, "checkpoint: corrupt dt");
+        const int n = in->topo.size();
+        std::vector<int64_t> hr(n);
+        get(hr.data(), 8 * n);
+        for (int i = 0; i < n; ++i) {
+            const Node& nd = in->topo.nodes[i];
+            check(hr[i] >= 1 && hr[i] <= nd.cap_hrank, ERR_INVALID, "checkpoint: bond dimension beyond capacity");
+        }
+        std::string rt(static_cast<size_t>(f[CK_RNGLEN]), '\0');
+        get(rt.data(), rt.size());
+        std::vector<double> pend(static_cast<size_t>(f[CK_PENDING]));
+        get(pend.data(), 8 * pend.size());
+        // shapes first: the node tensors are stored at their checkpointed shapes
+        for (int i = 0; i < n; ++i) {
+            Node& nd = in->topo.nodes[i];
+            nd.hrank = hr[i];
+            if (nd.parent >= 0) in->topo.nodes[nd.parent].dims[nd.child_id] = hr[i];
+        }
+        check(bytes == ckpt_bytes(in, f[CK_PENDING], f[CK_RNGLEN]), ERR_INVALID,
+              "checkpoint: size does not match its header");
+        for (int i = 0; i < n; ++i) {
+            const int64_t cnt = node_live(in->topo.nodes[i]);
+            hip_check(hipMemcpy(in->arena.p + in->node_A[i], r, 16 * cnt, hipMemcpyHostToDevice), "restore");
+            r += 16 * cnt;
+        }
+        for (const auto& reg : in->carried) {
+            hip_check(hipMemcpy(in->arena.p + reg.first, r, 16 * reg.second, hipMemcpyHostToDevice), "restore");
+            r += 16 * reg.second;
+        }
+        std::vector<int> hri(hr.begin(), hr.end());
+        hip_check(hipMemcpy(in->hr.p, hri.data(), n * sizeof(int), hipMemcpyHostToDevice), "restore hranks");
+        const int64_t c[2] = {f[CK_NH], f[CK_SWEEPS]};
+        hip_check(hipMemcpy(in->counters.p, c, sizeof(c), hipMemcpyHostToDevice), "restore counters");
+        int64_t w[ttnd::EXP_NWORDS] = {};
+        w[ttnd::EXP_ONESITE] = f[CK_ONESITE];
+        w[ttnd::EXP_TWOSITE] = f[CK_TWOSITE];
+        hip_check(hipMemcpy(in->d_exp.p, w, sizeof(w), hipMemcpyHostToDevice), "restore expansion words");
+        const int zero[2] = {0, 0};
+        hip_check(hipMemcpy(in->status.p, zero, sizeof(zero), hipMemcpyHostToDevice), "restore status");
+        std::istringstream is(rt);
+        is >> in->rng >> in->ndist;
+        check(!is.fail(), ERR_INVALID, "checkpoint: corrupt generator state");
+        in->pending = std::move(pend);
+        in->normals_used = f[CK_NORMALS];
+        in->n_onesite = f[CK_ONESITE];
+        in->n_twosite = f[CK_TWOSITE];
+        in->nsteps = f[CK_NSTEPS];
+        in->sweep_pairs = f[CK_PAIRS];
+        in->prepares = f[CK_PREPARES];
+        in->p.dt = dt;
+        in->inst.exp_log_cap = 0;
+        in->inst.exp_log = nullptr;
+        in->fm = flop_model(in->topo, in->ops, in->sop);
+    });
 }
 
 int ttn_set_expansion_params(ttn_integ* in, const ttn_integ_params* p) {
