@@ -10,10 +10,17 @@ the reference's prepare pass and six-sub-step ladder, so it is always part of th
 `value` is instance-steps per second of the whole batch.  Besides it the JSON line carries:
   roofline      algorithmic FP64 flops of the timed steps (H applications counted on the
                 device, per-node cost from the compiled term tables) / kernel time, against
-                the MI355X FP64 matrix peak (78.6 TFLOP/s)
-  plain_gpu     the same instances stepped one at a time (a sample of them)
-  cpu_baseline  the oracle (C++ CPU restatement, 1 thread) on a bounded sample, scaled to
-                the whole batch by flop count
+                the MI355X FP64 matrix peak (78.6 TFLOP/s); traffic is null (HBM bytes come
+                from rocprofv3 --pmc runs, which this script does not make)
+  plain_gpu     the plain per-instance path: a sample of the instances (the slowest, the
+                fastest and the median ones by their device time in the batch) restored from
+                checkpoints taken before the timed steps and stepped alone, each for the same
+                step as in the batch; the other instances' alone times are estimated from their
+                batch device time times the sample's median alone/batch ratio
+  cpu_baseline  the oracle (C++ CPU restatement, 1 thread) on a bounded sample of instances
+                (lightest first, then one heavier one if the budget allows), a regular step
+                each after an untimed first step, scaled to the whole batch by flop count;
+                the same steps on the GPU give check.oracle_max_dsigma_z
 """
 from __future__ import annotations
 
@@ -30,12 +37,6 @@ sys.path.insert(0, ROOT)
 
 METRIC = "TDVP steps/s + FP64-MFMA roofline %, batched vs plain GPU vs CPU, <sz> err<=1e-10"
 PEAK_FP64_TFLOPS = 78.6
-# HBM bytes moved per unit of work, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes on
-# MI355X (FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0: the factors that make the counters
-# read the 1 GiB device copy of `bench.py --workload copy` as 1 GiB each way); the commands and
-# raw counts are in profiles/hbm_traffic_counters.md.  Not measured by this script.
-TRAFFIC_PER_BATCH_STEP = 5.93e12   # the 64-instance batch, one regular TDVP step
-TRAFFIC_PER_NODE_HEFF = 8.45e8     # 64 x 64 x 64 node, 40 terms, one H-action (all GEMM stages)
 
 
 def make(lib, inst):
@@ -74,20 +75,27 @@ def workload(name, lib=None):
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(insts, flops_per_step, budget_s):
-    """Time the oracle (1 thread) on the cheapest instances within a time budget; scale the
-    measured CPU flop rate to the whole batch."""
+def cpu_baseline(lib, insts, flops_per_step, budget_s):
+    """The oracle (1 thread) on a bounded sample: the lightest instances first, then the lightest
+    instance above the median flop count if the measured rate says it fits the remaining
+    budget; an untimed first step, then one timed regular step each.  The CPU's flop rate over
+    the sample scales to the whole batch.  The same two steps on the GPU check the oracle's
+    <sigma_z> against the HIP path's."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from ttns_lib_amd.capi import Integrator, Library, sb_problem
+    from ttns_lib_amd.capi import SIGMA, Integrator, Library, sb_problem
 
     so = os.path.join(ROOT, "oracle", "_build", "libttn_oracle.so")
     if not os.path.exists(so):
-        return None
+        return None, None
     orc = Library(so, "orc_")
-    order = np.argsort(flops_per_step)
-    used, t_cpu, f_cpu, steps = [], 0.0, 0.0, 0
+    order = [int(i) for i in np.argsort(flops_per_step)]
+    heavy = next(i for i in order if flops_per_step[i] >= np.median(flops_per_step))
+    used, per, dsz = [], [], 0.0
+    t_cpu = f_cpu = 0.0
     t_start = time.perf_counter()
-    for i in order:
+
+    def one(i):
+        nonlocal t_cpu, f_cpu, dsz
         inst = insts[i]
         tree, sop = sb_problem(orc, inst.model)
         ig = Integrator(orc, tree, sop, dt=inst.dt, krylov_dim=inst.krylov_dim, krylov_tol=1e-12)
@@ -96,25 +104,36 @@ def cpu_baseline(insts, flops_per_step, budget_s):
         t0 = time.perf_counter()
         ig.step(1)
         dt = time.perf_counter() - t0
-        used.append(int(i))
+        g = make(lib, inst)
+        g.step(2)
+        dsz = max(dsz, abs(g.expect(0, SIGMA["z"]).real - ig.expect(0, SIGMA["z"]).real))
+        g.destroy()
+        used.append(i)
+        per.append(flops_per_step[i] / dt / 1e9)
         t_cpu += dt
         f_cpu += flops_per_step[i]
-        steps += 1
-        if time.perf_counter() - t_start > budget_s:
+
+    for i in order:
+        if i == heavy or (used and time.perf_counter() - t_start > budget_s / 2):
             break
-    rate = f_cpu / t_cpu  # flop/s of the CPU restatement
+        one(i)
+    rate = f_cpu / t_cpu
+    if 2.3 * flops_per_step[heavy] / rate < budget_s - (time.perf_counter() - t_start):
+        one(heavy)
+        rate = f_cpu / t_cpu
     total_time = float(np.sum(flops_per_step)) / rate  # one batched step on one core
     return {
         "value": len(insts) / total_time,
         "unit": "instance-steps/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"oracle, 1 thread, one regular step of instances {used} (after their first step); "
-                  f"measured {steps / t_cpu:.4g} steps/s = {rate / 1e9:.3g} GFLOP/s, scaled to the batch by "
-                  f"per-instance flop count",
+        "sample": f"oracle, 1 thread, one regular step (after an untimed first step) of instances {used} "
+                  f"({', '.join(f'{flops_per_step[i] / 1e9:.1f}' for i in used)} GFLOP per step; "
+                  f"{', '.join(f'{r:.2f}' for r in per)} GFLOP/s); {rate / 1e9:.3g} GFLOP/s over the sample "
+                  f"scales the batch's {np.sum(flops_per_step) / 1e12:.2f} TFLOP per step (extrapolated)",
         "cpu_model": _cpu_model(),
         "nproc": os.cpu_count(),
-    }
+    }, dsz
 
 
 def _cpu_worker(so, k, inst, barrier, queue):
@@ -260,7 +279,7 @@ def bench_c2(args):
         "config": {"workload": "c2: node 64x64x64, 40 product terms of random Hermitian 64x64 operators",
                    "terms": nt},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": TRAFFIC_PER_NODE_HEFF,
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
                      "kernel": "ttnd::zgemm_kernel (three GEMM stages of the H-action)",
                      "flops_per_heff": fl / args.steps},
         "mean_field": {"ms": mf_ms / args.steps, "tflops": mf_fl / (mf_ms / 1e3) / 1e12},
@@ -357,9 +376,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c3")
-    ap.add_argument("--plain-sample", type=int, default=4, help="instances stepped alone for the plain GPU rate")
+    ap.add_argument("--plain-sample", type=int, default=8, help="instances stepped alone for the plain GPU rate")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle time for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--report", action="store_true",
@@ -391,16 +410,29 @@ def main():
     t0 = time.perf_counter()
     batch.step(warm)
     print(f"[bench] warmup {warm} step(s) {time.perf_counter() - t0:.2f}s", file=sys.stderr, flush=True)
+    # the plain-path sample: the slowest, fastest and median instances by device time so far,
+    # checkpointed here so that each can later take the first timed step alone
+    t_warm = np.array([ig.profile()["total"] for ig in igs])
+    order = [int(i) for i in np.argsort(-t_warm)]
+    k = min(max(0, args.plain_sample), len(insts))
+    n_slow, n_fast = (k + 2) // 3, k // 3
+    n_mid = k - n_slow - n_fast
+    lo = len(order) // 2 - n_mid // 2
+    pick = sorted(set(order[:n_slow]) | set(order[len(order) - n_fast:]) | set(order[lo:lo + n_mid])) \
+        if k > 0 and len(insts) > 1 else []
+    ckpt = {i: igs[i].checkpoint() for i in pick}
     f0 = np.array([ig.flops() for ig in igs])
-    p0 = [ig.profile() for ig in igs] if args.report else None
+    p0 = [ig.profile() for ig in igs]
     s0 = [ig.node_stats() for ig in igs] if args.report else None
     t0 = time.perf_counter()
     batch.step(args.steps)
     wall = time.perf_counter() - t0
     kernel_ms = batch.last_step_ms()
     f1 = np.array([ig.flops() for ig in igs])
+    p1 = [ig.profile() for ig in igs]
     flops = float(np.sum(f1 - f0))
     per_inst_flops_step = (f1 - f0) / args.steps
+    t_batch = np.array([(b["total"] - a["total"]) / 1e6 / args.steps for a, b in zip(p0, p1)])  # s per step
     value = len(insts) * args.steps / wall
     achieved = flops / (kernel_ms / 1e3) / 1e12
     print(f"[bench] timed {args.steps} step(s): wall {wall:.3f}s kernel {kernel_ms / 1e3:.3f}s "
@@ -410,31 +442,33 @@ def main():
     sz = [float(ig.trajectory()[-1, 0]) for ig in igs]
     norm_dev = max(abs(float(ig.trajectory()[-1, 1]) - 1.0) for ig in igs)
 
-    # plain per-instance path on a sample spread over the flop range
+    # plain per-instance path: the sample, restored from its checkpoints, one step alone each
     plain = None
-    if args.plain_sample > 0 and len(insts) > 1:
-        order = np.argsort(per_inst_flops_step)
-        pick = sorted(set(int(order[int(round(q * (len(order) - 1)))])
-                          for q in np.linspace(0, 1, args.plain_sample)))
-        t_sum, f_sum = 0.0, 0.0
+    if pick:
+        alone = {}
         for i in pick:
             ig = make(lib, insts[i])
-            ig.step(warm)
-            a = ig.flops()
+            ig.restore(ckpt[i])
             t1 = time.perf_counter()
             ig.step(1)
-            t_sum += time.perf_counter() - t1
-            f_sum += ig.flops() - a
+            alone[i] = time.perf_counter() - t1
             ig.destroy()
-        # time of a whole batched step if every instance ran alone, scaled by flop count
-        est = t_sum * float(np.sum(per_inst_flops_step)) / f_sum
+            print(f"[bench] instance {i} alone: {alone[i]:.3f}s (in the batch {t_batch[i]:.3f}s)", file=sys.stderr,
+                  flush=True)
+        ratio = float(np.median([alone[i] / t_batch[i] for i in pick]))
+        est = sum(alone.values()) + sum(t_batch[j] * ratio for j in range(len(insts)) if j not in alone)
         plain = {"value": len(insts) / est, "unit": "instance-steps/s",
-                 "sample": f"instances {pick} stepped alone (1 regular step each), scaled by flop count",
+                 "sample": f"instances {pick} (slowest, fastest and median in the batch) restored from checkpoints "
+                           f"and stepped alone: {', '.join(f'{alone[i]:.2f}' for i in pick)} s for the step that "
+                           f"took them {', '.join(f'{t_batch[i]:.2f}' for i in pick)} s of device time in the "
+                           f"batch; the other {len(insts) - len(pick)} instances' alone times estimated as their "
+                           f"batch device time x the median ratio {ratio:.3f} (extrapolated)",
+                 "seconds_per_step_all_alone": est,
                  "batched_over_plain": value / (len(insts) / est)}
     elif len(insts) == 1:
         plain = {"value": value, "unit": "instance-steps/s", "sample": "single instance: batched == plain"}
 
-    cpu = None if args.no_cpu else cpu_baseline(insts, per_inst_flops_step, args.cpu_budget)
+    cpu, dsz = (None, None) if args.no_cpu else cpu_baseline(lib, insts, per_inst_flops_step, args.cpu_budget)
     cpu_all = None
     if not args.no_cpu and len(insts) > 1:
         cpu_all = cpu_baseline_all_cores(insts, per_inst_flops_step, min(16, os.cpu_count() or 1, len(insts)))
@@ -454,12 +488,13 @@ def main():
         "config": {"workload": f"{args.workload}: {desc}", "instances": len(insts), "krylov_dim": 12,
                    "krylov_tol": 1e-12},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS,
-                     "traffic": TRAFFIC_PER_BATCH_STEP * args.steps if args.workload == "c3" else None,
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
                      "kernel": "ttnd::tdvp_run_kernel", "flops_timed": flops},
         "plain_gpu": plain,
         "cpu_baseline": cpu,
-        "check": {"sigma_z_last": sz[:4], "max_norm_deviation": norm_dev},
+        "check": {"sigma_z_last": sz[:4], "max_norm_deviation": norm_dev, "oracle_max_dsigma_z": dsz},
+        "batch_device_seconds_per_step": {"max": float(t_batch.max()), "median": float(np.median(t_batch)),
+                                          "sum": float(t_batch.sum())},
     }
     if cpu:
         out["vs_cpu"] = value / cpu["value"]

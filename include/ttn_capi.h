@@ -183,10 +183,17 @@ int ttn_expansion_log(const ttn_integ* integ, int64_t* len, double* out);
 int ttn_expect(ttn_integ* integ, int64_t mode, const double* op, double* out);
 /* <psi|psi> */
 int ttn_norm2(ttn_integ* integ, double* out);
-/* Record Re<psi|O|psi> and <psi|psi> on the device after every completed step, as the
- * reference driver does per step (src/spin_boson.cpp:81-105).  O: d x d complex on mode. */
+/* Record <psi|O_j|psi> (j < nops <= 8) and <psi|psi> on the device after every completed step,
+ * as the reference driver evaluates every mel(mops[j], 0, A) per step (src/spin_boson.cpp:81-105).
+ * O_j: d_j x d_j complex on primitive mode modes[j], concatenated in ops.  Clears the records of
+ * an earlier set. */
+int ttn_set_observables(ttn_integ* integ, int64_t nops, const int64_t* modes, const double* ops);
+/* the records, one per completed step of 2 nops + 1 doubles: re, im of every <O_j>, then <1>;
+ * out may be NULL to query *len (steps) */
+int ttn_observables(const ttn_integ* integ, int64_t* len, double* out);
+/* one observable: ttn_set_observables with nops = 1 */
 int ttn_set_observable(ttn_integ* integ, int64_t mode, const double* op);
-/* recorded (Re<O>, <1>) pairs, one per completed step; out may be NULL to query *len */
+/* recorded (Re<O_0>, <1>) pairs, one per completed step; out may be NULL to query *len */
 int ttn_trajectory(const ttn_integ* integ, int64_t* len, double* out);
 
 /* ---- spin-boson driver helpers (src/spin_boson.cpp:447-649) ---- */

@@ -151,6 +151,33 @@ def test_batch_isolates_a_nan_instance(hip):
             assert np.array_equal(a.get_node(nd), b.get_node(nd))
 
 
+def test_several_observables_recorded_on_device(hip, orc):
+    # sigma_x, sigma_y, sigma_z, 1 on the spin and the occupation of bath mode 3, recorded after
+    # every step of one launch, against the oracle's expectation values step by step
+    print("[observables] 5 operators over 6 steps", flush=True)
+    model = sb_model(8, 6, 4)
+    _, _, g = make_integrator(hip, model)
+    _, _, o = make_integrator(orc, model)
+    num = np.diag(np.arange(6)).astype(complex)
+    obs = [(0, SIGMA["x"]), (0, SIGMA["y"]), (0, SIGMA["z"]), (0, np.eye(2)), (4, num)]
+    g.set_observables(obs)
+    g.step(6)
+    vals, norms = g.observables()
+    assert vals.shape == (6, 5) and norms.shape == (6,)
+    for k in range(6):
+        o.step(1)
+        want = np.array([o.expect(m, op) for m, op in obs])
+        assert np.abs(vals[k] - want).max() < 1e-10, k
+        assert abs(norms[k] - o.norm2()) < 1e-12
+    assert np.abs(vals[-1] - np.array([g.expect(m, op) for m, op in obs])).max() < 1e-14
+    # the one-observable trajectory reads the first slot
+    assert np.array_equal(g.trajectory()[:, 0], vals[:, 0].real)
+    # a new set clears the records
+    g.set_observables(obs[2:3])
+    g.step(2)
+    assert g.observables()[0].shape == (2, 1)
+
+
 def _op(a):
     import ctypes as C
 

@@ -108,6 +108,8 @@ _SIGS = {
     "expect": [vp, i64, dblp, dblp],
     "norm2": [vp, dblp],
     "set_observable": [vp, i64, dblp],
+    "set_observables": [vp, i64, i64p, dblp],
+    "observables": [vp, i64p, dblp],
     "trajectory": [vp, i64p, dblp],
     "sb_problem": [vp, C.POINTER(SBParamsC), C.POINTER(vp), C.POINTER(vp)],
     "sb_initial_state": [vp, C.POINTER(SBParamsC)],
@@ -561,7 +563,26 @@ class Integrator:
 
     def set_observable(self, mode: int, op):
         self._obs = _cplx_in(op)
+        self.nobs = 1
         self.lib._call("set_observable", self.h, mode, _dp(self._obs))
+
+    def set_observables(self, obs):
+        """[(mode, d x d operator), ...] (at most 8) recorded on the device after every step"""
+        modes = np.array([m for m, _ in obs], dtype=np.int64)
+        self._obs = np.ascontiguousarray(np.concatenate([_cplx_in(o).ravel() for _, o in obs])) if obs else \
+            np.zeros(2)
+        self.nobs = len(obs)
+        self.lib._call("set_observables", self.h, len(obs), _ip(modes), _dp(self._obs))
+
+    def observables(self):
+        """(values (steps, nobs) complex, norms (steps,)) recorded since set_observables"""
+        n = i64(0)
+        self.lib._call("observables", self.h, C.byref(n), None)
+        k = getattr(self, "nobs", 1)
+        out = np.zeros((2 * k + 1) * n.value)
+        self.lib._call("observables", self.h, C.byref(n), _dp(out))
+        rec = out.reshape(-1, 2 * k + 1)
+        return rec[:, 0:2 * k:2] + 1j * rec[:, 1:2 * k:2], rec[:, 2 * k]
 
     def trajectory(self) -> np.ndarray:
         n = i64(0)

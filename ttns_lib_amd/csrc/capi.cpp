@@ -86,14 +86,13 @@ struct ttn_integ {
     int64_t prepares = 0;
     int64_t arena_size = 0;
     std::vector<int64_t> node_A;
-    int64_t obs_op = -1;
+    int64_t obs_slot[ttnd::MAX_OBS] = {};  // arena offsets of the observable operators
     int64_t tmp_op = -1;
-    int obs_leaf = -1;
-    int64_t obs_dim = 0;
+    int nobs = 0;                   // observables recorded after every step
     int64_t nsteps = 0;
     int64_t n_onesite = 0, n_twosite = 0;  // expansion attempts (engine().Nonesite/Ntwosite)
     double last_ms = 0.0;
-    std::vector<double> trajectory;  // (re <O>, <1>) per completed step
+    std::vector<double> trajectory;  // per completed step: re, im of every <O_j>, then <1>
 };
 
 // A captured sweep launch (zero fill of the team words + tdvp_run_kernel), replayed while
@@ -329,7 +328,7 @@ void build_instance(ttn_integ* in) {
     // observable slot and work buffers
     int64_t maxdim = 1;
     for (i64 d : t.mode_dims) maxdim = std::max<int64_t>(maxdim, d);
-    in->obs_op = alloc(maxdim * maxdim);
+    for (int j = 0; j < ttnd::MAX_OBS; ++j) in->obs_slot[j] = alloc(maxdim * maxdim);
     in->tmp_op = alloc(maxdim * maxdim);
     ttnd::DInst& I = in->inst;
     for (int k = 0; k < ttnd::NWORK; ++k) I.work[k] = alloc(maxsize);
@@ -410,8 +409,11 @@ void build_instance(ttn_integ* in) {
     I.tour_len = static_cast<int>(tour_i.size());
     I.kdim = kdim;
     I.tol = in->p.krylov_tol;
-    I.obs_op = in->obs_op;
-    I.obs_leaf = -1;
+    for (int j = 0; j < ttnd::MAX_OBS; ++j) {
+        I.obs_ops[j] = in->obs_slot[j];
+        I.obs_leaves[j] = -1;
+    }
+    I.nobs = 0;
     I.traj = nullptr;
     I.counters = in->counters.p;
     I.node_stats = in->node_stats.p;
@@ -465,14 +467,25 @@ void make_args(const ttn_integ* in, double cr, double ci, int64_t nsteps, ttnd::
     }
 }
 
+int64_t traj_stride(const ttn_integ* in) { return 2 * in->nobs + 1; }
+
 void prepare_traj(ttn_integ* in, int64_t nsteps) {
-    if (in->obs_leaf < 0) {
+    in->inst.nobs = in->nobs;
+    if (in->nobs == 0) {
         in->inst.traj = nullptr;
         return;
     }
-    if (static_cast<int64_t>(in->traj.n) < 2 * nsteps) in->traj.alloc(2 * nsteps);
+    const int64_t need = traj_stride(in) * nsteps;
+    if (static_cast<int64_t>(in->traj.n) < need) in->traj.alloc(need);
     in->inst.traj = in->traj.p;
-    in->inst.obs_leaf = in->obs_leaf;
+}
+
+// the trajectory records of the first `steps` steps of the last launch(es)
+void keep_traj(ttn_integ* in, int64_t steps) {
+    if (in->nobs == 0 || steps <= 0) return;
+    std::vector<double> buf(traj_stride(in) * steps);
+    hip_check(hipMemcpy(buf.data(), in->traj.p, buf.size() * sizeof(double), hipMemcpyDeviceToHost), "traj");
+    in->trajectory.insert(in->trajectory.end(), buf.begin(), buf.end());
 }
 
 void account(ttn_integ* in, const ttnd::RunArgs& a) {
@@ -509,11 +522,7 @@ std::string failure_message(const int st[2]) {
 void finish(ttn_integ* in, int64_t nsteps) {
     int st[2] = {0, 0};
     hip_check(hipMemcpy(st, in->status.p, sizeof(st), hipMemcpyDeviceToHost), "status");
-    if (in->obs_leaf >= 0 && nsteps > 0) {
-        std::vector<double> buf(2 * nsteps);
-        hip_check(hipMemcpy(buf.data(), in->traj.p, buf.size() * sizeof(double), hipMemcpyDeviceToHost), "traj");
-        in->trajectory.insert(in->trajectory.end(), buf.begin(), buf.end());
-    }
+    keep_traj(in, nsteps);
     if (st[0] != 0) fail(ERR_NUMERIC, failure_message(st));
     in->nsteps += nsteps;
 }
@@ -989,22 +998,49 @@ int ttn_state_get(ttn_integ* in, int64_t node, double* data, int64_t rows, int64
     });
 }
 
-int ttn_set_observable(ttn_integ* in, int64_t mode, const double* op) {
+int ttn_set_observables(ttn_integ* in, int64_t nops, const int64_t* modes, const double* ops) {
     return guard(in->ctx, [&] {
-        check(mode >= 0 && mode < in->topo.nmodes(), ERR_INVALID, "mode out of range");
-        const i64 d = in->topo.mode_dims[mode];
-        hip_check(hipMemcpy(in->arena.p + in->obs_op, op, d * d * sizeof(double2), hipMemcpyHostToDevice),
-                  "observable");
-        in->obs_leaf = in->topo.leaf_node[mode];
-        in->obs_dim = d;
+        check(nops >= 0 && nops <= ttnd::MAX_OBS, ERR_INVALID, "at most 8 observables are recorded on the device");
+        check(nops == 0 || (modes != nullptr && ops != nullptr), ERR_INVALID, "null observables");
+        int64_t at = 0;
+        for (int64_t j = 0; j < nops; ++j) {
+            check(modes[j] >= 0 && modes[j] < in->topo.nmodes(), ERR_INVALID, "mode out of range");
+            const i64 d = in->topo.mode_dims[modes[j]];
+            hip_check(hipMemcpy(in->arena.p + in->obs_slot[j], ops + 2 * at, d * d * sizeof(double2),
+                                hipMemcpyHostToDevice),
+                      "observable");
+            in->inst.obs_leaves[j] = in->topo.leaf_node[modes[j]];
+            at += d * d;
+        }
+        in->nobs = static_cast<int>(nops);
+        in->trajectory.clear();  // records of another observable set
     });
 }
 
-int ttn_trajectory(const ttn_integ* in, int64_t* len, double* out) {
-    const int64_t n = static_cast<int64_t>(in->trajectory.size() / 2);
+int ttn_set_observable(ttn_integ* in, int64_t mode, const double* op) {
+    return ttn_set_observables(in, 1, &mode, op);
+}
+
+int ttn_observables(const ttn_integ* in, int64_t* len, double* out) {
+    const int64_t stride = traj_stride(in);
+    const int64_t n = in->nobs == 0 ? 0 : static_cast<int64_t>(in->trajectory.size()) / stride;
     if (out) {
         const int64_t m = std::min(*len, n);
-        std::copy(in->trajectory.begin(), in->trajectory.begin() + 2 * m, out);
+        std::copy(in->trajectory.begin(), in->trajectory.begin() + stride * m, out);
+    }
+    *len = n;
+    return TTN_OK;
+}
+
+int ttn_trajectory(const ttn_integ* in, int64_t* len, double* out) {
+    const int64_t stride = traj_stride(in);
+    const int64_t n = in->nobs == 0 ? 0 : static_cast<int64_t>(in->trajectory.size()) / stride;
+    if (out) {
+        const int64_t m = std::min(*len, n);
+        for (int64_t k = 0; k < m; ++k) {
+            out[2 * k] = in->trajectory[stride * k];
+            out[2 * k + 1] = in->trajectory[stride * k + stride - 1];
+        }
     }
     *len = n;
     return TTN_OK;
@@ -1069,11 +1105,7 @@ void step_expanding(ttn_integ* in, double cr, double ci, int64_t nsteps) {
     in->last_ms = ms_total;
     if (st[0] != 0) {
         // the steps completed before the failure are kept; the failing one is not
-        if (in->obs_leaf >= 0 && done > 0) {
-            std::vector<double> buf(2 * done);
-            hip_check(hipMemcpy(buf.data(), in->traj.p, buf.size() * sizeof(double), hipMemcpyDeviceToHost), "traj");
-            in->trajectory.insert(in->trajectory.end(), buf.begin(), buf.end());
-        }
+        keep_traj(in, done);
         in->nsteps += done;
         fail(ERR_NUMERIC, failure_message(st));
     }

@@ -98,6 +98,8 @@ enum ExpWord { EXP_RNG_POS = 0, EXP_ONESITE, EXP_TWOSITE, EXP_GREW, EXP_LOG_N, E
 // added, then the Ritz values (descending, MAXK slots)
 constexpr int EXP_LOG_REC = 4 + MAXK;
 
+constexpr int MAX_OBS = 8;  // observables recorded on the device after every step
+
 struct DInst {
     double2* base;                  // arena
     const DNode* nodes;
@@ -113,10 +115,12 @@ struct DInst {
     int64_t maxsize;               // largest node size (work-buffer stride)
     int64_t work[NWORK];           // work buffer offsets
     int64_t krylov;                // Krylov basis (kdim + 1) * maxsize
-    int64_t obs_op;                // observable operator (d x d) offset, -1 if none
-    int obs_leaf;                  // node id of the observable's leaf, -1 if none
+    int64_t obs_ops[MAX_OBS];      // observable operators (d x d each) offsets
+    int obs_leaves[MAX_OBS];       // node id of each observable's leaf
+    int nobs;                      // observables recorded after every step (0: none)
     int pad1;
-    double* traj;                  // per-step <O>, <1> (2 doubles per step) or null
+    double* traj;                  // per step: re, im of every <O_j>, then <1> (2 nobs + 1 doubles),
+                                   // or null
     int64_t* counters;             // [0] H applications (node updates), [1] steps done
     int64_t* node_stats;           // per node: node-update H applications, their Krylov sizes
                                    // summed, bond-update applications (by operator node), sizes

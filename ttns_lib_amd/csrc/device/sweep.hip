@@ -486,14 +486,15 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
 #pragma unroll
             for (int i = 0; i < 2 * MAXK; ++i) p[i] = 0.0;
             // two strides per iteration keep 2(j+2) loads in flight per thread
+            const Coh cV(V), cw(w);
             for (int64_t q0 = q00; q0 < n; q0 += 2 * st) {
                 const int64_t q1 = q0 + st;
                 const bool ok1 = q1 < n;
-                const double2 w0 = ldc(w + q0), w1 = ok1 ? ldc(w + q1) : cz();
+                const double2 w0 = cw[q0], w1 = ok1 ? cw[q1] : cz();
 #pragma unroll
                 for (int i = 0; i < MAXK; ++i) {
                     if (i <= j) {
-                        const double2 a0 = ldc(V + i * ld + q0), a1 = ok1 ? ldc(V + i * ld + q1) : cz();
+                        const double2 a0 = cV[i * ld + q0], a1 = ok1 ? cV[i * ld + q1] : cz();
                         const double2 pr = cadd(cmulc(a0, w0), cmulc(a1, w1));
                         p[2 * i] += pr.x;
                         p[2 * i + 1] += pr.y;
@@ -505,10 +506,10 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
             for (int i = 0; i < 2 * MAXK; ++i) p[i] = 0.0;
             for (int64_t q = q00; q < n; q += st) {
                 double2 vi[MAXK];
-                double2 wq = ldc(w + q);
+                double2 wq = cw[q];
 #pragma unroll
                 for (int i = 0; i < MAXK; ++i)
-                    if (i <= j) vi[i] = ldc(V + i * ld + q);
+                    if (i <= j) vi[i] = cV[i * ld + q];
 #pragma unroll
                 for (int i = 0; i < MAXK; ++i)
                     if (i <= j) wq = csub(wq, cmul(cmk(h1[2 * i], h1[2 * i + 1]), vi[i]));
@@ -527,12 +528,12 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
             for (int64_t q0 = q00; q0 < n; q0 += 2 * st) {
                 const int64_t q1 = q0 + st;
                 const bool ok1 = q1 < n;
-                double2 w0 = ldc(w + q0), w1 = ok1 ? ldc(w + q1) : cz();
+                double2 w0 = cw[q0], w1 = ok1 ? cw[q1] : cz();
 #pragma unroll
                 for (int i = 0; i < MAXK; ++i) {
                     if (i <= j) {
                         const double2 h = cmk(h2[2 * i], h2[2 * i + 1]);
-                        const double2 a0 = ldc(V + i * ld + q0), a1 = ok1 ? ldc(V + i * ld + q1) : cz();
+                        const double2 a0 = cV[i * ld + q0], a1 = ok1 ? cV[i * ld + q1] : cz();
                         w0 = csub(w0, cmul(h, a0));
                         w1 = csub(w1, cmul(h, a1));
                     }
@@ -1272,15 +1273,22 @@ __device__ void path_expect(Ctx& c, int leaf, int64_t op_off, double2& val, doub
     nrm = team_norm2(T, c.base + root.A, c.shape(0).size);
 }
 
+// every observable's <psi|O_j|psi> and <psi|psi> into the step's trajectory record (the
+// driver's mel(mops[j], 0, A) after each step, src/spin_boson.cpp:86-90)
 __device__ void step_done(Ctx& c, int64_t step_index) {
-    if (c.I.obs_leaf >= 0 && c.I.traj != nullptr) {
-        double2 v;
-        double nrm;
-        path_expect(c, c.I.obs_leaf, c.I.obs_op, v, nrm);
-        if (c.writer()) {
-            c.I.traj[2 * step_index] = v.x;
-            c.I.traj[2 * step_index + 1] = nrm;
+    if (c.I.nobs > 0 && c.I.traj != nullptr) {
+        const int no = c.I.nobs;
+        double* rec = c.I.traj + step_index * (2 * no + 1);
+        double nrm = 0.0;
+        for (int j = 0; j < no; ++j) {
+            double2 v;
+            path_expect(c, c.I.obs_leaves[j], c.I.obs_ops[j], v, nrm);
+            if (c.writer()) {
+                rec[2 * j] = v.x;
+                rec[2 * j + 1] = v.y;
+            }
         }
+        if (c.writer()) rec[2 * no] = nrm;
     }
     if (c.writer()) c.I.counters[1] += 1;
     __syncthreads();
@@ -1396,7 +1404,7 @@ __global__ __launch_bounds__(NT) void tdvp_run_kernel(const DInst* insts, RunArg
     }
 }
 
-// <psi|O|psi> on one instance (O stored at obs_op); out: re, im, norm2
+// <psi|O|psi> on one instance (O stored at op_off); out: re, im, norm2
 __global__ __launch_bounds__(NT) void expect_kernel(const DInst* inst, int leaf, int64_t op_off, double* out) {
     if (threadIdx.x == 0) g_sh.fail = 0;
     __syncthreads();

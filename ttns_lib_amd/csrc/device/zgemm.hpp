@@ -32,6 +32,29 @@ __device__ __forceinline__ double2 ldc(const double2* p) {
     return make_double2(ldc(q), ldc(q + 1));
 }
 
+// The same coherent loads, 16 bytes at a time, from a wave-uniform base: a buffer load with the
+// device-scope cache policy (sc1, as the atomic loads above get) through a raw buffer
+// descriptor.  The base must be the same in every lane of the wave (it is read from lane 0);
+// offsets are element indices below 2^27 (2 GiB, the descriptor's extent; the hardware returns
+// zero beyond it instead of faulting).
+struct Coh {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ explicit Coh(const double2* base) {
+        const uint64_t a = reinterpret_cast<uint64_t>(base);
+        const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(a));
+        const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(a >> 32));
+        void* u = reinterpret_cast<void*>((static_cast<uint64_t>(hi) << 32) | lo);
+        r = __builtin_amdgcn_make_buffer_rsrc(u, 0, 0x7fffffff, 0x00020000);
+    }
+    __device__ __forceinline__ double2 operator[](int64_t i) const {
+        typedef int v4i __attribute__((ext_vector_type(4)));
+        const v4i v = __builtin_amdgcn_raw_buffer_load_b128(r, static_cast<int>(i * 16), 0, 16);
+        double2 d;
+        __builtin_memcpy(&d, &v, 16);
+        return d;
+    }
+};
+
 struct ZAcc {
     d4 re, im;
 };

@@ -129,6 +129,10 @@ class Job:
         self.ig.set_sb_initial_state(model)
         self.caps = self.tree.capacities()
         self.ops = observables(cfg)
+        # every observable is recorded on the device after each step when they fit its 8 slots
+        self.on_device = 0 < len(self.ops) <= 8 and lib.has("set_observables")
+        if self.on_device:
+            self.ig.set_observables([(0, o) for o in self.ops])
         self.print_hranks = bool(cfg.get("printbonddimension"))
         self.nnodes = len(self.ig.shapes)
         self.tmax, self.steps, self.out = float(cfg["tmax"]), 0, out
@@ -148,7 +152,10 @@ class Job:
     def record(self):
         """The row of the step just taken."""
         self.steps += 1
-        vals = "".join(_g(self.ig.expect(0, o).real) + " " for o in self.ops)
+        if self.on_device:
+            vals = "".join(_g(v.real) + " " for v in self.ig.observables()[0][-1])
+        else:
+            vals = "".join(_g(self.ig.expect(0, o).real) + " " for o in self.ops)
         nh = self.ig.counters()[0]
         n1, n2 = self.ig.expansion_counters()
         self.out.write(f"{_g(self.ig.time())} {vals}{n1} {n2} {_g(nh / self.nnodes)} {self.hranks()}\n")
