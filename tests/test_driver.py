@@ -132,3 +132,34 @@ def test_driver_batch_of_inputs_on_gpu_matches_reference_output(tmp_path):
     for (run, _, _, out), steps, bounds, exact in made:
         with open(out) as f:
             check_rows(parse(f.read()), run, steps, bounds, exact)
+
+
+@pytest.mark.gpu
+def test_driver_discretises_the_bath_when_no_pairs_are_given(orc, tmp_path):
+    # the command without --bath discretises the input's spectral density (ttns_lib_amd.bath) and
+    # prints the pairs in the reference's log format; the rows equal the oracle's on those pairs
+    from ttns_lib_amd.bath import discretise_input
+
+    cfg = {"Hsys": [[0, 1], [1, 0]], "Scoup": [[1, 0], [0, -1]], "psi0": [1, 0], "nmodes": 16, "dt": 0.05,
+           "tmax": 0.1, "nspf": 6, "nspf lower": 6, "maximum dimension": 8, "target dimension": 8,
+           "krylov dimension": 12, "seed": 1, "bath integration error tolerance": 1e-3,
+           "bath": {"type": "exponential", "wc": 5.0, "alpha": 0.2, "s": 1},
+           "ops": [[[1, 0], [0, -1]]], "output file": str(tmp_path / "out.out")}
+    inp = tmp_path / "input.json"
+    inp.write_text(json.dumps(cfg))
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    print("[driver] discretised bath, 16 modes", flush=True)
+    r = subprocess.run([sys.executable, "-m", "ttns_lib_amd.driver", str(inp)], check=True, cwd=root, timeout=300,
+                       capture_output=True, text=True)
+    wk, gk = driver.load_bath_text(r.stderr)
+    c = driver.load_input(str(inp))
+    w0, g0 = discretise_input(c)
+    assert np.array_equal(wk, w0) and np.array_equal(gk, g0)
+    out = io.StringIO()
+    driver.run(orc, c, w0, g0, out, log=io.StringIO())
+    with open(cfg["output file"]) as f:
+        got = [ln.split() for ln in f.read().splitlines() if not ln.startswith("nh_app")]
+    want = [ln.split() for ln in out.getvalue().splitlines() if not ln.startswith("nh_app")]
+    assert len(got) == len(want) == 3
+    for a, b in zip(got, want):
+        assert abs(float(a[0]) - float(b[0])) < 1e-15 and abs(float(a[1]) - float(b[1])) < 1e-10

@@ -3,10 +3,10 @@
 
 * reads the reference's JSON input.  Keys are matched with spaces removed, as its loader does
   (e.g. "krylov tolerance" -> krylovtolerance, :253).
-* takes the discretised bath as w_k g_k pairs (`--bath`): a two-column file, or the log that the
-  reference writes to stderr, whose pairs sit between the "w_k \\t g_k" header and the line
-  "bath discretised." (:356-363).  The orthogonal-polynomial discretisation itself
-  (:292-354, un-vendored `orthopol`) is not part of this library.
+* discretises the input's spectral density into nmodes harmonic modes (:292-365; the Gauss rule
+  of ttns_lib_amd.bath), or takes the pairs w_k g_k from `--bath`: a two-column file, or the log
+  that the reference writes to stderr, whose pairs sit between the "w_k \\t g_k" header and the
+  line "bath discretised." (:356-363).
 * builds the leaves with partition_modes and the harmonic mode blocks (:418-441, 515-538), the
   tree, the Hamiltonian and the seeded initial state.
 * steps while t < tmax and writes the reference's output rows (`run`, :41-107):
@@ -28,7 +28,7 @@ reference's process pool over inputs (run_examples.py); each writes its own outp
 that share dt and whose bonds are all at capacity step in one persistent launch
 (ttn_batch_step); runs whose bonds can still grow step one after another.
 
-usage: python -m ttns_lib_amd.driver input.json [input2.json ...] --bath bath.txt [--bath bath2.txt ...]
+usage: python -m ttns_lib_amd.driver input.json [input2.json ...] [--bath bath.txt [--bath bath2.txt ...]]
                                      [--output out] [--steps N] [--max-capacity C]
 """
 from __future__ import annotations
@@ -58,7 +58,12 @@ def load_input(path: str) -> dict:
 def load_bath(path: str):
     """(wk, gk) from a two-column file or from the reference's stderr log."""
     with open(path, "rb") as f:
-        lines = f.read().decode().split("\n")
+        return load_bath_text(f.read().decode(), path)
+
+
+def load_bath_text(text: str, what: str = "bath"):
+    """(wk, gk) from the text of a two-column file or of the reference's stderr log."""
+    lines = text.split("\n")
     head = [i for i, ln in enumerate(lines) if ln.rstrip().endswith("g_k")]
     if head:
         end = next((i for i, ln in enumerate(lines) if ln.strip() == "bath discretised."), len(lines))
@@ -67,7 +72,7 @@ def load_bath(path: str):
     wk = np.array([float(p[0]) for p in pairs])
     gk = np.array([float(p[1]) for p in pairs])
     if len(wk) == 0:
-        raise ValueError(f"{path}: no w_k g_k pairs")
+        raise ValueError(f"{what}: no w_k g_k pairs")
     return wk, gk
 
 
@@ -218,8 +223,9 @@ def run_batch(lib: Library, jobs_in, max_steps: Optional[int] = None, log: TextI
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("inputs", nargs="+")
-    ap.add_argument("--bath", action="append", required=True,
-                    help="w_k g_k pairs, or the reference's stderr log of them (one per input, in order)")
+    ap.add_argument("--bath", action="append",
+                    help="w_k g_k pairs, or the reference's stderr log of them (one per input, in order); "
+                         "without it the input's spectral density is discretised (ttns_lib_amd.bath)")
     ap.add_argument("--output", help="output file of a single input (default: its 'output file', else stdout)")
     ap.add_argument("--steps", type=int, help="stop after this many steps")
     ap.add_argument("--max-capacity", type=int,
@@ -227,10 +233,23 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
     from . import load_library
 
-    if len(args.bath) != len(args.inputs):
-        raise SystemExit("give one --bath per input")
     cfgs = [load_input(p) for p in args.inputs]
-    baths = [load_bath(p) for p in args.bath]
+    if args.bath:
+        if len(args.bath) != len(args.inputs):
+            raise SystemExit("give one --bath per input, or none")
+        baths = [load_bath(p) for p in args.bath]
+    else:
+        from .bath import discretise_input
+
+        baths = []
+        for cfg in cfgs:
+            wk, gk = discretise_input(cfg)
+            # the reference prints the pairs it discretised to stderr (src/spin_boson.cpp:354-363)
+            print("w_k \t g_k", file=sys.stderr)
+            for w, g in zip(wk, gk):
+                print(f"{w!r} {g!r}", file=sys.stderr)
+            print("bath discretised.", file=sys.stderr)
+            baths.append((wk, gk))
     for p, cfg, (wk, _) in zip(args.inputs, cfgs, baths):
         if "nmodes" in cfg and int(cfg["nmodes"]) != len(wk):
             raise SystemExit(f"{p}: the input asks for {cfg['nmodes']} modes, the bath file has {len(wk)}")
