@@ -247,7 +247,7 @@ def main(argv=None) -> int:
             # the reference prints the pairs it discretised to stderr (src/spin_boson.cpp:354-363)
             print("w_k \t g_k", file=sys.stderr)
             for w, g in zip(wk, gk):
-                print(f"{w!r} {g!r}", file=sys.stderr)
+                print(f"{float(w)!r} {float(g)!r}", file=sys.stderr)
             print("bath discretised.", file=sys.stderr)
             baths.append((wk, gk))
     for p, cfg, (wk, _) in zip(args.inputs, cfgs, baths):
