@@ -235,6 +235,22 @@ int ttn_sb_mode_block(ttn_ctx* ctx, int64_t nmodes, const double* w, const doubl
                       int64_t* nstates, int64_t* nnz, int64_t* occupations, double* h0, int64_t* hc_rowptr,
                       int64_t* hc_colind, double* hc_vals);
 
+/* Gradient of J = Re<psi(T)|O|psi(T)> with respect to the bath couplings g_k and frequencies
+ * w_k of a spin-boson integrator built from p by ttn_sb_problem (fixed-step, real time, not
+ * yet stepped), T = nsteps * dt, O (d x d, complex) on leaf mode `mode`.  Costate method:
+ * dJ/dtheta = 2 int_0^T Im <chi(t)|dH/dtheta|psi(t)> dt with chi(t) = exp(iH(T - t)) O_s psi(T),
+ * O_s = (O + O^H)/2.  psi is advanced with a checkpoint every `every` calls and recomputed
+ * from the checkpoints; chi is a second tree state propagated backwards by the same TDVP
+ * step; each step's integral is an nquad-point Gauss-Legendre rule (1-16 nodes; 0: chosen
+ * from dt, max |w_k| and |Hsys| so that exp(i w dt s) is integrated to 1e-13) whose
+ * partial steps of psi and chi advance in one persistent launch, the matrix elements come
+ * from the leaves' transition densities (one workgroup per node pair).  value: J; dg, dw:
+ * nmodes doubles each.  On return the integrator holds psi(T).  The reference has no adjoint:
+ * the forward is projector_splitting_integrator.hpp:136-169, and g_k, w_k enter the leaves'
+ * H0 and Hc as src/mode_data.hpp:239-253, 309-361 builds them (Fock cutoffs held fixed). */
+int ttn_sb_adjoint_run(ttn_integ* integ, const ttn_sb_params* p, int64_t nsteps, int64_t every, int64_t nquad,
+                       int64_t mode, const double* op, double* value, double* dg, double* dw);
+
 /* ---- batched persistent sweep: many instances advance in one launch ---- */
 int ttn_batch_create(ttn_ctx* ctx, int64_t n, ttn_integ* const* integs, ttn_batch** out);
 int ttn_batch_destroy(ttn_batch* batch);
@@ -305,6 +321,14 @@ int ttn_node_mf(ttn_node* node, int64_t child, int64_t reps, double* out);
 int ttn_node_expv(ttn_node* node, double coeff_re, double coeff_im, double tau, int64_t krylov_dim, double tol,
                   int64_t* napply);
 int ttn_node_last(const ttn_node* node, double* ms, double* flops);
+/* Tile autotuner of the node-level GEMMs (on by default): the first time a context meets a GEMM
+ * shape it times every output tile (32x32, 64x32, 32x64, 64x64, 128x64, 64x128 per workgroup)
+ * with several K splits on the device and keeps the fastest; tiles never change results, K
+ * splits change their rounding.  The allocation-free entry point uses the table but never
+ * times.  ttn_autotune_table: *n entries (query with entries = ms = NULL), each M, N, K,
+ * batch, K block, tile (0-5 in the order above), K split, and its time in ms. */
+int ttn_ctx_set_autotune(ttn_ctx* ctx, int enabled);
+int ttn_autotune_table(const ttn_ctx* ctx, int64_t* n, int64_t* entries, double* ms);
 
 /* No counterpart in the reference.  For callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the

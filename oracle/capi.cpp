@@ -340,6 +340,49 @@ int orc_sb_initial_state(orc_integ* in, const ttn_sb_params* p) {
     });
 }
 
+int orc_sb_adjoint_run(orc_integ* in, const ttn_sb_params* p, int64_t nsteps, int64_t every, int64_t nquad,
+                       int64_t mode, const double* op, double* value, double* dg, double* dw) {
+    return guard(in->ctx, [&] {
+        const auto& topo = in->eng->topo();
+        check(mode >= 0 && mode < topo.nmodes(), ERR_INVALID, "mode out of range");
+        const i64 d = topo.mode_dims[mode];
+        std::vector<cplx> o(d * d);
+        for (i64 i = 0; i < d * d; ++i) o[i] = cget(op, i);
+        const SBParams sp = sb_from_c(p);
+        std::vector<double> g, w;
+        const double J = orc::sb_adjoint(*in->eng, sp, nsteps, every, static_cast<int>(nquad), static_cast<int>(mode),
+                                         o.data(), g, w);
+        std::copy(g.begin(), g.end(), dg);
+        std::copy(w.begin(), w.end(), dw);
+        if (value) *value = J;
+    });
+}
+int orc_transition_densities(orc_integ* ket, orc_integ* bra, int64_t ins, const double* S, int64_t len, double* out) {
+    return guard(ket->ctx, [&] {
+        const auto& topo = ket->eng->topo();
+        check(ins >= 0 && ins < topo.nmodes(), ERR_INVALID, "mode out of range");
+        const i64 d = topo.mode_dims[ins];
+        std::vector<cplx> s(d * d);
+        for (i64 i = 0; i < d * d; ++i) s[i] = cget(S, i);
+        std::vector<std::vector<cplx>> rho, rhoS;
+        orc::transition_densities(*ket->eng, *bra->eng, static_cast<int>(ins), s.data(), rho, rhoS);
+        // per leaf mode: rho (d x d) then rhoS (d x d, zeros on the insertion leaf)
+        i64 at = 0;
+        for (int m = 0; m < topo.nmodes(); ++m) {
+            const i64 dm = topo.mode_dims[m];
+            check(at + 2 * dm * dm <= len, ERR_INVALID, "transition densities: output too small");
+            for (i64 q = 0; q < dm * dm; ++q) {
+                const cplx a = rho[m][q], b = rhoS[m].empty() ? cplx(0.0, 0.0) : rhoS[m][q];
+                out[2 * (at + q)] = a.real();
+                out[2 * (at + q) + 1] = a.imag();
+                out[2 * (at + dm * dm + q)] = b.real();
+                out[2 * (at + dm * dm + q) + 1] = b.imag();
+            }
+            at += 2 * dm * dm;
+        }
+    });
+}
+
 int orc_sb_partition_modes(orc_ctx* ctx, int64_t nmodes, const double* wk, double ecut, double wc, int64_t nmax,
                            int64_t nmax4, int64_t nmax2, int64_t nmax3, int64_t* nblocks, int64_t* partition) {
     return guard(ctx, [&] { sb_partition_c(nmodes, wk, ecut, wc, nmax, nmax4, nmax2, nmax3, nblocks, partition); });

@@ -35,6 +35,7 @@
 #pragma once
 
 #include "../ttns_lib_amd/csrc/host/sop.hpp"
+#include "../ttns_lib_amd/csrc/host/spin_boson.hpp"
 #include "../ttns_lib_amd/csrc/host/topology.hpp"
 
 #include <functional>
@@ -100,6 +101,7 @@ public:
     std::vector<cplx>& A(int n) { return m_A[n]; }
     const ttnh::Topology& topo() const { return m_topo; }
     const ttnh::CompiledSOP& compiled() const { return m_ops; }
+    const ttnh::SOP& sop() const { return m_sop; }
 
     // one integrator call (projector_splitting_integrator.hpp:136-169, 281-326)
     void step(cplx coeff);
@@ -144,6 +146,26 @@ public:
     const std::vector<cplx>& mf(int n, int t) const { return m_mf[n][t]; }
     std::vector<cplx>& mf_mut(int n, int t) { return m_mf[n][t]; }
     std::vector<cplx>& spf_mut(int n, int t) { return m_spf[n][t]; }
+    const std::vector<cplx>& node(int n) const { return m_A[n]; }
+
+    // the whole integrator state (node tensors, gauge and environment matrices, counters, dt)
+    struct Snapshot {
+        std::vector<std::vector<cplx>> A, R;
+        std::vector<std::vector<std::vector<cplx>>> spf, mf;
+        i64 nsteps = 0, nh = 0;
+        double dt = 0.0, t = 0.0;
+    };
+    Snapshot snapshot() const { return {m_A, m_R, m_spf, m_mf, m_nsteps, m_nh, m_dt, m_t}; }
+    void restore(const Snapshot& s) {
+        m_A = s.A;
+        m_R = s.R;
+        m_spf = s.spf;
+        m_mf = s.mf;
+        m_nsteps = s.nsteps;
+        m_nh = s.nh;
+        m_dt = s.dt;
+        m_t = s.t;
+    }
 
 private:
     void kpo(int n, const std::vector<std::pair<int, int>>& list, const cplx* v, cplx* out, cplx* tmp) const;
@@ -191,5 +213,17 @@ private:
     std::mt19937 m_rng;
     std::normal_distribution<double> m_ndist{0.0, 1.0};
 };
+
+// ---- gradients by the costate method (adjoint.cpp; the reference has no adjoint) ----
+// rho[m] = <bra|(|j><l|)_m|ket> for every leaf mode m (d_m x d_m, row-major), and rhoS[m] the same
+// with S (d x d) inserted on leaf mode `ins` (rhoS[ins] empty).  Full contractions of the two
+// trees; no orthonormality is assumed.
+void transition_densities(const Engine& ket, const Engine& bra, int ins, const cplx* S,
+                          std::vector<std::vector<cplx>>& rho, std::vector<std::vector<cplx>>& rhoS);
+
+// J = Re<psi(T)|O|psi(T)> after nsteps real-time integrator calls from a state that has not
+// been stepped, and dJ/dg_k, dJ/dw_k of the spin-boson model p (adjoint.cpp has the method).
+double sb_adjoint(Engine& ket, const ttnh::SBParams& p, i64 nsteps, i64 every, int nquad, int mode, const cplx* O,
+                  std::vector<double>& dg, std::vector<double>& dw);
 
 }  // namespace orc

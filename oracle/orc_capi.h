@@ -63,6 +63,14 @@ int orc_sb_mode_block(orc_ctx* ctx, int64_t nmodes, const double* w, const doubl
                       int64_t* nstates, int64_t* nnz, int64_t* occupations, double* h0, int64_t* hc_rowptr,
                       int64_t* hc_colind, double* hc_vals);
 
+/* gradient of Re<psi(T)|O|psi(T)> in the spin-boson couplings and frequencies by the costate
+ * method (oracle/adjoint.cpp), same arguments as ttn_sb_adjoint_run */
+int orc_sb_adjoint_run(orc_integ* integ, const ttn_sb_params* p, int64_t nsteps, int64_t every, int64_t nquad,
+                       int64_t mode, const double* op, double* value, double* dg, double* dw);
+/* leaf transition densities <bra|(|j><l|)_m|ket>, and with S inserted on leaf mode ins: per
+ * leaf mode m, d_m^2 complex entries of each (len: complex entries of out) */
+int orc_transition_densities(orc_integ* ket, orc_integ* bra, int64_t ins, const double* S, int64_t len, double* out);
+
 /* building blocks for unit tests: apply the node / bond effective Hamiltonian with the
  * current environments, refresh environments, and the dense kernels */
 int orc_heff_node(orc_integ* integ, int64_t node, const double* v, double* out);

@@ -54,4 +54,5 @@ ORACLE_EXTRA = {
     "normals_used": [_C.c_void_p, _i],
     "singular_values": [_C.c_int64, _d, _d],
     "tridiag_eigen": [_C.c_int64, _d, _d, _d, _d],
+    "sb_adjoint_run": [_C.c_void_p, _C.c_void_p, _C.c_int64, _C.c_int64, _C.c_int64, _C.c_int64, _d, _d, _d, _d],
 }

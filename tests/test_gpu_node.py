@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 from node_fixture import NodeProblem
-from ttns_lib_amd.capi import Node
+from ttns_lib_amd.capi import GEMM_TILES, Node
 
 pytestmark = pytest.mark.gpu
 
@@ -57,3 +57,30 @@ def test_node_expv_matches_oracle(hip, orc, d0, d1, r, nt, tau, scale, na_want):
     # unitary propagation
     assert abs(np.linalg.norm(g.get()) - 1.0) < 1e-11
     g.destroy()
+
+
+def test_autotuner_keeps_results_and_reports_its_tiles(hip, orc):
+    """The contraction microbenchmark's node with the tile autotuner off and on: the tuned
+    H-action equals the oracle's, repeats itself bit for bit from the table, and the table holds
+    a valid tile and K split for every GEMM shape of the H-action."""
+    P = NodeProblem(orc, 64, 64, 64, 40, seed=3)
+    hip.set_autotune(True)
+    g = Node(hip, P.op0, P.op1, P.mf)
+    g.set(P.v)
+    out = g.heff()
+    table = hip.autotune_table()
+    print("[autotune] " + "; ".join(f"{e[:4]} tile {GEMM_TILES[e[5]]} split {e[6]} {e[7]:.3f} ms" for e in table),
+          flush=True)
+    assert len(table) >= 2
+    assert all(0 <= e[5] < len(GEMM_TILES) and e[6] >= 1 and e[7] > 0 for e in table)
+    assert _rel(out, P.heff(P.v)) < 1e-13
+    assert np.array_equal(out, g.heff())
+    g.destroy()
+    hip.set_autotune(False)
+    try:
+        g0 = Node(hip, P.op0, P.op1, P.mf)
+        g0.set(P.v)
+        assert _rel(g0.heff(), out) < 1e-14
+        g0.destroy()
+    finally:
+        hip.set_autotune(True)

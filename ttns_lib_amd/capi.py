@@ -139,6 +139,10 @@ _SIGS = {
     "node_expv": [vp, C.c_double, C.c_double, C.c_double, i64, C.c_double, i64p],
     "node_heff_noalloc": [vp, i64, i64, i64, i64, vp, vp, vp, vp, vp, vp, vp],
     "node_last": [vp, dblp, dblp],
+    "sb_adjoint_run": [vp, C.POINTER(SBParamsC), i64, i64, i64, i64, dblp, dblp, dblp, dblp],
+    "ctx_set_autotune": [vp, C.c_int],
+    "autotune_table": [vp, i64p, i64p, dblp],
+    "transition_densities": [vp, vp, i64, dblp, i64, dblp],
 }
 _RESTYPE = {"last_error": C.c_char_p}
 
@@ -154,6 +158,10 @@ def _ip(a: np.ndarray):
 def _cplx_in(a) -> np.ndarray:
     a = np.ascontiguousarray(np.asarray(a, dtype=np.complex128))
     return a.view(np.float64)
+
+
+# output tiles (rows, columns per workgroup) of the node-level GEMM, in ttn_autotune_table order
+GEMM_TILES = [(32, 32), (64, 32), (32, 64), (64, 64), (128, 64), (64, 128)]
 
 
 class Library:
@@ -186,6 +194,22 @@ class Library:
         team_min entries are spread over an instance's team; teams=False runs every
         instance on one workgroup."""
         self._call("ctx_set_tuning", self.ctx, int(team_min), int(bool(teams)))
+
+    def set_autotune(self, enabled: bool = True):
+        """tile autotuner of the node-level GEMMs (ttn_ctx_set_autotune)"""
+        self._call("ctx_set_autotune", self.ctx, int(bool(enabled)))
+
+    def autotune_table(self) -> list:
+        """[(M, N, K, batch, kblock, tile, ksplit, ms)] of every GEMM shape the autotuner timed;
+        tile indexes GEMM_TILES"""
+        n = i64(0)
+        self._call("autotune_table", self.ctx, C.byref(n), None, None)
+        if n.value == 0:
+            return []
+        e = np.zeros(7 * n.value, dtype=np.int64)
+        ms = np.zeros(n.value)
+        self._call("autotune_table", self.ctx, C.byref(n), _ip(e), _dp(ms))
+        return [tuple(int(x) for x in e[7 * i:7 * i + 7]) + (float(ms[i]),) for i in range(n.value)]
 
     def set_small(self, small_max: int = 512):
         """Krylov propagations of at most small_max entries run on one workgroup with the
@@ -391,6 +415,17 @@ def compiled_tables(lib: Library, tree: Tree, sop: SOPHandle) -> np.ndarray:
     return buf
 
 
+def sb_gradient(lib: Library, model: SpinBoson, dops) -> tuple:
+    """(dJ/dg_k, dJ/dw_k) from the cotangents of the spin-boson operator values
+    (ttn_sb_gradient; the Fock cutoffs held fixed)"""
+    d = _cplx_in(dops)
+    n = len(model.wk)
+    dg, dw = np.zeros(n), np.zeros(n)
+    p = model.to_c()
+    lib._call("sb_gradient", lib.ctx, C.byref(p), len(d) // 2, _dp(d), _dp(dg), _dp(dw))
+    return dg, dw
+
+
 def sb_problem(lib: Library, model: SpinBoson):
     t, s = vp(), vp()
     p = model.to_c()
@@ -532,6 +567,40 @@ class Integrator:
         out = C.c_double()
         self.lib._call("norm2", self.h, C.byref(out))
         return out.value
+
+    def sb_adjoint(self, model: "SpinBoson", nsteps: int, mode: int, op, *, every: int = 20, nquad: int = 0):
+        """(J, dJ/dg_k, dJ/dw_k) for J = Re<psi(T)|O|psi(T)> after nsteps real-time calls from a
+        state that has not been stepped (ttn_sb_adjoint_run: costate method, the state
+        checkpointed every `every` calls, nquad-point Gauss-Legendre quadrature per step, 0: chosen
+        from dt and the largest frequency).  The
+        integrator is left at the state after nsteps calls."""
+        o = _cplx_in(op)
+        val = C.c_double()
+        n = len(model.wk)
+        dg, dw = np.zeros(n), np.zeros(n)
+        p = model.to_c()
+        self.lib._call("sb_adjoint_run", self.h, C.byref(p), int(nsteps), int(every), int(nquad), int(mode), _dp(o),
+                       C.byref(val), _dp(dg), _dp(dw))
+        return val.value, dg, dw
+
+    def transition_densities(self, bra: "Integrator", ins: int, S) -> list:
+        """[(rho_m, rhoS_m)] per leaf mode m: rho_m(j, l) = <bra|(|j><l|)_m|self>, rhoS_m the same
+        with S inserted on leaf mode ins (zero on that leaf)"""
+        s = _cplx_in(S)
+        leaves = [self.tree.node(i) for i in range(len(self.shapes))]
+        ml = [0] * sum(1 for x in leaves if x["nchild"] == 0)
+        for x in leaves:
+            if x["nchild"] == 0:
+                ml[x["leaf_index"]] = x["rows"]
+        tot = sum(2 * d * d for d in ml)
+        out = np.zeros(2 * tot)
+        self.lib._call("transition_densities", self.h, bra.h, int(ins), _dp(s), tot, _dp(out))
+        z = out.view(np.complex128)
+        res, at = [], 0
+        for d in ml:
+            res.append((z[at:at + d * d].reshape(d, d).copy(), z[at + d * d:at + 2 * d * d].reshape(d, d).copy()))
+            at += 2 * d * d
+        return res
 
     def last_step_ms(self) -> float:
         out = C.c_double()

@@ -31,6 +31,9 @@ extern "C" hipError_t ttnd_launch_expect(const ttnd::DInst* d_inst, int leaf, in
 extern "C" hipError_t ttnd_run_occupancy(int* blocks_per_cu);
 extern "C" hipError_t ttnd_launch_census(int* d_out, int n, hipStream_t s);
 extern "C" hipError_t ttnd_launch_zero_u32(unsigned* p, int64_t n, hipStream_t s);
+extern "C" hipError_t ttnd_launch_densities(const ttnd::DInst* kets, const ttnd::DInst* bras, int npairs, int ins,
+                                            const double2* S, double2* work, int64_t work_stride, int64_t rr,
+                                            const int64_t* out_off, double2* out, int64_t out_stride, hipStream_t s);
 
 using namespace ttnh;
 using namespace ttnrt;
@@ -1504,6 +1507,288 @@ int ttn_sb_initial_state(ttn_integ* in, const ttn_sb_params* p) {
             hip_check(hipMemcpy(in->arena.p + in->node_A[n], st[n].data(), st[n].size() * sizeof(double2),
                                 hipMemcpyHostToDevice),
                       "initial state");
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- costate gradient
+namespace {
+
+// A device copy of an integrator's whole arena (node tensors, gauge and environment
+// matrices, operators) with its step count and dt: instances built from the same tree,
+// operator and parameters share the arena layout, so a copy moves a state between them.
+struct ArenaSnap {
+    DevBuf<double2> a;
+    int64_t nsteps = 0;
+    double dt = 0.0;
+};
+
+void snap_take(const ttn_integ* in, ArenaSnap& s) {
+    if (static_cast<int64_t>(s.a.n) < in->arena_size) s.a.alloc(in->arena_size);
+    hip_check(hipMemcpyAsync(s.a.p, in->arena.p, in->arena_size * sizeof(double2), hipMemcpyDeviceToDevice,
+                             in->ctx->stream),
+              "snapshot");
+    s.nsteps = in->nsteps;
+    s.dt = in->p.dt;
+}
+
+void snap_put(ttn_integ* in, const ArenaSnap& s) {
+    hip_check(hipMemcpyAsync(in->arena.p, s.a.p, in->arena_size * sizeof(double2), hipMemcpyDeviceToDevice,
+                             in->ctx->stream),
+              "restore");
+    in->nsteps = s.nsteps;
+    in->p.dt = s.dt;
+}
+
+void copy_state(ttn_integ* dst, const ttn_integ* src) {
+    check(dst->arena_size == src->arena_size, ERR_INVALID, "adjoint: instances with different layouts");
+    hip_check(hipMemcpyAsync(dst->arena.p, src->arena.p, src->arena_size * sizeof(double2), hipMemcpyDeviceToDevice,
+                             src->ctx->stream),
+              "copy state");
+    dst->nsteps = src->nsteps;
+    dst->p.dt = src->p.dt;
+}
+
+// a second integrator on the same tree, operator (current coefficients) and parameters
+std::unique_ptr<ttn_integ> twin(const ttn_integ* in) {
+    auto t = std::make_unique<ttn_integ>();
+    t->ctx = in->ctx;
+    t->topo = in->topo;
+    t->sop = in->sop;
+    t->p = in->p;
+    t->ops = compile_sop(t->topo, t->sop);
+    build_instance(t.get());
+    return t;
+}
+
+// one integrator call of one instance (ttn_step without the trajectory bookkeeping)
+void step_one(ttn_integ* in, double ci) {
+    ttnd::RunArgs a;
+    make_args(in, 0.0, ci, 1, a);
+    prepare_traj(in, 1);
+    hip_check(hipMemcpyAsync(in->d_inst.p, &in->inst, sizeof(in->inst), hipMemcpyHostToDevice, in->ctx->stream),
+              "inst");
+    launch(in->ctx, in->d_inst.p, {in->inst}, a, {auto_team(in)});
+    account(in, a);
+    finish(in, 1);
+}
+
+// one call of every instance, each with its own coefficient (-i forward, +i backward in time)
+// and dt, in one persistent launch (teams sized by the batch autotuner)
+void step_each(ttn_ctx* ctx, const std::vector<ttn_integ*>& ins, const std::vector<double>& ci,
+               DevBuf<ttnd::RunArgs>& d_args, DevBuf<ttnd::DInst>& d_insts) {
+    const size_t n = ins.size();
+    std::vector<ttnd::RunArgs> args(n);
+    for (size_t k = 0; k < n; ++k) make_args(ins[k], 0.0, ci[k], 1, args[k]);
+    if (d_args.n < n) d_args.alloc(n);
+    if (d_insts.n < n) d_insts.alloc(n);
+    hipStream_t s = ctx->stream;
+    hip_check(hipMemcpyAsync(d_args.p, args.data(), n * sizeof(ttnd::RunArgs), hipMemcpyHostToDevice, s), "args");
+    std::vector<ttnd::DInst> insts;
+    for (size_t k = 0; k < n; ++k) {
+        prepare_traj(ins[k], 1);
+        ttnd::DInst I = ins[k]->inst;
+        I.own = d_args.p + k;
+        insts.push_back(I);
+    }
+    hip_check(hipMemcpyAsync(d_insts.p, insts.data(), n * sizeof(ttnd::DInst), hipMemcpyHostToDevice, s), "insts");
+    ttn_batch b;
+    b.ctx = ctx;
+    b.integs = ins;
+    for (ttn_integ* in : ins) b.cost.push_back(static_cost(in));
+    launch(ctx, d_insts.p, insts, args[0], batch_teams(&b));
+    std::string err;
+    for (size_t k = 0; k < n; ++k) {
+        account(ins[k], args[k]);
+        try {
+            finish(ins[k], 1);
+        } catch (const Error& e) {
+            if (err.empty()) err = e.what();
+        }
+    }
+    if (!err.empty()) fail(ERR_NUMERIC, "adjoint: " + err);
+}
+
+// Gauss-Legendre nodes and weights on [0, 1] (Newton's method on P_n)
+void gauss_legendre01(int q, std::vector<double>& x, std::vector<double>& w) {
+    x.assign(q, 0.0);
+    w.assign(q, 0.0);
+    const double pi = std::acos(-1.0);
+    for (int i = 0; i < q; ++i) {
+        double z = std::cos(pi * (i + 0.75) / (q + 0.5)), dp = 1.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = z;
+            for (int k = 2; k <= q; ++k) {
+                const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = q * (z * p1 - p0) / (z * z - 1.0);
+            const double dz = p1 / dp;
+            z -= dz;
+            if (std::abs(dz) < 1e-16) break;
+        }
+        x[i] = 0.5 * (1.0 - z);
+        w[i] = 1.0 / ((1.0 - z * z) * dp * dp);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ttn_sb_adjoint_run(ttn_integ* in, const ttn_sb_params* p, int64_t nsteps, int64_t every, int64_t nquad,
+                       int64_t mode, const double* op, double* value, double* dg, double* dw) {
+    return guard(in->ctx, [&] {
+        const Topology& tp = in->topo;
+        check(!in->expand, ERR_INVALID, "adjoint: the fixed-step integrator only");
+        check(in->nsteps == 0, ERR_INVALID, "adjoint: the state must not have been stepped");
+        check(nsteps >= 1 && every >= 1 && nquad >= 0 && nquad <= 16, ERR_INVALID,
+              "adjoint: nsteps and the checkpoint interval must be positive, 0-16 quadrature nodes");
+        check(mode >= 0 && mode < tp.nmodes(), ERR_INVALID, "adjoint: mode out of range");
+        const SBParams sp = sb_from_c(p);
+        check(static_cast<int64_t>(sb_blocks(sp).size()) == tp.nmodes() - 1, ERR_INVALID,
+              "adjoint: the model's leaves do not match the tree");
+        hip_check(hipSetDevice(in->ctx->device), "hipSetDevice");
+        hipStream_t st = in->ctx->stream;
+        const double dt = in->p.dt;
+        const int Q = nquad > 0 ? static_cast<int>(nquad) : sb_quadrature_nodes(sp, dt);
+        // forward with checkpoints
+        std::vector<std::unique_ptr<ArenaSnap>> ck;
+        for (int64_t s = 0; s < nsteps; ++s) {
+            if (s % every == 0) {
+                ck.push_back(std::make_unique<ArenaSnap>());
+                snap_take(in, *ck.back());
+            }
+            step_one(in, -1.0);
+        }
+        ArenaSnap fin;
+        snap_take(in, fin);
+        // J and chi(T) = O_s psi(T)
+        const int64_t dm = tp.mode_dims[mode];
+        std::vector<cplx> O(dm * dm), Os(dm * dm);
+        for (int64_t i = 0; i < dm * dm; ++i) O[i] = cget(op, i);
+        for (int64_t a = 0; a < dm; ++a)
+            for (int64_t b = 0; b < dm; ++b) Os[a * dm + b] = 0.5 * (O[a * dm + b] + std::conj(O[b * dm + a]));
+        hip_check(hipMemcpy(in->arena.p + in->tmp_op, O.data(), dm * dm * sizeof(double2), hipMemcpyHostToDevice), "op");
+        double e3[3];
+        run_expect(in, tp.leaf_node[mode], in->tmp_op, e3);
+        auto chi = twin(in);
+        copy_state(chi.get(), in);
+        {
+            const int ln = tp.leaf_node[mode];
+            const int64_t r = tp.nodes[ln].hrank;
+            std::vector<cplx> A(dm * r), B(dm * r);
+            hip_check(hipStreamSynchronize(st), "sync");
+            hip_check(hipMemcpy(A.data(), chi->arena.p + chi->node_A[ln], A.size() * sizeof(double2),
+                                hipMemcpyDeviceToHost),
+                      "leaf");
+            for (int64_t a = 0; a < dm; ++a)
+                for (int64_t c = 0; c < r; ++c) {
+                    cplx s(0.0, 0.0);
+                    for (int64_t b = 0; b < dm; ++b) s += Os[a * dm + b] * A[b * r + c];
+                    B[a * r + c] = s;
+                }
+            hip_check(hipMemcpy(chi->arena.p + chi->node_A[ln], B.data(), B.size() * sizeof(double2),
+                                hipMemcpyHostToDevice),
+                      "leaf");
+        }
+        chi->nsteps = 0;
+        chi->p.dt = dt;
+        // the 2Q instances of one step's quadrature: psi at t_n + x_q dt, chi at the same times
+        std::vector<std::unique_ptr<ttn_integ>> W;
+        std::vector<ttn_integ*> wp;
+        for (int q = 0; q < 2 * Q; ++q) {
+            W.push_back(twin(in));
+            wp.push_back(W.back().get());
+        }
+        std::vector<double> xs, ws;
+        gauss_legendre01(Q, xs, ws);
+        std::vector<double> ci(2 * Q);
+        for (int q = 0; q < Q; ++q) {
+            ci[q] = -1.0;
+            ci[Q + q] = 1.0;
+        }
+        // transition densities: pairs (W[q], W[Q + q]), S = Scoup on the system leaf
+        const int64_t d0 = tp.mode_dims[0];
+        check(static_cast<int64_t>(sp.scoup.size()) == d0 * d0, ERR_INVALID, "adjoint: Scoup has the wrong size");
+        DevBuf<double2> dS;
+        dS.alloc(d0 * d0);
+        hip_check(hipMemcpy(dS.p, sp.scoup.data(), d0 * d0 * sizeof(double2), hipMemcpyHostToDevice), "Scoup");
+        int64_t rr = 1;
+        for (const Node& nd : tp.nodes) rr = std::max<int64_t>(rr, nd.hrank * nd.hrank);
+        const int64_t work_stride = 4 * static_cast<int64_t>(tp.size()) * rr + 3 * in->inst.maxsize;
+        DevBuf<double2> dwork;
+        dwork.alloc(work_stride * Q);
+        std::vector<int64_t> off(tp.size(), -1), moff(tp.nmodes());
+        int64_t out_stride = 0;
+        for (int m = 0; m < tp.nmodes(); ++m) {
+            moff[m] = out_stride;
+            off[tp.leaf_node[m]] = out_stride;
+            out_stride += 2 * tp.mode_dims[m] * tp.mode_dims[m];
+        }
+        DevBuf<int64_t> doff;
+        doff.alloc(tp.size());
+        hip_check(hipMemcpy(doff.p, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice), "offsets");
+        DevBuf<double2> dout;
+        dout.alloc(out_stride * Q);
+        std::vector<cplx> hout(out_stride * Q);
+        DevBuf<ttnd::RunArgs> d_args;
+        DevBuf<ttnd::DInst> d_insts, d_pairs;
+        d_pairs.alloc(2 * Q);
+        std::vector<double> g(sp.wk.size(), 0.0), w(sp.wk.size(), 0.0);
+        std::vector<std::vector<cplx>> rho(tp.nmodes()), rhoS(tp.nmodes());
+        ArenaSnap c1;
+        std::vector<std::unique_ptr<ArenaSnap>> starts;
+        for (int64_t gi = static_cast<int64_t>(ck.size()) - 1; gi >= 0; --gi) {
+            snap_put(in, *ck[gi]);
+            const int64_t s0 = gi * every, s1 = std::min(nsteps, s0 + every);
+            for (int64_t s = s0; s < s1; ++s) {
+                const size_t k = static_cast<size_t>(s - s0);
+                if (starts.size() <= k) starts.push_back(std::make_unique<ArenaSnap>());
+                snap_take(in, *starts[k]);
+                if (s + 1 < s1) step_one(in, -1.0);
+            }
+            for (int64_t s = s1 - 1; s >= s0; --s) {
+                snap_take(chi.get(), c1);  // chi(t_{s+1})
+                for (int q = 0; q < Q; ++q) {
+                    snap_put(W[q].get(), *starts[s - s0]);
+                    W[q]->p.dt = xs[q] * dt;
+                    snap_put(W[Q + q].get(), c1);
+                    W[Q + q]->p.dt = (1.0 - xs[q]) * dt;
+                }
+                step_each(in->ctx, wp, ci, d_args, d_insts);
+                std::vector<ttnd::DInst> pr;
+                for (int q = 0; q < 2 * Q; ++q) pr.push_back(W[q]->inst);
+                hip_check(hipMemcpyAsync(d_pairs.p, pr.data(), pr.size() * sizeof(ttnd::DInst), hipMemcpyHostToDevice,
+                                         st),
+                          "pairs");
+                hip_check(ttnd_launch_densities(d_pairs.p, d_pairs.p + Q, Q, tp.leaf_node[0], dS.p, dwork.p,
+                                                work_stride, rr, doff.p, dout.p, out_stride, st),
+                          "launch densities_kernel");
+                hip_check(hipMemcpyAsync(hout.data(), dout.p, hout.size() * sizeof(double2), hipMemcpyDeviceToHost, st),
+                          "densities");
+                hip_check(hipStreamSynchronize(st), "sync");
+                for (int q = 0; q < Q; ++q) {
+                    for (int m = 0; m < tp.nmodes(); ++m) {
+                        const int64_t dd = tp.mode_dims[m] * tp.mode_dims[m];
+                        const cplx* base = hout.data() + q * out_stride + moff[m];
+                        rho[m].assign(base, base + dd);
+                        rhoS[m].assign(base + dd, base + 2 * dd);
+                    }
+                    sb_density_gradient(sp, rho, rhoS, ws[q] * dt, g, w);
+                }
+                snap_put(chi.get(), c1);
+                chi->p.dt = dt;
+                step_one(chi.get(), 1.0);
+            }
+        }
+        snap_put(in, fin);
+        hip_check(hipStreamSynchronize(st), "sync");
+        std::copy(g.begin(), g.end(), dg);
+        std::copy(w.begin(), w.end(), dw);
+        if (value) *value = e3[0];
     });
 }
 

@@ -100,6 +100,8 @@ constexpr int EXP_LOG_REC = 4 + MAXK;
 
 constexpr int MAX_OBS = 8;  // observables recorded on the device after every step
 
+struct RunArgs;
+
 struct DInst {
     double2* base;                  // arena
     const DNode* nodes;
@@ -141,6 +143,9 @@ struct DInst {
     int nt2, pad2;                 // largest number of two-site terms of a node
     double* exp_log;               // expansion log (EXP_LOG_REC doubles per two-site attempt) or null
     int64_t exp_log_cap;           // records it holds
+    // this instance's own call arguments (coefficient, dt, prepare and ladder), or null for the
+    // launch's: lets one launch advance instances by different lengths and directions in time
+    const RunArgs* own;
 };
 
 // ---------------------------------------------------------------- teams

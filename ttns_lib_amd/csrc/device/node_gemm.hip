@@ -3,9 +3,10 @@
 // Hamiltonian, build mean fields and run the Krylov propagator of a single node tensor.
 //
 // zgemm_kernel: C[b] (M x N) = alpha op(A[b]) (M x K) B[b] (K x N) (+ C), b = b1 * nb0 + b0.
-//   * 64 x 64 output tile per workgroup (256 threads, 4 wave64, each wave a 32 x 32 block of
-//     four 16 x 16 MFMA tiles), K in chunks of 16 staged through LDS with re and im planes
-//     kept apart so every MFMA operand is one LDS read;
+//   * one output tile per workgroup (256 threads, 4 wave64 in a 2 x 2 grid, each wave a
+//     WM x WN block of 16 x 16 MFMA tiles: 32 x 32 up to 128 x 64 and 64 x 128, the shape
+//     chosen per GEMM by the node path's autotuner), K in chunks of 16 staged through LDS
+//     with re and im planes kept apart so every MFMA operand is one LDS read;
 //   * the next chunk's global loads are issued before the current chunk's MFMAs;
 //   * K may be blocked: k = kblk * kb + kin, with a separate stride per block for A and B, so
 //     a sum of products over terms (each term a contiguous K block) is one GEMM;
@@ -25,8 +26,7 @@ namespace ttnd {
 namespace {
 
 constexpr int GT = 256;     // threads
-constexpr int BM = 64, BN = 64, BK = 16;
-constexpr int LDP = 80;     // padded LDS row (doubles): rows kq and kq+1 fall 32 banks apart
+constexpr int BK = 16;
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
@@ -41,14 +41,22 @@ __device__ __forceinline__ void mfma_c(Acc& acc, double ar, double ai, double br
     acc.im = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, acc.im, 0, 0, 0);
 }
 
-struct Panels {
-    double ar[BK][LDP], ai[BK][LDP];  // A(row, k) at [k][row]
-    double br[BK][LDP], bi[BK][LDP];  // B(k, col) at [k][col]
+// LDS row length (doubles) of a panel of width W: rows kq and kq + 1 fall 32 banks apart
+template <int W>
+struct Pad {
+    static constexpr int v = W + 16;
 };
 
-// Global -> register staging of one K chunk: four A and four B elements per thread.
+template <int BM, int BN>
+struct Panels {
+    double ar[BK][Pad<BM>::v], ai[BK][Pad<BM>::v];  // A(row, k) at [k][row]
+    double br[BK][Pad<BN>::v], bi[BK][Pad<BN>::v];  // B(k, col) at [k][col]
+};
+
+// Global -> register staging of one K chunk: BM / 16 A and BN / 16 B elements per thread.
+template <int BM, int BN>
 struct Stage {
-    double2 a[4], b[4];
+    double2 a[BM / 16], b[BN / 16];
 };
 
 __device__ __forceinline__ int64_t a_off(const GemmDesc& g, int row, int k) {
@@ -60,53 +68,66 @@ __device__ __forceinline__ int64_t b_off(const GemmDesc& g, int k, int col) {
     return static_cast<int64_t>(ki) * g.b_k + static_cast<int64_t>(col) * g.b_c + static_cast<int64_t>(kb) * g.b_kblk;
 }
 
-// element e (0..3) of this thread in the A panel: (row, k) within the tile
+// element e (0 .. BM/16 - 1) of this thread in the A panel: (row, k) within the tile; with
+// k fastest in memory 16 consecutive threads take one row's 16 k, otherwise BM consecutive
+// threads take one k's rows
+template <int BM>
 __device__ __forceinline__ void a_slot(bool kfast, int t, int e, int& row, int& k) {
     if (kfast) {
         k = t & 15;
         row = (t >> 4) + 16 * e;
     } else {
-        row = t & 63;
-        k = (t >> 6) + 4 * e;
+        row = t % BM;
+        k = t / BM + (GT / BM) * e;
     }
 }
+template <int BN>
 __device__ __forceinline__ void b_slot(bool cfast, int t, int e, int& k, int& col) {
     if (cfast) {
-        col = t & 63;
-        k = (t >> 6) + 4 * e;
+        col = t % BN;
+        k = t / BN + (GT / BN) * e;
     } else {
         k = t & 15;
         col = (t >> 4) + 16 * e;
     }
 }
 
+template <int BM, int BN>
 __device__ __forceinline__ void load_stage(const GemmDesc& g, const double2* A, const double2* B, int row0, int col0,
-                                           int k0, int kend, bool kfast, bool cfast, Stage& s) {
+                                           int k0, int kend, bool kfast, bool cfast, Stage<BM, BN>& s) {
     const int t = threadIdx.x;
     const double2 z = make_double2(0.0, 0.0);
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
+    for (int e = 0; e < BM / 16; ++e) {
         int row, k;
-        a_slot(kfast, t, e, row, k);
+        a_slot<BM>(kfast, t, e, row, k);
         const int gr = row0 + row, gk = k0 + k;
         s.a[e] = (gr < g.M && gk < kend) ? A[a_off(g, gr, gk)] : z;
+    }
+#pragma unroll
+    for (int e = 0; e < BN / 16; ++e) {
         int kk, col;
-        b_slot(cfast, t, e, kk, col);
+        b_slot<BN>(cfast, t, e, kk, col);
         const int gk2 = k0 + kk, gc = col0 + col;
         s.b[e] = (gc < g.N && gk2 < kend) ? B[b_off(g, gk2, gc)] : z;
     }
 }
 
-__device__ __forceinline__ void store_stage(Panels& p, bool kfast, bool cfast, bool conja, const Stage& s) {
+template <int BM, int BN>
+__device__ __forceinline__ void store_stage(Panels<BM, BN>& p, bool kfast, bool cfast, bool conja,
+                                            const Stage<BM, BN>& s) {
     const int t = threadIdx.x;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
+    for (int e = 0; e < BM / 16; ++e) {
         int row, k;
-        a_slot(kfast, t, e, row, k);
+        a_slot<BM>(kfast, t, e, row, k);
         p.ar[k][row] = s.a[e].x;
         p.ai[k][row] = conja ? -s.a[e].y : s.a[e].y;
+    }
+#pragma unroll
+    for (int e = 0; e < BN / 16; ++e) {
         int kk, col;
-        b_slot(cfast, t, e, kk, col);
+        b_slot<BN>(cfast, t, e, kk, col);
         p.br[kk][col] = s.b[e].x;
         p.bi[kk][col] = s.b[e].y;
     }
@@ -114,8 +135,13 @@ __device__ __forceinline__ void store_stage(Panels& p, bool kfast, bool cfast, b
 
 }  // namespace
 
+// One (2 WM 16) x (2 WN 16) output tile per workgroup: four waves in a 2 x 2 grid, each a
+// WM x WN block of 16 x 16 MFMA tiles.  Every output element sums its K range in the same
+// order for every WM, WN, so all tilings give the same bits.
+template <int WM, int WN>
 __global__ __launch_bounds__(GT) void zgemm_kernel(GemmDesc g) {
-    __shared__ Panels P;
+    constexpr int BM = 32 * WM, BN = 32 * WN;
+    __shared__ Panels<BM, BN> P;
     const int tiles_n = (g.N + BN - 1) / BN;
     const int tile = blockIdx.x;
     const int row0 = (tile / tiles_n) * BM, col0 = (tile % tiles_n) * BN;
@@ -134,35 +160,38 @@ __global__ __launch_bounds__(GT) void zgemm_kernel(GemmDesc g) {
     const int wr = wv >> 1, wc = wv & 1;
     const int li = lane & 15, kq = lane >> 4;
 
-    Acc acc[2][2];
+    Acc acc[WM][WN];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < WM; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < WN; ++j) {
             acc[i][j].re = d4{0.0, 0.0, 0.0, 0.0};
             acc[i][j].im = d4{0.0, 0.0, 0.0, 0.0};
         }
-    Stage st;
-    if (c0 < c1) load_stage(g, A, B, row0, col0, c0 * BK, g.K, kfast, cfast, st);
+    Stage<BM, BN> st;
+    if (c0 < c1) load_stage<BM, BN>(g, A, B, row0, col0, c0 * BK, g.K, kfast, cfast, st);
     for (int c = c0; c < c1; ++c) {
         __syncthreads();
-        store_stage(P, kfast, cfast, g.conja != 0, st);
+        store_stage<BM, BN>(P, kfast, cfast, g.conja != 0, st);
         __syncthreads();
-        if (c + 1 < c1) load_stage(g, A, B, row0, col0, (c + 1) * BK, g.K, kfast, cfast, st);
+        if (c + 1 < c1) load_stage<BM, BN>(g, A, B, row0, col0, (c + 1) * BK, g.K, kfast, cfast, st);
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 4) {
-            double ar[2], ai[2], br[2], bi[2];
+            double ar[WM], ai[WM], br[WN], bi[WN];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                ar[i] = P.ar[ks + kq][wr * 32 + i * 16 + li];
-                ai[i] = P.ai[ks + kq][wr * 32 + i * 16 + li];
-                br[i] = P.br[ks + kq][wc * 32 + i * 16 + li];
-                bi[i] = P.bi[ks + kq][wc * 32 + i * 16 + li];
+            for (int i = 0; i < WM; ++i) {
+                ar[i] = P.ar[ks + kq][wr * 16 * WM + i * 16 + li];
+                ai[i] = P.ai[ks + kq][wr * 16 * WM + i * 16 + li];
             }
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < WN; ++j) {
+                br[j] = P.br[ks + kq][wc * 16 * WN + j * 16 + li];
+                bi[j] = P.bi[ks + kq][wc * 16 * WN + j * 16 + li];
+            }
 #pragma unroll
-                for (int j = 0; j < 2; ++j) mfma_c(acc[i][j], ar[i], ai[i], br[j], bi[j]);
+            for (int i = 0; i < WM; ++i)
+#pragma unroll
+                for (int j = 0; j < WN; ++j) mfma_c(acc[i][j], ar[i], ai[i], br[j], bi[j]);
         }
     }
     // epilogue: workspace (split K) or C
@@ -172,13 +201,13 @@ __global__ __launch_bounds__(GT) void zgemm_kernel(GemmDesc g) {
     const int64_t cc = nsplit > 1 ? 1 : g.c_c;
     const bool direct = nsplit == 1;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < WM; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < WN; ++j)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int row = row0 + wr * 32 + i * 16 + kq + 4 * q;
-                const int col = col0 + wc * 32 + j * 16 + li;
+                const int row = row0 + wr * 16 * WM + i * 16 + kq + 4 * q;
+                const int col = col0 + wc * 16 * WN + j * 16 + li;
                 if (row < g.M && col < g.N) {
                     double2 v = make_double2(acc[i][j].re[q], acc[i][j].im[q]);
                     double2* p = C + row * cr + col * cc;
@@ -368,11 +397,29 @@ extern "C" hipError_t ttnd_zmft(const double2* coeff, const double2* mf, int64_t
     hipLaunchKernelGGL(zmft_kernel, dim3(blocks), dim3(GT), 0, s, coeff, mf, nt, r, out);
     return hipGetLastError();
 }
-extern "C" hipError_t ttnd_zgemm(const ttnd::GemmDesc* g, int nsplit, hipStream_t s) {
-    using namespace ttnd;
+namespace {
+template <int WM, int WN>
+void launch_tiles(const ttnd::GemmDesc* g, int nsplit, hipStream_t s) {
+    constexpr int BM = 32 * WM, BN = 32 * WN;
     const int tiles = ((g->M + BM - 1) / BM) * ((g->N + BN - 1) / BN);
     const int batch = g->nb0 * g->nb1;
-    hipLaunchKernelGGL(zgemm_kernel, dim3(tiles, batch, nsplit), dim3(GT), 0, s, *g);
+    hipLaunchKernelGGL((ttnd::zgemm_kernel<WM, WN>), dim3(tiles, batch, nsplit), dim3(ttnd::GT), 0, s, *g);
+}
+}  // namespace
+
+// tile: one of ttnd::GEMM_TILES (output tile of the workgroup, see node_gemm.hpp)
+extern "C" hipError_t ttnd_zgemm(const ttnd::GemmDesc* g, int nsplit, int tile, hipStream_t s) {
+    using namespace ttnd;
+    switch (tile) {
+        case TILE_32x32: launch_tiles<1, 1>(g, nsplit, s); break;
+        case TILE_64x32: launch_tiles<2, 1>(g, nsplit, s); break;
+        case TILE_32x64: launch_tiles<1, 2>(g, nsplit, s); break;
+        case TILE_64x64: launch_tiles<2, 2>(g, nsplit, s); break;
+        case TILE_128x64: launch_tiles<4, 2>(g, nsplit, s); break;
+        case TILE_64x128: launch_tiles<2, 4>(g, nsplit, s); break;
+        default: return hipErrorInvalidValue;
+    }
+    const int batch = g->nb0 * g->nb1;
     if (nsplit > 1) {
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;

@@ -17,6 +17,9 @@ namespace ttnd {
 
 constexpr int MAXV = 32;  // Krylov vectors handled by the node-level Gram-Schmidt kernels
 
+// output tiles of zgemm_kernel (rows x columns per workgroup)
+enum GemmTile { TILE_32x32, TILE_64x32, TILE_32x64, TILE_64x64, TILE_128x64, TILE_64x128, GEMM_TILES };
+
 struct GemmDesc {
     int M, N, K;
     int nb0, nb1;

@@ -1302,7 +1302,7 @@ __device__ __forceinline__ int xcd_id() {
 // plan == null: workgroup b runs instance b alone.  Otherwise every workgroup takes the
 // next slot of the XCD it runs on (xcd_fill: one counter per 128-byte line) and joins
 // the team that slot names; words[i] are the team words of instance i.
-__global__ __launch_bounds__(NT) void tdvp_run_kernel(const DInst* insts, RunArgs a, const TeamPlan* plan,
+__global__ __launch_bounds__(NT) void tdvp_run_kernel(const DInst* insts, RunArgs a_in, const TeamPlan* plan,
                                                       TeamWords* words, unsigned* xcd_fill) {
     // visit count of every node in the current sweep (at most 1 + MAXC)
     __shared__ unsigned char visits[MAX_NODES];
@@ -1335,6 +1335,12 @@ __global__ __launch_bounds__(NT) void tdvp_run_kernel(const DInst* insts, RunArg
     const int inst = slot_info[0], rank = slot_info[1], size = slot_info[2];
     if (inst < 0) return;
     const DInst& I = insts[inst];
+    RunArgs a = a_in;
+    if (I.own != nullptr) {
+        a = *I.own;
+        a.team_min = a_in.team_min;
+        a.small_max = a_in.small_max;
+    }
     Team T;
     T.w = size > 1 ? &words[inst] : nullptr;
     T.rank = rank;
@@ -1434,6 +1440,162 @@ __global__ __launch_bounds__(NT) void expect_kernel(const DInst* inst, int leaf,
 // ---------------------------------------------------------------- host launchers
 // Team launches start `grid` workgroups (one per CU); plan == null runs n instances one
 // workgroup each.
+namespace ttnd {
+
+// Leaf transition densities of pairs of instances of one tree, one workgroup per pair (the
+// costate gradient's matrix elements):  rho_m(j, l) = <bra|(|j><l|)_m|ket> for every leaf
+// and rhoS_m, the same with S (the ket's leaf node `ins` operator, d x d) inserted, by full
+// contraction of the two trees: bottom-up transfer matrices T = Abra^H (children) Aket (TS
+// along the insertion leaf's path), then top-down environments E (ES: S outside the
+// subtree), then rho = Abra^* E Aket^T on every leaf.  No orthonormality is assumed.
+// work: per pair 4 * nnodes * rr + 3 * maxsize entries (rr >= largest hrank^2); out: per
+// pair, rho then rhoS (d x d each) at out_off[node] of every leaf node.
+__global__ __launch_bounds__(NT) void densities_kernel(const DInst* kets, const DInst* bras, int ins, const double2* S,
+                                                       double2* work, int64_t work_stride, int64_t rr,
+                                                       const int64_t* out_off, double2* out, int64_t out_stride) {
+    __shared__ unsigned char onpath[MAX_NODES], esdef[MAX_NODES];
+    const int pair = blockIdx.x;
+    const DInst& K = kets[pair];
+    const DInst& B = bras[pair];
+    if (threadIdx.x == 0) g_sh.fail = 0;
+    Team T;
+    T.w = nullptr;
+    T.rank = 0;
+    T.size = 1;
+    T.gen = 0;
+    T.pbuf = 0;
+    T.part = reinterpret_cast<double*>(K.base + K.team_part);
+    T.tiles = K.base + K.team_tiles;
+    Ctx c(K, &T, 0);
+    const int n = K.nnodes;
+    for (int e = threadIdx.x; e < n; e += NT) {
+        g_hr[e] = static_cast<unsigned char>(K.hr[e]);
+        onpath[e] = 0;
+        esdef[e] = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = ins; q >= 0; q = K.nodes[q].parent) onpath[q] = 1;
+        // ES exists below the insertion path (pre-order: parents first)
+        for (int i = 1; i < n; ++i) {
+            const int p = K.nodes[i].parent;
+            esdef[i] = !onpath[i] && (onpath[p] || esdef[p]);
+        }
+    }
+    __syncthreads();
+    const double2 one = cmk(1.0, 0.0);
+    double2* wk = work + static_cast<int64_t>(pair) * work_stride;
+    double2* y0 = wk + 4 * static_cast<int64_t>(n) * rr;
+    double2* y1 = y0 + K.maxsize;
+    auto TM = [&](int i) { return wk + static_cast<int64_t>(i) * rr; };
+    auto TSM = [&](int i) { return wk + static_cast<int64_t>(n + i) * rr; };
+    auto EM = [&](int i) { return wk + static_cast<int64_t>(2 * n + i) * rr; };
+    auto ESM = [&](int i) { return wk + static_cast<int64_t>(3 * n + i) * rr; };
+    // bottom-up transfer matrices
+    for (int i = n - 1; i >= 0; --i) {
+        const DNode& nd = K.nodes[i];
+        const Shape sh = c.shape(i);
+        const double2* Ak = K.base + nd.A;
+        const double2* Ab = B.base + nd.A;
+        if (nd.leaf >= 0) {
+            t_adjoint_mul(T, Ab, Ak, sh.dimen, sh.hrank, TM(i), one, false);
+            if (onpath[i]) {
+                const int d = static_cast<int>(sh.dimen), r = sh.hrank;
+                for (int64_t e = threadIdx.x; e < sh.size; e += NT) {
+                    const int a = static_cast<int>(e / r), col = static_cast<int>(e % r);
+                    double2 acc = cz();
+                    for (int b = 0; b < d; ++b) acc = cadd(acc, cmul(S[a * d + b], Ak[b * r + col]));
+                    y0[e] = acc;
+                }
+                T.sync();
+                t_adjoint_mul(T, Ab, y0, sh.dimen, sh.hrank, TSM(i), one, false);
+            }
+            continue;
+        }
+        for (int pass = 0; pass < (onpath[i] ? 2 : 1); ++pass) {
+            const double2* src = Ak;
+            for (int k = 0; k < nd.nmodes; ++k) {
+                const int ch = nd.children[k];
+                int64_t L, R;
+                mode_view(sh, k, L, R);
+                double2* dst = (k % 2 == 0) ? y0 : y1;
+                t_mode_contract(T, (pass == 1 && onpath[ch]) ? TSM(ch) : TM(ch), L, sh.dims[k], R, src, dst, one,
+                                false);
+                src = dst;
+            }
+            t_adjoint_mul(T, Ab, src, sh.dimen, sh.hrank, pass == 1 ? TSM(i) : TM(i), one, false);
+        }
+    }
+    // top-down environments; the root's is the identity on its (dummy) upper index
+    {
+        const int r0 = c.hr(0);
+        for (int e = threadIdx.x; e < r0 * r0; e += NT) EM(0)[e] = (e / r0 == e % r0) ? one : cz();
+        T.sync();
+    }
+    for (int p = 0; p < n; ++p) {
+        const DNode& nd = K.nodes[p];
+        if (nd.leaf >= 0) continue;
+        const Shape sh = c.shape(p);
+        const double2* Ak = K.base + nd.A;
+        const double2* Ab = B.base + nd.A;
+        for (int k = 0; k < nd.nmodes; ++k) {
+            const int ch = nd.children[k];
+            for (int which = 0; which < 2; ++which) {
+                if (which == 1 && !esdef[ch]) continue;
+                const double2* Ep = (which == 1 && !onpath[p]) ? ESM(p) : EM(p);
+                const double2* src = Ak;
+                int flip = 0;
+                for (int s2 = 0; s2 < nd.nmodes; ++s2) {
+                    if (s2 == k) continue;
+                    const int sc = nd.children[s2];
+                    int64_t L, R;
+                    mode_view(sh, s2, L, R);
+                    double2* dst = flip ? y1 : y0;
+                    flip ^= 1;
+                    t_mode_contract(T, (which == 1 && onpath[p] && onpath[sc]) ? TSM(sc) : TM(sc), L, sh.dims[s2], R,
+                                    src, dst, one, false);
+                    src = dst;
+                }
+                double2* ye = (src == y0) ? y1 : y0;
+                t_mul_transposed(T, src, sh.dimen, sh.hrank, Ep, ye, one, false);
+                int64_t L, R;
+                mode_view(sh, k, L, R);
+                t_outer_contract(T, Ab, ye, L, sh.dims[k], R, which == 1 ? ESM(ch) : EM(ch), one, false);
+            }
+        }
+    }
+    // leaf densities
+    double2* o = out + static_cast<int64_t>(pair) * out_stride;
+    for (int i = 0; i < n; ++i) {
+        const DNode& nd = K.nodes[i];
+        if (nd.leaf < 0) continue;
+        const Shape sh = c.shape(i);
+        const int d = static_cast<int>(sh.dimen);
+        const double2* Ak = K.base + nd.A;
+        const double2* Ab = B.base + nd.A;
+        double2* dst = o + out_off[i];
+        t_mul_transposed(T, Ak, sh.dimen, sh.hrank, EM(i), y0, one, false);
+        t_outer_contract(T, Ab, y0, 1, d, sh.hrank, dst, one, false);
+        if (esdef[i]) {
+            t_mul_transposed(T, Ak, sh.dimen, sh.hrank, ESM(i), y0, one, false);
+            t_outer_contract(T, Ab, y0, 1, d, sh.hrank, dst + d * d, one, false);
+        } else {
+            for (int e = threadIdx.x; e < d * d; e += NT) dst[d * d + e] = cz();
+            T.sync();
+        }
+    }
+}
+
+}  // namespace ttnd
+
+extern "C" hipError_t ttnd_launch_densities(const ttnd::DInst* kets, const ttnd::DInst* bras, int npairs, int ins,
+                                            const double2* S, double2* work, int64_t work_stride, int64_t rr,
+                                            const int64_t* out_off, double2* out, int64_t out_stride, hipStream_t s) {
+    hipLaunchKernelGGL(ttnd::densities_kernel, dim3(npairs), dim3(ttnd::NT), 0, s, kets, bras, ins, S, work,
+                       work_stride, rr, out_off, out, out_stride);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t ttnd_launch_run(const ttnd::DInst* d_insts, int n, const ttnd::RunArgs* a,
                                       const ttnd::TeamPlan* d_plan, ttnd::TeamWords* d_words, unsigned* d_fill,
                                       int grid, hipStream_t s) {

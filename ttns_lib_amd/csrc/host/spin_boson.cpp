@@ -1,5 +1,6 @@
 #include "spin_boson.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <random>
 
@@ -144,6 +145,58 @@ SBProblem sb_build(const SBParams& p) {
     prob.sop = std::move(H);
     prob.state = sb_initial_state(prob.topo, p.psi0, p.seed);
     return prob;
+}
+
+int sb_quadrature_nodes(const SBParams& p, double dt) {
+    double wmax = 0.0, hs = 0.0;
+    for (double w : p.wk) wmax = std::max(wmax, std::abs(w));
+    for (const cplx& h : p.hsys) hs += std::norm(h);
+    const double half = 0.75 * std::abs(dt) * (wmax + 2.0 * std::sqrt(hs));
+    for (int n = 4; n < 16; ++n) {
+        // log of (phi/2)^(2n) / (2n)!
+        const double lg = 2.0 * n * std::log(std::max(half, 1e-300)) - std::lgamma(2.0 * n + 1.0);
+        if (lg < std::log(1e-13)) return n;
+    }
+    return 16;
+}
+
+void sb_density_gradient(const SBParams& p, const std::vector<std::vector<cplx>>& rho,
+                         const std::vector<std::vector<cplx>>& rhoS, double weight, std::vector<double>& dg,
+                         std::vector<double>& dw) {
+    const std::vector<ModeBlock> blocks = sb_blocks(p);
+    check(rho.size() == blocks.size() + 1 && rhoS.size() == blocks.size() + 1, ERR_INVALID,
+          "sb gradient: one density per leaf mode expected");
+    if (dg.size() != p.wk.size()) dg.assign(p.wk.size(), 0.0);
+    if (dw.size() != p.wk.size()) dw.assign(p.wk.size(), 0.0);
+    size_t first = 0;  // global index of the block's first mode
+    for (size_t b = 0; b < blocks.size(); ++b) {
+        const ModeBlock& B = blocks[b];
+        const i64 d = B.nstates;
+        const std::vector<cplx>& r1 = rho[b + 1];
+        const std::vector<cplx>& rS = rhoS[b + 1];
+        check(static_cast<i64>(r1.size()) == d * d && static_cast<i64>(rS.size()) == d * d, ERR_INVALID,
+              "sb gradient: density of the wrong size");
+        const size_t N = B.w.size();
+        // H0[s] = sum_k w_k occ(s, k): <chi|dH0/dw_k|psi> = sum_s occ(s, k) rho(s, s)
+        for (i64 s = 0; s < d; ++s)
+            for (size_t k = 0; k < N; ++k)
+                dw[first + k] += weight * 2.0 * r1[s * d + s].imag() * static_cast<double>(B.occ(s, k));
+        // Hc in the order block_hc writes it: lowering in mode order, then raising in reverse
+        for (i64 s = 0; s < d; ++s) {
+            for (size_t k = 0; k < N; ++k) {
+                const i64 n = B.occ(s, k);
+                if (n > 0)
+                    dg[first + k] += weight * 2.0 * rS[s * d + (s - B.stride[k])].imag() * std::sqrt(static_cast<double>(n));
+            }
+            for (size_t k = N; k-- > 0;) {
+                const i64 n = B.occ(s, k);
+                if (n + 1 < B.nper[k])
+                    dg[first + k] += weight * 2.0 * rS[s * d + (s + B.stride[k])].imag() *
+                                     std::sqrt(static_cast<double>(n) + 1.0);
+            }
+        }
+        first += N;
+    }
 }
 
 }  // namespace ttnh

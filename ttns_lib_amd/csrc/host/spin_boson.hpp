@@ -56,4 +56,20 @@ SBProblem sb_build(const SBParams& p);
 std::vector<std::vector<cplx>> sb_initial_state(const Topology& t, const std::vector<cplx>& psi0,
                                                 uint64_t seed);
 
+// Adds weight * 2 Im <chi|dH/dtheta|psi> to dg (theta = g_k) and dw (theta = w_k) from the
+// leaves' transition densities rho_b(j, l) = <chi|(|j><l|)_b|psi> (rho) and the same with the
+// coupling operator Scoup inserted on the system leaf (rhoS), both indexed by leaf mode (1 + b;
+// d_b x d_b row-major).  dH/dw_k is the occupation of mode k in its block's H0, dH/dg_k is
+// Scoup times the lowering + raising part of mode k in its block's Hc (sqrt(n), sqrt(n + 1);
+// src/mode_data.hpp:239-253, 309-361); the Fock cutoffs are held fixed.
+// Gauss-Legendre nodes per step for the costate gradient's step integrals: the integrand
+// <chi|dH|psi> oscillates at most at phi = 1.5 dt (max |w_k| + 2 |Hsys|_F) per step, and the
+// n-point rule's error on exp(i phi s) is about (phi / 2)^(2n) / (2n)!; the smallest n in
+// [4, 16] that brings this below 1e-13.
+int sb_quadrature_nodes(const SBParams& p, double dt);
+
+void sb_density_gradient(const SBParams& p, const std::vector<std::vector<cplx>>& rho,
+                         const std::vector<std::vector<cplx>>& rhoS, double weight, std::vector<double>& dg,
+                         std::vector<double>& dw);
+
 }  // namespace ttnh

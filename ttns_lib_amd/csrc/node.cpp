@@ -21,7 +21,7 @@
 #include "host/smallexp.hpp"
 #include "runtime.hpp"
 
-extern "C" hipError_t ttnd_zgemm(const ttnd::GemmDesc* g, int nsplit, hipStream_t s);
+extern "C" hipError_t ttnd_zgemm(const ttnd::GemmDesc* g, int nsplit, int tile, hipStream_t s);
 extern "C" hipError_t ttnd_zdots(const double2* V, int64_t ld, int m, const double2* w, int64_t n, double* partial,
                                  int nblk, double* h, hipStream_t s);
 extern "C" hipError_t ttnd_zorth(const double2* V, int64_t ld, int m, const double* h, double2* w, int64_t n,
@@ -61,15 +61,19 @@ struct ttn_node {
 
 namespace {
 
-constexpr int TILE = 64;     // GEMM output tile (node_gemm.hip)
 constexpr int KCHUNK = 16;   // GEMM K chunk
 constexpr int TARGET_WG = 512;
+// output tile (rows, columns) of every ttnd::GemmTile
+constexpr int TILE_DIMS[ttnd::GEMM_TILES][2] = {{32, 32}, {64, 32}, {32, 64}, {64, 64}, {128, 64}, {64, 128}};
 
-int tiles(int64_t M, int64_t N) { return static_cast<int>(((M + TILE - 1) / TILE) * ((N + TILE - 1) / TILE)); }
+int tiles(int64_t M, int64_t N, int tile = ttnd::TILE_64x64) {
+    const int64_t tm = TILE_DIMS[tile][0], tn = TILE_DIMS[tile][1];
+    return static_cast<int>(((M + tm - 1) / tm) * ((N + tn - 1) / tn));
+}
 
 // K split that gives the GEMM about TARGET_WG workgroups
-int split_for(int64_t M, int64_t N, int64_t batch, int64_t K) {
-    const int64_t wg = static_cast<int64_t>(tiles(M, N)) * batch;
+int split_for(int64_t M, int64_t N, int64_t batch, int64_t K, int tile = ttnd::TILE_64x64) {
+    const int64_t wg = static_cast<int64_t>(tiles(M, N, tile)) * batch;
     if (wg >= TARGET_WG / 2) return 1;
     const int64_t want = (TARGET_WG + wg - 1) / wg;
     const int64_t chunks = (K + KCHUNK - 1) / KCHUNK;
@@ -93,27 +97,95 @@ struct NodeView {
     int64_t ws_n;                    // its entries
     bool ws_cap;                     // run unsplit when a split would not fit (else an error)
     hipStream_t s;
+    ttn_ctx* ctx;                    // its tile table; new shapes are timed only when alloc_ok
+    bool alloc_ok;
 };
 
 NodeView view(ttn_node* nd) {
     return NodeView{nd->d0, nd->d1, nd->r, nd->nt, nd->D, nd->size, nd->op0.p, nd->op1.p, nd->mft.p,
-                    nd->Y.p, nd->Z.p, nd->ws.p, static_cast<int64_t>(nd->ws.n), false, nd->ctx->stream};
+                    nd->Y.p, nd->Z.p, nd->ws.p, static_cast<int64_t>(nd->ws.n), false, nd->ctx->stream, nd->ctx,
+                    true};
 }
 
-void run(const NodeView& v, ttnd::GemmDesc g) {
-    if (g.kb <= 0) g.kb = g.K;
-    int ns = split_for(g.M, g.N, static_cast<int64_t>(g.nb0) * g.nb1, g.K);
-    if (ns > 1) {
-        const int64_t need = static_cast<int64_t>(ns) * g.M * g.N * g.nb0 * g.nb1;
-        if (need > v.ws_n && v.ws_cap) ns = 1;
-        check(need <= v.ws_n || ns == 1, ERR_INVALID, "node workspace too small for the K split");
-    }
+int64_t split_need(const ttnd::GemmDesc& g, int ns) { return static_cast<int64_t>(ns) * g.M * g.N * g.nb0 * g.nb1; }
+
+void launch_gemm(const NodeView& v, ttnd::GemmDesc g, int tile, int ns) {
     if (ns > 1) {
         g.W = v.ws;
         g.w_batch = static_cast<int64_t>(g.M) * g.N;
         g.w_split = g.w_batch * g.nb0 * g.nb1;
     }
-    hip_check(ttnd_zgemm(&g, ns, v.s), "zgemm");
+    hip_check(ttnd_zgemm(&g, ns, tile, v.s), "zgemm");
+}
+
+// The autotuner: the first time a context meets a GEMM shape it times every output tile with
+// the K split that gives that tile about TARGET_WG workgroups, twice and half that split and
+// no split (those whose partial products fit the workspace), into a scratch output, and keeps
+// the fastest for the shape.  Every tile sums each output element in the same order, so the
+// choice of tile never changes results; a different K split changes their rounding.
+std::array<int64_t, 8> tune_key(const ttnd::GemmDesc& g) {
+    return {g.M, g.N, g.K, static_cast<int64_t>(g.nb0) * g.nb1, g.kb, g.a_k == 1 ? 1 : 0, g.b_c == 1 ? 1 : 0,
+            g.accumulate};
+}
+
+std::array<double, 3> tune(const NodeView& v, const ttnd::GemmDesc& g) {
+    ttn_ctx* c = v.ctx;
+    int64_t extent = 1 + (g.nb0 - 1) * g.c_b0 + (g.nb1 - 1) * g.c_b1 + (g.M - 1) * g.c_r + (g.N - 1) * g.c_c;
+    if (static_cast<int64_t>(c->tune_out.n) < extent) c->tune_out.alloc(extent);
+    ttnd::GemmDesc t = g;
+    t.C = c->tune_out.p;
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    std::array<double, 3> best = {static_cast<double>(ttnd::TILE_64x64), 1.0, 1e300};
+    const int64_t batch = static_cast<int64_t>(g.nb0) * g.nb1;
+    const int64_t chunks = (g.K + KCHUNK - 1) / KCHUNK;
+    for (int tile = 0; tile < ttnd::GEMM_TILES; ++tile) {
+        const int base = split_for(g.M, g.N, batch, g.K, tile);
+        int splits[4] = {1, base, std::max(1, base / 2), static_cast<int>(std::min<int64_t>(chunks, 2 * base))};
+        for (int si = 0; si < 4; ++si) {
+            const int ns = splits[si];
+            bool seen = false;
+            for (int q = 0; q < si; ++q) seen = seen || splits[q] == ns;
+            if (seen || (ns > 1 && split_need(g, ns) > v.ws_n)) continue;
+            launch_gemm(v, t, tile, ns);  // warm-up
+            float ms = 1e30f;
+            for (int rep = 0; rep < 3; ++rep) {
+                hip_check(hipEventRecord(e0, v.s), "event");
+                launch_gemm(v, t, tile, ns);
+                hip_check(hipEventRecord(e1, v.s), "event");
+                hip_check(hipEventSynchronize(e1), "sync");
+                float x = 0.f;
+                hip_check(hipEventElapsedTime(&x, e0, e1), "elapsed");
+                ms = std::min(ms, x);
+            }
+            if (ms < best[2]) best = {static_cast<double>(tile), static_cast<double>(ns), static_cast<double>(ms)};
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best;
+}
+
+void run(const NodeView& v, ttnd::GemmDesc g) {
+    if (g.kb <= 0) g.kb = g.K;
+    int tile = ttnd::TILE_64x64;
+    int ns = split_for(g.M, g.N, static_cast<int64_t>(g.nb0) * g.nb1, g.K);
+    if (v.ctx && v.ctx->autotune) {
+        const auto key = tune_key(g);
+        auto it = v.ctx->tune.find(key);
+        if (it == v.ctx->tune.end() && v.alloc_ok) it = v.ctx->tune.emplace(key, tune(v, g)).first;
+        if (it != v.ctx->tune.end() && (it->second[1] <= 1.0 || split_need(g, static_cast<int>(it->second[1])) <= v.ws_n)) {
+            tile = static_cast<int>(it->second[0]);
+            ns = static_cast<int>(it->second[1]);
+        }
+    }
+    if (ns > 1) {
+        const int64_t need = split_need(g, ns);
+        if (need > v.ws_n && v.ws_cap) ns = 1;
+        check(need <= v.ws_n || ns == 1, ERR_INVALID, "node workspace too small for the K split");
+    }
+    launch_gemm(v, g, tile, ns);
 }
 void run(ttn_node* nd, const ttnd::GemmDesc& g) { run(view(nd), g); }
 
@@ -559,12 +631,36 @@ int ttn_node_heff_noalloc(ttn_ctx* ctx, int64_t d0, int64_t d1, int64_t r, int64
             NodeView v{d0, d1, r, cnt, d0 * d1, size,
                        reinterpret_cast<const double2*>(op0) + t0 * d0 * d0,
                        reinterpret_cast<const double2*>(op1) + t0 * d1 * d1, mft + t0 * rr,
-                       Y, Z, ws, ws_n, true, s};
+                       Y, Z, ws, ws_n, true, s, ctx, false};
             stage_mode1(v, reinterpret_cast<const double2*>(A));
             stage_mode0(v, v.Y, v.size, v.Z);
             stage_mf_sum(v, reinterpret_cast<double2*>(res), t0 > 0);
         }
     });
+}
+
+int ttn_ctx_set_autotune(ttn_ctx* ctx, int enabled) {
+    return guard(ctx, [&] { ctx->autotune = enabled != 0; });
+}
+
+int ttn_autotune_table(const ttn_ctx* ctx, int64_t* n, int64_t* entries, double* ms) {
+    if (!ctx || !n) return TTN_ERR_INVALID;
+    if (entries || ms) {
+        int64_t i = 0;
+        for (const auto& kv : ctx->tune) {
+            if (i >= *n) break;
+            if (entries) {
+                for (int q = 0; q < 4; ++q) entries[7 * i + q] = kv.first[q];
+                entries[7 * i + 4] = kv.first[4];
+                entries[7 * i + 5] = static_cast<int64_t>(kv.second[0]);
+                entries[7 * i + 6] = static_cast<int64_t>(kv.second[1]);
+            }
+            if (ms) ms[i] = kv.second[2];
+            ++i;
+        }
+    }
+    *n = static_cast<int64_t>(ctx->tune.size());
+    return TTN_OK;
 }
 
 int ttn_node_last(const ttn_node* nd, double* ms, double* flops) {

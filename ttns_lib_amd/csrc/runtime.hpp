@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 
 #include <exception>
+#include <array>
+#include <map>
 #include <string>
 
 #include "../../include/ttn_capi.h"
@@ -59,6 +61,12 @@ struct ttn_ctx {
     int64_t team_words_n = 0;     // instances the buffer holds
     ttnrt::DevBuf<ttnd::TeamPlan> plan;
     void* noalloc_scratch = nullptr;  // this device's copy of the allocation-free scratch (node.cpp)
+    // tile autotuner of the node path's GEMMs (node.cpp): for every GEMM shape (M, N, K, batch,
+    // K block, operand layouts, accumulation) the output tile and K split measured fastest,
+    // with that time in ms
+    bool autotune = true;
+    std::map<std::array<int64_t, 8>, std::array<double, 3>> tune;
+    ttnrt::DevBuf<double2> tune_out;  // output of the timing runs (the caller's C is untouched)
 };
 
 // Takes the device address of the allocation-free scratch on the context's (current) device.

@@ -1,17 +1,24 @@
 """Gradients of a spin-boson TDVP observable with respect to the bath couplings g_k and
-frequencies w_k, by central differences over trajectories advanced together.
+frequencies w_k.
+
+adjoint_gradient: the costate method of ttn_sb_adjoint_run,
+
+    d<O(T)>/dtheta = 2 int_0^T Im <chi(t)| dH/dtheta |psi(t)> dt,  chi(t) = exp(iH(T - t)) O psi(T),
+
+one forward trajectory checkpointed every K steps, a second tree state chi propagated
+backwards by the same TDVP step, and Gauss-Legendre quadrature of every step from the
+leaves' transition densities; the cost does not grow with the number of parameters.
+
+observable_and_gradient: central differences over 4 N + 1 trajectories (base, +-h in each
+g_k and w_k) advanced in one batched launch (ttn_batch_step):
 
     d<O(T)>/dg_k ~ (<O(T)>[g_k + h] - <O(T)>[g_k - h]) / (2 h)
 
-The 4 N + 1 trajectories (base, +-h in each g_k and w_k) are independent instances of the
-same tree; on the HIP library they run in one batched launch (ttn_batch_step), so the cost
-is one batched trajectory rather than 4 N + 1 sequential ones.  This is a finite-difference
-gradient, not a reverse (adjoint) sweep: its error is O(h^2) truncation plus the rounding
-noise of the trajectories divided by h, which is why the Krylov tolerance defaults to 1e-14.
+with O(h^2) truncation plus the trajectories' rounding noise divided by h, which is why the
+Krylov tolerance defaults to 1e-14.  It checks the adjoint.
 
-`SigmaGradient` wraps it as a torch.autograd.Function: forward returns <O(T)> for the
-coupling and frequency tensors, backward returns the stored central differences times the
-incoming gradient.
+`SigmaGradient` wraps either as a torch.autograd.Function: forward returns <O(T)> for the
+coupling and frequency tensors, backward returns the stored gradient times the incoming one.
 """
 from __future__ import annotations
 
@@ -94,18 +101,41 @@ def observable_and_gradient(lib: Library, model: SpinBoson, nsteps: int, dt: flo
     return GradientResult(value, d[:n].copy(), d[n:].copy(), len(models), bool(batched))
 
 
+def adjoint_gradient(lib: Library, model: SpinBoson, nsteps: int, dt: float, *, op: str = "z",
+                     every: int = 20, nquad: int = 0, krylov_dim: int = 12,
+                     krylov_tol: float = 1e-14) -> GradientResult:
+    """<sigma_op(T)> at T = nsteps * dt and its gradient in (g_k, w_k) by the costate method
+    (ttn_sb_adjoint_run): checkpoints every `every` steps, nquad-point quadrature per step
+    (0: chosen from dt and the largest bath frequency)."""
+    tree, sop = sb_problem(lib, model)
+    ig = Integrator(lib, tree, sop, dt=dt, krylov_dim=krylov_dim, krylov_tol=krylov_tol)
+    try:
+        ig.set_sb_initial_state(model)
+        value, dg, dw = ig.sb_adjoint(model, nsteps, 0, SIGMA[op], every=every, nquad=nquad)
+    finally:
+        ig.destroy()
+    return GradientResult(float(value), dg, dw, 1, False)
+
+
 try:
     import torch
 
     class SigmaGradient(torch.autograd.Function):
         """<sigma_op(T)> as a differentiable function of the couplings g and frequencies w
-        (float64 tensors of the bath size); gradients by batched central differences."""
+        (float64 tensors of the bath size); gradients by the costate method (method="adjoint",
+        checkpoints every `every` steps) or batched central differences (method="fd", step h)."""
 
         @staticmethod
-        def forward(ctx, g, w, lib, model, nsteps, dt, op="z", h=1e-6, krylov_tol=1e-14):
+        def forward(ctx, g, w, lib, model, nsteps, dt, op="z", h=1e-6, krylov_tol=1e-14, method="adjoint",
+                    every=20):
             m = dataclasses.replace(model, gk=g.detach().cpu().numpy().astype(float),
                                     wk=w.detach().cpu().numpy().astype(float))
-            r = observable_and_gradient(lib, m, nsteps, dt, op=op, h=h, krylov_tol=krylov_tol)
+            if method == "adjoint":
+                r = adjoint_gradient(lib, m, nsteps, dt, op=op, every=every, krylov_tol=krylov_tol)
+            elif method == "fd":
+                r = observable_and_gradient(lib, m, nsteps, dt, op=op, h=h, krylov_tol=krylov_tol)
+            else:
+                raise ValueError(f"unknown gradient method {method!r}")
             ctx.save_for_backward(torch.as_tensor(r.dg, dtype=g.dtype, device=g.device),
                                   torch.as_tensor(r.dw, dtype=w.dtype, device=w.device))
             return torch.tensor(r.value, dtype=g.dtype, device=g.device)
@@ -113,6 +143,6 @@ try:
         @staticmethod
         def backward(ctx, grad_out):
             dg, dw = ctx.saved_tensors
-            return grad_out * dg, grad_out * dw, None, None, None, None, None, None, None
+            return grad_out * dg, grad_out * dw, None, None, None, None, None, None, None, None, None
 except ImportError:  # pragma: no cover - torch is part of the environment
     SigmaGradient = None
