@@ -354,6 +354,53 @@ def bench_c4grad(args):
     print(json.dumps(out))
 
 
+def bench_c4adj(args):
+    """Gradient workload by the costate method: d<sigma_z(T)>/d(g_k, w_k) for the 64-mode Ohmic
+    bath (alpha 0.2, wc 5, T = 0, boson dim 12, bond dim 24, dt 0.05) over T = steps * dt with a
+    checkpoint every --checkpoint-every steps (ttn_sb_adjoint_run).  One "step" is one step of
+    the forward trajectory and its share of the backward pass; a full gradient at T = 10 is 200
+    of them.  The timed region is the whole call (forward with checkpoints, recomputation,
+    quadrature launches, transition densities)."""
+    import ttns_lib_amd
+    from ttns_lib_amd import workloads as W
+    from ttns_lib_amd.capi import SIGMA, Integrator, sb_problem
+
+    lib = ttns_lib_amd.load_library()
+    inst = W.c4_instance()
+    tree, sop = sb_problem(lib, inst.model)
+    ig = Integrator(lib, tree, sop, dt=inst.dt, krylov_dim=inst.krylov_dim, krylov_tol=1e-14)
+    ig.set_sb_initial_state(inst.model)
+    every = max(1, min(args.checkpoint_every, args.steps))
+    t0 = time.perf_counter()
+    value, dg, dw = ig.sb_adjoint(inst.model, args.steps, 0, SIGMA["z"], every=every, nquad=0)
+    wall = time.perf_counter() - t0
+    ig.destroy()
+    out = {
+        "metric": METRIC,
+        "value": args.steps / wall,
+        "unit": "gradient-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 0,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Ohmic bath discretisation, random orthonormal unoccupied functions)",
+        "config": {"workload": "c4adj: costate gradient, 64-mode Ohmic alpha 0.2 wc 5 T=0, boson 12, bond 24, "
+                               "dt 0.05", "checkpoint_every": every, "krylov_dim": inst.krylov_dim,
+                   "krylov_tol": 1e-14},
+        "gradient": {"T": args.steps * inst.dt, "sigma_z": value, "seconds": wall,
+                     "max_abs_dg": float(np.abs(dg).max()), "max_abs_dw": float(np.abs(dw).max()),
+                     "projected_s_for_T10": 200 * wall / args.steps},
+        "roofline": None,
+        "plain_gpu": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def bench_copy(args):
     """Calibration for HBM traffic counters: a device-to-device copy of 1 GiB (read 1 GiB,
     write 1 GiB per repetition) by torch's elementwise copy kernel."""
@@ -381,6 +428,7 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=8, help="instances stepped alone for the plain GPU rate")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle time for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--checkpoint-every", type=int, default=20, help="c4adj: steps between checkpoints")
     ap.add_argument("--report", action="store_true",
                     help="print each instance's team size, flops and device phase times of the timed steps to stderr")
     args = ap.parse_args()
@@ -397,6 +445,9 @@ def main():
         return
     if args.workload == "c4grad":
         bench_c4grad(args)
+        return
+    if args.workload == "c4adj":
+        bench_c4adj(args)
         return
 
     import ttns_lib_amd

@@ -1562,16 +1562,16 @@ std::unique_ptr<ttn_integ> twin(const ttn_integ* in) {
     return t;
 }
 
-// one integrator call of one instance (ttn_step without the trajectory bookkeeping)
-void step_one(ttn_integ* in, double ci) {
+// n integrator calls of one instance in one launch (ttn_step without the trajectory bookkeeping)
+void step_n(ttn_integ* in, double ci, int64_t n) {
     ttnd::RunArgs a;
-    make_args(in, 0.0, ci, 1, a);
-    prepare_traj(in, 1);
+    make_args(in, 0.0, ci, n, a);
+    prepare_traj(in, n);
     hip_check(hipMemcpyAsync(in->d_inst.p, &in->inst, sizeof(in->inst), hipMemcpyHostToDevice, in->ctx->stream),
               "inst");
     launch(in->ctx, in->d_inst.p, {in->inst}, a, {auto_team(in)});
     account(in, a);
-    finish(in, 1);
+    finish(in, n);
 }
 
 // one call of every instance, each with its own coefficient (-i forward, +i backward in time)
@@ -1654,14 +1654,12 @@ int ttn_sb_adjoint_run(ttn_integ* in, const ttn_sb_params* p, int64_t nsteps, in
         hipStream_t st = in->ctx->stream;
         const double dt = in->p.dt;
         const int Q = nquad > 0 ? static_cast<int>(nquad) : sb_quadrature_nodes(sp, dt);
-        // forward with checkpoints
+        // forward: one launch per checkpoint interval
         std::vector<std::unique_ptr<ArenaSnap>> ck;
-        for (int64_t s = 0; s < nsteps; ++s) {
-            if (s % every == 0) {
-                ck.push_back(std::make_unique<ArenaSnap>());
-                snap_take(in, *ck.back());
-            }
-            step_one(in, -1.0);
+        for (int64_t s0 = 0; s0 < nsteps; s0 += every) {
+            ck.push_back(std::make_unique<ArenaSnap>());
+            snap_take(in, *ck.back());
+            step_n(in, -1.0, std::min(every, nsteps - s0));
         }
         ArenaSnap fin;
         snap_take(in, fin);
@@ -1696,20 +1694,19 @@ int ttn_sb_adjoint_run(ttn_integ* in, const ttn_sb_params* p, int64_t nsteps, in
         }
         chi->nsteps = 0;
         chi->p.dt = dt;
-        // the 2Q instances of one step's quadrature: psi at t_n + x_q dt, chi at the same times
+        // one launch per step: the 2Q instances of the step's quadrature (psi at t_n + x_q dt,
+        // chi at the same times) and chi's own full step back to t_n
         std::vector<std::unique_ptr<ttn_integ>> W;
         std::vector<ttn_integ*> wp;
         for (int q = 0; q < 2 * Q; ++q) {
             W.push_back(twin(in));
             wp.push_back(W.back().get());
         }
+        wp.push_back(chi.get());
         std::vector<double> xs, ws;
         gauss_legendre01(Q, xs, ws);
-        std::vector<double> ci(2 * Q);
-        for (int q = 0; q < Q; ++q) {
-            ci[q] = -1.0;
-            ci[Q + q] = 1.0;
-        }
+        std::vector<double> ci(2 * Q + 1, 1.0);
+        for (int q = 0; q < Q; ++q) ci[q] = -1.0;
         // transition densities: pairs (W[q], W[Q + q]), S = Scoup on the system leaf
         const int64_t d0 = tp.mode_dims[0];
         check(static_cast<int64_t>(sp.scoup.size()) == d0 * d0, ERR_INVALID, "adjoint: Scoup has the wrong size");
@@ -1737,9 +1734,9 @@ int ttn_sb_adjoint_run(ttn_integ* in, const ttn_sb_params* p, int64_t nsteps, in
         DevBuf<ttnd::RunArgs> d_args;
         DevBuf<ttnd::DInst> d_insts, d_pairs;
         d_pairs.alloc(2 * Q);
+        // (the pairs read the quadrature instances only; chi has moved on by then)
         std::vector<double> g(sp.wk.size(), 0.0), w(sp.wk.size(), 0.0);
         std::vector<std::vector<cplx>> rho(tp.nmodes()), rhoS(tp.nmodes());
-        ArenaSnap c1;
         std::vector<std::unique_ptr<ArenaSnap>> starts;
         for (int64_t gi = static_cast<int64_t>(ck.size()) - 1; gi >= 0; --gi) {
             snap_put(in, *ck[gi]);
@@ -1748,17 +1745,17 @@ int ttn_sb_adjoint_run(ttn_integ* in, const ttn_sb_params* p, int64_t nsteps, in
                 const size_t k = static_cast<size_t>(s - s0);
                 if (starts.size() <= k) starts.push_back(std::make_unique<ArenaSnap>());
                 snap_take(in, *starts[k]);
-                if (s + 1 < s1) step_one(in, -1.0);
+                if (s + 1 < s1) step_n(in, -1.0, 1);
             }
             for (int64_t s = s1 - 1; s >= s0; --s) {
-                snap_take(chi.get(), c1);  // chi(t_{s+1})
                 for (int q = 0; q < Q; ++q) {
                     snap_put(W[q].get(), *starts[s - s0]);
                     W[q]->p.dt = xs[q] * dt;
-                    snap_put(W[Q + q].get(), c1);
+                    copy_state(W[Q + q].get(), chi.get());  // chi(t_{s+1})
                     W[Q + q]->p.dt = (1.0 - xs[q]) * dt;
                 }
-                step_each(in->ctx, wp, ci, d_args, d_insts);
+                chi->p.dt = dt;
+                step_each(in->ctx, wp, ci, d_args, d_insts);  // chi -> chi(t_s)
                 std::vector<ttnd::DInst> pr;
                 for (int q = 0; q < 2 * Q; ++q) pr.push_back(W[q]->inst);
                 hip_check(hipMemcpyAsync(d_pairs.p, pr.data(), pr.size() * sizeof(ttnd::DInst), hipMemcpyHostToDevice,
@@ -1779,9 +1776,6 @@ int ttn_sb_adjoint_run(ttn_integ* in, const ttn_sb_params* p, int64_t nsteps, in
                     }
                     sb_density_gradient(sp, rho, rhoS, ws[q] * dt, g, w);
                 }
-                snap_put(chi.get(), c1);
-                chi->p.dt = dt;
-                step_one(chi.get(), 1.0);
             }
         }
         snap_put(in, fin);
