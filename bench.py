@@ -244,11 +244,23 @@ def bench_c2(args):
     op1 = np.array([herm(d) for _ in range(nt)])
     mf = np.array([herm(d) for _ in range(nt)])
     A = rng.standard_normal((d * d, d)) + 1j * rng.standard_normal((d * d, d))
+    # the fixed 64 x 64 tile first, then the autotuner's choice per GEMM shape
+    lib.set_autotune(False)
+    node0 = Node(lib, op0, op1, mf)
+    node0.set(A)
+    node0.heff_timed(max(1, args.warmup))
+    ms0, _ = node0.heff_timed(args.steps)
+    node0.destroy()
+    lib.set_autotune(True)
     node = Node(lib, op0, op1, mf)
     node.set(A)
     node.heff_timed(max(1, args.warmup))
     ms, fl = node.heff_timed(args.steps)
     heff_ms = ms / args.steps
+    from ttns_lib_amd.capi import GEMM_TILES
+
+    tuned = [{"M": e[0], "N": e[1], "K": e[2], "batch": e[3], "tile": list(GEMM_TILES[e[5]]), "k_split": e[6],
+              "ms": round(e[7], 4)} for e in lib.autotune_table()]
     node.mf(0, reps=1)
     node.mf(0, reps=args.steps)
     mf_ms, mf_fl = node.last()
@@ -282,6 +294,7 @@ def bench_c2(args):
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
                      "kernel": "ttnd::zgemm_kernel (three GEMM stages of the H-action)",
                      "flops_per_heff": fl / args.steps},
+        "autotune": {"heff_ms_fixed_64x64": ms0 / args.steps, "heff_ms_tuned": heff_ms, "table": tuned},
         "mean_field": {"ms": mf_ms / args.steps, "tflops": mf_fl / (mf_ms / 1e3) / 1e12},
         "krylov16": {"ms": kms, "tau": tau, "h_applications": na, "tflops": kfl / (kms / 1e3) / 1e12,
                      "ms_per_iteration": kms / max(na, 1)},

@@ -1,47 +1,73 @@
 """Writes tests/golden/c3_oracle.json: the oracle's <sigma_x,y,z>(t) and <psi|psi> over the
-first steps of selected instances of the batched workload (full size, complex128).  The GPU
-tests compare the HIP library against these numbers; the oracle needs minutes per step at
-this size, so they are computed once here.
+first steps of selected instances of the batched workload (full size, complex128), and for
+each step the oracle's own rounding spread: the largest |<sigma>| difference between the run
+and two runs whose initial node tensors differ by relative 1e-15 (tests/helpers.py
+perturb_nodes).  The GPU tests compare the HIP library against these numbers within
+max(1e-10, 10 x spread so far); the oracle needs seconds to minutes per step at this size, so
+they are computed once here, one process per instance.
 
-Usage: python tests/golden/make_c3_oracle.py [instance ...]   (default: 0)
+Usage: python tests/golden/make_c3_oracle.py [--steps N] [--jobs J] [instance ...]   (default: 0)
 """
+import argparse
 import json
 import os
 import sys
 import time
-
-import numpy as np
+from multiprocessing import Pool
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
-
-from ttns_lib_amd.capi import SIGMA, Integrator, Library, sb_problem  # noqa: E402
-from ttns_lib_amd.workloads import c3_instances  # noqa: E402
+sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 OUT = os.path.join(ROOT, "tests", "golden", "c3_oracle.json")
-NSTEPS = 2
+
+
+def run_instance(args):
+    k, nsteps = args
+    from helpers import perturb_nodes
+    from ttns_lib_amd.capi import SIGMA, Integrator, Library, sb_problem
+    from ttns_lib_amd.workloads import c3_instances
+
+    orc = Library(os.path.join(ROOT, "oracle", "_build", "libttn_oracle.so"), "orc_")
+    inst = c3_instances()[k]
+    runs, keep = [], []
+    for seed in (None, 1, 2):
+        tree, sop = sb_problem(orc, inst.model)
+        ig = Integrator(orc, tree, sop, dt=inst.dt, krylov_dim=inst.krylov_dim, krylov_tol=1e-12)
+        ig.set_sb_initial_state(inst.model)
+        if seed is not None:
+            perturb_nodes(ig, seed)
+        keep.append((tree, sop))
+        runs.append(ig)
+    rec = {"steps": [], "krylov_dim": inst.krylov_dim, "krylov_tol": 1e-12}
+    for s in range(nsteps):
+        t0 = time.time()
+        vals = []
+        for ig in runs:
+            ig.step(1)
+            vals.append({o: complex(ig.expect(0, SIGMA[o])).real for o in "xyz"})
+        base = runs[0]
+        spread = max(abs(v[o] - vals[0][o]) for v in vals[1:] for o in "xyz")
+        rec["steps"].append({"t": base.time(), **vals[0], "norm2": base.norm2(), "nh": base.counters()[0],
+                             "spread": spread})
+        print(f"instance {k} step {s + 1}: spread {spread:.2e} ({time.time() - t0:.0f}s)", flush=True)
+    return k, rec
+
 
 if __name__ == "__main__":
-    picks = [int(a) for a in sys.argv[1:]] or [0]
-    orc = Library(os.path.join(ROOT, "oracle", "_build", "libttn_oracle.so"), "orc_")
-    ins = c3_instances()
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--jobs", type=int, default=1)
+    ap.add_argument("instances", nargs="*", type=int)
+    a = ap.parse_args()
+    picks = a.instances or [0]
     data = {}
     if os.path.exists(OUT):
         with open(OUT) as f:
             data = json.load(f)
-    for k in picks:
-        inst = ins[k]
-        tree, sop = sb_problem(orc, inst.model)
-        ig = Integrator(orc, tree, sop, dt=inst.dt, krylov_dim=inst.krylov_dim, krylov_tol=1e-12)
-        ig.set_sb_initial_state(inst.model)
-        rec = {"steps": [], "krylov_dim": inst.krylov_dim, "krylov_tol": 1e-12}
-        for s in range(NSTEPS):
-            t0 = time.time()
-            ig.step(1)
-            sig = {o: complex(ig.expect(0, SIGMA[o])).real for o in "xyz"}
-            rec["steps"].append({"t": ig.time(), **sig, "norm2": ig.norm2(), "nh": ig.counters()[0]})
-            print(f"instance {k} step {s + 1}: {rec['steps'][-1]} ({time.time() - t0:.0f}s)", flush=True)
-        data[str(k)] = rec
-    with open(OUT, "w") as f:
-        json.dump(data, f, indent=1)
+    with Pool(max(1, a.jobs)) as pool:
+        for k, rec in pool.imap_unordered(run_instance, [(k, a.steps) for k in picks]):
+            data[str(k)] = rec
+            with open(OUT, "w") as f:
+                json.dump(data, f, indent=1)
     print(f"wrote {OUT}")

@@ -7,8 +7,8 @@
 //   A (16 x 4): lane l holds A[l % 16][l / 16];  B (4 x 16): lane l holds B[l / 16][l % 16];
 //   D (16 x 16): lane l holds D[l / 16 + 4 * j][l % 16] in accumulator element j
 //   (measured on gfx950 with a lane-id probe).
-// Operands come straight from L2 (node tensors are too large for LDS); the loads of up
-// to eight k-steps are issued before their MFMAs.
+// Operands come straight from L2 (node tensors are too large for LDS) as 16-byte coherent
+// buffer loads; the loads of up to eight k-steps are issued before their MFMAs.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -87,6 +87,10 @@ __device__ __forceinline__ void ztile(ZAcc& acc, const double2* __restrict__ a, 
     const int lane = threadIdx.x & 63, i = lane & 15, kq = lane >> 4;
     const bool arow = i < mv, bcol = i < nv;
     const double2 z = make_double2(0.0, 0.0);
+    // global operands: one 16-byte coherent buffer load per element (the bases are
+    // wave-uniform, every offset lies inside the instance's arena)
+    constexpr bool coh = __is_same(LD, LoadL2);
+    const Coh ca(coh ? a : nullptr), cb(coh ? b : nullptr);
     // up to 8 k-steps (k <= 32) of operands are loaded before the first MFMA
     for (int64_t k0 = 0; k0 < K; k0 += 32) {
         const int nks = static_cast<int>((K - k0 + 3) / 4 < 8 ? (K - k0 + 3) / 4 : 8);  // wave-uniform
@@ -96,8 +100,13 @@ __device__ __forceinline__ void ztile(ZAcc& acc, const double2* __restrict__ a, 
             if (s < nks) {
                 const int64_t k = k0 + 4 * s + kq;
                 const bool ok = k < K;
-                av[s] = (arow && ok) ? ld(a + i * sar + k * sak) : z;
-                bv[s] = (bcol && ok) ? ld(b + k * sbk + i * sbc) : z;
+                if constexpr (coh) {
+                    av[s] = (arow && ok) ? ca[i * sar + k * sak] : z;
+                    bv[s] = (bcol && ok) ? cb[k * sbk + i * sbc] : z;
+                } else {
+                    av[s] = (arow && ok) ? ld(a + i * sar + k * sak) : z;
+                    bv[s] = (bcol && ok) ? ld(b + k * sbk + i * sbc) : z;
+                }
             }
         }
 #pragma unroll
