@@ -252,6 +252,17 @@ def bench_c2(args):
     ms0, _ = node0.heff_timed(args.steps)
     node0.destroy()
     lib.set_autotune(True)
+    # the two forms of the H-action's mode-0 and mean-field stages, then the autotuner's pick
+    forms = {}
+    for form, name in ((1, "staged"), (2, "fused")):
+        lib.set_heff_form(form)
+        nf = Node(lib, op0, op1, mf)
+        nf.set(A)
+        nf.heff_timed(max(1, args.warmup))
+        forms[name] = nf.heff_timed(args.steps)[0] / args.steps
+        nf.destroy()
+    lib.set_heff_form(0)
+    fused = forms["fused"] < forms["staged"]
     node = Node(lib, op0, op1, mf)
     node.set(A)
     node.heff_timed(max(1, args.warmup))
@@ -292,9 +303,11 @@ def bench_c2(args):
                    "terms": nt},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "kernel": "ttnd::zgemm_kernel (three GEMM stages of the H-action)",
+                     "kernel": ("ttnd::zgemm_kernel (mode-1 stage) + ttnd::zheff_fused_kernel" if fused
+                                else "ttnd::zgemm_kernel (three GEMM stages of the H-action)"),
                      "flops_per_heff": fl / args.steps},
-        "autotune": {"heff_ms_fixed_64x64": ms0 / args.steps, "heff_ms_tuned": heff_ms, "table": tuned},
+        "autotune": {"heff_ms_fixed_64x64": ms0 / args.steps, "heff_ms_tuned": heff_ms, "table": tuned,
+                     "heff_ms_by_form": forms},
         "mean_field": {"ms": mf_ms / args.steps, "tflops": mf_fl / (mf_ms / 1e3) / 1e12},
         "krylov16": {"ms": kms, "tau": tau, "h_applications": na, "tflops": kfl / (kms / 1e3) / 1e12,
                      "ms_per_iteration": kms / max(na, 1)},

@@ -142,6 +142,7 @@ _SIGS = {
     "sb_adjoint_run": [vp, C.POINTER(SBParamsC), i64, i64, i64, i64, dblp, dblp, dblp, dblp],
     "ctx_set_autotune": [vp, C.c_int],
     "autotune_table": [vp, i64p, i64p, dblp],
+    "ctx_set_heff_form": [vp, C.c_int],
     "transition_densities": [vp, vp, i64, dblp, i64, dblp],
 }
 _RESTYPE = {"last_error": C.c_char_p}
@@ -198,6 +199,11 @@ class Library:
     def set_autotune(self, enabled: bool = True):
         """tile autotuner of the node-level GEMMs (ttn_ctx_set_autotune)"""
         self._call("ctx_set_autotune", self.ctx, int(bool(enabled)))
+
+    def set_heff_form(self, form: int = 0):
+        """node H-action's mode-0 and mean-field stages: 0 autotuned, 1 staged, 2 fused
+        (ttn_ctx_set_heff_form)"""
+        self._call("ctx_set_heff_form", self.ctx, int(form))
 
     def autotune_table(self) -> list:
         """[(M, N, K, batch, kblock, tile, ksplit, ms)] of every GEMM shape the autotuner timed;

@@ -1222,6 +1222,7 @@ int ttn_reset_nsteps(ttn_integ* in) {
 // node, the std::mt19937 / std::normal_distribution state as text, the pending normal draws,
 // every node tensor at its current shape and the carried matrices (gauge factors,
 // single-particle and mean-field matrices) at capacity.
+extern "C++" {
 namespace {
 constexpr char CKPT_TAG[8] = {'T', 'T', 'N', 'C', 'K', 'P', 'T', '1'};
 constexpr int CKPT_NFIELDS = 13;
@@ -1264,6 +1265,7 @@ int64_t ckpt_bytes(const ttn_integ* in, int64_t pending, int64_t rt) {
     return b;
 }
 }  // namespace
+}  // extern "C++"
 
 int ttn_checkpoint_save(ttn_integ* in, int64_t* bytes, void* buf) {
     return guard(in->ctx, [&] {

@@ -374,6 +374,121 @@ __global__ __launch_bounds__(GT) void znorm_kernel(const double2* x, int64_t n, 
     }
 }
 
+// Stages 2 and 3 of the H-action fused, for d0 <= 64 and r <= 64:
+//   res[(i0, i1)][c] (=|+=) sum_t sum_k (sum_j0 op0_t[i0][j0] Y_t[j0][i1][k]) mft_t[k][c]
+// One workgroup per (i1, group of terms).  Per term, Y_t[:, i1, :] is staged in LDS, the mode-0
+// product Z = op0_t Y_t[:, i1, :] is formed in registers and written back over it (k-major, after
+// a barrier: nothing reads Y any more), and the mean-field product Z mft_t accumulates in
+// registers over the group's terms.  Each wave owns up to four 16 x 16 output tiles (the same
+// ones in both products) and advances all of them per k-step, so its MFMAs form eight
+// independent accumulation chains; one 64 x 72 plane pair (74 KB) leaves room for two
+// workgroups per CU.  With several groups the partial sums go to W[g] (D x r each) for
+// zreduce_kernel, which adds them in group order.
+constexpr int FL = 72;  // LDS row length (doubles)
+
+__global__ __launch_bounds__(GT) void zheff_fused_kernel(FusedDesc f) {
+    __shared__ double pr[64][FL], pi[64][FL];  // Y_t[j0][i1][k] at [j0][k], then Z[i0][k] at [k][i0]
+    const int i1 = blockIdx.x, g = blockIdx.y, ng = gridDim.y;
+    const int t0 = static_cast<int>(static_cast<int64_t>(f.nt) * g / ng);
+    const int t1 = static_cast<int>(static_cast<int64_t>(f.nt) * (g + 1) / ng);
+    const int d0 = f.d0, r = f.r;
+    const int64_t size = static_cast<int64_t>(d0) * f.d1 * r;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, li = lane & 15, kq = lane >> 4;
+    const int mt = (d0 + 15) >> 4, nt = (r + 15) >> 4, tiles = mt * nt;
+    // this wave's tiles wv, wv + 4, ...: row block tm[q], column block tn[q]
+    const int cnt = tiles > wv ? min(4, (tiles - wv + 3) / 4) : 0;
+    int tm[4], tn[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int it = wv + 4 * q;
+        tm[q] = it / nt;
+        tn[q] = it % nt;
+    }
+    Acc acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        acc[q].re = d4{0.0, 0.0, 0.0, 0.0};
+        acc[q].im = d4{0.0, 0.0, 0.0, 0.0};
+    }
+    for (int t = t0; t < t1; ++t) {
+        const double2* Y = f.Y + t * size + static_cast<int64_t>(i1) * r;
+        for (int e = threadIdx.x; e < 64 * 64; e += GT) {
+            const int j0 = e >> 6, k = e & 63;
+            const double2 y = (j0 < d0 && k < r) ? Y[static_cast<int64_t>(j0) * f.d1 * r + k] : make_double2(0.0, 0.0);
+            pr[j0][k] = y.x;
+            pi[j0][k] = y.y;
+        }
+        __syncthreads();
+        // Z tiles = op0_t (global) x Y (LDS)
+        const double2* O = f.op0 + static_cast<int64_t>(t) * d0 * d0;
+        Acc z[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            z[q].re = d4{0.0, 0.0, 0.0, 0.0};
+            z[q].im = d4{0.0, 0.0, 0.0, 0.0};
+        }
+        for (int ks = 0; ks < d0; ks += 4) {
+            const int k = ks + kq;
+            double2 a[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = 16 * tm[q] + li;
+                a[q] = (q < cnt && row < d0 && k < d0) ? O[static_cast<int64_t>(row) * d0 + k] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (q < cnt) mfma_c(z[q], a[q].x, a[q].y, pr[k][16 * tn[q] + li], pi[k][16 * tn[q] + li]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (q < cnt)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    pr[16 * tn[q] + li][16 * tm[q] + kq + 4 * j] = z[q].re[j];
+                    pi[16 * tn[q] + li][16 * tm[q] + kq + 4 * j] = z[q].im[j];
+                }
+        __syncthreads();
+        // acc += Z (LDS) x mft_t (global)
+        const double2* M = f.mft + static_cast<int64_t>(t) * r * r;
+        for (int ks = 0; ks < r; ks += 4) {
+            const int k = ks + kq;
+            double2 b[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int col = 16 * tn[q] + li;
+                b[q] = (q < cnt && col < r && k < r) ? M[static_cast<int64_t>(k) * r + col] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (q < cnt) mfma_c(acc[q], pr[k][16 * tm[q] + li], pi[k][16 * tm[q] + li], b[q].x, b[q].y);
+        }
+        __syncthreads();
+    }
+    // out[(i0 d1 + i1) r + c]
+    double2* out = ng > 1 ? f.W + static_cast<int64_t>(g) * f.w_split : f.res;
+    const bool direct = ng == 1;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (q >= cnt) break;
+        const int col = 16 * tn[q] + li;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int i0 = 16 * tm[q] + kq + 4 * j;
+            if (i0 < d0 && col < r) {
+                double2* p = out + (static_cast<int64_t>(i0) * f.d1 + i1) * r + col;
+                double2 v = make_double2(acc[q].re[j], acc[q].im[j]);
+                if (direct && f.accumulate) {
+                    const double2 o = *p;
+                    v.x += o.x;
+                    v.y += o.y;
+                }
+                *p = v;
+            }
+        }
+    }
+}
+
 // out[t](p, j) = coeff[t] * mf[t](j, p): the mean-field factors of the term-blocked sum
 __global__ __launch_bounds__(GT) void zmft_kernel(const double2* coeff, const double2* mf, int64_t nt, int64_t r,
                                                   double2* out) {
@@ -397,6 +512,32 @@ extern "C" hipError_t ttnd_zmft(const double2* coeff, const double2* mf, int64_t
     hipLaunchKernelGGL(zmft_kernel, dim3(blocks), dim3(GT), 0, s, coeff, mf, nt, r, out);
     return hipGetLastError();
 }
+extern "C" hipError_t ttnd_zheff_fused(const ttnd::FusedDesc* f, int ngroups, hipStream_t s) {
+    using namespace ttnd;
+    if (f->d0 > 64 || f->r > 64 || f->d0 < 1 || f->r < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(zheff_fused_kernel, dim3(f->d1, ngroups), dim3(GT), 0, s, *f);
+    if (ngroups > 1) {
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        GemmDesc g{};
+        g.M = static_cast<int>(f->d0 * f->d1);
+        g.N = static_cast<int>(f->r);
+        g.nb0 = g.nb1 = 1;
+        g.W = f->W;
+        g.w_split = f->w_split;
+        g.w_batch = 0;
+        g.C = f->res;
+        g.c_r = f->r;
+        g.c_c = 1;
+        g.accumulate = f->accumulate;
+        g.alpha = make_double2(1.0, 0.0);
+        const int64_t total = static_cast<int64_t>(g.M) * g.N;
+        const int blocks = static_cast<int>(total / GT + 1 < 4096 ? total / GT + 1 : 4096);
+        hipLaunchKernelGGL(zreduce_kernel, dim3(blocks), dim3(GT), 0, s, g, ngroups);
+    }
+    return hipGetLastError();
+}
+
 namespace {
 template <int WM, int WN>
 void launch_tiles(const ttnd::GemmDesc* g, int nsplit, hipStream_t s) {

@@ -36,4 +36,19 @@ struct GemmDesc {
     double2 alpha;
 };
 
+// The fused stages 2 and 3 of a node's H-action (zheff_fused_kernel), d0, r <= 64:
+//   res[(i0, i1)][c] (=|+=) sum_t sum_k (sum_j0 op0[t][i0][j0] Y[t][j0][i1][k]) mft[t][k][c]
+// Y[t] row-major d0 x d1 x r, op0[t] d0 x d0, mft[t] r x r; groups > 1 write partial sums to
+// W (w_split entries apart, D x r each) and sum them in group order.
+struct FusedDesc {
+    int d0, d1, r, nt;
+    int accumulate, pad;
+    const double2* Y;
+    const double2* op0;
+    const double2* mft;
+    double2* res;
+    double2* W;
+    int64_t w_split;
+};
+
 }  // namespace ttnd

@@ -22,6 +22,7 @@
 #include "runtime.hpp"
 
 extern "C" hipError_t ttnd_zgemm(const ttnd::GemmDesc* g, int nsplit, int tile, hipStream_t s);
+extern "C" hipError_t ttnd_zheff_fused(const ttnd::FusedDesc* f, int ngroups, hipStream_t s);
 extern "C" hipError_t ttnd_zdots(const double2* V, int64_t ld, int m, const double2* w, int64_t n, double* partial,
                                  int nblk, double* h, hipStream_t s);
 extern "C" hipError_t ttnd_zorth(const double2* V, int64_t ld, int m, const double* h, double2* w, int64_t n,
@@ -257,8 +258,72 @@ void stage_mf_sum(const NodeView& v, double2* res, bool accumulate) {
     run(v, g);
 }
 
+// Stages 2 and 3 fused (zheff_fused_kernel): term groups so that d1 x groups fills the GPU at two
+// workgroups per CU, fewer when their partial sums would not fit the workspace.  Returns false if the shape does
+// not fit the kernel.
+bool fused_mf(const NodeView& v, double2* res, bool accumulate) {
+    if (v.d0 > 64 || v.r > 64) return false;
+    int groups = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(v.nt, (512 + v.d1 - 1) / v.d1)));
+    while (groups > 1 && groups * v.D * v.r > v.ws_n) --groups;
+    ttnd::FusedDesc f{};
+    f.d0 = static_cast<int>(v.d0);
+    f.d1 = static_cast<int>(v.d1);
+    f.r = static_cast<int>(v.r);
+    f.nt = static_cast<int>(v.nt);
+    f.accumulate = accumulate ? 1 : 0;
+    f.Y = v.Y;
+    f.op0 = v.op0;
+    f.mft = v.mft;
+    f.res = res;
+    f.W = v.ws;
+    f.w_split = v.D * v.r;
+    hip_check(ttnd_zheff_fused(&f, groups, v.s), "fused H-action");
+    return true;
+}
+
+// Whether this H-action shape runs its mode-0 and mean-field stages fused: the autotuner times
+// both forms once per shape (into a scratch result) and keeps the faster; without it the staged
+// form runs, unless ttn_ctx_set_heff_form fixed the form.
+bool use_fused(const NodeView& v) {
+    if (!v.ctx || v.d0 > 64 || v.r > 64 || v.ctx->heff_form == 1) return false;
+    if (v.ctx->heff_form == 2) return true;
+    if (!v.ctx->autotune) return false;
+    const std::array<int64_t, 4> key = {v.d0, v.d1, v.r, v.nt};
+    auto it = v.ctx->fuse.find(key);
+    if (it == v.ctx->fuse.end()) {
+        if (!v.alloc_ok) return false;
+        DevBuf<double2> scratch;
+        scratch.alloc(v.D * v.r);
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "event");
+        hip_check(hipEventCreate(&e1), "event");
+        float best[2] = {1e30f, 1e30f};
+        for (int form = 0; form < 2; ++form)
+            for (int rep = 0; rep < 4; ++rep) {
+                hip_check(hipEventRecord(e0, v.s), "event");
+                if (form == 0) {
+                    stage_mode0(v, v.Y, v.size, v.Z);
+                    stage_mf_sum(v, scratch.p, false);
+                } else {
+                    fused_mf(v, scratch.p, false);
+                }
+                hip_check(hipEventRecord(e1, v.s), "event");
+                hip_check(hipEventSynchronize(e1), "sync");
+                float ms = 0.f;
+                hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+                if (rep > 0) best[form] = std::min(best[form], ms);
+            }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        const int form = best[1] < best[0] ? 1 : 0;
+        it = v.ctx->fuse.emplace(key, std::array<double, 2>{static_cast<double>(form), best[form]}).first;
+    }
+    return it->second[0] > 0.5;
+}
+
 void heff(const NodeView& v, const double2* x, double2* res) {
     stage_mode1(v, x);
+    if (use_fused(v) && fused_mf(v, res, false)) return;
     stage_mode0(v, v.Y, v.size, v.Z);
     stage_mf_sum(v, res, false);
 }
@@ -641,6 +706,13 @@ int ttn_node_heff_noalloc(ttn_ctx* ctx, int64_t d0, int64_t d1, int64_t r, int64
 
 int ttn_ctx_set_autotune(ttn_ctx* ctx, int enabled) {
     return guard(ctx, [&] { ctx->autotune = enabled != 0; });
+}
+
+int ttn_ctx_set_heff_form(ttn_ctx* ctx, int form) {
+    return guard(ctx, [&] {
+        check(form >= 0 && form <= 2, ERR_INVALID, "form must be 0 (autotuned), 1 (staged) or 2 (fused)");
+        ctx->heff_form = form;
+    });
 }
 
 int ttn_autotune_table(const ttn_ctx* ctx, int64_t* n, int64_t* entries, double* ms) {

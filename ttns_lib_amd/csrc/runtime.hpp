@@ -67,6 +67,10 @@ struct ttn_ctx {
     bool autotune = true;
     std::map<std::array<int64_t, 8>, std::array<double, 3>> tune;
     ttnrt::DevBuf<double2> tune_out;  // output of the timing runs (the caller's C is untouched)
+    // for every node H-action shape (d0, d1, r, terms): whether the mode-0 and mean-field
+    // stages run fused (1) or staged (0), with the faster form's time in ms
+    std::map<std::array<int64_t, 4>, std::array<double, 2>> fuse;
+    int heff_form = 0;  // 0: the autotuner's choice, 1: always staged, 2: fused where it fits
 };
 
 // Takes the device address of the allocation-free scratch on the context's (current) device.
