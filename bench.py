@@ -205,12 +205,15 @@ def report(insts, igs, teams, dflops, p0, steps, s0=None):
         st = (ig.node_stats() - s0[k]) / steps
         groups = {}
         for n, shp in enumerate(ig.shapes):
-            g = groups.setdefault(shp, np.zeros(5))
+            g = groups.setdefault(shp, np.zeros(9))
             g += np.array([1, *st[n]])
         for shp, g in sorted(groups.items(), key=lambda kv: -kv[1][1]):
             print(f"[report] inst {k} nodes {shp[0]}x{shp[1]} x{int(g[0])}: node H-apps {g[1]:.0f} "
-                  f"(mean Krylov size {g[2] / max(g[1], 1):.1f}), bond H-apps {g[3]:.0f} "
-                  f"(mean size {g[4] / max(g[3], 1):.1f}) per step", file=sys.stderr, flush=True)
+                  f"(mean Krylov size {g[2] / max(g[1], 1):.1f}, {g[5] / 1e8:.2f} s, "
+                  f"{g[5] / 100 / max(g[1], 1):.1f} us per iteration of which H-action {g[7] / 100 / max(g[1], 1):.1f}), "
+                  f"bond H-apps {g[3]:.0f} (mean size {g[4] / max(g[3], 1):.1f}, {g[6] / 1e8:.2f} s, "
+                  f"{g[6] / 100 / max(g[3], 1):.1f} us per iteration of which H-action {g[8] / 100 / max(g[3], 1):.1f}) "
+                  "per step", file=sys.stderr, flush=True)
 
 
 def _cpu_model():

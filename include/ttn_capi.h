@@ -274,9 +274,11 @@ int ttn_flops(const ttn_integ* integ, double* flops);
  * those propagations.  Instrumentation only. */
 int ttn_profile(const ttn_integ* integ, int64_t* slots);
 
-/* Per node (out[4 n .. 4 n + 3], n in pre-order): H applications of its node propagations,
+/* Per node (out[8 n .. 8 n + 7], n in pre-order): H applications of its node propagations,
  * their Krylov sizes summed over iterations, H applications of the bond propagations that
- * use its operators, and their Krylov sizes summed.  Instrumentation only. */
+ * use its operators, their Krylov sizes summed, the device wall-clock ticks (100 MHz) of the
+ * node and of the bond propagations, and the ticks of the H-actions inside those two (outside
+ * the small-vector propagator).  Instrumentation only. */
 int ttn_node_stats(const ttn_integ* integ, int64_t* out);
 
 /* ---- workgroup teams (no reference counterpart: GPU execution tuning) ----

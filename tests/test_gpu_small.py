@@ -120,3 +120,5 @@ def test_node_stats_account_for_every_node_application(hip):
             used = napp > 0
             assert np.all(sumj[used] >= napp[used]) and np.all(sumj[used] <= 12 * napp[used])
         assert st[:, 2].sum() > 0
+        # device ticks of the node and bond propagations where there were any
+        assert np.all(st[st[:, 0] > 0, 4] > 0) and np.all(st[st[:, 2] > 0, 5] > 0)

@@ -631,10 +631,12 @@ class Integrator:
         return d
 
     def node_stats(self) -> np.ndarray:
-        """per node: node H applications, their Krylov sizes summed, bond H applications, sizes"""
-        out = np.zeros(4 * len(self.shapes), dtype=np.int64)
+        """per node: node H applications, their Krylov sizes summed, bond H applications, sizes,
+        the device ticks of the node and of the bond propagations (100 MHz), and of the H-actions
+        inside them (outside the small-vector propagator)"""
+        out = np.zeros(8 * len(self.shapes), dtype=np.int64)
         self.lib._call("node_stats", self.h, _ip(out))
-        return out.reshape(-1, 4)
+        return out.reshape(-1, 8)
 
     def set_observable(self, mode: int, op):
         self._obs = _cplx_in(op)

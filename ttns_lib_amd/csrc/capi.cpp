@@ -393,8 +393,9 @@ void build_instance(ttn_integ* in) {
     in->status.alloc(2);
     hip_check(hipMemsetAsync(in->status.p, 0, 2 * sizeof(int), in->ctx->stream), "memset status");
     in->scratch.alloc(4);
-    in->node_stats.alloc(4 * n);
-    hip_check(hipMemsetAsync(in->node_stats.p, 0, 4 * n * sizeof(int64_t), in->ctx->stream), "memset stats");
+    in->node_stats.alloc(ttnd::NODE_STATS * n);
+    hip_check(hipMemsetAsync(in->node_stats.p, 0, ttnd::NODE_STATS * n * sizeof(int64_t), in->ctx->stream),
+              "memset stats");
     in->fm = flop_model(t, ops, in->sop);
     in->prof.alloc(ttnd::PROF_NSLOTS);
     hip_check(hipMemsetAsync(in->prof.p, 0, ttnd::PROF_NSLOTS * sizeof(int64_t), in->ctx->stream), "memset prof");
@@ -738,13 +739,14 @@ void launch(ttn_ctx* c, ttnd::DInst* d_insts, const std::vector<ttnd::DInst>& in
 // Algorithmic flops executed by an instance so far (the ttn_flops model)
 double measured_flops(const ttn_integ* in) {
     const int n = in->topo.size();
-    std::vector<int64_t> st(4 * n);
+    constexpr int S = ttnd::NODE_STATS;
+    std::vector<int64_t> st(S * n);
     hip_check(hipMemcpy(st.data(), in->node_stats.p, st.size() * sizeof(int64_t), hipMemcpyDeviceToHost), "stats");
     const FlopModel& f = in->fm;
     double tot = static_cast<double>(in->sweep_pairs) * f.sweep_pair + static_cast<double>(in->prepares) * f.prepare_spf;
     for (int i = 0; i < n; ++i) {
-        tot += static_cast<double>(st[4 * i]) * f.heff_node[i] + 32.0 * f.node_size[i] * static_cast<double>(st[4 * i + 1]);
-        tot += static_cast<double>(st[4 * i + 2]) * f.heff_bond[i] + 32.0 * f.bond_size[i] * static_cast<double>(st[4 * i + 3]);
+        tot += static_cast<double>(st[S * i]) * f.heff_node[i] + 32.0 * f.node_size[i] * static_cast<double>(st[S * i + 1]);
+        tot += static_cast<double>(st[S * i + 2]) * f.heff_bond[i] + 32.0 * f.bond_size[i] * static_cast<double>(st[S * i + 3]);
     }
     return tot;
 }
@@ -1896,7 +1898,8 @@ int ttn_profile(const ttn_integ* in, int64_t* slots) {
 int ttn_node_stats(const ttn_integ* in, int64_t* out) {
     return guard(in->ctx, [&] {
         const int n = in->topo.size();
-        hip_check(hipMemcpy(out, in->node_stats.p, 4 * n * sizeof(int64_t), hipMemcpyDeviceToHost), "stats");
+        hip_check(hipMemcpy(out, in->node_stats.p, ttnd::NODE_STATS * n * sizeof(int64_t), hipMemcpyDeviceToHost),
+                  "stats");
     });
 }
 

@@ -18,6 +18,11 @@ constexpr int NWORK = 6;       // generic work buffers, each max node size
 constexpr int MAX_NODES = 4096;  // nodes per tree (one byte of visit count each in LDS)
 
 // device wall-clock phase slots (100 MHz ticks, accumulated by thread 0)
+// entries per node of the per-node statistics: node-update H applications, their Krylov sizes
+// summed, bond-update H applications, their sizes, then the wall-clock ticks of the node and of
+// the bond propagations, and of the H-actions inside them (general path only; the small-vector
+// propagator's H-actions are not timed apart)
+constexpr int NODE_STATS = 8;
 enum ProfSlot { PROF_HEFF = 0, PROF_HBOND, PROF_ORTH, PROF_EXPM, PROF_QR, PROF_SPF, PROF_MF, PROF_GAUGE,
                 PROF_EXPV_NODE, PROF_EXPV_BOND, PROF_OBS, PROF_TOTAL,
                 PROF_SMALL,    // ticks in propagations run by the small-vector propagator
@@ -124,7 +129,7 @@ struct DInst {
     double* traj;                  // per step: re, im of every <O_j>, then <1> (2 nobs + 1 doubles),
                                    // or null
     int64_t* counters;             // [0] H applications (node updates), [1] steps done
-    int64_t* node_stats;           // per node: node-update H applications, their Krylov sizes
+    int64_t* node_stats;           // NODE_STATS per node: node-update H applications, their Krylov sizes
                                    // summed, bond-update applications (by operator node), sizes
     int* status;                   // [0]: failure record (0 = ok, see FailReason), [1]: step index
     int64_t* prof;                 // per-phase device wall-clock ticks (PROF_* slots), or null

@@ -206,16 +206,31 @@ __device__ void heff_node(Ctx& c, int n, const double2* v, double2* res) {
 
 // dst (=|+=) alpha * P Q on one workgroup, P and Q r x r matrices copied into LDS first, with
 // Q transposed when TRANS (Q^T: the mean-field product x mf^T).  The tiles and their k order
-// are those of t_mode_contract (L = 1) and t_mul_transposed, so the sums are the same.
+// are those of t_mode_contract (L = 1) and t_mul_transposed, so the sums are the same.  The copy
+// issues the 16-byte coherent loads of UNR strides of both matrices before storing any of them.
 template <bool TRANS>
 __device__ void wg_square_product_lds(const double2* P, const double2* Q, int r, double2* dst, double2 alpha,
                                       bool accumulate) {
     double2* a = reinterpret_cast<double2*>(&g_small);  // the whole 4608-entry area
     double2* b = a + LDS_STAGE;
     const int rr = r * r;
-    for (int e = threadIdx.x; e < rr; e += NT) {
-        a[e] = ldc(P + e);
-        b[e] = ldc(Q + e);
+    const Coh cp(P), cq(Q);
+    for (int e0 = threadIdx.x; e0 < rr; e0 += UNR * NT) {
+        double2 pv[UNR], qv[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const int e = e0 + u * NT;
+            pv[u] = e < rr ? cp[e] : cz();
+            qv[u] = e < rr ? cq[e] : cz();
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const int e = e0 + u * NT;
+            if (e < rr) {
+                a[e] = pv[u];
+                b[e] = qv[u];
+            }
+        }
     }
     __syncthreads();
     const int wv = threadIdx.x >> 6;
@@ -1119,7 +1134,7 @@ __device__ void node_update(Ctx& c, SweepState& st, int n) {
     const int size = c.shape(n).size;
     c.where = n;
     c.bond = 0;
-    int64_t na = 0, sumj = 0;
+    int64_t na = 0, sumj = 0, hticks = 0;
     const uint64_t pt0 = prof_now(c);
     if (small_eligible(c, n, false, size)) {
         // one workgroup (the leader, other members wait at the barrier that follows)
@@ -1128,15 +1143,26 @@ __device__ void node_update(Ctx& c, SweepState& st, int n) {
     } else {
         on_team(c, size, [&](Ctx& cc) {
             na = expv(
-                cc, [&](const double2* v, double2* res) { PROFILE(PROF_HEFF, heff_node(cc, n, v, res)); },
+                cc,
+                [&](const double2* v, double2* res) {
+                    const uint64_t h0 = prof_now(c);
+                    heff_node(cc, n, v, res);
+                    const uint64_t dh = prof_now(c) - h0;
+                    prof_add(c, PROF_HEFF, h0);
+                    hticks += static_cast<int64_t>(dh);
+                },
                 cc.base + nd.A, size, st.coeff, st.sub_dt / 2.0, sumj);
         });
     }
     prof_add(c, PROF_EXPV_NODE, pt0);
     st.nh += na;
     if (c.writer()) {
-        c.I.node_stats[4 * n] += na;
-        c.I.node_stats[4 * n + 1] += sumj;
+        c.I.node_stats[NODE_STATS * n] += na;
+        c.I.node_stats[NODE_STATS * n + 1] += sumj;
+        if (c.I.prof) {
+            c.I.node_stats[NODE_STATS * n + 4] += static_cast<int64_t>(prof_now(c) - pt0);
+            c.I.node_stats[NODE_STATS * n + 6] += hticks;
+        }
     }
 }
 
@@ -1144,7 +1170,7 @@ __device__ void bond_update(Ctx& c, SweepState& st, int n, int ops) {
     const int r = c.hr(ops);
     c.where = ops;
     c.bond = 1;
-    int64_t na = 0, sumj = 0;
+    int64_t na = 0, sumj = 0, hticks = 0;
     const uint64_t pt0 = prof_now(c);
     if (small_eligible(c, ops, true, static_cast<int64_t>(r) * r)) {
         on_team(c, 0, [&](Ctx& cc) {
@@ -1154,14 +1180,25 @@ __device__ void bond_update(Ctx& c, SweepState& st, int n, int ops) {
     } else {
         on_team(c, static_cast<int64_t>(r) * r, [&](Ctx& cc) {
             na = expv(
-                cc, [&](const double2* v, double2* res) { PROFILE(PROF_HBOND, heff_bond(cc, ops, v, res)); },
+                cc,
+                [&](const double2* v, double2* res) {
+                    const uint64_t h0 = prof_now(c);
+                    heff_bond(cc, ops, v, res);
+                    const uint64_t dh = prof_now(c) - h0;
+                    prof_add(c, PROF_HBOND, h0);
+                    hticks += static_cast<int64_t>(dh);
+                },
                 cc.base + cc.node(n).R, static_cast<int64_t>(r) * r, st.coeff, -st.sub_dt / 2.0, sumj);
         });
     }
     prof_add(c, PROF_EXPV_BOND, pt0);
     if (c.writer()) {
-        c.I.node_stats[4 * ops + 2] += na;
-        c.I.node_stats[4 * ops + 3] += sumj;
+        c.I.node_stats[NODE_STATS * ops + 2] += na;
+        c.I.node_stats[NODE_STATS * ops + 3] += sumj;
+        if (c.I.prof) {
+            c.I.node_stats[NODE_STATS * ops + 5] += static_cast<int64_t>(prof_now(c) - pt0);
+            c.I.node_stats[NODE_STATS * ops + 7] += hticks;
+        }
     }
 }
 
