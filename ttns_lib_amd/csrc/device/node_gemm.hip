@@ -252,31 +252,35 @@ __global__ __launch_bounds__(GT) void zreduce_kernel(GemmDesc g, int nsplit) {
 
 // ---------------------------------------------------------------- Krylov vector kernels
 // partial[blk][i] = sum over this block's entries of conj(V_i) w, i < m (two doubles each)
+template <int MB>
 __global__ __launch_bounds__(GT) void zdots_kernel(const double2* V, int64_t ld, int m, const double2* w, int64_t n,
                                                    double* partial) {
-    __shared__ double red[GT / 64][2 * MAXV];
-    double acc[2 * MAXV];
+    __shared__ double red[GT / 64][2 * MB];
+    double acc[2 * MB];
 #pragma unroll
-    for (int i = 0; i < 2 * MAXV; ++i) acc[i] = 0.0;
+    for (int i = 0; i < 2 * MB; ++i) acc[i] = 0.0;
     for (int64_t q = blockIdx.x * static_cast<int64_t>(GT) + threadIdx.x; q < n;
          q += static_cast<int64_t>(gridDim.x) * GT) {
+        // the m basis loads are issued before any is used (m <= MB, uniform)
         const double2 x = w[q];
+        double2 v[MB];
 #pragma unroll
-        for (int i = 0; i < MAXV; ++i) {
-            if (i < m) {
-                const double2 v = V[i * ld + q];
-                acc[2 * i] += v.x * x.x + v.y * x.y;
-                acc[2 * i + 1] += v.x * x.y - v.y * x.x;
-            }
+        for (int i = 0; i < MB; ++i) v[i] = i < m ? V[i * ld + q] : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int i = 0; i < MB; ++i) {
+            acc[2 * i] += v[i].x * x.x + v[i].y * x.y;
+            acc[2 * i + 1] += v[i].x * x.y - v[i].y * x.x;
         }
     }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-    for (int i = 0; i < 2 * MAXV; ++i) {
-        double s = acc[i];
+    for (int i = 0; i < 2 * MB; ++i) {
+        if (i < 2 * m) {
+            double s = acc[i];
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-        if (lane == 0) red[wv][i] = s;
+            for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+            if (lane == 0) red[wv][i] = s;
+        }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < 2 * m; i += GT) {
@@ -303,18 +307,29 @@ __global__ __launch_bounds__(GT) void zdots_finish_kernel(const double* partial,
 }
 
 // w -= sum_i h[i] V_i ; then partial norms of w per block
+template <int MB>
 __global__ __launch_bounds__(GT) void zorth_kernel(const double2* V, int64_t ld, int m, const double* h, double2* w,
                                                    int64_t n, double* partial) {
     __shared__ double red[GT / 64];
+    double hr[MB], hi[MB];
+#pragma unroll
+    for (int i = 0; i < MB; ++i) {
+        hr[i] = i < m ? h[2 * i] : 0.0;
+        hi[i] = i < m ? h[2 * i + 1] : 0.0;
+    }
     double nn = 0.0;
     for (int64_t q = blockIdx.x * static_cast<int64_t>(GT) + threadIdx.x; q < n;
          q += static_cast<int64_t>(gridDim.x) * GT) {
         double2 x = w[q];
-        for (int i = 0; i < m; ++i) {
-            const double2 v = V[i * ld + q];
-            const double hr = h[2 * i], hi = h[2 * i + 1];
-            x.x -= hr * v.x - hi * v.y;
-            x.y -= hr * v.y + hi * v.x;
+        double2 v[MB];
+#pragma unroll
+        for (int i = 0; i < MB; ++i) v[i] = i < m ? V[i * ld + q] : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int i = 0; i < MB; ++i) {
+            if (i < m) {
+                x.x -= hr[i] * v[i].x - hi[i] * v[i].y;
+                x.y -= hr[i] * v[i].y + hi[i] * v[i].x;
+            }
         }
         w[q] = x;
         nn += x.x * x.x + x.y * x.y;
@@ -574,14 +589,26 @@ extern "C" hipError_t ttnd_zgemm(const ttnd::GemmDesc* g, int nsplit, int tile, 
 extern "C" hipError_t ttnd_zdots(const double2* V, int64_t ld, int m, const double2* w, int64_t n, double* partial,
                                  int nblk, double* h, hipStream_t s) {
     using namespace ttnd;
-    hipLaunchKernelGGL(zdots_kernel, dim3(nblk), dim3(GT), 0, s, V, ld, m, w, n, partial);
+    if (m < 1 || m > MAXV) return hipErrorInvalidValue;
+    if (m <= 8)
+        hipLaunchKernelGGL(zdots_kernel<8>, dim3(nblk), dim3(GT), 0, s, V, ld, m, w, n, partial);
+    else if (m <= 16)
+        hipLaunchKernelGGL(zdots_kernel<16>, dim3(nblk), dim3(GT), 0, s, V, ld, m, w, n, partial);
+    else
+        hipLaunchKernelGGL(zdots_kernel<MAXV>, dim3(nblk), dim3(GT), 0, s, V, ld, m, w, n, partial);
     hipLaunchKernelGGL(zdots_finish_kernel, dim3(2 * m), dim3(GT), 0, s, partial, nblk, h);
     return hipGetLastError();
 }
 extern "C" hipError_t ttnd_zorth(const double2* V, int64_t ld, int m, const double* h, double2* w, int64_t n,
                                  double* partial, int nblk, hipStream_t s) {
     using namespace ttnd;
-    hipLaunchKernelGGL(zorth_kernel, dim3(nblk), dim3(GT), 0, s, V, ld, m, h, w, n, partial);
+    if (m < 1 || m > MAXV) return hipErrorInvalidValue;
+    if (m <= 8)
+        hipLaunchKernelGGL(zorth_kernel<8>, dim3(nblk), dim3(GT), 0, s, V, ld, m, h, w, n, partial);
+    else if (m <= 16)
+        hipLaunchKernelGGL(zorth_kernel<16>, dim3(nblk), dim3(GT), 0, s, V, ld, m, h, w, n, partial);
+    else
+        hipLaunchKernelGGL(zorth_kernel<MAXV>, dim3(nblk), dim3(GT), 0, s, V, ld, m, h, w, n, partial);
     return hipGetLastError();
 }
 extern "C" hipError_t ttnd_zscale(double2* y, const double2* x, double ar, double ai, int64_t n, int nblk,

@@ -395,15 +395,20 @@ int64_t node_expv(ttn_node* nd, cplx coeff, double tau, int kdim, double tol) {
         for (int j = 0; j < kdim; ++j) {
             heff(nd, V + j * ld, w);
             ++napply;
+            // CGS2: both passes on the device (coefficients of pass p at h + 2 MAXV p), then one
+            // read-back of both coefficient sets and the partial norms of the final w
             for (int pass = 0; pass < 2; ++pass) {
-                hip_check(ttnd_zdots(V, ld, j + 1, w, n, nd->part.p, nb, nd->h.p, s), "dots");
-                hip_check(ttnd_zorth(V, ld, j + 1, nd->h.p, w, n, nd->part.p, nb, s), "orth");
-                hip_check(hipMemcpyAsync(hv.data(), nd->h.p, 2 * (j + 1) * sizeof(double), hipMemcpyDeviceToHost, s),
-                          "h");
-                const double nn = sum_partials(nd, nb);  // synchronises the stream
-                for (int i = 0; i <= j; ++i) Hm[i * kdim + j] += cplx(hv[2 * i], hv[2 * i + 1]);
-                hn = std::sqrt(nn);
+                double* hp = nd->h.p + 2 * ttnd::MAXV * pass;
+                hip_check(ttnd_zdots(V, ld, j + 1, w, n, nd->part.p, nb, hp, s), "dots");
+                hip_check(ttnd_zorth(V, ld, j + 1, hp, w, n, nd->part.p, nb, s), "orth");
             }
+            hip_check(hipMemcpyAsync(hv.data(), nd->h.p, 4 * ttnd::MAXV * sizeof(double), hipMemcpyDeviceToHost, s),
+                      "h");
+            const double nn = sum_partials(nd, nb);  // synchronises the stream
+            for (int pass = 0; pass < 2; ++pass)
+                for (int i = 0; i <= j; ++i)
+                    Hm[i * kdim + j] += cplx(hv[2 * ttnd::MAXV * pass + 2 * i], hv[2 * ttnd::MAXV * pass + 2 * i + 1]);
+            hn = std::sqrt(nn);
             check(std::isfinite(hn), ERR_NUMERIC, "node expv: non-finite Krylov vector");
             if (j + 1 < kdim) Hm[(j + 1) * kdim + j] = hn;
             m = j + 1;
