@@ -117,10 +117,11 @@ def cpu_baseline(lib, insts, flops_per_step, budget_s):
         if i == heavy or (used and time.perf_counter() - t_start > budget_s / 2):
             break
         one(i)
-    rate = f_cpu / t_cpu
-    if 2.3 * flops_per_step[heavy] / rate < budget_s - (time.perf_counter() - t_start):
+    # a single instance (or a batch whose lightest instance is already the median) is timed
+    # as it is; otherwise the heavy one joins when the rate so far says it fits the budget
+    if not used or 2.3 * flops_per_step[heavy] / (f_cpu / t_cpu) < budget_s - (time.perf_counter() - t_start):
         one(heavy)
-        rate = f_cpu / t_cpu
+    rate = f_cpu / t_cpu
     total_time = float(np.sum(flops_per_step)) / rate  # one batched step on one core
     return {
         "value": len(insts) / total_time,

@@ -215,7 +215,7 @@ __device__ bool wg_eigs(Ctx& c, FF&& F, double2* x, int64_t m, int kd, double to
                             p[2 * i + 1] += pr.y;
                         }
                 }
-                block_sum_lds<2 * MAXK>(p, g_sh.hb[pass]);
+                block_sum_lds<2 * MAXK>(p, g_sh.hb[pass], 2 * (j + 1));
                 const double* hp = g_sh.hb[pass];
                 for (int64_t q = threadIdx.x; q < m; q += NT) {
                     double2 wq = ldc(w + q);
@@ -225,7 +225,8 @@ __device__ bool wg_eigs(Ctx& c, FF&& F, double2* x, int64_t m, int kd, double to
                 __builtin_amdgcn_s_waitcnt(0);
                 __syncthreads();
 #pragma unroll
-                for (int i = 0; i < 2 * MAXK; ++i) h[i] += hp[i];
+                for (int i = 0; i < 2 * MAXK; ++i)
+                    if (i < 2 * (j + 1)) h[i] += hp[i];
                 __syncthreads();
             }
             const double bj = sqrt(wg_norm2(w, m));

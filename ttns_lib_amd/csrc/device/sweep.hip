@@ -516,7 +516,7 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
                     }
                 }
             }
-            team_sum_lds<2 * MAXK>(T, p, g_sh.hb[0]);
+            team_sum_lds<2 * MAXK>(T, p, g_sh.hb[0], 2 * (j + 1));
 #pragma unroll
             for (int i = 0; i < 2 * MAXK; ++i) p[i] = 0.0;
             for (int64_t q = q00; q < n; q += st) {
@@ -538,7 +538,7 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
                     }
                 }
             }
-            team_sum_lds<2 * MAXK>(T, p, g_sh.hb[1]);
+            team_sum_lds<2 * MAXK>(T, p, g_sh.hb[1], 2 * (j + 1));
             double nn[1] = {0.0}, nno[1];
             for (int64_t q0 = q00; q0 < n; q0 += 2 * st) {
                 const int64_t q1 = q0 + st;
@@ -1024,7 +1024,7 @@ __device__ int64_t expv_small(Ctx& c, int n_op, bool bond, double2* x, int n, do
                             p[2 * i] += pr.x;
                             p[2 * i + 1] += pr.y;
                         }
-                block_sum_lds<2 * KM>(p, g_sh.hb[pass]);
+                block_sum_lds<2 * KM>(p, g_sh.hb[pass], 2 * (j + 1));
                 const double* h = g_sh.hb[pass];
 #pragma unroll
                 for (int i = 0; i < KM; ++i)

@@ -95,17 +95,20 @@ __device__ void block_sum(double (&v)[K], double (&out)[K]) {
     __syncthreads();
 }
 
-// sum of K doubles over the workgroup, totals left in the LDS array dst[K]
+// sum of the first cnt (uniform, <= K) of K doubles over the workgroup, totals left in the LDS
+// array dst[cnt]; the others are skipped
 template <int K>
-__device__ void block_sum_lds(double (&v)[K], double* dst) {
+__device__ void block_sum_lds(double (&v)[K], double* dst, int cnt = K) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const double s = wave_sum(v[k]);
-        if (lane == 0) g_sh.red[wv * K + k] = s;
+        if (k < cnt) {
+            const double s = wave_sum(v[k]);
+            if (lane == 0) g_sh.red[wv * K + k] = s;
+        }
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < K; k += NT) {
+    for (int k = threadIdx.x; k < cnt; k += NT) {
         double s = 0.0;
         for (int w = 0; w < NWAVE; ++w) s += g_sh.red[w * K + k];
         dst[k] = s;
@@ -276,11 +279,11 @@ __device__ void team_sum(Team& T, double (&v)[K], double (&out)[K]) {
     __syncthreads();
 }
 
-// team sum of K doubles, totals left in the LDS array dst[K]
+// team sum of the first cnt (uniform, <= K) of K doubles, totals left in the LDS array dst[cnt]
 template <int K>
-__device__ void team_sum_lds(Team& T, double (&v)[K], double* dst) {
-    block_sum_lds<K>(v, dst);
-    team_reduce_lds(T, dst, K);
+__device__ void team_sum_lds(Team& T, double (&v)[K], double* dst, int cnt = K) {
+    block_sum_lds<K>(v, dst, cnt);
+    team_reduce_lds(T, dst, cnt);
 }
 
 __device__ double team_max(Team& T, double v) {
