@@ -69,9 +69,28 @@ struct Shared {
 // reference passed between non-inlined functions it would be a flat access).
 __shared__ Shared g_sh;
 
+// v from another lane of the same row of 16 by a DPP lane move (a VALU operation, no LDS
+// round trip), one 32-bit half at a time
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint32_t>(b)), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint32_t>(b >> 32)), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) | static_cast<uint32_t>(lo));
+}
+
+// Sum over the wave, every lane receives it.  Butterfly: lane ^ 1 and lane ^ 2 (quad
+// permutations), then the other quad of the half-row and the other half of the row (half-row and
+// row mirrors: after the previous steps every lane of a quad, then of a half-row, holds the same
+// value, so a mirror pairs the same partial sums as lane ^ 4 and lane ^ 8), then lane ^ 16 and
+// lane ^ 32 across rows through ds_bpermute.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    v += dpp_move<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+    v += dpp_move<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+    v += dpp_move<0x141>(v);  // row_half_mirror
+    v += dpp_move<0x140>(v);  // row_mirror
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
     return v;
 }
 
