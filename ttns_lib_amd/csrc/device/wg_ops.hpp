@@ -762,17 +762,22 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-// max over the wave with the lowest lane winning ties
-__device__ __forceinline__ void wave_argmax(double& v, int& idx) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const double ov = __shfl_xor(v, o, 64);
-        const int oi = __shfl_xor(idx, o, 64);
-        if (ov > v || (ov == v && oi < idx)) {
-            v = ov;
-            idx = oi;
-        }
+__device__ __forceinline__ void argmax_take(double& v, int& idx, double ov, int oi) {
+    if (ov > v || (ov == v && oi < idx)) {
+        v = ov;
+        idx = oi;
     }
+}
+
+// max over the wave with the lowest index winning ties (for finite values the result does not
+// depend on the order of the pairings); the same butterfly as wave_sum
+__device__ __forceinline__ void wave_argmax(double& v, int& idx) {
+    argmax_take(v, idx, dpp_move<0xB1>(v), __builtin_amdgcn_update_dpp(0, idx, 0xB1, 0xF, 0xF, false));
+    argmax_take(v, idx, dpp_move<0x4E>(v), __builtin_amdgcn_update_dpp(0, idx, 0x4E, 0xF, 0xF, false));
+    argmax_take(v, idx, dpp_move<0x141>(v), __builtin_amdgcn_update_dpp(0, idx, 0x141, 0xF, 0xF, false));
+    argmax_take(v, idx, dpp_move<0x140>(v), __builtin_amdgcn_update_dpp(0, idx, 0x140, 0xF, 0xF, false));
+    argmax_take(v, idx, __shfl_xor(v, 16, 64), __shfl_xor(idx, 16, 64));
+    argmax_take(v, idx, __shfl_xor(v, 32, 64), __shfl_xor(idx, 32, 64));
 }
 
 // one entry of C = A B per thread (n <= 16), summed over q in ascending order as in
