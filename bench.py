@@ -257,13 +257,16 @@ def bench_c2(args):
     node0.destroy()
     lib.set_autotune(True)
     # the two forms of the H-action's mode-0 and mean-field stages, then the autotuner's pick
-    forms = {}
+    forms, mf_forms = {}, {}
     for form, name in ((1, "staged"), (2, "fused")):
         lib.set_heff_form(form)
         nf = Node(lib, op0, op1, mf)
         nf.set(A)
         nf.heff_timed(max(1, args.warmup))
         forms[name] = nf.heff_timed(args.steps)[0] / args.steps
+        nf.mf(0, reps=1)
+        nf.mf(0, reps=args.steps)
+        mf_forms[name] = nf.last()[0] / args.steps
         nf.destroy()
     lib.set_heff_form(0)
     fused = forms["fused"] < forms["staged"]
@@ -312,7 +315,7 @@ def bench_c2(args):
                      "flops_per_heff": fl / args.steps},
         "autotune": {"heff_ms_fixed_64x64": ms0 / args.steps, "heff_ms_tuned": heff_ms, "table": tuned,
                      "heff_ms_by_form": forms},
-        "mean_field": {"ms": mf_ms / args.steps, "tflops": mf_fl / (mf_ms / 1e3) / 1e12},
+        "mean_field": {"ms": mf_ms / args.steps, "tflops": mf_fl / (mf_ms / 1e3) / 1e12, "ms_by_form": mf_forms},
         "krylov16": {"ms": kms, "tau": tau, "h_applications": na, "tflops": kfl / (kms / 1e3) / 1e12,
                      "ms_per_iteration": kms / max(na, 1)},
     }

@@ -331,13 +331,13 @@ int ttn_node_last(const ttn_node* node, double* ms, double* flops);
  * batch, K block, tile (0-5 in the order above), K split, and its time in ms. */
 int ttn_ctx_set_autotune(ttn_ctx* ctx, int enabled);
 int ttn_autotune_table(const ttn_ctx* ctx, int64_t* n, int64_t* entries, double* ms);
-/* Form of the node-level H-action's last two stages (no counterpart in the reference; the
- * reference contracts term by term, coefficient_evolution_branch,
- * projector_splitting_evolution_functions.hpp:120-258).  0 (default): with the autotuner on,
- * the first call on a shape times both forms and keeps the faster; 1: staged, three batched
- * GEMMs with the per-term intermediates O0_t x_0 (O1_t x_1 A) in memory; 2: fused where
- * d0 <= 64 and r <= 64, one kernel that keeps each term's intermediate slice in LDS and sums the
- * mean-field products in registers.  The forms differ only in the order of rounding. */
+/* Form of the last two stages of the node-level H-action and mean-field update (no
+ * counterpart in the reference; the reference contracts term by term, coefficient_evolution_branch,
+ * projector_splitting_evolution_functions.hpp:120-258, mean_field_operator.hpp:37-184).
+ * 0 (default): with the autotuner on, the first call on a shape times both forms and keeps the
+ * faster; 1: staged, batched GEMMs with the per-term intermediates in memory; 2: fused where the
+ * child dimension and r are at most 64, one kernel that keeps each slab of a term's intermediate
+ * in LDS and sums the products in registers.  The forms differ only in the order of rounding. */
 int ttn_ctx_set_heff_form(ttn_ctx* ctx, int form);
 
 /* No counterpart in the reference.  For callers that cannot allocate device memory (code

@@ -52,6 +52,32 @@ def test_node_heff_forms_match_oracle(hip, orc, d0, d1, r, nt):
     assert _rel(outs[2], outs[1]) < 1e-14
 
 
+@pytest.mark.parametrize("d0,d1,r,nt", [(16, 16, 16, 5), (20, 12, 24, 3), (40, 50, 36, 7), (64, 64, 64, 40),
+                                         (64, 8, 64, 3)])
+@pytest.mark.parametrize("child", [0, 1])
+def test_node_mean_field_forms_match_oracle(hip, orc, d0, d1, r, nt, child):
+    """The staged mean-field update (three batched GEMMs) and the fused one (one kernel over the
+    other child's slabs, the slab's Z in LDS, chunks summed in order) against the oracle, each
+    repeating itself bit for bit."""
+    print(f"[node mf forms] {d0}x{d1}x{r} terms {nt} child {child}", flush=True)
+    P = NodeProblem(orc, d0, d1, r, nt, seed=d0 + 2 * nt + child)
+    want = P.mf_child(child)
+    outs = {}
+    try:
+        for form in (1, 2):
+            hip.set_heff_form(form)
+            g = Node(hip, P.op0, P.op1, P.mf)
+            g.set(P.v)
+            outs[form] = g.mf(child)
+            assert np.array_equal(outs[form], g.mf(child)), form
+            g.destroy()
+    finally:
+        hip.set_heff_form(0)
+    for form, out in outs.items():
+        assert _rel(out, want) < 1e-13, form
+    assert _rel(outs[2], outs[1]) < 1e-14
+
+
 @pytest.mark.parametrize("d0,d1,r,nt", [(16, 16, 16, 5), (20, 12, 24, 3), (64, 64, 64, 40)])
 @pytest.mark.parametrize("child", [0, 1])
 def test_node_mean_fields_match_oracle(hip, orc, d0, d1, r, nt, child):

@@ -504,6 +504,112 @@ __global__ __launch_bounds__(GT) void zheff_fused_kernel(FusedDesc f) {
     }
 }
 
+// The mean-field update's last two stages fused (MfDesc).  One workgroup per (chunk of slabs,
+// term).  Per slab, Z_s = Y_s MF_t^T is formed with MFMAs whose operands come straight from L2 (Y_s
+// rows and MF_t rows), written to LDS transposed (c-major) once every wave has finished the previous
+// slab, and conj(A_s) Z_s^T accumulates in registers over the chunk's slabs.  Waves own up to four
+// 16 x 16 tiles of each product and advance them together per k-step.
+__global__ __launch_bounds__(GT, 2) void zmf_fused_kernel(MfDesc f) {
+    __shared__ double zr[64][FL], zi[64][FL];  // Z_s^T at [c][j]
+    const int g = blockIdx.x, ng = gridDim.x, t = blockIdx.y;
+    const int s0 = static_cast<int>(static_cast<int64_t>(f.ns) * g / ng);
+    const int s1 = static_cast<int>(static_cast<int64_t>(f.ns) * (g + 1) / ng);
+    const int dc = f.dc, r = f.r;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, li = lane & 15, kq = lane >> 4;
+    const int md = (dc + 15) >> 4, mr = (r + 15) >> 4;
+    const int tz = md * mr, to = md * md;  // tiles of Z_s (dc x r) and of out (dc x dc)
+    const int cz = tz > wv ? min(4, (tz - wv + 3) / 4) : 0;
+    const int co = to > wv ? min(4, (to - wv + 3) / 4) : 0;
+    int zm[4], zn[4], om[4], on[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int it = wv + 4 * q;
+        zm[q] = it / mr;
+        zn[q] = it % mr;
+        om[q] = it / md;
+        on[q] = it % md;
+    }
+    Acc acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        acc[q].re = d4{0.0, 0.0, 0.0, 0.0};
+        acc[q].im = d4{0.0, 0.0, 0.0, 0.0};
+    }
+    const double2* M = f.mf + static_cast<int64_t>(t) * r * r;
+    for (int s = s0; s < s1; ++s) {
+        // Z_s(j, c) = sum_p Y_s(j, p) MF_t(c, p)
+        const double2* Ys = f.Y + static_cast<int64_t>(t) * f.size + s * f.ss;
+        Acc z[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            z[q].re = d4{0.0, 0.0, 0.0, 0.0};
+            z[q].im = d4{0.0, 0.0, 0.0, 0.0};
+        }
+        // the operands of two k-steps are loaded before their MFMAs
+        for (int k0 = 0; k0 < r; k0 += 8) {
+            double2 a[2][4], b[2][4];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int k = k0 + 4 * u + kq;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int row = 16 * zm[q] + li, col = 16 * zn[q] + li;
+                    a[u][q] = (q < cz && row < dc && k < r) ? Ys[row * f.si + k] : make_double2(0.0, 0.0);
+                    b[u][q] = (q < cz && col < r && k < r) ? M[static_cast<int64_t>(col) * r + k] : make_double2(0.0, 0.0);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (q < cz) mfma_c(z[q], a[u][q].x, a[u][q].y, b[u][q].x, b[u][q].y);
+        }
+        __syncthreads();  // the previous slab's readers of the LDS planes are done
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (q < cz)
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) {
+                    zr[16 * zn[q] + li][16 * zm[q] + kq + 4 * jj] = z[q].re[jj];
+                    zi[16 * zn[q] + li][16 * zm[q] + kq + 4 * jj] = z[q].im[jj];
+                }
+        __syncthreads();
+        // out(i, j) += sum_c conj(A_s(i, c)) Z_s(j, c)
+        const double2* As = f.A + s * f.ss;
+        for (int k0 = 0; k0 < r; k0 += 8) {
+            double2 a[2][4];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int k = k0 + 4 * u + kq;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int row = 16 * om[q] + li;
+                    a[u][q] = (q < co && row < dc && k < r) ? As[row * f.si + k] : make_double2(0.0, 0.0);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int k = k0 + 4 * u + kq;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (q < co) mfma_c(acc[q], a[u][q].x, -a[u][q].y, zr[k][16 * on[q] + li], zi[k][16 * on[q] + li]);
+            }
+        }
+    }
+    double2* out = ng > 1 ? f.W + static_cast<int64_t>(g) * f.w_split : f.out;
+    out += static_cast<int64_t>(t) * dc * dc;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (q >= co) break;
+        const int col = 16 * on[q] + li;
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+            const int row = 16 * om[q] + kq + 4 * jj;
+            if (row < dc && col < dc) out[row * dc + col] = make_double2(acc[q].re[jj], acc[q].im[jj]);
+        }
+    }
+}
+
 // out[t](p, j) = coeff[t] * mf[t](j, p): the mean-field factors of the term-blocked sum
 __global__ __launch_bounds__(GT) void zmft_kernel(const double2* coeff, const double2* mf, int64_t nt, int64_t r,
                                                   double2* out) {
@@ -525,6 +631,31 @@ extern "C" hipError_t ttnd_zmft(const double2* coeff, const double2* mf, int64_t
     const int64_t total = nt * r * r;
     const int blocks = static_cast<int>(total / GT + 1 < 1024 ? total / GT + 1 : 1024);
     hipLaunchKernelGGL(zmft_kernel, dim3(blocks), dim3(GT), 0, s, coeff, mf, nt, r, out);
+    return hipGetLastError();
+}
+extern "C" hipError_t ttnd_zmf_fused(const ttnd::MfDesc* f, int nchunks, hipStream_t s) {
+    using namespace ttnd;
+    if (f->dc > 64 || f->r > 64 || f->dc < 1 || f->r < 1 || nchunks < 1 || nchunks > f->ns) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(zmf_fused_kernel, dim3(nchunks, f->nt), dim3(GT), 0, s, *f);
+    if (nchunks > 1) {
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        GemmDesc g{};
+        g.M = g.N = f->dc;
+        g.nb0 = f->nt;
+        g.nb1 = 1;
+        g.W = f->W;
+        g.w_split = f->w_split;
+        g.w_batch = static_cast<int64_t>(f->dc) * f->dc;
+        g.C = f->out;
+        g.c_b0 = static_cast<int64_t>(f->dc) * f->dc;
+        g.c_r = f->dc;
+        g.c_c = 1;
+        g.alpha = make_double2(1.0, 0.0);
+        const int64_t total = static_cast<int64_t>(f->dc) * f->dc * f->nt;
+        const int blocks = static_cast<int>(total / GT + 1 < 4096 ? total / GT + 1 : 4096);
+        hipLaunchKernelGGL(zreduce_kernel, dim3(blocks), dim3(GT), 0, s, g, nchunks);
+    }
     return hipGetLastError();
 }
 extern "C" hipError_t ttnd_zheff_fused(const ttnd::FusedDesc* f, int ngroups, hipStream_t s) {

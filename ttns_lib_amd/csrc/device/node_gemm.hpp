@@ -51,4 +51,20 @@ struct FusedDesc {
     int64_t w_split;
 };
 
+// The fused last two stages of a child's mean-field update (zmf_fused_kernel), dc, r <= 64:
+//   out[t](i, j) = sum_s sum_c conj(A[s, i, c]) sum_p Y[t][s, j, p] MF[t](c, p)
+// over the slabs s of the other child's index; element (s, i, c) of A and of each Y[t] at
+// s * ss + i * si + c.  chunks > 1 write partial sums to W (w_split entries apart, nt dc x dc
+// each) and sum them in chunk order.
+struct MfDesc {
+    int dc, ns, r, nt;
+    int64_t ss, si, size;
+    const double2* Y;
+    const double2* A;
+    const double2* mf;
+    double2* out;
+    double2* W;
+    int64_t w_split;
+};
+
 }  // namespace ttnd

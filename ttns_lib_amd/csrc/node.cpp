@@ -23,6 +23,7 @@
 
 extern "C" hipError_t ttnd_zgemm(const ttnd::GemmDesc* g, int nsplit, int tile, hipStream_t s);
 extern "C" hipError_t ttnd_zheff_fused(const ttnd::FusedDesc* f, int ngroups, hipStream_t s);
+extern "C" hipError_t ttnd_zmf_fused(const ttnd::MfDesc* f, int nchunks, hipStream_t s);
 extern "C" hipError_t ttnd_zdots(const double2* V, int64_t ld, int m, const double2* w, int64_t n, double* partial,
                                  int nblk, double* h, hipStream_t s);
 extern "C" hipError_t ttnd_zorth(const double2* V, int64_t ld, int m, const double* h, double2* w, int64_t n,
@@ -329,6 +330,122 @@ void heff(const NodeView& v, const double2* x, double2* res) {
 }
 void heff(ttn_node* nd, const double2* x, double2* res) { heff(view(nd), x, res); }
 
+// The mean fields of child `child` for every term, res[t] (dc x dc): the other child's operator
+// on A (stage 1, batched GEMM), then either two more batched GEMMs (Z_t = Y_t MF_t^T to memory,
+// then the long-K product with conj(A)) or the fused kernel that keeps each slab's Z_t in LDS.
+void mean_field(ttn_node* nd, int child, double2* res, bool fused) {
+    const NodeView v = view(nd);
+    if (child == 0)
+        stage_mode1(v, nd->A.p);
+    else
+        stage_mode0(v, nd->A.p, 0, nd->Y.p);
+    const int64_t dc = child == 0 ? nd->d0 : nd->d1, ns = child == 0 ? nd->d1 : nd->d0;
+    if (fused) {
+        ttnd::MfDesc f{};
+        f.dc = static_cast<int>(dc);
+        f.ns = static_cast<int>(ns);
+        f.r = static_cast<int>(nd->r);
+        f.nt = static_cast<int>(nd->nt);
+        f.ss = child == 0 ? nd->r : nd->d1 * nd->r;
+        f.si = child == 0 ? nd->d1 * nd->r : nd->r;
+        f.size = nd->size;
+        f.Y = nd->Y.p;
+        f.A = nd->A.p;
+        f.mf = nd->mf.p;
+        f.out = res;
+        f.W = v.ws;
+        f.w_split = nd->nt * dc * dc;
+        // chunks of slabs so that chunks x terms fills the GPU at two workgroups per CU, fewer
+        // when their partial sums would not fit the workspace
+        int chunks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ns, (512 + nd->nt - 1) / nd->nt)));
+        while (chunks > 1 && chunks * f.w_split > v.ws_n) --chunks;
+        hip_check(ttnd_zmf_fused(&f, chunks, v.s), "fused mean field");
+        return;
+    }
+    ttnd::GemmDesc g = desc0();
+    g.M = static_cast<int>(nd->D);
+    g.N = static_cast<int>(nd->r);
+    g.K = static_cast<int>(nd->r);
+    g.nb0 = static_cast<int>(nd->nt);
+    g.A = nd->Y.p;
+    g.a_b0 = nd->size;
+    g.a_r = nd->r;
+    g.a_k = 1;
+    g.B = nd->mf.p;  // B(p, j) = MF_t(j, p)
+    g.b_b0 = nd->r * nd->r;
+    g.b_k = 1;
+    g.b_c = nd->r;
+    g.C = nd->Z.p;
+    g.c_b0 = nd->size;
+    g.c_r = nd->r;
+    g.c_c = 1;
+    run(v, g);
+    // mf_t(i, j) = sum over the other indices of conj(A)(.., i, ..) Z_t(.., j, ..)
+    ttnd::GemmDesc m = desc0();
+    m.M = m.N = static_cast<int>(dc);
+    m.nb0 = static_cast<int>(nd->nt);
+    m.conja = 1;
+    m.A = nd->A.p;
+    m.B = nd->Z.p;
+    m.b_b0 = nd->size;
+    if (child == 0) {
+        m.K = static_cast<int>(nd->d1 * nd->r);
+        m.a_r = nd->d1 * nd->r;
+        m.a_k = 1;
+        m.b_k = 1;
+        m.b_c = nd->d1 * nd->r;
+    } else {
+        m.K = static_cast<int>(nd->d0 * nd->r);
+        m.kb = static_cast<int>(nd->r);
+        m.a_r = nd->r;
+        m.a_k = 1;
+        m.a_kblk = nd->d1 * nd->r;
+        m.b_k = 1;
+        m.b_c = nd->r;
+        m.b_kblk = nd->d1 * nd->r;
+    }
+    m.C = res;
+    m.c_b0 = dc * dc;
+    m.c_r = dc;
+    m.c_c = 1;
+    run(v, m);
+}
+
+// Whether the mean fields of this child and shape run fused: fixed by ttn_ctx_set_heff_form, or
+// timed once per shape (both forms into a scratch result) when the autotuner is on.
+bool use_fused_mf(ttn_node* nd, int child) {
+    ttn_ctx* ctx = nd->ctx;
+    const int64_t dc = child == 0 ? nd->d0 : nd->d1;
+    if (dc > 64 || nd->r > 64 || ctx->heff_form == 1) return false;
+    if (ctx->heff_form == 2) return true;
+    if (!ctx->autotune) return false;
+    const std::array<int64_t, 5> key = {child, nd->d0, nd->d1, nd->r, nd->nt};
+    auto it = ctx->fuse_mf.find(key);
+    if (it == ctx->fuse_mf.end()) {
+        DevBuf<double2> scratch;
+        scratch.alloc(static_cast<size_t>(nd->nt * dc * dc));
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "event");
+        hip_check(hipEventCreate(&e1), "event");
+        float best[2] = {1e30f, 1e30f};
+        for (int form = 0; form < 2; ++form)
+            for (int rep = 0; rep < 4; ++rep) {
+                hip_check(hipEventRecord(e0, ctx->stream), "event");
+                mean_field(nd, child, scratch.p, form == 1);
+                hip_check(hipEventRecord(e1, ctx->stream), "event");
+                hip_check(hipEventSynchronize(e1), "sync");
+                float ms = 0.f;
+                hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+                if (rep > 0) best[form] = std::min(best[form], ms);
+            }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        const int form = best[1] < best[0] ? 1 : 0;
+        it = ctx->fuse_mf.emplace(key, std::array<double, 2>{static_cast<double>(form), best[form]}).first;
+    }
+    return it->second[0] > 0.5;
+}
+
 double heff_flops(const ttn_node* nd) {
     return 8.0 * static_cast<double>(nd->nt) * static_cast<double>(nd->size) *
            static_cast<double>(nd->d0 + nd->d1 + nd->r);
@@ -586,61 +703,9 @@ int ttn_node_mf(ttn_node* nd, int64_t child, int64_t reps, double* out) {
         const int64_t dc = child == 0 ? nd->d0 : nd->d1;
         DevBuf<double2> res;
         res.alloc(static_cast<size_t>(nd->nt * dc * dc));
+        const bool fused = use_fused_mf(nd, static_cast<int>(child));
         Timer tm(nd);
-        for (int64_t k = 0; k < reps; ++k) {
-            // the other child's operator on A, then MF_t^T: Z_t
-            if (child == 0)
-                stage_mode1(view(nd), nd->A.p);
-            else
-                stage_mode0(view(nd), nd->A.p, 0, nd->Y.p);
-            ttnd::GemmDesc g = desc0();
-            g.M = static_cast<int>(nd->D);
-            g.N = static_cast<int>(nd->r);
-            g.K = static_cast<int>(nd->r);
-            g.nb0 = static_cast<int>(nd->nt);
-            g.A = nd->Y.p;
-            g.a_b0 = nd->size;
-            g.a_r = nd->r;
-            g.a_k = 1;
-            g.B = nd->mf.p;  // B(p, j) = MF_t(j, p)
-            g.b_b0 = nd->r * nd->r;
-            g.b_k = 1;
-            g.b_c = nd->r;
-            g.C = nd->Z.p;
-            g.c_b0 = nd->size;
-            g.c_r = nd->r;
-            g.c_c = 1;
-            run(nd, g);
-            // mf_t(i, j) = sum over the other indices of conj(A)(.., i, ..) Z_t(.., j, ..)
-            ttnd::GemmDesc m = desc0();
-            m.M = m.N = static_cast<int>(dc);
-            m.nb0 = static_cast<int>(nd->nt);
-            m.conja = 1;
-            m.A = nd->A.p;
-            m.B = nd->Z.p;
-            m.b_b0 = nd->size;
-            if (child == 0) {
-                m.K = static_cast<int>(nd->d1 * nd->r);
-                m.a_r = nd->d1 * nd->r;
-                m.a_k = 1;
-                m.b_k = 1;
-                m.b_c = nd->d1 * nd->r;
-            } else {
-                m.K = static_cast<int>(nd->d0 * nd->r);
-                m.kb = static_cast<int>(nd->r);
-                m.a_r = nd->r;
-                m.a_k = 1;
-                m.a_kblk = nd->d1 * nd->r;
-                m.b_k = 1;
-                m.b_c = nd->r;
-                m.b_kblk = nd->d1 * nd->r;
-            }
-            m.C = res.p;
-            m.c_b0 = dc * dc;
-            m.c_r = dc;
-            m.c_c = 1;
-            run(nd, m);
-        }
+        for (int64_t k = 0; k < reps; ++k) mean_field(nd, static_cast<int>(child), res.p, fused);
         tm.stop();
         nd->last_flops = static_cast<double>(reps) * 8.0 * static_cast<double>(nd->nt) *
                          static_cast<double>(nd->size) * static_cast<double>((child == 0 ? nd->d1 : nd->d0) + nd->r + dc);

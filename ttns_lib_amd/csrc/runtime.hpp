@@ -70,6 +70,7 @@ struct ttn_ctx {
     // for every node H-action shape (d0, d1, r, terms): whether the mode-0 and mean-field
     // stages run fused (1) or staged (0), with the faster form's time in ms
     std::map<std::array<int64_t, 4>, std::array<double, 2>> fuse;
+    std::map<std::array<int64_t, 5>, std::array<double, 2>> fuse_mf;  // the same per (child, shape)
     int heff_form = 0;  // 0: the autotuner's choice, 1: always staged, 2: fused where it fits
 };
 
