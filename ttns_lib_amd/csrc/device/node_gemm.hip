@@ -401,7 +401,7 @@ __global__ __launch_bounds__(GT) void znorm_kernel(const double2* x, int64_t n, 
 // zreduce_kernel, which adds them in group order.
 constexpr int FL = 72;  // LDS row length (doubles)
 
-__global__ __launch_bounds__(GT) void zheff_fused_kernel(FusedDesc f) {
+__global__ __launch_bounds__(GT, 2) void zheff_fused_kernel(FusedDesc f) {
     __shared__ double pr[64][FL], pi[64][FL];  // Y_t[j0][i1][k] at [j0][k], then Z[i0][k] at [k][i0]
     const int i1 = blockIdx.x, g = blockIdx.y, ng = gridDim.y;
     const int t0 = static_cast<int>(static_cast<int64_t>(f.nt) * g / ng);
@@ -442,17 +442,26 @@ __global__ __launch_bounds__(GT) void zheff_fused_kernel(FusedDesc f) {
             z[q].re = d4{0.0, 0.0, 0.0, 0.0};
             z[q].im = d4{0.0, 0.0, 0.0, 0.0};
         }
-        for (int ks = 0; ks < d0; ks += 4) {
-            const int k = ks + kq;
-            double2 a[4];
+        // the operator loads of two k-steps are issued before their MFMAs
+        for (int k0 = 0; k0 < d0; k0 += 8) {
+            double2 a[2][4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int row = 16 * tm[q] + li;
-                a[q] = (q < cnt && row < d0 && k < d0) ? O[static_cast<int64_t>(row) * d0 + k] : make_double2(0.0, 0.0);
+            for (int u = 0; u < 2; ++u) {
+                const int k = k0 + 4 * u + kq;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int row = 16 * tm[q] + li;
+                    a[u][q] = (q < cnt && row < d0 && k < d0) ? O[static_cast<int64_t>(row) * d0 + k]
+                                                               : make_double2(0.0, 0.0);
+                }
             }
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (q < cnt) mfma_c(z[q], a[q].x, a[q].y, pr[k][16 * tn[q] + li], pi[k][16 * tn[q] + li]);
+            for (int u = 0; u < 2; ++u) {
+                const int k = k0 + 4 * u + kq;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (q < cnt) mfma_c(z[q], a[u][q].x, a[u][q].y, pr[k][16 * tn[q] + li], pi[k][16 * tn[q] + li]);
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -466,17 +475,24 @@ __global__ __launch_bounds__(GT) void zheff_fused_kernel(FusedDesc f) {
         __syncthreads();
         // acc += Z (LDS) x mft_t (global)
         const double2* M = f.mft + static_cast<int64_t>(t) * r * r;
-        for (int ks = 0; ks < r; ks += 4) {
-            const int k = ks + kq;
-            double2 b[4];
+        for (int k0 = 0; k0 < r; k0 += 8) {
+            double2 b[2][4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int col = 16 * tn[q] + li;
-                b[q] = (q < cnt && col < r && k < r) ? M[static_cast<int64_t>(k) * r + col] : make_double2(0.0, 0.0);
+            for (int u = 0; u < 2; ++u) {
+                const int k = k0 + 4 * u + kq;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int col = 16 * tn[q] + li;
+                    b[u][q] = (q < cnt && col < r && k < r) ? M[static_cast<int64_t>(k) * r + col] : make_double2(0.0, 0.0);
+                }
             }
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (q < cnt) mfma_c(acc[q], pr[k][16 * tm[q] + li], pi[k][16 * tm[q] + li], b[q].x, b[q].y);
+            for (int u = 0; u < 2; ++u) {
+                const int k = k0 + 4 * u + kq;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (q < cnt) mfma_c(acc[q], pr[k][16 * tm[q] + li], pi[k][16 * tm[q] + li], b[u][q].x, b[u][q].y);
+            }
         }
         __syncthreads();
     }
