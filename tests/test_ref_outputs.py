@@ -75,12 +75,14 @@ BOND_DIMENSION_RUNS = ["ohmic_debye", "ohmic_exponential", "ohmic_sudden"]
 
 
 def check_trajectory(ig, run, steps, bound):
-    for r in run["rows"][1:1 + steps]:
+    """bound: one for every step, or a sequence of per-step bounds"""
+    bounds = list(bound) if hasattr(bound, "__len__") else [bound] * steps
+    for r, b in zip(run["rows"][1:1 + steps], bounds):
         ig.step(1)
         t, sx, sy, sz, one = r["values"]
         assert abs(ig.time() - t) < 1e-15
         got = [ig.expect(0, SIGMA[q]).real for q in "xyz"]
-        assert np.abs(np.array(got) - np.array([sx, sy, sz])).max() < bound, (t, got, [sx, sy, sz])
+        assert np.abs(np.array(got) - np.array([sx, sy, sz])).max() < b, (t, got, [sx, sy, sz])
         assert abs(ig.norm2() - one) < 1e-12
 
 
