@@ -410,14 +410,19 @@ __global__ __launch_bounds__(GT, 2) void zheff_fused_kernel(FusedDesc f) {
     const int64_t size = static_cast<int64_t>(d0) * f.d1 * r;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, li = lane & 15, kq = lane >> 4;
     const int mt = (d0 + 15) >> 4, nt = (r + 15) >> 4, tiles = mt * nt;
-    // this wave's tiles wv, wv + 4, ...: row block tm[q], column block tn[q]
+    // this wave's tiles wv, wv + 4, ...  The mode-0 product's tiles (zm row block, zn column
+    // block) share the wave's row block when mt divides 4, so one op0 load per k-step serves all
+    // of them; the mean-field product's tiles (om, on) share the column block of mft likewise.
     const int cnt = tiles > wv ? min(4, (tiles - wv + 3) / 4) : 0;
-    int tm[4], tn[4];
+    const bool zshare = (4 % mt) == 0, oshare = (4 % nt) == 0;
+    int zm[4], zn[4], om[4], on[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int it = wv + 4 * q;
-        tm[q] = it / nt;
-        tn[q] = it % nt;
+        zm[q] = it % mt;
+        zn[q] = it / mt;
+        om[q] = it / nt;
+        on[q] = it % nt;
     }
     Acc acc[4];
 #pragma unroll
@@ -450,9 +455,13 @@ __global__ __launch_bounds__(GT, 2) void zheff_fused_kernel(FusedDesc f) {
                 const int k = k0 + 4 * u + kq;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const int row = 16 * tm[q] + li;
-                    a[u][q] = (q < cnt && row < d0 && k < d0) ? O[static_cast<int64_t>(row) * d0 + k]
-                                                               : make_double2(0.0, 0.0);
+                    if (q > 0 && zshare) {
+                        a[u][q] = a[u][0];
+                    } else {
+                        const int row = 16 * zm[q] + li;
+                        a[u][q] = (q < cnt && row < d0 && k < d0) ? O[static_cast<int64_t>(row) * d0 + k]
+                                                                   : make_double2(0.0, 0.0);
+                    }
                 }
             }
 #pragma unroll
@@ -460,7 +469,7 @@ __global__ __launch_bounds__(GT, 2) void zheff_fused_kernel(FusedDesc f) {
                 const int k = k0 + 4 * u + kq;
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    if (q < cnt) mfma_c(z[q], a[u][q].x, a[u][q].y, pr[k][16 * tn[q] + li], pi[k][16 * tn[q] + li]);
+                    if (q < cnt) mfma_c(z[q], a[u][q].x, a[u][q].y, pr[k][16 * zn[q] + li], pi[k][16 * zn[q] + li]);
             }
         }
         __syncthreads();
@@ -469,8 +478,8 @@ __global__ __launch_bounds__(GT, 2) void zheff_fused_kernel(FusedDesc f) {
             if (q < cnt)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    pr[16 * tn[q] + li][16 * tm[q] + kq + 4 * j] = z[q].re[j];
-                    pi[16 * tn[q] + li][16 * tm[q] + kq + 4 * j] = z[q].im[j];
+                    pr[16 * zn[q] + li][16 * zm[q] + kq + 4 * j] = z[q].re[j];
+                    pi[16 * zn[q] + li][16 * zm[q] + kq + 4 * j] = z[q].im[j];
                 }
         __syncthreads();
         // acc += Z (LDS) x mft_t (global)
@@ -482,8 +491,13 @@ __global__ __launch_bounds__(GT, 2) void zheff_fused_kernel(FusedDesc f) {
                 const int k = k0 + 4 * u + kq;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const int col = 16 * tn[q] + li;
-                    b[u][q] = (q < cnt && col < r && k < r) ? M[static_cast<int64_t>(k) * r + col] : make_double2(0.0, 0.0);
+                    if (q > 0 && oshare) {
+                        b[u][q] = b[u][0];
+                    } else {
+                        const int col = 16 * on[q] + li;
+                        b[u][q] = (q < cnt && col < r && k < r) ? M[static_cast<int64_t>(k) * r + col]
+                                                                 : make_double2(0.0, 0.0);
+                    }
                 }
             }
 #pragma unroll
@@ -491,7 +505,7 @@ __global__ __launch_bounds__(GT, 2) void zheff_fused_kernel(FusedDesc f) {
                 const int k = k0 + 4 * u + kq;
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    if (q < cnt) mfma_c(acc[q], pr[k][16 * tm[q] + li], pi[k][16 * tm[q] + li], b[u][q].x, b[u][q].y);
+                    if (q < cnt) mfma_c(acc[q], pr[k][16 * om[q] + li], pi[k][16 * om[q] + li], b[u][q].x, b[u][q].y);
             }
         }
         __syncthreads();
@@ -502,10 +516,10 @@ __global__ __launch_bounds__(GT, 2) void zheff_fused_kernel(FusedDesc f) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         if (q >= cnt) break;
-        const int col = 16 * tn[q] + li;
+        const int col = 16 * on[q] + li;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const int i0 = 16 * tm[q] + kq + 4 * j;
+            const int i0 = 16 * om[q] + kq + 4 * j;
             if (i0 < d0 && col < r) {
                 double2* p = out + (static_cast<int64_t>(i0) * f.d1 + i1) * r + col;
                 double2 v = make_double2(acc[q].re[j], acc[q].im[j]);
@@ -536,14 +550,17 @@ __global__ __launch_bounds__(GT, 2) void zmf_fused_kernel(MfDesc f) {
     const int tz = md * mr, to = md * md;  // tiles of Z_s (dc x r) and of out (dc x dc)
     const int cz = tz > wv ? min(4, (tz - wv + 3) / 4) : 0;
     const int co = to > wv ? min(4, (to - wv + 3) / 4) : 0;
+    // both products take their L2 operand by rows (Y_s rows, A_s rows): the tiles of a wave share
+    // its row block when md divides 4, so one load per k-step serves all of them
+    const bool share = (4 % md) == 0;
     int zm[4], zn[4], om[4], on[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int it = wv + 4 * q;
-        zm[q] = it / mr;
-        zn[q] = it % mr;
-        om[q] = it / md;
-        on[q] = it % md;
+        zm[q] = it % md;
+        zn[q] = it / md;
+        om[q] = it % md;
+        on[q] = it / md;
     }
     Acc acc[4];
 #pragma unroll
@@ -570,7 +587,10 @@ __global__ __launch_bounds__(GT, 2) void zmf_fused_kernel(MfDesc f) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int row = 16 * zm[q] + li, col = 16 * zn[q] + li;
-                    a[u][q] = (q < cz && row < dc && k < r) ? Ys[row * f.si + k] : make_double2(0.0, 0.0);
+                    if (q > 0 && share)
+                        a[u][q] = a[u][0];
+                    else
+                        a[u][q] = (q < cz && row < dc && k < r) ? Ys[row * f.si + k] : make_double2(0.0, 0.0);
                     b[u][q] = (q < cz && col < r && k < r) ? M[static_cast<int64_t>(col) * r + k] : make_double2(0.0, 0.0);
                 }
             }
@@ -600,7 +620,10 @@ __global__ __launch_bounds__(GT, 2) void zmf_fused_kernel(MfDesc f) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int row = 16 * om[q] + li;
-                    a[u][q] = (q < co && row < dc && k < r) ? As[row * f.si + k] : make_double2(0.0, 0.0);
+                    if (q > 0 && share)
+                        a[u][q] = a[u][0];
+                    else
+                        a[u][q] = (q < co && row < dc && k < r) ? As[row * f.si + k] : make_double2(0.0, 0.0);
                 }
             }
 #pragma unroll
