@@ -674,7 +674,9 @@ extern "C" hipError_t ttnd_zmft(const double2* coeff, const double2* mf, int64_t
 }
 extern "C" hipError_t ttnd_zmf_fused(const ttnd::MfDesc* f, int nchunks, hipStream_t s) {
     using namespace ttnd;
-    if (f->dc > 64 || f->r > 64 || f->dc < 1 || f->r < 1 || nchunks < 1 || nchunks > f->ns) return hipErrorInvalidValue;
+    if (f->dc > 64 || f->r > 64 || f->dc < 1 || f->r < 1 || f->nt < 1 || f->nt > 65535 || nchunks < 1 ||
+        nchunks > f->ns)
+        return hipErrorInvalidValue;
     hipLaunchKernelGGL(zmf_fused_kernel, dim3(nchunks, f->nt), dim3(GT), 0, s, *f);
     if (nchunks > 1) {
         hipError_t e = hipGetLastError();

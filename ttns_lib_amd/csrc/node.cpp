@@ -416,7 +416,8 @@ void mean_field(ttn_node* nd, int child, double2* res, bool fused) {
 bool use_fused_mf(ttn_node* nd, int child) {
     ttn_ctx* ctx = nd->ctx;
     const int64_t dc = child == 0 ? nd->d0 : nd->d1;
-    if (dc > 64 || nd->r > 64 || ctx->heff_form == 1) return false;
+    // the fused kernel puts the terms on the grid's second dimension (at most 65,535)
+    if (dc > 64 || nd->r > 64 || nd->nt > 65535 || ctx->heff_form == 1) return false;
     if (ctx->heff_form == 2) return true;
     if (!ctx->autotune) return false;
     const std::array<int64_t, 5> key = {child, nd->d0, nd->d1, nd->r, nd->nt};
