@@ -1178,7 +1178,9 @@ __device__ void bond_update(Ctx& c, SweepState& st, int n, int ops) {
         });
         count_small(c, pt0);
     } else {
-        on_team(c, static_cast<int64_t>(r) * r, [&](Ctx& cc) {
+        // a bond propagation spreads over the team from half the element count of other
+        // operations: its H-action is r x r x r products, more work per entry
+        on_team(c, 2 * static_cast<int64_t>(r) * r, [&](Ctx& cc) {
             na = expv(
                 cc,
                 [&](const double2* v, double2* res) {
