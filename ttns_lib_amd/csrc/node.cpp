@@ -125,6 +125,39 @@ void launch_gemm(const NodeView& v, ttnd::GemmDesc g, int tile, int ns) {
 // no split (those whose partial products fit the workspace), into a scratch output, and keeps
 // the fastest for the shape.  Every tile sums each output element in the same order, so the
 // choice of tile never changes results; a different K split changes their rounding.
+// A pair of HIP events for timing work on a stream, released on every path out.
+class EventTimer {
+public:
+    EventTimer() {
+        hip_check(hipEventCreate(&a_), "event");
+        const hipError_t e = hipEventCreate(&b_);
+        if (e != hipSuccess) {
+            (void)hipEventDestroy(a_);
+            hip_check(e, "event");
+        }
+    }
+    ~EventTimer() {
+        (void)hipEventDestroy(a_);
+        (void)hipEventDestroy(b_);
+    }
+    EventTimer(const EventTimer&) = delete;
+    EventTimer& operator=(const EventTimer&) = delete;
+    // device time of f() in ms
+    template <class F>
+    float time(hipStream_t s, F&& f) {
+        hip_check(hipEventRecord(a_, s), "event");
+        f();
+        hip_check(hipEventRecord(b_, s), "event");
+        hip_check(hipEventSynchronize(b_), "sync");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, a_, b_), "elapsed");
+        return ms;
+    }
+
+private:
+    hipEvent_t a_ = nullptr, b_ = nullptr;
+};
+
 std::array<int64_t, 8> tune_key(const ttnd::GemmDesc& g) {
     return {g.M, g.N, g.K, static_cast<int64_t>(g.nb0) * g.nb1, g.kb, g.a_k == 1 ? 1 : 0, g.b_c == 1 ? 1 : 0,
             g.accumulate};
@@ -136,9 +169,7 @@ std::array<double, 3> tune(const NodeView& v, const ttnd::GemmDesc& g) {
     if (static_cast<int64_t>(c->tune_out.n) < extent) c->tune_out.alloc(extent);
     ttnd::GemmDesc t = g;
     t.C = c->tune_out.p;
-    hipEvent_t e0, e1;
-    hip_check(hipEventCreate(&e0), "event");
-    hip_check(hipEventCreate(&e1), "event");
+    EventTimer timer;
     std::array<double, 3> best = {static_cast<double>(ttnd::TILE_64x64), 1.0, 1e300};
     const int64_t batch = static_cast<int64_t>(g.nb0) * g.nb1;
     const int64_t chunks = (g.K + KCHUNK - 1) / KCHUNK;
@@ -152,20 +183,10 @@ std::array<double, 3> tune(const NodeView& v, const ttnd::GemmDesc& g) {
             if (seen || (ns > 1 && split_need(g, ns) > v.ws_n)) continue;
             launch_gemm(v, t, tile, ns);  // warm-up
             float ms = 1e30f;
-            for (int rep = 0; rep < 3; ++rep) {
-                hip_check(hipEventRecord(e0, v.s), "event");
-                launch_gemm(v, t, tile, ns);
-                hip_check(hipEventRecord(e1, v.s), "event");
-                hip_check(hipEventSynchronize(e1), "sync");
-                float x = 0.f;
-                hip_check(hipEventElapsedTime(&x, e0, e1), "elapsed");
-                ms = std::min(ms, x);
-            }
+            for (int rep = 0; rep < 3; ++rep) ms = std::min(ms, timer.time(v.s, [&] { launch_gemm(v, t, tile, ns); }));
             if (ms < best[2]) best = {static_cast<double>(tile), static_cast<double>(ns), static_cast<double>(ms)};
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     return best;
 }
 
@@ -260,9 +281,9 @@ void stage_mf_sum(const NodeView& v, double2* res, bool accumulate) {
 }
 
 // Stages 2 and 3 fused (zheff_fused_kernel): term groups so that d1 x groups fills the GPU at two
-// workgroups per CU, fewer when their partial sums would not fit the workspace.  Returns false if the shape does
-// not fit the kernel.
-bool fused_mf(const NodeView& v, double2* res, bool accumulate) {
+// workgroups per CU, fewer when their partial sums would not fit the workspace.  Returns false if
+// the shape does not fit the kernel.
+bool fused_tail(const NodeView& v, double2* res, bool accumulate) {
     if (v.d0 > 64 || v.r > 64) return false;
     int groups = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(v.nt, (512 + v.d1 - 1) / v.d1)));
     while (groups > 1 && groups * v.D * v.r > v.ws_n) --groups;
@@ -295,27 +316,20 @@ bool use_fused(const NodeView& v) {
         if (!v.alloc_ok) return false;
         DevBuf<double2> scratch;
         scratch.alloc(v.D * v.r);
-        hipEvent_t e0, e1;
-        hip_check(hipEventCreate(&e0), "event");
-        hip_check(hipEventCreate(&e1), "event");
+        EventTimer timer;
         float best[2] = {1e30f, 1e30f};
         for (int form = 0; form < 2; ++form)
             for (int rep = 0; rep < 4; ++rep) {
-                hip_check(hipEventRecord(e0, v.s), "event");
-                if (form == 0) {
-                    stage_mode0(v, v.Y, v.size, v.Z);
-                    stage_mf_sum(v, scratch.p, false);
-                } else {
-                    fused_mf(v, scratch.p, false);
-                }
-                hip_check(hipEventRecord(e1, v.s), "event");
-                hip_check(hipEventSynchronize(e1), "sync");
-                float ms = 0.f;
-                hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
-                if (rep > 0) best[form] = std::min(best[form], ms);
+                const float ms = timer.time(v.s, [&] {
+                    if (form == 0) {
+                        stage_mode0(v, v.Y, v.size, v.Z);
+                        stage_mf_sum(v, scratch.p, false);
+                    } else {
+                        fused_tail(v, scratch.p, false);
+                    }
+                });
+                if (rep > 0) best[form] = std::min(best[form], ms);  // the first is a warm-up
             }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         const int form = best[1] < best[0] ? 1 : 0;
         it = v.ctx->fuse.emplace(key, std::array<double, 2>{static_cast<double>(form), best[form]}).first;
     }
@@ -324,7 +338,7 @@ bool use_fused(const NodeView& v) {
 
 void heff(const NodeView& v, const double2* x, double2* res) {
     stage_mode1(v, x);
-    if (use_fused(v) && fused_mf(v, res, false)) return;
+    if (use_fused(v) && fused_tail(v, res, false)) return;
     stage_mode0(v, v.Y, v.size, v.Z);
     stage_mf_sum(v, res, false);
 }
@@ -425,22 +439,13 @@ bool use_fused_mf(ttn_node* nd, int child) {
     if (it == ctx->fuse_mf.end()) {
         DevBuf<double2> scratch;
         scratch.alloc(static_cast<size_t>(nd->nt * dc * dc));
-        hipEvent_t e0, e1;
-        hip_check(hipEventCreate(&e0), "event");
-        hip_check(hipEventCreate(&e1), "event");
+        EventTimer timer;
         float best[2] = {1e30f, 1e30f};
         for (int form = 0; form < 2; ++form)
             for (int rep = 0; rep < 4; ++rep) {
-                hip_check(hipEventRecord(e0, ctx->stream), "event");
-                mean_field(nd, child, scratch.p, form == 1);
-                hip_check(hipEventRecord(e1, ctx->stream), "event");
-                hip_check(hipEventSynchronize(e1), "sync");
-                float ms = 0.f;
-                hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
-                if (rep > 0) best[form] = std::min(best[form], ms);
+                const float ms = timer.time(ctx->stream, [&] { mean_field(nd, child, scratch.p, form == 1); });
+                if (rep > 0) best[form] = std::min(best[form], ms);  // the first is a warm-up
             }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         const int form = best[1] < best[0] ? 1 : 0;
         it = ctx->fuse_mf.emplace(key, std::array<double, 2>{static_cast<double>(form), best[form]}).first;
     }
