@@ -456,7 +456,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=2,
+                    help="untimed steps: the first (the first-step ladder) in a launch of its own, then the rest")
     ap.add_argument("--workload", default="c3")
     ap.add_argument("--plain-sample", type=int, default=8, help="instances stepped alone for the plain GPU rate")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle time for the CPU baseline")
@@ -492,7 +493,12 @@ def main():
     igs = [make(lib, i) for i in insts]
     batch = Batch(lib, igs)
     t0 = time.perf_counter()
-    batch.step(warm)
+    # the first step (the first-step ladder, six sub-steps) in a launch of its own, then the rest:
+    # the team-size autotuner sizes the next launch's teams from the last launch's device times,
+    # which should be those of regular steps
+    batch.step(1)
+    if warm > 1:
+        batch.step(warm - 1)
     print(f"[bench] warmup {warm} step(s) {time.perf_counter() - t0:.2f}s", file=sys.stderr, flush=True)
     # the plain-path sample: the slowest, fastest and median instances by device time so far,
     # checkpointed here so that each can later take the first timed step alone

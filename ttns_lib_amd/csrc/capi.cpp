@@ -813,8 +813,8 @@ std::vector<int> batch_teams(const ttn_batch* b) {
     const int cap = c->nxcd * c->xcd_cap;
     auto time = [&](size_t i, int ti) { return b->cost[i] * (s[i] + (1.0 - s[i]) / ti); };
     // an instance takes another workgroup only while that cuts its predicted time by at
-    // least 3% (every member adds barrier traffic)
-    auto grows = [&](size_t i) { return time(i, t[i] + 1) < 0.97 * time(i, t[i]); };
+    // least team_gain (every member adds barrier traffic)
+    auto grows = [&](size_t i) { return time(i, t[i] + 1) < (1.0 - c->team_gain) * time(i, t[i]); };
     while (used < cap) {
         size_t best = n;
         for (size_t i = 0; i < n; ++i)
@@ -861,8 +861,10 @@ int ttn_ctx_create(int device, ttn_ctx** out) {
         ttnrt_noalloc_setup(c.get());
         // tuning knobs: TTN_TEAM_MIN (elements below which an operation runs on a team's
         // leader alone), TTN_TEAMS=0 switches teams off, TTN_SMALL_MAX (largest propagation
-        // run in one workgroup's LDS, 0: none)
+        // run in one workgroup's LDS, 0: none), TTN_TEAM_GAIN (smallest predicted relative gain
+        // for which the team-size autotuner adds a member)
         if (const char* e = std::getenv("TTN_TEAM_MIN")) c->team_min = std::max<int64_t>(1, std::atoll(e));
+        if (const char* e = std::getenv("TTN_TEAM_GAIN")) c->team_gain = std::max(0.0, std::atof(e));
         if (const char* e = std::getenv("TTN_SMALL_MAX")) c->small_max = std::max<int64_t>(0, std::atoll(e));
         if (const char* e = std::getenv("TTN_TEAMS"))
             if (std::atoi(e) == 0) c->teams_ok = false;

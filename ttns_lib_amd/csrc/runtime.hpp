@@ -56,6 +56,7 @@ struct ttn_ctx {
     bool teams_ok = false;
     int grid = 0, nxcd = 0, xcd_cap = 0;
     int64_t team_min = 2048;      // smaller operations run on a team's leader alone
+    double team_gain = 0.01;      // a team grows only while a member cuts its predicted time by this much
     int64_t small_max = 512;      // propagations of at most this many entries run in one workgroup's LDS
     ttnrt::DevBuf<unsigned> team_words;  // XCD slot counters (MAX_XCD lines) + TeamWords per instance
     int64_t team_words_n = 0;     // instances the buffer holds
