@@ -267,15 +267,15 @@ __device__ void team_reduce_lds(Team& T, double* v, int count) {
     T.sync();
     const double* all = T.part + static_cast<int64_t>(T.pbuf) * TEAM_MAX * PART_K;
     for (int e = threadIdx.x; e < count; e += NT) {
-        // the members' partials are loaded 8 at a time (independent loads in flight) and
-        // added in rank order
+        // the members' partials are loaded 16 at a time (independent loads in flight: two
+        // round trips for a team of a whole XCD) and added in rank order
         double s = 0.0;
-        for (int r0 = 0; r0 < T.size; r0 += 8) {
-            double p[8];
+        for (int r0 = 0; r0 < T.size; r0 += 16) {
+            double p[16];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) p[u] = r0 + u < T.size ? ldc(all + (r0 + u) * PART_K + e) : 0.0;
+            for (int u = 0; u < 16; ++u) p[u] = r0 + u < T.size ? ldc(all + (r0 + u) * PART_K + e) : 0.0;
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
+            for (int u = 0; u < 16; ++u)
                 if (r0 + u < T.size) s += p[u];
         }
         v[e] = s;
