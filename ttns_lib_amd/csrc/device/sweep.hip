@@ -558,20 +558,21 @@ __device__ int64_t expv(Ctx& c, HF&& H, double2* x, int64_t n, double2 coeff, do
                 nn[0] += cabs2(w0) + cabs2(w1);
             }
             team_sum<1>(T, nn, nno);
+            hn = sqrt(nno[0]);
+            // column j of the Hessenberg matrix and its subdiagonal entry (different entries),
+            // published by one barrier
             if (static_cast<int>(threadIdx.x) <= j) {
                 const int i = threadIdx.x;
                 g_sh.Hm[i * MAXK + j] =
                     cadd(cadd(g_sh.Hm[i * MAXK + j], cmk(h1[2 * i], h1[2 * i + 1])), cmk(h2[2 * i], h2[2 * i + 1]));
             }
+            if (j + 1 < kdim && threadIdx.x == 0) g_sh.Hm[(j + 1) * MAXK + j] = cmk(hn, 0.0);
             __syncthreads();
             prof_add(c, PROF_ORTH, ot0);
-            hn = sqrt(nno[0]);
             if (!isfinite(hn)) {
                 c.fail(FAIL_KRYLOV_NONFINITE);
                 return napply;
             }
-            if (j + 1 < kdim && threadIdx.x == 0) g_sh.Hm[(j + 1) * MAXK + j] = cmk(hn, 0.0);
-            __syncthreads();
             m = j + 1;
             // leading Taylor term of the Saad estimate (prod of subdiagonal entries / j!);
             // the exponential is evaluated only when it is within 1e3 of the tolerance
@@ -1036,21 +1037,21 @@ __device__ int64_t expv_small(Ctx& c, int n_op, bool bond, double2* x, int n, do
 #pragma unroll
             for (int e = 0; e < SMALL_E; ++e) nn[0] += cabs2(wv[e]);
             block_sum<1>(nn, nno);
+            hn = sqrt(nno[0]);
+            // column j of the Hessenberg matrix and its subdiagonal entry, one barrier
             if (static_cast<int>(threadIdx.x) <= j) {
                 const int i = threadIdx.x;
                 g_sh.Hm[i * MAXK + j] = cadd(cadd(g_sh.Hm[i * MAXK + j], cmk(g_sh.hb[0][2 * i], g_sh.hb[0][2 * i + 1])),
                                            cmk(g_sh.hb[1][2 * i], g_sh.hb[1][2 * i + 1]));
             }
+            if (j + 1 < kdim && threadIdx.x == 0) g_sh.Hm[(j + 1) * MAXK + j] = cmk(hn, 0.0);
             __syncthreads();
             prof_add(c, PROF_ORTH, ot0);
-            hn = sqrt(nno[0]);
             if (!isfinite(hn)) {
                 c.fail(FAIL_KRYLOV_NONFINITE);
                 ok = false;
                 break;
             }
-            if (j + 1 < kdim && threadIdx.x == 0) g_sh.Hm[(j + 1) * MAXK + j] = cmk(hn, 0.0);
-            __syncthreads();
             m = j + 1;
             double lead = hn;
             for (int i = 0; i < j; ++i)
